@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Trial evaluations per second of the island DE engine on one MI355X.
+
+Workload (one "step" = one generation over every island): 64 islands x
+NP=128, D=100, DE/rand/1/bin with F=0.5, CR=0.9, bounds [-100, 100];
+objectives assigned round-robin per island: shifted-rotated elliptic,
+Rastrigin and Ackley, each with its own seeded o ~ U[-80, 80]^D and Haar M;
+ring migration of the best 2 every 50 generations among the islands of one
+objective; FP32 evaluation.  Synthetic data, populations resident in HBM.
+
+Prints one JSON line with the throughput, the dominant kernel's roofline
+(kernel time measured with HIP events on the engine's stream) and a CPU
+baseline timed on this host (the reference's own CEC2014::EvaluateFitness
+built from its source in oracle/_ref when present, plus the oracle's DE step).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "speo-gpu-single-control_amd"))
+sys.path.insert(0, str(REPO / "oracle"))
+
+METRIC = "trial evals/sec + time-to-error≤1e-2 at D=100, 1 GPU: surrogate/batched vs plain"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+BASE_SEED = 2026
+
+
+def build_engine(S, args):
+    D, I, NP = args.dim, args.islands, args.island_size
+    objs = [S.OBJ_ELLIPTIC, S.OBJ_RASTRIGIN, S.OBJ_ACKLEY]
+    problems, data = [], []
+    for k, obj in enumerate(objs):
+        o, M = S.generate_problem_data(S.problem_seed(BASE_SEED, k), D)
+        problems.append(S.Problem([obj], o[None, :], M[None, :, :]))
+        data.append((obj, o, M))
+    prec = {v: k for k, v in S.PREC_NAMES.items()}[args.precision]
+    mode = S.MODE_FUSED if args.mode == "fused" else S.MODE_PLAIN
+    eng = S.Engine(problems, [NP] * I, [i % 3 for i in range(I)], F=0.5, CR=0.9, seed=BASE_SEED,
+                   interval=50, migration_size=2, precision=prec, mode=mode,
+                   use_graph=not args.no_graph, rows_per_block=args.rows_per_block)
+    return eng, problems, data
+
+
+def cpu_baseline(data, seconds, D, NP):
+    """Single-thread CPU trial evals/s: objective evaluation + DE step."""
+    import oracle as O
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-100, 100, D)
+    eval_rates, eval_rates_o0 = {}, {}
+    kind = "reference" if O.ref_available() else "port"
+    fid_of = {O.OBJ_ELLIPTIC: 1, O.OBJ_ACKLEY: 5, O.OBJ_RASTRIGIN: 9}
+    for obj, o, M in data:
+        fid = fid_of[obj]
+        if kind == "reference":
+            with tempfile.TemporaryDirectory() as td:
+                O.write_cec_files(td, fid, o, M)
+                eval_rates[fid] = O.ref_time(td, fid, x, seconds, "O2")
+                if O.ref_binary("O0").exists():
+                    eval_rates_o0[fid] = O.ref_time(td, fid, x, seconds / 3, "O0")
+        else:
+            n, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < seconds:
+                O.evaluate(obj, x[None, :], o, M)
+                n += 1
+            eval_rates[fid] = n / (time.perf_counter() - t0)
+    # DE step (generate + select) of one island, FP32, the oracle's C code
+    X = O.init_population(NP, D, D, -100.0, 100.0, 1)
+    fX = rng.uniform(0, 1, NP).astype(np.float32)
+    n, t0, gen = 0, time.perf_counter(), 0
+    while time.perf_counter() - t0 < seconds:
+        U, *_ = O.de_generate(X, D, NP, D, seed=1, gen=gen)
+        fU = rng.uniform(0, 1, NP).astype(np.float32)
+        X, fX, _ = O.de_select(X, fX, U, fU, D, NP, D)
+        n += NP
+        gen += 1
+    de_rate = n / (time.perf_counter() - t0)
+    t_eval = float(np.mean([1.0 / r for r in eval_rates.values()]))
+    value = 1.0 / (t_eval + 1.0 / de_rate)
+    out = {"value": value, "unit": "trial evals/s", "cores": 1, "kind": kind,
+           "sample": (f"{'reference CEC2014::EvaluateFitness (-O2, built from CEC2014.cc)' if kind == 'reference' else 'oracle FP64 evaluation'}"
+                      f" on this workload's elliptic/Rastrigin/Ackley at D={D}, ~{seconds:.0f} s each,"
+                      f" plus the oracle's FP32 DE/rand/1/bin step on one NP={NP} island for ~{seconds:.0f} s;"
+                      f" 1 thread"),
+           "eval_per_s": {f"F{k}": v for k, v in eval_rates.items()},
+           "de_step_trials_per_s": de_rate}
+    if eval_rates_o0:
+        t0e = float(np.mean([1.0 / r for r in eval_rates_o0.values()]))
+        out["value_reference_O0"] = 1.0 / (t0e + 1.0 / de_rate)
+    try:
+        import platform
+        out["host_cpu"] = platform.processor() or open("/proc/cpuinfo").read().split(
+            "model name")[1].split("\n")[0].strip(": ")
+    except Exception:
+        pass
+    out["host_nproc"] = os.cpu_count()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="timed generations")
+    ap.add_argument("--warmup", type=int, default=50, help="untimed generations")
+    ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--precision", default="fp32",
+                    choices=["fp32", "bf16", "fp16", "bf16x3", "fp16x3", "bf16x6"])
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--rows-per-block", type=int, default=0)
+    ap.add_argument("--islands", type=int, default=64)
+    ap.add_argument("--island-size", type=int, default=128)
+    ap.add_argument("--dim", type=int, default=100)
+    ap.add_argument("--profile-gens", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=4.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: every box in this build has a single MI355X, "
+              "and the multi-GPU island exchange is not built; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import speo_de as S
+
+    S.check(S.lib().speo_device_check())
+    eng, problems, data = build_engine(S, args)
+    rows = eng.total_rows
+    D = args.dim
+    print(f"# engine ready: {args.islands} islands x {args.island_size}, D={D}, "
+          f"mode={args.mode}, precision={args.precision}, graph={not args.no_graph}, "
+          f"rows_per_block={eng.stats()['rows_per_block']}", file=sys.stderr, flush=True)
+
+    eng.run(args.warmup)
+    eng.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.run(args.steps)
+    eng.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = args.steps * rows / elapsed
+    print(f"# timed {args.steps} generations in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # best-so-far, re-verified in FP64, per objective
+    _, f32, f64 = eng.best()
+    pidx = np.arange(args.islands) % 3
+    names = {0: "elliptic", 1: "rastrigin", 2: "ackley"}
+    final = {names[k]: float(f64[pidx == k].min()) for k in range(3) if (pidx == k).any()}
+
+    kms = eng.profile(args.profile_gens)
+    dominant = "fused" if args.mode == "fused" else "evaluate"
+    kernel_ms = kms[dominant]
+    flops = 2.0 * D * D * rows
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    de_bytes = 4.0 * (3 * D + 3) * rows
+    traffic = None
+    tfile = REPO / "profiles" / f"traffic_{args.mode}_{args.precision}_i{args.islands}_np{args.island_size}_d{D}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "trial evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else args.precision,
+        "data": "synthetic (seeded populations, shifts and Haar rotations)",
+        "config": {
+            "workload": (f"{args.islands} islands x NP={args.island_size}, D={D}, DE/rand/1/bin "
+                         "F=0.5 CR=0.9, elliptic/Rastrigin/Ackley round-robin, ring best-2 "
+                         "every 50 generations"),
+            "islands": args.islands, "island_size": args.island_size, "dim": D,
+            "mode": args.mode, "precision": args.precision, "graph": not args.no_graph,
+            "rows_per_block": eng.stats()["rows_per_block"],
+        },
+        "roofline": {
+            "kernel": "k_generation (" + dominant + ")",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic,
+            "algorithmic": f"2*D^2 = {2 * D * D} flop per trial x {rows} trials per launch",
+            "kernel_ms": kernel_ms,
+        },
+        "hbm_view": {
+            "de_bytes_per_launch": de_bytes,
+            "achieved_GBs": de_bytes / (kernel_ms * 1e-3) / 1e9,
+            "peak_GBs": PEAK_HBM_GBS,
+            "frac": de_bytes / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+        },
+        "kernel_ms": kms,
+        "final_error_f64": final,
+        "generations_total": eng.stats()["generations"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(data, args.cpu_seconds, D, args.island_size)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,321 @@
+/*
+ * speo_de.h -- C ABI of the MI355X island-model DE/rand/1/bin engine.
+ *
+ * Two layers, both plain C (raw pointers, sizes, POD structs by value):
+ *
+ *  1. Drop-in entry points with the reference's names and roles.  The
+ *     reference's host classes EA_CUDA / DE_CUDA / CEC2014_CUDA call these
+ *     (CUDA_EA_library/EA_CUDA.cc:41-43, DE_CUDA.cc:210-212,
+ *     CEC2014_CUDA.cc:292-298).  They keep the reference's population layout
+ *     (row stride = next power of two of dim, at least 32; EA_CUDA.cc:197-206)
+ *     and take the reference built with GPU_SINGLE_PRECISION (real = float,
+ *     config.h:35-37).  The one signature change: cuRAND's device state array
+ *     becomes a host-side counter-based Philox4x32-10 state (SPEO_RandState).
+ *     Errors are returned (or recorded for the void API_evaluateFitness), never
+ *     exit(); API_CheckCUDAError() synchronises and reports them.
+ *
+ *  2. The native engine: islands of uneven size and dimension held in one
+ *     GPU's HBM, a fused select/mutate/crossover/evaluate generation kernel,
+ *     ring migration in HBM, FP64 re-verification of each island's best,
+ *     split-precision rotation, hipGraph capture and an MLP surrogate.
+ *     Every call takes a HIP stream (NULL = the default stream) and is
+ *     stream-ordered; nothing synchronises unless its name says so.
+ *
+ * Status codes: 0 = success, negative = SPEO_ERR_*; speo_last_error() returns
+ * the message of the last failure on the calling thread.
+ */
+#ifndef SPEO_DE_H_
+#define SPEO_DE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPEO_EXPORT __attribute__((visibility("default")))
+
+typedef struct ihipStream_t *speo_stream_t; /* == hipStream_t */
+
+enum {
+    SPEO_OK = 0,
+    SPEO_ERR_ARG = -1,         /* invalid argument (size, pointer, layout)   */
+    SPEO_ERR_HIP = -2,         /* a HIP runtime call or kernel launch failed */
+    SPEO_ERR_UNSUPPORTED = -3, /* valid in the reference, not built here      */
+    SPEO_ERR_NO_DEVICE = -4,   /* no usable gfx950 device                     */
+    SPEO_ERR_ALLOC = -5        /* device or host allocation failed            */
+};
+
+/* ==================================================================== *
+ *  1. Drop-in layer (reference names and struct layouts, real = float)  *
+ * ==================================================================== */
+
+typedef float real;
+
+/* CUDA_EA_library/EA_CUDA.h:21-25 */
+typedef struct {
+    real *elements;      /* [island_size][next_pow2(dim)], padding = 0 */
+    real *fitness_value; /* [island_size] */
+} GPU_Population;
+
+/* CUDA_EA_library/EA_CUDA.h:26-31 */
+typedef struct {
+    int island_size;
+    int subisland_num;
+    int *permutated_index; /* device, [island_size] */
+} GPU_IslandInfo;
+
+/* CUDA_EA_library/EA_CUDA.h:33-39 */
+typedef struct {
+    real CR;
+    real F;
+    int strategy_ID; /* 0..8, CUDA_EA_library/DE_CUDA.cc:36-53; 4 = rand/1/bin */
+    int pop_size;
+} DEInfo;
+
+/* CUDA_EA_library/config.h:48-59 */
+typedef struct {
+    int dim;
+    int function_ID;
+    int run_ID;
+    int max_base_FEs;
+    int seed;
+    int running_time;
+    int computing_time;
+    real max_bound;
+    real min_bound;
+} ProblemInfo;
+
+/* Replaces `curandState *d_rand_states` (EA_CUDA.h:61).  Counter-based:
+ * draw k of individual i in generation g is Philox4x32-10 with key = seed
+ * and counter = {k/4, row_base + i, g, purpose}.  The library advances
+ * `generation` once per API_DE_GenerateNewPopulation call. */
+typedef struct {
+    uint64_t seed;
+    uint32_t generation;
+    uint32_t row_base;
+} SPEO_RandState;
+
+/* Row stride of the drop-in layout: next power of two >= dim, at least 32
+ * (CUDA_EA_library/EA_CUDA.cc:197-206). */
+SPEO_EXPORT int SPEO_NextPow2Dim(int dim);
+
+/* Replaces API_Initilize (CUDA_EA_library/EA_CUDA.h:41-42,
+ * kernel_EA_CUDA.cu:121-134): seeds *rand_state from problem_info.seed and
+ * fills d_population.elements with min + u*(max-min), u in (0,1]. */
+SPEO_EXPORT int API_Initilize(GPU_Population d_population, SPEO_RandState *rand_state,
+                              GPU_IslandInfo GPU_island_info, ProblemInfo problem_info);
+
+/* Replaces API_CheckCUDAError (CUDA_EA_library/EA_CUDA.h:43-44,
+ * kernel_EA_CUDA.cu:64-69): synchronises the device and returns the first
+ * recorded or pending error (0 if none), instead of exit(-1). */
+SPEO_EXPORT int API_CheckCUDAError(void);
+
+/* Replaces API_DE_GenerateNewPopulation (CUDA_EA_library/DE_CUDA.cc:2-3,
+ * kernel_DE_CUDA.cu:291-302): one DE generation of d_population into
+ * d_candidate.  d_best_individual_ID (device, [subisland_num]) is filled with
+ * each sub-island's best slot (used by strategies 2, 3, 6, 7).  d_DE_info is
+ * a device array of subisland_num entries.  Every individual is processed,
+ * also when island_size is not a multiple of 16. */
+SPEO_EXPORT int API_DE_GenerateNewPopulation(GPU_Population d_candidate,
+                                             GPU_Population d_population,
+                                             int *d_best_individual_ID,
+                                             GPU_IslandInfo d_island_info, DEInfo *d_DE_info,
+                                             SPEO_RandState *rand_state,
+                                             ProblemInfo problem_info);
+
+/* Replaces API_DE_SelectSurvival (CUDA_EA_library/DE_CUDA.cc:4-5,
+ * kernel_DE_CUDA.cu:329-338): slot k takes the candidate when its fitness is
+ * strictly smaller. */
+SPEO_EXPORT int API_DE_SelectSurvival(GPU_Population d_population, GPU_Population d_candidate,
+                                      GPU_IslandInfo GPU_island_info, ProblemInfo problem_info);
+
+/* Replaces API_evaluateFitness (CUDA_EA_library/CEC2014_CUDA.h:35-37,
+ * kernel_CEC2014_CUDA.cu:1672-1797) for function_ID 1 (rotated elliptic),
+ * 5 (rotated Ackley), 8 (shifted Rastrigin, rate 0.0512) and 9 (rotated
+ * Rastrigin, rate 0.0512), in the reference's layouts: d_original
+ * [pop_size][next_pow2(dim)], d_shift [next_pow2(dim)], d_rotated_data =
+ * M transposed and row-padded, [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255).
+ * The shift-rotate GEMM and the objective run as one fused MFMA kernel, so
+ * d_shuffled_elements, d_rotated_elements, d_shuffle_data, d_tmp_fitness,
+ * d_wi and d_sigma are accepted and unused.  Other IDs record
+ * SPEO_ERR_UNSUPPORTED, reported by API_CheckCUDAError(). */
+SPEO_EXPORT void API_evaluateFitness(real *d_fitness_value, real *d_shuffled_elements,
+                                     real *d_rotated_elements, real *d_original_elements,
+                                     real *d_shift_data, real *d_rotated_data,
+                                     int *d_shuffle_data, real *d_tmp_fitness, real *d_wi,
+                                     real *d_sigma, int pop_size, int problem_dim, real bias,
+                                     int function_ID);
+
+/* ==================================================================== *
+ *  2. Native engine                                                     *
+ * ==================================================================== */
+
+SPEO_EXPORT const char *speo_last_error(void);
+SPEO_EXPORT int speo_device_check(void); /* SPEO_OK if a gfx950 device is current */
+
+/* ---- objectives ---------------------------------------------------- */
+
+enum { SPEO_OBJ_ELLIPTIC = 1, SPEO_OBJ_RASTRIGIN = 2, SPEO_OBJ_ACKLEY = 3 };
+
+/* Precision of the rotation GEMM (accumulation is always FP32). */
+enum {
+    SPEO_PREC_FP32 = 0,   /* FP32 operands, v_mfma_f32_16x16x4_f32          */
+    SPEO_PREC_BF16 = 1,   /* BF16 operands                                   */
+    SPEO_PREC_FP16 = 2,   /* FP16 operands (x and M scaled to FP16 range)    */
+    SPEO_PREC_BF16X3 = 3, /* split BF16: hi*hi + hi*lo + lo*hi               */
+    SPEO_PREC_FP16X3 = 4, /* split FP16, same three products                 */
+    SPEO_PREC_BF16X6 = 5  /* three-part BF16 split, six products (~FP32)     */
+};
+
+typedef struct speo_problem_s *speo_problem;
+
+/*
+ * A shifted-rotated problem of dimension `dim` with `ncomp` components.
+ * ncomp == 1: f(x) = lambda_0 * g_0(rate * M_0 (x - o_0)) + bias_0.
+ * ncomp > 1: the composition of CEC2014.cc:1481-1519 (cf_cal) over
+ * components c with sigma[c], bias[c], lambda[c].
+ * shift: [ncomp][dim] (FP64; stored on the device in FP32 and FP64),
+ * rotation: [ncomp][dim][dim] row-major, z = M (x - o); NULL = identity.
+ * sigma/bias/lambda may be NULL (10 / 0 / 1).  rate: NULL = 1.
+ */
+SPEO_EXPORT int speo_problem_create(int dim, int ncomp, const int *objectives,
+                                    const double *shift, const double *rotation,
+                                    const double *sigma, const double *bias,
+                                    const double *lambda, const double *rate,
+                                    speo_problem *out);
+SPEO_EXPORT int speo_problem_destroy(speo_problem p);
+SPEO_EXPORT int speo_problem_dim(speo_problem p);
+
+/* Seeded problem data: o_j = fl32(-80 + 160 u), M = Haar orthogonal (QR of a
+ * Philox/Box-Muller N(0,1) matrix, sign-fixed so diag(R) > 0).  Host-side,
+ * FP64.  shift_out [dim], rotation_out [dim][dim] (may be NULL). */
+SPEO_EXPORT int speo_generate_problem_data(uint64_t seed, int dim, double *shift_out,
+                                           double *rotation_out);
+SPEO_EXPORT uint64_t speo_problem_seed(uint64_t base_seed, uint32_t index);
+
+/* f_out[i] = f(X[i]) for n rows of stride ld (device pointers), FP32
+ * evaluation with the rotation GEMM in `precision`.  Rows must have ld >= dim. */
+SPEO_EXPORT int speo_evaluate(speo_problem p, const float *X, int ld, int n, float *f_out,
+                              int precision, speo_stream_t stream);
+
+/* FP64 evaluation on the device (re-verification path). */
+SPEO_EXPORT int speo_evaluate_f64(speo_problem p, const float *X, int ld, int n, double *f_out,
+                                  speo_stream_t stream);
+
+/* ---- DE operators on caller-owned buffers (device pointers) ---------- */
+
+/* x = lo + u (hi - lo) for rows [0, n) of stride ld (padding set to 0);
+ * draw j of row i is (seed, purpose 2, row_base + i, 0, j). */
+SPEO_EXPORT int speo_init_population(float *X, int ld, int n, int dim, float lo, float hi,
+                                     uint64_t seed, uint32_t row_base, speo_stream_t stream);
+
+/*
+ * DE generation for one population of n rows split into subisland_num
+ * equal sub-islands through perm (NULL = identity), with per-sub-island
+ * de_info (device, [subisland_num]) and best (device, [subisland_num] or
+ * NULL).  Row i writes slot perm[i] of U.  Optional test outputs (device,
+ * may be NULL): r_out [n][5] donor slots, jrand_out [n].  Optional injected
+ * draws (device, may be NULL): inj_r [n][n_inj] replace the first n_inj donor
+ * draws of each row, inj_cr [n][dim] replace the crossover draws.
+ */
+SPEO_EXPORT int speo_de_generate(float *U, const float *X, int ld, int n, int dim,
+                                 int subisland_num, const int *perm, const DEInfo *de_info,
+                                 const int *best, float lo, float hi, SPEO_RandState rng,
+                                 const float *inj_r, int n_inj, const float *inj_cr, int *r_out,
+                                 int *jrand_out, speo_stream_t stream);
+
+/* Strict-"<" greedy selection of n rows; accept_out (device, [n]) optional. */
+SPEO_EXPORT int speo_de_select(float *X, float *fX, const float *U, const float *fU, int ld,
+                               int n, int dim, uint8_t *accept_out, speo_stream_t stream);
+
+/* Per-segment argmin of fitness (ties to the lower row, NaN last):
+ * segments are rows [off[s], off[s+1]) (device), best_out [nseg] (device). */
+SPEO_EXPORT int speo_segment_argmin(const float *f, const int *off, int nseg, int *best_out,
+                                    speo_stream_t stream);
+
+/* ---- the island engine ---------------------------------------------- */
+
+enum {
+    SPEO_MODE_PLAIN = 0, /* generate, evaluate, select: three launches per generation */
+    SPEO_MODE_FUSED = 1  /* one launch per generation: select(g-1)+generate(g)+evaluate(g) */
+};
+
+typedef struct {
+    int num_islands;
+    const int *island_size;    /* host [num_islands]: NP of each island (>= 8)      */
+    const int *island_problem; /* host [num_islands]: index into problems[]          */
+    int num_problems;
+    const speo_problem *problems; /* host [num_problems]                             */
+    const float *island_F;     /* host [num_islands] or NULL (then F for all)       */
+    const float *island_CR;    /* host [num_islands] or NULL (then CR for all)      */
+    float F, CR;
+    float min_bound, max_bound;
+    uint64_t seed;
+    int interval;        /* migrate when generation % interval == 0 (0 = never)     */
+    int migration_size;  /* best-k sent along the ring of islands of one problem    */
+    int precision;       /* SPEO_PREC_*                                             */
+    int mode;            /* SPEO_MODE_*                                             */
+    int use_graph;       /* capture the generation loop between migrations in a hipGraph */
+    int rows_per_block;  /* 0 = autotuned; else 16, 32 or 64                        */
+} speo_engine_config;
+
+typedef struct speo_engine_s *speo_engine;
+
+typedef struct {
+    int64_t generations;   /* generations run                        */
+    int64_t evaluations;   /* true objective evaluations (FEs)       */
+    int64_t migrations;    /* migration events                        */
+    int64_t graph_launches;
+    int rows_per_block;    /* tile in use                            */
+} speo_engine_stats;
+
+/* Allocates the populations, initialises them (Philox) and evaluates them
+ * (counted as FEs, island_EA.cc:81).  Synchronous. */
+SPEO_EXPORT int speo_engine_create(const speo_engine_config *cfg, speo_stream_t stream,
+                                   speo_engine *out);
+SPEO_EXPORT int speo_engine_destroy(speo_engine e);
+
+/* Runs `generations` generations; after generation g (counted from 0 over the
+ * engine's life) islands migrate when g % interval == 0 (island_EA.cc:93-99).
+ * Asynchronous on the engine's stream. */
+SPEO_EXPORT int speo_engine_run(speo_engine e, int generations);
+SPEO_EXPORT int speo_engine_sync(speo_engine e);
+
+/* Applies any pending selection, then per island: best row, its FP32 fitness
+ * and its FP64 re-verified fitness.  Host outputs [num_islands], each may be
+ * NULL.  Synchronous. */
+SPEO_EXPORT int speo_engine_best(speo_engine e, int *row_out, float *f32_out, double *f64_out);
+
+/* Host copies of the current population (pending selection applied):
+ * X_out [total_rows][max_dim] (row r of island i holds dim_i values, rest 0),
+ * f_out [total_rows].  Synchronous. */
+SPEO_EXPORT int speo_engine_read(speo_engine e, float *X_out, float *f_out);
+/* Host copies of the most recent trial vectors and their fitness, same
+ * layout as speo_engine_read; call before anything that applies a pending
+ * selection (speo_engine_best / _read / _write), which clears the trial
+ * fitness.  Synchronous. */
+SPEO_EXPORT int speo_engine_read_trials(speo_engine e, float *U_out, float *fU_out);
+/* Overwrites the population and fitness from host arrays (same layout). */
+SPEO_EXPORT int speo_engine_write(speo_engine e, const float *X_in, const float *f_in);
+SPEO_EXPORT int speo_engine_get_stats(speo_engine e, speo_engine_stats *out);
+SPEO_EXPORT int speo_engine_total_rows(speo_engine e);
+SPEO_EXPORT int speo_engine_max_dim(speo_engine e);
+
+/* Runs `generations` generations eagerly with HIP events recorded on the
+ * engine's stream around every launch and returns the average duration (ms)
+ * of each per-generation kernel role: [0] generate, [1] evaluate, [2] select
+ * (plain mode) and [3] fused generation (fused mode); unused roles are 0.
+ * Migration follows the same schedule as speo_engine_run.  Synchronous. */
+SPEO_EXPORT int speo_engine_profile(speo_engine e, int generations, double *avg_ms_out);
+
+/* Device pointers of the engine's buffers (for profiling and tests). */
+SPEO_EXPORT int speo_engine_buffers(speo_engine e, float **X, float **fX, int64_t *elem_off,
+                                    int *row_off, int *ld);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPEO_DE_H_ */

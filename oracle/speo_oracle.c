@@ -1,0 +1,413 @@
+/*
+ * speo_oracle.c -- CPU oracle for the island-model DE/rand/1/bin hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links or calls
+ * this file; only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg load it, and only as the checker (or as the timed CPU baseline).
+ *
+ * What it restates (reference paths relative to SPEO-GPU-single-control/):
+ *   - DE operator math of the GPU kernels, in FP32 (the reference's
+ *     GPU_SINGLE_PRECISION branch):
+ *       donor draw with its redraw quirks   CUDA_EA_library/kernel_DE_CUDA.cu:183-216
+ *       j_rand without "== dim" guard       CUDA_EA_library/kernel_DE_CUDA.cu:218-222
+ *       binomial crossover test             CUDA_EA_library/kernel_DE_CUDA.cu:229-241
+ *       strategies 0-8 (4 = rand/1/bin)     CUDA_EA_library/kernel_DE_CUDA.cu:243-282
+ *       reflective bound                    CUDA_EA_library/kernel_DE_CUDA.cu:160-170
+ *       greedy strict "<" selection         CUDA_EA_library/kernel_DE_CUDA.cu:305-327
+ *       uniform population init             CUDA_EA_library/kernel_EA_CUDA.cu:91-118
+ *   - objectives in FP64, following the reference CPU code:
+ *       shift / rotate                      CUDA_EA_library/CEC2014.cc:1373-1393
+ *       elliptic                            CUDA_EA_library/CEC2014.cc:481-497
+ *       Ackley                              CUDA_EA_library/CEC2014.cc:590-612
+ *       Rastrigin                           CUDA_EA_library/CEC2014.cc:668-683
+ *       composition weights (cf_cal)        CUDA_EA_library/CEC2014.cc:1481-1519
+ *   - Philox4x32-10 (Salmon et al., "Parallel random numbers: as easy as
+ *     1, 2, 3", SC'11), replacing cuRAND XORWOW, which is closed and is not
+ *     restated.  The mapping of draws to (purpose, individual, generation,
+ *     index) is this project's own and is shared with the HIP kernels.
+ *
+ * Pinning (tests/test_oracle_*.py): Philox against the Random123 known-answer
+ * triplets and the open-source rocRAND header compiled on the CPU; DE operators
+ * against hand-worked cases, forced edge draws and invariants; objectives
+ * against closed forms and against the reference's own CEC2014.cc compiled
+ * from its sources (oracle/_ref), via committed golden vectors.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORACLE_EXPORT __attribute__((visibility("default")))
+
+/* ---------------------------------------------------------------- Philox */
+
+ORACLE_EXPORT void oracle_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
+                                        uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        if (round < 9) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* u32 -> float in (0,1]: the top 24 bits plus one, times 2^-24.  Every value
+ * is exactly representable, so 1.0f occurs (probability 2^-24), as with the
+ * reference's (0,1] uniforms, and CPU and GPU agree bit for bit. */
+ORACLE_EXPORT float oracle_u01(uint32_t x) {
+    return (float)((x >> 8) + 1u) * 0x1p-24f;
+}
+
+/* Draw `idx` of stream (purpose, a, b): counter = {idx/4, a, b, purpose},
+ * key = seed; word idx%4 of the block. */
+ORACLE_EXPORT float oracle_draw(uint64_t seed, uint32_t purpose, uint32_t a, uint32_t b,
+                                uint32_t idx) {
+    uint32_t ctr[4] = {idx >> 2, a, b, purpose};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t out[4];
+    oracle_philox4x32_10(ctr, key, out);
+    return oracle_u01(out[idx & 3u]);
+}
+
+enum { PURPOSE_DONOR = 0, PURPOSE_CROSS = 1, PURPOSE_INIT = 2 };
+
+/* ----------------------------------------------------------- DE operators */
+
+/* Reflective bound, kernel_DE_CUDA.cu:160-170.  The loop is capped at 64
+ * reflections and then clamped; finite inputs within (1+4)x the range never
+ * reach the cap.  NaN passes through unchanged, as in the reference. */
+ORACLE_EXPORT float oracle_check_bound(float v, float lo, float hi) {
+    int it = 0;
+    while ((v < lo) || (v > hi)) {
+        if (v < lo) v = lo + (lo - v);
+        if (v > hi) v = hi - (v - hi);
+        if (++it >= 64) { v = v < lo ? lo : (v > hi ? hi : v); break; }
+    }
+    return v;
+}
+
+#define MAX_R 5
+#define MAX_DRAWS 4096
+
+typedef struct {
+    uint64_t seed;
+    uint32_t row, gen;
+    uint32_t k;              /* draws consumed */
+    const float *inj;        /* injected draws for this row, or NULL */
+    int n_inj;
+} draw_stream;
+
+static float next_draw(draw_stream *s) {
+    uint32_t k = s->k++;
+    if (s->inj && (int)k < s->n_inj) return s->inj[k];
+    return oracle_draw(s->seed, PURPOSE_DONOR, s->row, s->gen, k);
+}
+
+/* (int)(u S) clamped to [0, S]: injected draws outside (0,1] stay in range;
+ * u in (0,1] is unaffected. */
+static int scale_draw(float u, int S) {
+    float p = u * (float)S;
+    if (!(p >= 0.0f)) return 0;             /* negative or NaN */
+    if (p >= (float)S) return p > (float)S ? S : (int)p;
+    return (int)p;
+}
+
+/* Donor indices and j_rand for one individual, kernel_DE_CUDA.cu:183-222.
+ * S = sub-island size; r[] are sub-island-local (before the permutation). */
+ORACLE_EXPORT void oracle_draw_donors(uint64_t seed, uint32_t row, uint32_t gen, int S, int D,
+                                      const float *inj, int n_inj, int r_out[MAX_R],
+                                      int *jrand_out, int *draws_used) {
+    draw_stream s = {seed, row, gen, 0, inj, n_inj};
+    int r[MAX_R] = {0};
+    for (int i = 0; i < MAX_R; ++i) {
+        r[i] = scale_draw(next_draw(&s), S);
+        if (r[i] == S) r[i]--;
+        for (int j = 0; j < i; ++j) {
+            while (r[i] == r[j] && s.k < MAX_DRAWS) r[i] = scale_draw(next_draw(&s), S);
+        }
+        if (r[i] == S) r[i]--;
+    }
+    int jr = (int)(next_draw(&s) * (float)D);
+    for (int i = 0; i < MAX_R; ++i) r_out[i] = r[i];
+    *jrand_out = jr;
+    if (draws_used) *draws_used = (int)s.k;
+}
+
+/* Mutant coordinate for strategy 0-8, kernel_DE_CUDA.cu:243-277.  The sums
+ * are evaluated left to right with each F*(a-b) fused into the running sum
+ * (fmaf), the contraction nvcc applies to the reference expression. */
+static float mutant(int strategy, float F, const float *X, int ld, int j, int sel, const int *r,
+                    int best) {
+#define E(i) X[(size_t)(i) * ld + j]
+    float v;
+    switch (strategy) {
+    case 0: v = fmaf(F, E(r[0]) - E(r[1]), E(sel)); break;
+    case 1: v = fmaf(F, E(r[2]) - E(r[3]), fmaf(F, E(r[0]) - E(r[1]), E(sel))); break;
+    case 2: v = fmaf(F, E(r[0]) - E(r[1]), fmaf(F, E(best) - E(sel), E(sel))); break;
+    case 3:
+        v = fmaf(F, E(r[1]) - E(r[2]),
+                 fmaf(F, E(r[0]) - E(r[1]), fmaf(F, E(best) - E(sel), E(sel))));
+        break;
+    case 4: v = fmaf(F, E(r[1]) - E(r[2]), E(r[0])); break;
+    case 5: v = fmaf(F, E(r[3]) - E(r[4]), fmaf(F, E(r[1]) - E(r[2]), E(r[0]))); break;
+    case 6: v = fmaf(F, E(r[0]) - E(r[1]), E(best)); break;
+    case 7: v = fmaf(F, E(r[2]) - E(r[3]), fmaf(F, E(r[0]) - E(r[1]), E(best))); break;
+    case 8: v = fmaf(F, E(r[1]) - E(r[2]), fmaf(F, E(r[0]) - E(sel), E(sel))); break;
+    default: v = E(sel); break;
+    }
+#undef E
+    return v;
+}
+
+/*
+ * One DE generation over one population of NP rows split into
+ * `subisland_num` sub-islands through `perm` (kernel_DE_CUDA.cu:172-289):
+ * row i is processed by "thread" i; it writes the trial of slot perm[i].
+ * The random stream of row i is (seed, row_base + i, gen).  Per-sub-island
+ * parameters CR/F/strategy come from cr[], f[], strategy[]; best[] holds the
+ * per-sub-island best slot for strategies 2, 3, 6, 7.
+ * Optional outputs (may be NULL): r_out[NP*5] (after the permutation),
+ * jrand_out[NP], mask_out[NP*D] (1 = coordinate taken from the mutant).
+ * Injected draws: inj_r[NP*n_inj] replace the first n_inj donor draws of each
+ * row; inj_cr[NP*D] replace the crossover draws.
+ * Returns 0, or -1 on invalid arguments.
+ */
+ORACLE_EXPORT int oracle_de_generate(float *U, const float *X, int ld, int NP, int D,
+                                     int subisland_num, const int *perm, const float *cr,
+                                     const float *f, const int *strategy, const int *best,
+                                     float lo, float hi, uint64_t seed, uint32_t gen,
+                                     uint32_t row_base, const float *inj_r, int n_inj,
+                                     const float *inj_cr, int *r_out, int *jrand_out,
+                                     unsigned char *mask_out) {
+    if (NP <= 0 || D <= 0 || ld < D || subisland_num <= 0 || NP % subisland_num) return -1;
+    int S = NP / subisland_num;
+    if (S < 4) return -1;
+    for (int i = 0; i < NP; ++i) {
+        int sel = perm ? perm[i] : i;
+        int isl = i / S;
+        int r[MAX_R], jr;
+        oracle_draw_donors(seed, row_base + (uint32_t)i, gen, S, D,
+                           inj_r ? inj_r + (size_t)i * n_inj : NULL, n_inj, r, &jr, NULL);
+        for (int k = 0; k < MAX_R; ++k) r[k] = perm ? perm[r[k] + isl * S] : r[k] + isl * S;
+        if (r_out) for (int k = 0; k < MAX_R; ++k) r_out[i * MAX_R + k] = r[k];
+        if (jrand_out) jrand_out[i] = jr;
+        float F = f[isl], CR = cr[isl];
+        int st = strategy ? strategy[isl] : 4;
+        int b = best ? best[isl] : sel;
+        for (int j = 0; j < D; ++j) {
+            float u = inj_cr ? inj_cr[(size_t)i * D + j]
+                             : oracle_draw(seed, PURPOSE_CROSS, row_base + (uint32_t)i, gen,
+                                           (uint32_t)j);
+            int take = (j == jr) || (u < CR);
+            float v = take ? mutant(st, F, X, ld, j, sel, r, b) : X[(size_t)sel * ld + j];
+            if (mask_out) mask_out[(size_t)i * D + j] = (unsigned char)take;
+            U[(size_t)sel * ld + j] = oracle_check_bound(v, lo, hi);
+        }
+    }
+    return 0;
+}
+
+/* Greedy selection, kernel_DE_CUDA.cu:305-327: strict "<", so ties and NaN
+ * keep the parent.  Returns the number of accepted trials. */
+ORACLE_EXPORT int oracle_de_select(float *X, float *fX, const float *U, const float *fU, int ld,
+                                   int NP, int D, unsigned char *accept_out) {
+    int n = 0;
+    for (int i = 0; i < NP; ++i) {
+        int acc = fU[i] < fX[i];
+        if (accept_out) accept_out[i] = (unsigned char)acc;
+        if (acc) {
+            memcpy(X + (size_t)i * ld, U + (size_t)i * ld, sizeof(float) * (size_t)D);
+            fX[i] = fU[i];
+            ++n;
+        }
+    }
+    return n;
+}
+
+/* Uniform init, kernel_EA_CUDA.cu:91-118: x = lo + u*(hi-lo), u in (0,1]. */
+ORACLE_EXPORT void oracle_init_population(float *X, int ld, int NP, int D, float lo, float hi,
+                                          uint64_t seed, uint32_t row_base) {
+    for (int i = 0; i < NP; ++i)
+        for (int j = 0; j < D; ++j) {
+            float u = oracle_draw(seed, PURPOSE_INIT, row_base + (uint32_t)i, 0u, (uint32_t)j);
+            X[(size_t)i * ld + j] = fmaf(u, hi - lo, lo);
+        }
+}
+
+/* ------------------------------------------------------------- objectives */
+
+enum { OBJ_ELLIPTIC = 1, OBJ_RASTRIGIN = 2, OBJ_ACKLEY = 3 };
+
+static const double ORACLE_PI = 3.1415926535897932384626433832795029;
+static const double ORACLE_E = 2.7182818284590452353602874713526625;
+
+/* z = M (x - o), CEC2014.cc:1373-1393 (xrot[i] = sum_j x_j M[i*nx+j]). */
+ORACLE_EXPORT void oracle_shift_rotate(const double *x, const double *o, const double *M, int D,
+                                       double *z, double *scratch) {
+    for (int j = 0; j < D; ++j) scratch[j] = x[j] - o[j];
+    for (int i = 0; i < D; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < D; ++j) acc = acc + scratch[j] * M[(size_t)i * D + j];
+        z[i] = acc;
+    }
+}
+
+/* CEC2014.cc:488-496 */
+ORACLE_EXPORT double oracle_elliptic(const double *z, int D) {
+    double f = 0.0;
+    for (int i = 0; i < D; ++i) f += pow(10.0, 6.0 * i / (D - 1)) * z[i] * z[i];
+    return f;
+}
+
+/* CEC2014.cc:679-682 */
+ORACLE_EXPORT double oracle_rastrigin(const double *z, int D) {
+    double f = 0.0;
+    for (int i = 0; i < D; ++i) f += (z[i] * z[i] - 10.0 * cos(2.0 * ORACLE_PI * z[i]) + 10.0);
+    return f;
+}
+
+/* CEC2014.cc:604-611 */
+ORACLE_EXPORT double oracle_ackley(const double *z, int D) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < D; ++i) {
+        s1 += z[i] * z[i];
+        s2 += cos(2.0 * ORACLE_PI * z[i]);
+    }
+    s1 = -0.2 * sqrt(s1 / (double)D);
+    s2 /= (double)D;
+    return ORACLE_E - 20.0 * exp(s1) + 20.0 - exp(s2);
+}
+
+ORACLE_EXPORT double oracle_basic(int objective, const double *z, int D) {
+    switch (objective) {
+    case OBJ_ELLIPTIC: return oracle_elliptic(z, D);
+    case OBJ_RASTRIGIN: return oracle_rastrigin(z, D);
+    case OBJ_ACKLEY: return oracle_ackley(z, D);
+    default: return NAN;
+    }
+}
+
+/* Shifted-rotated objective for n points (rows of x with stride ldx). */
+ORACLE_EXPORT void oracle_eval(int objective, const double *x, int ldx, int n, const double *o,
+                               const double *M, int D, double *f_out) {
+    double *z = (double *)malloc(sizeof(double) * 2 * (size_t)D);
+    for (int p = 0; p < n; ++p) {
+        oracle_shift_rotate(x + (size_t)p * ldx, o, M, D, z, z + D);
+        f_out[p] = oracle_basic(objective, z, D);
+    }
+    free(z);
+}
+
+/* Same, from FP32 points (the device population), evaluated in FP64. */
+ORACLE_EXPORT void oracle_eval_f32(int objective, const float *x, int ldx, int n, const double *o,
+                                   const double *M, int D, double *f_out) {
+    double *xd = (double *)malloc(sizeof(double) * (size_t)D);
+    for (int p = 0; p < n; ++p) {
+        for (int j = 0; j < D; ++j) xd[j] = (double)x[(size_t)p * ldx + j];
+        oracle_eval(objective, xd, D, 1, o, M, D, f_out + p);
+    }
+    free(xd);
+}
+
+/*
+ * Composition of `ncomp` shifted-rotated components, cf_cal
+ * (CEC2014.cc:1481-1519): fit_i = lambda_i * g_i(M_i (x - o_i)) + bias_i,
+ * w_i = d_i^-1/2 exp(-d_i / (2 D sigma_i^2)) with d_i = |x - o_i|^2
+ * (w_i = 1e99 when d_i == 0), all-zero weights fall back to uniform, and
+ * f = sum_i w_i / sum_j w_j * fit_i.  os: [ncomp][D], Ms: [ncomp][D][D].
+ */
+ORACLE_EXPORT void oracle_composition(const float *x, int ldx, int n, int ncomp,
+                                      const int *objectives, const double *os, const double *Ms,
+                                      const double *sigma, const double *bias,
+                                      const double *lambda, int D, double *f_out) {
+    double *xd = (double *)malloc(sizeof(double) * (size_t)D * 3);
+    double *z = xd + D, *tmp = xd + 2 * D;
+    double w[16], fit[16];
+    for (int p = 0; p < n; ++p) {
+        for (int j = 0; j < D; ++j) xd[j] = (double)x[(size_t)p * ldx + j];
+        double wmax = 0.0, wsum = 0.0;
+        for (int c = 0; c < ncomp; ++c) {
+            const double *o = os + (size_t)c * D;
+            oracle_shift_rotate(xd, o, Ms + (size_t)c * D * D, D, z, tmp);
+            fit[c] = lambda[c] * oracle_basic(objectives[c], z, D) + bias[c];
+            double d = 0.0;
+            for (int j = 0; j < D; ++j) d += pow(xd[j] - o[j], 2.0);
+            if (d != 0.0)
+                w[c] = pow(1.0 / d, 0.5) * exp(-d / 2.0 / D / pow(sigma[c], 2.0));
+            else
+                w[c] = 1.0e99;
+            if (w[c] > wmax) wmax = w[c];
+        }
+        for (int c = 0; c < ncomp; ++c) wsum += w[c];
+        if (wmax == 0.0) {
+            for (int c = 0; c < ncomp; ++c) w[c] = 1.0;
+            wsum = ncomp;
+        }
+        double fv = 0.0;
+        for (int c = 0; c < ncomp; ++c) fv += w[c] / wsum * fit[c];
+        f_out[p] = fv;
+    }
+    free(xd);
+}
+
+/* ------------------------------------------------------- island driver */
+
+/*
+ * Ring migration among islands held in one population array (replaces the
+ * reference's MPI migration, src/migrate.cc:46-144).  Islands are ordered by
+ * the total order "ascending fitness, then ascending row", NaN after every
+ * number.  Island i's first k rows in that order (its best) replace the last k
+ * rows (its worst) of island dst[i] (dst[i] < 0: island i sends nothing), all
+ * chosen from the state before any copy.  Fitness travels with the row, so
+ * dst[i] must solve the same problem (objective instance and dimension) as i.
+ * Islands: rows off[i]..off[i+1]-1 of X (stride ld) and fX, dimension dims[i].
+ */
+static int before(float a, int ia, float b, int ib) {
+    int an = isnan(a), bn = isnan(b);
+    if (an != bn) return bn;        /* a number sorts before NaN */
+    if (!an && a != b) return a < b;
+    return ia < ib;
+}
+
+ORACLE_EXPORT void oracle_migrate_ring(float *X, float *fX, int ld, const int *off,
+                                       const int *dims, const int *dst, int I, int k) {
+    int total = off[I];
+    float *Xs = (float *)malloc(sizeof(float) * (size_t)total * ld);
+    float *fs = (float *)malloc(sizeof(float) * (size_t)total);
+    memcpy(Xs, X, sizeof(float) * (size_t)total * ld);
+    memcpy(fs, fX, sizeof(float) * (size_t)total);
+    int *best = (int *)malloc(sizeof(int) * (size_t)I * k);
+    int *worst = (int *)malloc(sizeof(int) * (size_t)I * k);
+    for (int i = 0; i < I; ++i) {
+        int n = off[i + 1] - off[i];
+        char *used_b = (char *)calloc((size_t)n, 1), *used_w = (char *)calloc((size_t)n, 1);
+        for (int t = 0; t < k; ++t) {
+            int bi = -1, wi = -1;
+            for (int r = 0; r < n; ++r) {
+                int g = off[i] + r;
+                if (!used_b[r] && (bi < 0 || before(fs[g], g, fs[off[i] + bi], off[i] + bi))) bi = r;
+                if (!used_w[r] && (wi < 0 || before(fs[off[i] + wi], off[i] + wi, fs[g], g))) wi = r;
+            }
+            used_b[bi] = 1; used_w[wi] = 1;
+            best[i * k + t] = off[i] + bi;
+            worst[i * k + t] = off[i] + wi;
+        }
+        free(used_b); free(used_w);
+    }
+    for (int i = 0; i < I; ++i) {
+        if (dst[i] < 0) continue;
+        for (int t = 0; t < k; ++t) {
+            int s = best[i * k + t], d = worst[dst[i] * k + t];
+            memcpy(X + (size_t)d * ld, Xs + (size_t)s * ld, sizeof(float) * (size_t)dims[i]);
+            fX[d] = fs[s];
+        }
+    }
+    free(Xs); free(fs); free(best); free(worst);
+}

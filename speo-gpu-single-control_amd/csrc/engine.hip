@@ -1,0 +1,961 @@
+// engine.hip -- the island engine: islands of uneven size and dimension in one
+// GPU's HBM, evolved by DE/rand/1/bin with ring migration.
+//
+// Reference counterparts: the generation loop of IslandEA::Execute
+// (src/island_EA.cc:78-113), DE_CUDA::Run (CUDA_EA_library/DE_CUDA.cc:200-216),
+// EA_CUDA::FindBestIndividual (EA_CUDA.cc:104-120) and the MPI migration of
+// src/migrate.cc:46-144, which here becomes a ring among islands in HBM.
+//
+// One kernel, k_generation, covers the four per-generation roles over a table
+// of (island, row tile) work items, so every launch spans all islands:
+//   kModeGen     trials U <- X                       (plain engine, launch 1)
+//   kModeEval    f(U) through the MFMA evaluation    (plain engine, launch 2)
+//   kModeSelect  X, fX <- U, fU where fU < fX        (plain engine, launch 3)
+//   kModeFused   select(g-1) + generate(g) + evaluate(g) in one launch.
+// The fused mode needs no grid-wide barrier: the selection of generation g-1
+// is not applied before the donors are read; instead a donor row r is read as
+// U_prev[r] when fU_prev[r] < fX_prev[r] and as X[r] otherwise, and X[r] is
+// only written for accepted rows (which nobody reads as X), while U and the
+// fitness arrays are double-buffered.  The result equals the plain engine's.
+#pragma clang fp contract(off)
+
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "de.h"
+#include "eval.h"
+#include "eval_f64.h"
+
+namespace speo {
+
+enum GenMode { kModeGen = 0, kModeEval = 1, kModeSelect = 2, kModeFused = 3 };
+
+struct TileDesc {
+    int island, row0, nrows, pad;
+};
+
+struct FusedProblem {
+    int ncomp, dim;
+    float lambda[kMaxComp], bias[kMaxComp], sigma[kMaxComp];
+    CompDev comp[kMaxComp];
+};
+
+struct GenerationArgs {
+    const IslandDesc *isl;
+    const TileDesc *tiles;
+    const FusedProblem *probs;
+    float *X;
+    const float *fX_prev;
+    float *fX_new;
+    const float *U_prev;
+    float *fU_prev;
+    float *U_new;
+    float *fU_new;
+    const float *eval_src;
+    float *eval_out;
+    float lo, hi;
+    uint64_t seed;
+    const uint32_t *gen_base;
+    uint32_t gen_offset;
+    int trial_in_lds;
+    int clear_trials;
+};
+
+constexpr int kTLD = 128;  // LDS row stride of the trial tile (dim <= 128)
+
+template <int RT, int PREC>
+constexpr int generation_lds_bytes(bool trial_in_lds) {
+    return (trial_in_lds ? RT * kTLD * 4 : 0) + kMaxComp * RT * 4 + gemm_lds_bytes<RT, PREC>();
+}
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+__device__ __forceinline__ float comp4(const float4 &v, int q) {
+    return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
+}
+
+template <int RT, int PREC, int MODE>
+__global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    const TileDesc t = a.tiles[blockIdx.x];
+    const IslandDesc I = a.isl[t.island];
+    const int tid = threadIdx.x;
+
+    if constexpr (MODE == kModeSelect) {
+        // 16 lanes per row: trials (U_prev, fU_prev) into (X, fX_new).
+        for (int pass = 0; pass < RT / 16; ++pass) {
+            const int lr = pass * 16 + (tid >> 4), l = tid & 15;
+            if (lr >= t.nrows) continue;
+            const int li = t.row0 + lr, gr = I.row_off + li;
+            const float fu = a.fU_prev[gr], fx = a.fX_new[gr];
+            const bool acc = fu < fx;
+            if (acc) {
+                const float *u = a.U_prev + I.elem_off + (size_t)li * I.ld;
+                float *x = a.X + I.elem_off + (size_t)li * I.ld;
+                for (int g = l; g < (I.ld >> 2); g += 16) st4(x + 4 * g, ld4(u + 4 * g));
+            }
+            if (l == 0) {
+                if (acc) a.fX_new[gr] = fu;
+                if (a.clear_trials) a.fU_prev[gr] = INFINITY;
+            }
+        }
+        return;
+    }
+
+    const bool lds_trial = (MODE == kModeFused) && a.trial_in_lds;
+    float *trial = reinterpret_cast<float *>(lds);
+    float *gval = trial + (lds_trial ? RT * kTLD : 0);
+    char *gemm_lds = reinterpret_cast<char *>(gval + kMaxComp * RT);
+
+    if constexpr (MODE == kModeGen || MODE == kModeFused) {
+        const uint32_t gen = (a.gen_base ? *a.gen_base : 0u) + a.gen_offset;
+        const int ngroups = (I.dim + 3) >> 2;
+        for (int pass = 0; pass < RT / 16; ++pass) {
+            const int lr = pass * 16 + (tid >> 4), l = tid & 15;
+            if (lr >= t.nrows) continue;
+            const int li = t.row0 + lr;
+            const uint32_t gr = (uint32_t)(I.row_off + li);
+            DonorStream ds(a.seed, gr, gen);
+            int r[kMaxR], jr;
+            draw_donors(ds, nullptr, 0, I.np, I.dim, r, jr);
+            // Row `k` of the island as the parent population of this generation.
+            auto parent = [&](int k) -> const float * {
+                const int g = I.row_off + k;
+                if (MODE == kModeFused && a.fU_prev[g] < a.fX_prev[g])
+                    return a.U_prev + I.elem_off + (size_t)k * I.ld;
+                return a.X + I.elem_off + (size_t)k * I.ld;
+            };
+            const float *p0 = parent(r[0]), *p1 = parent(r[1]), *p2 = parent(r[2]);
+            bool own_acc = false;
+            float fx_own = 0.f, fu_own = 0.f;
+            if (MODE == kModeFused) {
+                fx_own = a.fX_prev[gr];
+                fu_own = a.fU_prev[gr];
+                own_acc = fu_own < fx_own;
+            }
+            const float *own = own_acc ? a.U_prev + I.elem_off + (size_t)li * I.ld
+                                       : a.X + I.elem_off + (size_t)li * I.ld;
+            float *urow = a.U_new + I.elem_off + (size_t)li * I.ld;
+            float *xrow = a.X + I.elem_off + (size_t)li * I.ld;
+            for (int g = l; g < ngroups; g += 16) {
+                const U4 w = philox_block(a.seed, (uint32_t)g, gr, gen, kPurposeCross);
+                const float4 e0 = ld4(p0 + 4 * g), e1 = ld4(p1 + 4 * g), e2 = ld4(p2 + 4 * g);
+                const float4 xo = ld4(own + 4 * g);
+                float v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int j = 4 * g + q;
+                    if (j < I.dim) {
+                        const bool take = (j == jr) || (u01(word(w, q)) < I.CR);
+                        const float m = take ? fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q))
+                                             : comp4(xo, q);
+                        v[q] = check_bound(m, a.lo, a.hi);
+                    } else {
+                        v[q] = 0.f;
+                    }
+                }
+                const float4 out{v[0], v[1], v[2], v[3]};
+                st4(urow + 4 * g, out);
+                if (lds_trial) st4(trial + lr * kTLD + 4 * g, out);
+                if (MODE == kModeFused && own_acc) st4(xrow + 4 * g, xo);
+            }
+            if (MODE == kModeFused && l == 0) a.fX_new[gr] = own_acc ? fu_own : fx_own;
+        }
+    }
+
+    if constexpr (MODE == kModeEval || MODE == kModeFused) {
+        __syncthreads();
+        const FusedProblem *P = a.probs + I.problem;
+        const int ncomp = P->ncomp, dim = I.dim;
+        RowSource src;
+        if (MODE == kModeEval)
+            src = RowSource{a.eval_src + I.elem_off + (size_t)t.row0 * I.ld, I.ld, t.nrows};
+        else if (lds_trial)
+            src = RowSource{trial, kTLD, t.nrows};
+        else
+            src = RowSource{a.U_new + I.elem_off + (size_t)t.row0 * I.ld, I.ld, t.nrows};
+        float *red = reinterpret_cast<float *>(gemm_lds);
+        float *o1 = red + 8 * RT, *o2 = o1 + RT;
+        for (int c = 0; c < ncomp; ++c) {
+            const CompDev comp = P->comp[c];
+            float s1[4] = {0.f, 0.f, 0.f, 0.f}, s2[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int n0 = 0; n0 < dim; n0 += kNC) gemm_chunk<RT, PREC>(comp, src, dim, n0, gemm_lds, s1, s2);
+            reduce_rows<RT>(s1, s2, red, o1, o2);
+            if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);
+        }
+        if (tid < t.nrows) {
+            float f;
+            if (ncomp == 1) {
+                f = fmaf(P->lambda[0], gval[tid], P->bias[0]);
+            } else {
+                // cf_cal weights (CEC2014.cc:1481-1519) in the log domain.
+                const float *x = src.rows + (size_t)tid * src.ld;
+                float lw[kMaxComp], lwmax = -INFINITY;
+                int zero_c = -1;
+                for (int c = 0; c < ncomp; ++c) {
+                    const float *o = P->comp[c].shift;
+                    float d = 0.f;
+                    for (int j = 0; j < dim; ++j) {
+                        const float v = x[j] - o[j];
+                        d = fmaf(v, v, d);
+                    }
+                    if (d == 0.f && zero_c < 0) zero_c = c;
+                    const float s = P->sigma[c];
+                    lw[c] = -0.5f * logf(d) - d / (2.0f * (float)dim * s * s);
+                    lwmax = fmaxf(lwmax, lw[c]);
+                }
+                float wsum = 0.f, acc = 0.f;
+                for (int c = 0; c < ncomp; ++c) {
+                    const float w = zero_c >= 0 ? (c == zero_c ? 1.f : 0.f) : expf(lw[c] - lwmax);
+                    wsum += w;
+                    acc = fmaf(w, fmaf(P->lambda[c], gval[c * RT + tid], P->bias[c]), acc);
+                }
+                f = acc / wsum;
+            }
+            float *out = MODE == kModeEval ? a.eval_out : a.fU_new;
+            out[I.row_off + t.row0 + tid] = f;
+        }
+    }
+}
+
+__global__ void k_fill(float *p, int n, float v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_add_u32(uint32_t *p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] += v;
+}
+__global__ void k_eval_f64_rows(const CompParams *const *comps, const int *ncomp, const int *dims,
+                                const IslandDesc *isl, const int *rows, const float *X,
+                                double *out) {
+    extern __shared__ __align__(16) char lds_raw[];
+    const IslandDesc I = isl[blockIdx.x];
+    const int li = rows[blockIdx.x] - I.row_off;
+    const double f = eval_row_f64(comps[I.problem], ncomp[I.problem], dims[I.problem],
+                                  X + I.elem_off + (size_t)li * I.ld,
+                                  reinterpret_cast<double *>(lds_raw));
+    if (threadIdx.x == 0) out[blockIdx.x] = f;
+}
+
+// ------------------------------------------------------------------ launch
+
+template <int RT, int PREC, int MODE>
+static int launch_generation(const GenerationArgs &a, int ntiles, hipStream_t s) {
+    const int lds = (MODE == kModeSelect || MODE == kModeGen)
+                        ? 0
+                        : generation_lds_bytes<RT, PREC>(a.trial_in_lds != 0);
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_generation<RT, PREC, MODE>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  generation_lds_bytes<RT, PREC>(true));
+    });
+    hipLaunchKernelGGL((k_generation<RT, PREC, MODE>), dim3(ntiles), dim3(256), lds, s, a);
+    return check_launch("k_generation");
+}
+
+template <int RT, int PREC>
+static int launch_mode(int mode, const GenerationArgs &a, int ntiles, hipStream_t s) {
+    switch (mode) {
+    case kModeGen: return launch_generation<RT, PREC, kModeGen>(a, ntiles, s);
+    case kModeEval: return launch_generation<RT, PREC, kModeEval>(a, ntiles, s);
+    case kModeSelect: return launch_generation<RT, SPEO_PREC_FP32, kModeSelect>(a, ntiles, s);
+    default: return launch_generation<RT, PREC, kModeFused>(a, ntiles, s);
+    }
+}
+
+template <int RT>
+static int launch_prec(int prec, int mode, const GenerationArgs &a, int ntiles, hipStream_t s) {
+    if (mode == kModeGen || mode == kModeSelect) return launch_mode<RT, SPEO_PREC_FP32>(mode, a, ntiles, s);
+    switch (prec) {
+    case SPEO_PREC_FP32: return launch_mode<RT, SPEO_PREC_FP32>(mode, a, ntiles, s);
+    case SPEO_PREC_BF16: return launch_mode<RT, SPEO_PREC_BF16>(mode, a, ntiles, s);
+    case SPEO_PREC_FP16: return launch_mode<RT, SPEO_PREC_FP16>(mode, a, ntiles, s);
+    case SPEO_PREC_BF16X3: return launch_mode<RT, SPEO_PREC_BF16X3>(mode, a, ntiles, s);
+    case SPEO_PREC_FP16X3: return launch_mode<RT, SPEO_PREC_FP16X3>(mode, a, ntiles, s);
+    case SPEO_PREC_BF16X6: return launch_mode<RT, SPEO_PREC_BF16X6>(mode, a, ntiles, s);
+    default: return set_error(SPEO_ERR_ARG, "unknown precision");
+    }
+}
+
+static int launch_any(int rt, int prec, int mode, const GenerationArgs &a, int ntiles,
+                      hipStream_t s) {
+    switch (rt) {
+    case 16: return launch_prec<16>(prec, mode, a, ntiles, s);
+    case 32: return launch_prec<32>(prec, mode, a, ntiles, s);
+    case 64: return launch_prec<64>(prec, mode, a, ntiles, s);
+    default: return set_error(SPEO_ERR_ARG, "rows_per_block must be 16, 32 or 64");
+    }
+}
+
+// ------------------------------------------------------------------ engine
+
+struct Engine {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    speo_engine_config cfg{};
+    int I = 0, total_rows = 0, max_dim = 0, max_ld = 0;
+    int64_t total_elems = 0;
+    std::vector<IslandDesc> isl;
+    std::vector<int> row_off, dst;
+    std::vector<speo_problem> problems;
+    bool all_small = true;  // every dim <= kTLD
+
+    IslandDesc *isl_dev = nullptr;
+    int *row_off_dev = nullptr, *dst_dev = nullptr, *best_buf = nullptr, *worst_buf = nullptr;
+    int *best_row_dev = nullptr;
+    double *best_f64_dev = nullptr;
+    FusedProblem *probs_dev = nullptr;
+    const CompParams **params_tab_dev = nullptr;
+    int *ncomp_dev = nullptr, *dims_dev = nullptr;
+    std::map<int, TileDesc *> tiles_dev;  // per rows-per-block
+    std::map<int, int> ntiles;
+    uint32_t *gen_base = nullptr;
+
+    float *X = nullptr, *fX[2] = {nullptr, nullptr}, *U[2] = {nullptr, nullptr};
+    float *fU[2] = {nullptr, nullptr};
+    int p = 0;
+    bool pending = false;
+
+    int rt = 32;
+    int64_t gens = 0, fes = 0, migrations = 0, graph_launches = 0;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+    bool graph_failed = false;
+
+    ~Engine();
+    int create(const speo_engine_config &c, hipStream_t s);
+    int upload_tables();
+    int build_tiles(int rt_);
+    int launch(int mode, int gen_offset, int rt_, hipStream_t s, int clear = 0);
+    int enqueue_chunk(int L, bool migrate, hipStream_t s);
+    int finalize_pending(hipStream_t s);
+    int migrate(hipStream_t s);
+    int run(int generations);
+    int profile(int generations, double *avg_ms);
+    int autotune();
+    int best(int *row_out, float *f32_out, double *f64_out);
+    int read(float *Xo, float *fo);
+    int read_rows(const float *src, const float *fsrc, float *Xo, float *fo);
+    int write(const float *Xi, const float *fi);
+};
+
+Engine::~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : tiles_dev) (void)hipFree(kv.second);
+    void *bufs[] = {isl_dev, row_off_dev, dst_dev, best_buf, worst_buf, best_row_dev,
+                    best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
+                    X, fX[0], fX[1], U[0], U[1], fU[0], fU[1]};
+    for (void *b : bufs) (void)hipFree(b);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+static int upload_vec(T **dst, const std::vector<T> &src) {
+    SPEO_TRY_HIP(hipMalloc(reinterpret_cast<void **>(dst), sizeof(T) * std::max<size_t>(src.size(), 1)));
+    if (!src.empty())
+        SPEO_TRY_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return SPEO_OK;
+}
+
+int Engine::build_tiles(int rt_) {
+    if (tiles_dev.count(rt_)) return SPEO_OK;
+    std::vector<TileDesc> tiles;
+    for (int i = 0; i < I; ++i)
+        for (int r0 = 0; r0 < isl[(size_t)i].np; r0 += rt_)
+            tiles.push_back(TileDesc{i, r0, std::min(rt_, isl[(size_t)i].np - r0), 0});
+    TileDesc *d = nullptr;
+    SPEO_TRY(upload_vec(&d, tiles));
+    tiles_dev[rt_] = d;
+    ntiles[rt_] = (int)tiles.size();
+    return SPEO_OK;
+}
+
+int Engine::upload_tables() {
+    SPEO_TRY(upload_vec(&isl_dev, isl));
+    SPEO_TRY(upload_vec(&row_off_dev, row_off));
+    SPEO_TRY(upload_vec(&dst_dev, dst));
+    std::vector<FusedProblem> fp(problems.size());
+    std::vector<const CompParams *> tab(problems.size());
+    std::vector<int> nc(problems.size()), dm(problems.size());
+    for (size_t q = 0; q < problems.size(); ++q) {
+        const ProblemHost &P = problems[q]->h;
+        FusedProblem f{};
+        f.ncomp = P.ncomp;
+        f.dim = P.dim;
+        for (int c = 0; c < P.ncomp; ++c) {
+            f.lambda[c] = (float)P.comp[c].lambda;
+            f.bias[c] = (float)P.comp[c].bias;
+            f.sigma[c] = (float)P.comp[c].sigma;
+            f.comp[c] = P.view(c, cfg.precision);
+        }
+        fp[q] = f;
+        tab[q] = P.params_dev;
+        nc[q] = P.ncomp;
+        dm[q] = P.dim;
+    }
+    SPEO_TRY(upload_vec(&probs_dev, fp));
+    SPEO_TRY(upload_vec(&params_tab_dev, tab));
+    SPEO_TRY(upload_vec(&ncomp_dev, nc));
+    SPEO_TRY(upload_vec(&dims_dev, dm));
+    const int k = std::max(cfg.migration_size, 1);
+    SPEO_TRY_HIP(hipMalloc(&best_buf, sizeof(int) * (size_t)I * k));
+    SPEO_TRY_HIP(hipMalloc(&worst_buf, sizeof(int) * (size_t)I * k));
+    SPEO_TRY_HIP(hipMalloc(&best_row_dev, sizeof(int) * (size_t)I));
+    SPEO_TRY_HIP(hipMalloc(&best_f64_dev, sizeof(double) * (size_t)I));
+    SPEO_TRY_HIP(hipMalloc(&gen_base, sizeof(uint32_t)));
+    SPEO_TRY_HIP(hipMemset(gen_base, 0, sizeof(uint32_t)));
+    return SPEO_OK;
+}
+
+int Engine::create(const speo_engine_config &c, hipStream_t s) {
+    cfg = c;
+    if (c.num_islands < 1 || !c.island_size || !c.island_problem || c.num_problems < 1 ||
+        !c.problems)
+        return set_error(SPEO_ERR_ARG, "speo_engine_create: islands and problems are required");
+    if (c.mode != SPEO_MODE_PLAIN && c.mode != SPEO_MODE_FUSED)
+        return set_error(SPEO_ERR_ARG, "speo_engine_create: unknown mode");
+    if (c.precision < SPEO_PREC_FP32 || c.precision > SPEO_PREC_BF16X6)
+        return set_error(SPEO_ERR_ARG, "speo_engine_create: unknown precision");
+    if (c.migration_size < 0 || c.interval < 0 || !(c.min_bound < c.max_bound))
+        return set_error(SPEO_ERR_ARG, "speo_engine_create: bad migration or bounds");
+    if (c.rows_per_block != 0 && c.rows_per_block != 16 && c.rows_per_block != 32 &&
+        c.rows_per_block != 64)
+        return set_error(SPEO_ERR_ARG, "speo_engine_create: rows_per_block must be 0, 16, 32, 64");
+    problems.assign(c.problems, c.problems + c.num_problems);
+    for (auto *q : problems)
+        if (!q) return set_error(SPEO_ERR_ARG, "speo_engine_create: null problem");
+    I = c.num_islands;
+    for (int i = 0; i < I; ++i) {
+        const int pr = c.island_problem[i];
+        if (pr >= 0 && pr < c.num_problems && problems[(size_t)pr]->h.dim > kTLD) all_small = false;
+    }
+    // When the fused kernel reads its trial rows back from global memory
+    // (dim > kTLD), rows start on 128-byte lines so that no cache line holds
+    // rows of two tiles (two blocks).
+    const int ld_align = all_small ? 4 : 32;
+    int64_t eoff = 0;
+    row_off.assign((size_t)I + 1, 0);
+    for (int i = 0; i < I; ++i) {
+        const int np = c.island_size[i], pr = c.island_problem[i];
+        if (np < 8 || pr < 0 || pr >= c.num_problems || np < 2 * c.migration_size)
+            return set_error(SPEO_ERR_ARG, "speo_engine_create: island size must be >= 8 and "
+                                           ">= 2 * migration_size; problem index in range");
+        IslandDesc d{};
+        d.elem_off = eoff;
+        d.row_off = row_off[(size_t)i];
+        d.np = np;
+        d.dim = problems[(size_t)pr]->h.dim;
+        d.ld = round_up(d.dim, ld_align);
+        d.problem = pr;
+        d.F = c.island_F ? c.island_F[i] : c.F;
+        d.CR = c.island_CR ? c.island_CR[i] : c.CR;
+        isl.push_back(d);
+        eoff += (int64_t)np * d.ld;
+        row_off[(size_t)i + 1] = row_off[(size_t)i] + np;
+        max_dim = std::max(max_dim, d.dim);
+        max_ld = std::max(max_ld, d.ld);
+    }
+    total_rows = row_off[(size_t)I];
+    total_elems = eoff;
+    // Ring per problem: island i sends to the next island (by index) that
+    // solves the same problem; islands alone on their problem send nothing.
+    dst.assign((size_t)I, -1);
+    for (int i = 0; i < I; ++i) {
+        for (int k = 1; k < I; ++k) {
+            const int j = (i + k) % I;
+            if (c.island_problem[j] == c.island_problem[i]) {
+                dst[(size_t)i] = j;
+                break;
+            }
+        }
+    }
+    if (s) {
+        stream = s;
+    } else {
+        SPEO_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    SPEO_TRY(upload_tables());
+    const size_t E = (size_t)total_elems, R = (size_t)total_rows;
+    SPEO_TRY_HIP(hipMalloc(&X, sizeof(float) * E));
+    for (int b = 0; b < 2; ++b) {
+        SPEO_TRY_HIP(hipMalloc(&U[b], sizeof(float) * E));
+        SPEO_TRY_HIP(hipMalloc(&fX[b], sizeof(float) * R));
+        SPEO_TRY_HIP(hipMalloc(&fU[b], sizeof(float) * R));
+        SPEO_TRY_HIP(hipMemsetAsync(U[b], 0, sizeof(float) * E, stream));
+    }
+    for (int i = 0; i < I; ++i) {
+        const IslandDesc &d = isl[(size_t)i];
+        SPEO_TRY(init_population(X + d.elem_off, d.ld, d.np, d.dim, c.min_bound, c.max_bound,
+                                 c.seed, (uint32_t)d.row_off, stream));
+    }
+    for (int b = 0; b < 2; ++b) {
+        hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, fU[b], total_rows, INFINITY);
+        SPEO_TRY(check_launch("k_fill"));
+    }
+    rt = c.rows_per_block;
+    if (rt == 0) SPEO_TRY(autotune());
+    SPEO_TRY(build_tiles(rt));
+    // Initial evaluation (island_EA.cc:81 counts it as island_size FEs).
+    GenerationArgs a{};
+    a.isl = isl_dev;
+    a.tiles = tiles_dev[rt];
+    a.probs = probs_dev;
+    a.eval_src = X;
+    a.eval_out = fX[0];
+    SPEO_TRY(launch_any(rt, cfg.precision, kModeEval, a, ntiles[rt], stream));
+    hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, fX[1], total_rows, INFINITY);
+    SPEO_TRY(check_launch("k_fill"));
+    fes = total_rows;
+    SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    return SPEO_OK;
+}
+
+int Engine::launch(int mode, int gen_offset, int rt_, hipStream_t s, int clear) {
+    GenerationArgs a{};
+    a.isl = isl_dev;
+    a.tiles = tiles_dev[rt_];
+    a.probs = probs_dev;
+    a.X = X;
+    a.lo = cfg.min_bound;
+    a.hi = cfg.max_bound;
+    a.seed = cfg.seed;
+    a.gen_base = gen_base;
+    a.gen_offset = (uint32_t)gen_offset;
+    a.clear_trials = clear;
+    switch (mode) {
+    case kModeGen:
+        a.U_new = U[0];
+        break;
+    case kModeEval:
+        a.eval_src = U[0];
+        a.eval_out = fU[0];
+        break;
+    case kModeSelect:
+        if (cfg.mode == SPEO_MODE_FUSED) {  // apply the pending selection
+            a.U_prev = U[p];
+            a.fU_prev = fU[p];
+            a.fX_new = fX[p];
+        } else {
+            a.U_prev = U[0];
+            a.fU_prev = fU[0];
+            a.fX_new = fX[0];
+        }
+        break;
+    default:
+        a.fX_prev = fX[p];
+        a.U_prev = U[p];
+        a.fU_prev = fU[p];
+        a.fX_new = fX[1 - p];
+        a.U_new = U[1 - p];
+        a.fU_new = fU[1 - p];
+        a.trial_in_lds = all_small ? 1 : 0;
+        break;
+    }
+    return launch_any(rt_, cfg.precision, mode, a, ntiles[rt_], s);
+}
+
+int Engine::finalize_pending(hipStream_t s) {
+    if (cfg.mode != SPEO_MODE_FUSED || !pending) return SPEO_OK;
+    SPEO_TRY(launch(kModeSelect, 0, rt, s, 1));
+    pending = false;
+    return SPEO_OK;
+}
+
+int Engine::migrate(hipStream_t s) {
+    if (cfg.migration_size <= 0) return SPEO_OK;
+    SPEO_TRY(finalize_pending(s));
+    const int cur = cfg.mode == SPEO_MODE_FUSED ? p : 0;
+    return migrate_ring(X, fX[cur], isl_dev, row_off_dev, dst_dev, best_buf, worst_buf, I,
+                        cfg.migration_size, s);
+}
+
+// L generations, then (optionally) migration, then gen_base += L.  The parity
+// `p` and `pending` advance on the host as the launches are enqueued.
+int Engine::enqueue_chunk(int L, bool do_migrate, hipStream_t s) {
+    for (int g = 0; g < L; ++g) {
+        if (cfg.mode == SPEO_MODE_FUSED) {
+            SPEO_TRY(launch(kModeFused, g, rt, s));
+            p = 1 - p;
+            pending = true;
+        } else {
+            SPEO_TRY(launch(kModeGen, g, rt, s));
+            SPEO_TRY(launch(kModeEval, g, rt, s));
+            SPEO_TRY(launch(kModeSelect, g, rt, s));
+        }
+    }
+    if (do_migrate) SPEO_TRY(migrate(s));
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, s, gen_base, (uint32_t)L);
+    return check_launch("k_add_u32");
+}
+
+int Engine::run(int generations) {
+    int left = generations;
+    while (left > 0) {
+        // Generations are counted from 0; migration follows generation g when
+        // g % interval == 0 (island_EA.cc:93-99).
+        int L = left;
+        bool mig = false;
+        if (cfg.interval > 0 && cfg.migration_size > 0) {
+            const int64_t g0 = gens;
+            const int64_t next = (g0 % cfg.interval == 0) ? g0 : (g0 / cfg.interval + 1) * cfg.interval;
+            const int64_t to_mig = next - g0 + 1;  // generations up to and including `next`
+            if (to_mig <= left) {
+                L = (int)to_mig;
+                mig = true;
+            }
+        }
+        const bool use_graph = cfg.use_graph && !graph_failed;
+        if (use_graph) {
+            const auto key = std::make_tuple(L, cfg.mode == SPEO_MODE_FUSED ? p : 0,
+                                             (mig ? 1 : 0) | (pending ? 2 : 0));
+            auto it = graphs.find(key);
+            const int p_before = p;
+            const bool pend_before = pending;
+            if (it == graphs.end()) {
+                hipGraph_t g = nullptr;
+                hipGraphExec_t ge = nullptr;
+                SPEO_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                const int rc = enqueue_chunk(L, mig, stream);
+                hipError_t e = hipStreamEndCapture(stream, &g);
+                if (rc != SPEO_OK || e != hipSuccess ||
+                    hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    (void)hipGetLastError();
+                    graph_failed = true;  // fall back to eager launches
+                    p = p_before;
+                    pending = pend_before;
+                    continue;
+                }
+                (void)hipGraphDestroy(g);
+                it = graphs.emplace(key, ge).first;
+                // enqueue_chunk advanced the host-side parity during capture
+                p = p_before;
+                pending = pend_before;
+            }
+            SPEO_TRY_HIP(hipGraphLaunch(it->second, stream));
+            ++graph_launches;
+            if (cfg.mode == SPEO_MODE_FUSED) {
+                if (L % 2) p = 1 - p;
+                pending = L > 0 ? true : pending;
+                if (mig && cfg.migration_size > 0) pending = false;
+            }
+        } else {
+            SPEO_TRY(enqueue_chunk(L, mig, stream));
+        }
+        gens += L;
+        fes += (int64_t)L * total_rows;
+        if (mig) ++migrations;
+        left -= L;
+    }
+    return SPEO_OK;
+}
+
+int Engine::profile(int generations, double *avg_ms) {
+    const int roles = 4;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> role;
+    auto record = [&](int r) -> int {
+        hipEvent_t a, b;
+        SPEO_TRY_HIP(hipEventCreate(&a));
+        SPEO_TRY_HIP(hipEventCreate(&b));
+        ev.push_back(a);
+        ev.push_back(b);
+        role.push_back(r);
+        return SPEO_OK;
+    };
+    for (int g = 0; g < generations; ++g) {
+        const int64_t gi = gens;
+        const int modes[3] = {kModeGen, kModeEval, kModeSelect};
+        if (cfg.mode == SPEO_MODE_FUSED) {
+            SPEO_TRY(record(3));
+            SPEO_TRY_HIP(hipEventRecord(ev[ev.size() - 2], stream));
+            SPEO_TRY(launch(kModeFused, 0, rt, stream));
+            SPEO_TRY_HIP(hipEventRecord(ev.back(), stream));
+            p = 1 - p;
+            pending = true;
+        } else {
+            for (int m = 0; m < 3; ++m) {
+                SPEO_TRY(record(m));
+                SPEO_TRY_HIP(hipEventRecord(ev[ev.size() - 2], stream));
+                SPEO_TRY(launch(modes[m], 0, rt, stream));
+                SPEO_TRY_HIP(hipEventRecord(ev.back(), stream));
+            }
+        }
+        const bool mig = cfg.interval > 0 && cfg.migration_size > 0 && gi % cfg.interval == 0;
+        if (mig) SPEO_TRY(migrate(stream));
+        hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, stream, gen_base, 1u);
+        SPEO_TRY(check_launch("k_add_u32"));
+        ++gens;
+        fes += total_rows;
+        if (mig) ++migrations;
+    }
+    SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    double sum[roles] = {0, 0, 0, 0};
+    int cnt[roles] = {0, 0, 0, 0};
+    for (size_t i = 0; i < role.size(); ++i) {
+        float ms = 0.f;
+        SPEO_TRY_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        sum[role[i]] += ms;
+        cnt[role[i]] += 1;
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    for (int r = 0; r < roles; ++r) avg_ms[r] = cnt[r] ? sum[r] / cnt[r] : 0.0;
+    return SPEO_OK;
+}
+
+// Times the generation kernel with each row tile on scratch copies of the
+// state and keeps the fastest.  Results are cached per configuration shape.
+int Engine::autotune() {
+    static std::mutex mu;
+    static std::map<std::vector<int>, int> cache;
+    std::vector<int> key{cfg.precision, cfg.mode};
+    for (const auto &d : isl) {
+        key.push_back(d.np);
+        key.push_back(d.dim);
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            rt = it->second;
+            return SPEO_OK;
+        }
+    }
+    float *saveX = X, *savefX[2] = {fX[0], fX[1]}, *saveU[2] = {U[0], U[1]},
+          *savefU[2] = {fU[0], fU[1]};
+    const size_t E = (size_t)total_elems, R = (size_t)total_rows;
+    float *tX = nullptr, *tf[4] = {nullptr, nullptr, nullptr, nullptr}, *tU[2] = {nullptr, nullptr};
+    SPEO_TRY_HIP(hipMalloc(&tX, sizeof(float) * E));
+    SPEO_TRY_HIP(hipMalloc(&tU[0], sizeof(float) * E));
+    SPEO_TRY_HIP(hipMalloc(&tU[1], sizeof(float) * E));
+    for (auto &b : tf) SPEO_TRY_HIP(hipMalloc(&b, sizeof(float) * R));
+    SPEO_TRY_HIP(hipMemcpyAsync(tX, saveX, sizeof(float) * E, hipMemcpyDeviceToDevice, stream));
+    SPEO_TRY_HIP(hipMemsetAsync(tU[0], 0, sizeof(float) * E, stream));
+    SPEO_TRY_HIP(hipMemsetAsync(tU[1], 0, sizeof(float) * E, stream));
+    for (auto &b : tf) {
+        hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, b, total_rows, INFINITY);
+        SPEO_TRY(check_launch("k_fill"));
+    }
+    X = tX;
+    fX[0] = tf[0];
+    fX[1] = tf[1];
+    fU[0] = tf[2];
+    fU[1] = tf[3];
+    U[0] = tU[0];
+    U[1] = tU[1];
+    hipEvent_t e0, e1;
+    SPEO_TRY_HIP(hipEventCreate(&e0));
+    SPEO_TRY_HIP(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    int best_rt = 32;
+    const int saved_p = p;
+    for (int cand : {16, 32, 64}) {
+        SPEO_TRY(build_tiles(cand));
+        const int mode = cfg.mode == SPEO_MODE_FUSED ? kModeFused : kModeEval;
+        for (int rep = 0; rep < 2; ++rep) SPEO_TRY(launch(mode, rep, cand, stream));
+        SPEO_TRY_HIP(hipEventRecord(e0, stream));
+        const int reps = 6;
+        for (int rep = 0; rep < reps; ++rep) {
+            SPEO_TRY(launch(mode, rep, cand, stream));
+            if (mode == kModeFused) p = 1 - p;
+        }
+        SPEO_TRY_HIP(hipEventRecord(e1, stream));
+        SPEO_TRY_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SPEO_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_rt = cand;
+        }
+    }
+    p = saved_p;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    X = saveX;
+    fX[0] = savefX[0];
+    fX[1] = savefX[1];
+    U[0] = saveU[0];
+    U[1] = saveU[1];
+    fU[0] = savefU[0];
+    fU[1] = savefU[1];
+    (void)hipFree(tX);
+    (void)hipFree(tU[0]);
+    (void)hipFree(tU[1]);
+    for (auto &b : tf) (void)hipFree(b);
+    rt = best_rt;
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = rt;
+    return SPEO_OK;
+}
+
+int Engine::best(int *row_out, float *f32_out, double *f64_out) {
+    SPEO_TRY(finalize_pending(stream));
+    const int cur = cfg.mode == SPEO_MODE_FUSED ? p : 0;
+    SPEO_TRY(segment_argmin(fX[cur], row_off_dev, I, best_row_dev, stream));
+    size_t lds = sizeof(double) * ((size_t)max_dim + 16);
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval_f64_rows),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    });
+    hipLaunchKernelGGL(k_eval_f64_rows, dim3(I), dim3(256), lds, stream, params_tab_dev,
+                       ncomp_dev, dims_dev, isl_dev, best_row_dev, X, best_f64_dev);
+    SPEO_TRY(check_launch("k_eval_f64_rows"));
+    std::vector<int> rows((size_t)I);
+    std::vector<double> f64((size_t)I);
+    std::vector<float> fall((size_t)total_rows);
+    SPEO_TRY_HIP(hipMemcpyAsync(rows.data(), best_row_dev, sizeof(int) * I, hipMemcpyDeviceToHost, stream));
+    SPEO_TRY_HIP(hipMemcpyAsync(f64.data(), best_f64_dev, sizeof(double) * I, hipMemcpyDeviceToHost, stream));
+    if (f32_out)
+        SPEO_TRY_HIP(hipMemcpyAsync(fall.data(), fX[cur], sizeof(float) * total_rows,
+                                    hipMemcpyDeviceToHost, stream));
+    SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    for (int i = 0; i < I; ++i) {
+        if (row_out) row_out[i] = rows[(size_t)i];
+        if (f32_out) f32_out[i] = fall[(size_t)rows[(size_t)i]];
+        if (f64_out) f64_out[i] = f64[(size_t)i];
+    }
+    return SPEO_OK;
+}
+
+int Engine::read(float *Xo, float *fo) {
+    SPEO_TRY(finalize_pending(stream));
+    const int cur = cfg.mode == SPEO_MODE_FUSED ? p : 0;
+    return read_rows(X, fX[cur], Xo, fo);
+}
+
+int Engine::read_rows(const float *src, const float *fsrc, float *Xo, float *fo) {
+    if (Xo) {
+        std::vector<float> h((size_t)total_elems);
+        SPEO_TRY_HIP(hipMemcpyAsync(h.data(), src, sizeof(float) * h.size(), hipMemcpyDeviceToHost, stream));
+        SPEO_TRY_HIP(hipStreamSynchronize(stream));
+        for (const auto &d : isl)
+            for (int r = 0; r < d.np; ++r)
+                for (int j = 0; j < max_dim; ++j)
+                    Xo[(size_t)(d.row_off + r) * max_dim + j] =
+                        j < d.dim ? h[(size_t)d.elem_off + (size_t)r * d.ld + j] : 0.f;
+    }
+    if (fo) {
+        SPEO_TRY_HIP(hipMemcpyAsync(fo, fsrc, sizeof(float) * total_rows, hipMemcpyDeviceToHost, stream));
+        SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    }
+    return SPEO_OK;
+}
+
+int Engine::write(const float *Xi, const float *fi) {
+    SPEO_TRY(finalize_pending(stream));
+    const int cur = cfg.mode == SPEO_MODE_FUSED ? p : 0;
+    if (Xi) {
+        std::vector<float> h((size_t)total_elems, 0.f);
+        for (const auto &d : isl)
+            for (int r = 0; r < d.np; ++r)
+                for (int j = 0; j < d.dim; ++j)
+                    h[(size_t)d.elem_off + (size_t)r * d.ld + j] = Xi[(size_t)(d.row_off + r) * max_dim + j];
+        SPEO_TRY_HIP(hipMemcpyAsync(X, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, stream));
+        SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    }
+    if (fi) {
+        SPEO_TRY_HIP(hipMemcpyAsync(fX[cur], fi, sizeof(float) * total_rows, hipMemcpyHostToDevice, stream));
+        SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    }
+    return SPEO_OK;
+}
+
+}  // namespace speo
+
+struct speo_engine_s {
+    speo::Engine e;
+};
+
+using namespace speo;
+
+extern "C" {
+
+int speo_engine_create(const speo_engine_config *cfg, speo_stream_t stream, speo_engine *out) {
+    if (!cfg || !out) return set_error(SPEO_ERR_ARG, "speo_engine_create: null argument");
+    *out = nullptr;
+    auto *h = new (std::nothrow) speo_engine_s;
+    if (!h) return set_error(SPEO_ERR_ALLOC, "speo_engine_create: out of host memory");
+    const int rc = h->e.create(*cfg, as_stream(stream));
+    if (rc != SPEO_OK) {
+        const std::string msg = speo_last_error();
+        delete h;
+        return set_error(rc, msg);
+    }
+    *out = h;
+    return SPEO_OK;
+}
+
+int speo_engine_destroy(speo_engine e) {
+    delete e;
+    return SPEO_OK;
+}
+
+int speo_engine_run(speo_engine e, int generations) {
+    if (!e || generations < 0) return set_error(SPEO_ERR_ARG, "speo_engine_run: bad arguments");
+    return e->e.run(generations);
+}
+
+int speo_engine_sync(speo_engine e) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_sync: null engine");
+    return check_hip(hipStreamSynchronize(e->e.stream), "speo_engine_sync");
+}
+
+int speo_engine_best(speo_engine e, int *row_out, float *f32_out, double *f64_out) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_best: null engine");
+    return e->e.best(row_out, f32_out, f64_out);
+}
+
+int speo_engine_read(speo_engine e, float *X_out, float *f_out) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_read: null engine");
+    return e->e.read(X_out, f_out);
+}
+
+int speo_engine_read_trials(speo_engine e, float *U_out, float *fU_out) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_read_trials: null engine");
+    const int cur = e->e.cfg.mode == SPEO_MODE_FUSED ? e->e.p : 0;
+    return e->e.read_rows(e->e.U[cur], e->e.fU[cur], U_out, fU_out);
+}
+
+int speo_engine_write(speo_engine e, const float *X_in, const float *f_in) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_write: null engine");
+    return e->e.write(X_in, f_in);
+}
+
+int speo_engine_get_stats(speo_engine e, speo_engine_stats *out) {
+    if (!e || !out) return set_error(SPEO_ERR_ARG, "speo_engine_get_stats: null argument");
+    out->generations = e->e.gens;
+    out->evaluations = e->e.fes;
+    out->migrations = e->e.migrations;
+    out->graph_launches = e->e.graph_launches;
+    out->rows_per_block = e->e.rt;
+    return SPEO_OK;
+}
+
+int speo_engine_profile(speo_engine e, int generations, double *avg_ms_out) {
+    if (!e || generations < 1 || !avg_ms_out)
+        return set_error(SPEO_ERR_ARG, "speo_engine_profile: bad arguments");
+    return e->e.profile(generations, avg_ms_out);
+}
+
+int speo_engine_total_rows(speo_engine e) { return e ? e->e.total_rows : 0; }
+int speo_engine_max_dim(speo_engine e) { return e ? e->e.max_dim : 0; }
+
+int speo_engine_buffers(speo_engine e, float **X, float **fX, int64_t *elem_off, int *row_off,
+                        int *ld) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_buffers: null engine");
+    if (X) *X = e->e.X;
+    if (fX) *fX = e->e.fX[e->e.cfg.mode == SPEO_MODE_FUSED ? e->e.p : 0];
+    for (int i = 0; i < e->e.I; ++i) {
+        if (elem_off) elem_off[i] = e->e.isl[(size_t)i].elem_off;
+        if (row_off) row_off[i] = e->e.isl[(size_t)i].row_off;
+        if (ld) ld[i] = e->e.isl[(size_t)i].ld;
+    }
+    return SPEO_OK;
+}
+
+}  // extern "C"

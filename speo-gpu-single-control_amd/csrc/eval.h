@@ -1,0 +1,37 @@
+// eval.h -- host entry points of the evaluation kernels.
+#pragma once
+
+#include "problem.h"
+
+namespace speo {
+
+struct EvalLaunch {
+    CompDev comp;
+    const float *X;
+    int ld, n, dim;
+    float *f_out;   // [n]
+    float *part;    // [nchunks][2][n] when nchunks > 1
+    int nchunks;
+    float lambda, bias;
+    int raw;        // 1: write g (composition component), 0: lambda * g + bias
+};
+
+// One component over n rows; `part` is needed when dim > 128.
+int eval_component(const CompDev &comp, const float *X, int ld, int n, int dim, float *f_out,
+                   float *part, float lambda, float bias, int raw, int precision,
+                   int rows_per_block, hipStream_t s);
+
+// Scratch (floats) eval_problem needs for n rows.
+size_t eval_scratch_floats(int ncomp, int dim, int n);
+
+// Whole problem (single component or composition) over n rows.
+int eval_problem(const ProblemHost &P, const float *X, int ld, int n, float *f_out,
+                 float *scratch, int precision, int rows_per_block, hipStream_t s);
+
+// FP64 evaluation, one block per row.
+int eval_problem_f64(const ProblemHost &P, const float *X, int ld, int n, double *f_out,
+                     hipStream_t s);
+
+int default_rows_per_block(int n, int dim);
+
+}  // namespace speo

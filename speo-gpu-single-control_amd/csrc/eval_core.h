@@ -1,0 +1,323 @@
+// eval_core.h -- device building blocks of the shifted-rotated objective.
+//
+// z = rate * M (x - o) for a tile of RT rows, as the GEMM Z = A B with
+// A = rate (X - 1 o^T) (staged in LDS with the shift subtracted) and
+// B = M^T, on MFMA:
+//   FP32:        v_mfma_f32_16x16x4_f32, A and B as FP32
+//   16-bit:      v_mfma_f32_16x16x32_{bf16,f16}, operands split into 1-3 parts
+//                (a = a0 + a1 [+ a2]) and the cross products summed in FP32.
+// Then per row sum_j g(z_j) (elliptic / Rastrigin / Ackley) with the per
+// coordinate terms reduced across the 16 lanes that hold one row's columns.
+//
+// MFMA 16x16 operand layout (lane l of 64): A row l%16, B column l%16,
+// k = (l/16) (FP32, one value) or 8*(l/16)+0..7 (16-bit, eight values);
+// the 4 accumulator values of lane l are rows 4*(l/16)+i, column l%16.
+//
+// Reference counterparts: global_ShiftAndRotation (kernel_CEC2014_CUDA.cu:
+// 147-181, shift fused into the A-tile load at 161), global_f1 / f5 / f8
+// (254-286, 389-442, 557-587) and device_ParallelSum (6-26).
+#pragma once
+
+#include <type_traits>
+
+#include "common.h"
+
+namespace speo {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+constexpr int kNC = 128;  // columns of Z per N chunk (8 MFMA tiles)
+constexpr int kKC = 64;   // K per LDS chunk
+
+// FP16 operands are scaled into FP16's normal range: A by 2^7 (|x-o| <= 2^8
+// maps below 2^15), M by 2^14 (|M_ij| <= 1); Z is rescaled by 2^-21.
+constexpr float kF16ScaleA = 128.0f;
+constexpr float kF16ScaleB = 16384.0f;
+constexpr float kF16Unscale = 1.0f / (128.0f * 16384.0f);
+
+template <int PREC> struct Prec;
+template <> struct Prec<SPEO_PREC_FP32> {
+    static constexpr int parts = 0;
+    static constexpr bool half = false;
+};
+template <> struct Prec<SPEO_PREC_BF16> {
+    static constexpr int parts = 1;
+    static constexpr bool half = false;
+};
+template <> struct Prec<SPEO_PREC_FP16> {
+    static constexpr int parts = 1;
+    static constexpr bool half = true;
+};
+template <> struct Prec<SPEO_PREC_BF16X3> {
+    static constexpr int parts = 2;
+    static constexpr bool half = false;
+};
+template <> struct Prec<SPEO_PREC_FP16X3> {
+    static constexpr int parts = 2;
+    static constexpr bool half = true;
+};
+template <> struct Prec<SPEO_PREC_BF16X6> {
+    static constexpr int parts = 3;
+    static constexpr bool half = false;
+};
+
+// One objective component as the kernels see it.
+struct CompDev {
+    int objective;           // SPEO_OBJ_*
+    float rate;              // z = rate * M (x - o)
+    const float *shift;      // [dim] FP32
+    const float *B32;        // FP32: B[k][c] = M[c][k], row stride ldb32
+    int ldb32;
+    const uint16_t *M16[3];  // 16-bit parts of M, row-major [c][k], row stride ldm16
+    int ldm16;
+    int is_half;             // M16 holds FP16 (scaled by 2^14) instead of BF16
+    const float *colw;       // elliptic column weights 10^(6j/(D-1)), [dim]
+};
+
+// LDS footprint (bytes) of the GEMM tile for RT rows.
+template <int RT, int PREC> constexpr int gemm_lds_bytes() {
+    return Prec<PREC>::parts == 0
+               ? (RT * (kKC + 2) + kKC * (kNC + 16)) * 4
+               : Prec<PREC>::parts * (RT * (kKC + 8) + kNC * (kKC + 8)) * 2;
+}
+
+__device__ __forceinline__ uint16_t to_bits(float v, bool half) {
+    if (half) {
+        _Float16 h = (_Float16)v;
+        return __builtin_bit_cast(uint16_t, h);
+    }
+    __bf16 b = (__bf16)v;
+    return __builtin_bit_cast(uint16_t, b);
+}
+__device__ __forceinline__ float from_bits(uint16_t v, bool half) {
+    if (half) return (float)__builtin_bit_cast(_Float16, v);
+    return (float)__builtin_bit_cast(__bf16, v);
+}
+
+// Sum over the 16 lanes holding one row (lanes differing in bits 0..3).
+__device__ __forceinline__ float sum16(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 8, 64);
+    return v;
+}
+
+// Per-coordinate terms.  s1/s2 accumulate, per row:
+//   elliptic:  s1 += w_j z^2
+//   Rastrigin: s1 += z^2 + 20 sin^2(pi z)   (== z^2 - 10 cos(2 pi z) + 10)
+//   Ackley:    s1 += z^2, s2 += sin^2(pi z) (cos(2 pi z) - 1 == -2 sin^2(pi z))
+__device__ __forceinline__ void add_terms(int objective, float z, float w, float &s1,
+                                          float &s2) {
+    if (objective == SPEO_OBJ_ELLIPTIC) {
+        s1 = fmaf(w * z, z, s1);
+    } else {
+        const float s = sinpif(z);
+        if (objective == SPEO_OBJ_RASTRIGIN) {
+            s1 += fmaf(z, z, 20.0f * s * s);
+        } else {
+            s1 = fmaf(z, z, s1);
+            s2 = fmaf(s, s, s2);
+        }
+    }
+}
+
+// Objective value from the two row sums.  Ackley in the cancellation-free
+// form -20 expm1(-0.2 r) - e expm1(mean(cos) - 1) of
+// e - 20 exp(-0.2 r) + 20 - exp(mean(cos)) (CEC2014.cc:604-611).
+__device__ __forceinline__ float finish_objective(int objective, float s1, float s2, int dim) {
+    if (objective == SPEO_OBJ_ACKLEY) {
+        const float r = sqrtf(s1 / (float)dim);
+        return -20.0f * expm1f(-0.2f * r) - 2.71828182845904523536f * expm1f(-2.0f * s2 / (float)dim);
+    }
+    return s1;
+}
+
+// Rows of the A tile: row r of the tile is `rows + r * ld` (global or LDS),
+// r < nrows; coordinates j < dim.
+struct RowSource {
+    const float *rows;
+    int ld;
+    int nrows;
+};
+
+// ------------------------------------------------------------------ GEMM
+//
+// Computes the Z columns [n0, n0 + kNC) of the tile's RT rows for component
+// `c` and adds their terms to s1[i], s2[i] (row 4*(lane/16)+i of this wave's
+// row tile; only lanes with lane%16 == 0 hold the full sums after the final
+// sum16, done by the caller).  All 256 threads of the block must call it.
+// `lds` must hold gemm_lds_bytes<RT, PREC>() bytes.
+template <int RT, int PREC>
+__device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int dim, int n0,
+                                           char *lds, float (&s1)[4], float (&s2)[4]) {
+    constexpr int NSPLIT = 4 / (RT / 16);  // waves sharing one 16-row tile
+    constexpr int TPW = 8 / NSPLIT;        // 16-column tiles per wave
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int rt = wave / NSPLIT, wn = wave % NSPLIT;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    floatx4 acc[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const int ncols = min(kNC, dim - n0);
+
+    if constexpr (Prec<PREC>::parts == 0) {
+        float *As = reinterpret_cast<float *>(lds);  // [RT][kKC+2]
+        float *Bs = As + RT * (kKC + 2);             // [kKC][kNC+16]
+        constexpr int LDA = kKC + 2, LDB = kNC + 16;
+        for (int k0 = 0; k0 < dim; k0 += kKC) {
+            const int kc = min(kKC, dim - k0);
+            __syncthreads();
+            for (int e = tid; e < RT * kKC; e += 256) {
+                const int r = e / kKC, k = e % kKC;
+                float v = 0.f;
+                if (r < src.nrows && k < kc)
+                    v = (src.rows[(size_t)r * src.ld + k0 + k] - c.shift[k0 + k]) * c.rate;
+                As[r * LDA + k] = v;
+            }
+            for (int e = tid; e < kKC * kNC; e += 256) {
+                const int k = e / kNC, n = e % kNC;
+                float v = 0.f;
+                if (k < kc && n < ncols) v = c.B32[(size_t)(k0 + k) * c.ldb32 + n0 + n];
+                Bs[k * LDB + n] = v;
+            }
+            __syncthreads();
+            const int ksteps = (kc + 3) >> 2;
+            for (int ks = 0; ks < ksteps; ++ks) {
+                const float a = As[(rt * 16 + lr) * LDA + ks * 4 + lk];
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    const int tile = wn + t * NSPLIT;
+                    if (tile * 16 < ncols) {
+                        const float b = Bs[(ks * 4 + lk) * LDB + tile * 16 + lr];
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    } else {
+        constexpr int P = Prec<PREC>::parts;
+        constexpr bool H = Prec<PREC>::half;
+        constexpr int LDA = kKC + 8, LDB = kKC + 8;
+        uint16_t *As = reinterpret_cast<uint16_t *>(lds);  // [P][RT][LDA]
+        uint16_t *Bs = As + P * RT * LDA;                  // [P][kNC][LDB] (B^T: column-major K)
+        const float sa = H ? kF16ScaleA : 1.0f;
+        for (int k0 = 0; k0 < dim; k0 += kKC) {
+            const int kc = min(kKC, dim - k0);
+            __syncthreads();
+            for (int e = tid; e < RT * kKC; e += 256) {
+                const int r = e / kKC, k = e % kKC;
+                float v = 0.f;
+                if (r < src.nrows && k < kc)
+                    v = (src.rows[(size_t)r * src.ld + k0 + k] - c.shift[k0 + k]) * c.rate * sa;
+                float rem = v;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const uint16_t b = to_bits(rem, H);
+                    As[(p * RT + r) * LDA + k] = b;
+                    rem -= from_bits(b, H);
+                }
+            }
+            for (int e = tid; e < kNC * kKC; e += 256) {
+                const int n = e / kKC, k = e % kKC;
+                const bool ok = (k < kc && n < ncols);
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    Bs[(p * kNC + n) * LDB + k] =
+                        ok ? c.M16[p][(size_t)(n0 + n) * c.ldm16 + k0 + k] : (uint16_t)0;
+            }
+            __syncthreads();
+            const int ksteps = (kc + 31) >> 5;
+            for (int ks = 0; ks < ksteps; ++ks) {
+                const int kb = ks * 32 + lk * 8;
+                typedef typename std::conditional<H, halfx8, bf16x8>::type vec8;
+                vec8 a[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    a[p] = *reinterpret_cast<const vec8 *>(&As[(p * RT + rt * 16 + lr) * LDA + kb]);
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    const int tile = wn + t * NSPLIT;
+                    if (tile * 16 < ncols) {
+                        vec8 b[P];
+#pragma unroll
+                        for (int p = 0; p < P; ++p)
+                            b[p] = *reinterpret_cast<const vec8 *>(
+                                &Bs[(p * kNC + tile * 16 + lr) * LDB + kb]);
+                        // small products first, then hi*hi
+#pragma unroll
+                        for (int sum = P - 1; sum >= 0; --sum) {
+#pragma unroll
+                            for (int pa = 0; pa <= sum; ++pa) {
+                                const int pb = sum - pa;
+                                if (pa < P && pb < P) {
+                                    if constexpr (H)
+                                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                            a[pa], b[pb], acc[t], 0, 0, 0);
+                                    else
+                                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                            a[pa], b[pb], acc[t], 0, 0, 0);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // epilogue: terms of this wave's columns
+    const float unscale = Prec<PREC>::half ? kF16Unscale : 1.0f;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int col = n0 + (wn + t * NSPLIT) * 16 + lr;
+        if (col < dim) {
+            const float w = (c.objective == SPEO_OBJ_ELLIPTIC) ? c.colw[col] : 1.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) add_terms(c.objective, acc[t][i] * unscale, w, s1[i], s2[i]);
+        }
+    }
+}
+
+// Reduces s1/s2 of all waves of the block into per-row sums:
+// out1[r], out2[r] for r < RT (LDS, caller-provided, >= 2 * 4 * RT floats of
+// scratch in `red`).  Deterministic order.  All threads call it.
+template <int RT>
+__device__ __forceinline__ void reduce_rows(float (&s1)[4], float (&s2)[4], float *red,
+                                           float *out1, float *out2) {
+    constexpr int NSPLIT = 4 / (RT / 16);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int rt = wave / NSPLIT, wn = wave % NSPLIT;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        s1[i] = sum16(s1[i]);
+        s2[i] = sum16(s2[i]);
+    }
+    __syncthreads();
+    if ((lane & 15) == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = rt * 16 + (lane >> 4) * 4 + i;
+            red[(wn * RT + r) * 2 + 0] = s1[i];
+            red[(wn * RT + r) * 2 + 1] = s2[i];
+        }
+    }
+    __syncthreads();
+    if (tid < RT) {
+        float a = 0.f, b = 0.f;
+        for (int w = 0; w < NSPLIT; ++w) {
+            a += red[(w * RT + tid) * 2 + 0];
+            b += red[(w * RT + tid) * 2 + 1];
+        }
+        out1[tid] = a;
+        out2[tid] = b;
+    }
+    __syncthreads();
+}
+
+}  // namespace speo

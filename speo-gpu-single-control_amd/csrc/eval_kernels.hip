@@ -1,0 +1,281 @@
+// eval_kernels.hip -- fused shift + rotation-GEMM + objective kernels, the FP64
+// re-verification kernel and the composition combine.
+//
+// Replaces the reference's per-objective launch chains (API_evaluateFitness,
+// kernel_CEC2014_CUDA.cu:1672-1797: ShiftAndRotation -> global_fN, plus
+// CalW/CombineFitness for compositions) by one kernel per N chunk of Z.
+#include <mutex>
+#include <unordered_map>
+
+#include "eval.h"
+#include "eval_f64.h"
+
+namespace speo {
+
+// ------------------------------------------------------------------ kernels
+
+template <int RT, int PREC>
+__global__ __launch_bounds__(256) void k_eval(EvalLaunch p) {
+    extern __shared__ __align__(16) char lds[];
+    const int row0 = blockIdx.x * RT;
+    const int chunk = blockIdx.y;
+    RowSource src{p.X + (size_t)row0 * p.ld, p.ld, min(RT, p.n - row0)};
+    float s1[4] = {0.f, 0.f, 0.f, 0.f}, s2[4] = {0.f, 0.f, 0.f, 0.f};
+    gemm_chunk<RT, PREC>(p.comp, src, p.dim, chunk * kNC, lds, s1, s2);
+    float *red = reinterpret_cast<float *>(lds);
+    float *o1 = red + 4 * RT * 2, *o2 = o1 + RT;
+    reduce_rows<RT>(s1, s2, red, o1, o2);
+    const int t = threadIdx.x;
+    if (t < RT && row0 + t < p.n) {
+        const int row = row0 + t;
+        if (p.nchunks == 1) {
+            const float g = finish_objective(p.comp.objective, o1[t], o2[t], p.dim);
+            p.f_out[row] = p.raw ? g : fmaf(p.lambda, g, p.bias);
+        } else {
+            p.part[((size_t)chunk * 2 + 0) * p.n + row] = o1[t];
+            p.part[((size_t)chunk * 2 + 1) * p.n + row] = o2[t];
+        }
+    }
+}
+
+__global__ void k_eval_finalize(const float *part, int nchunks, int n, int dim, int objective,
+                                float lambda, float bias, int raw, float *f_out) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    float a = 0.f, b = 0.f;
+    for (int c = 0; c < nchunks; ++c) {
+        a += part[((size_t)c * 2 + 0) * n + row];
+        b += part[((size_t)c * 2 + 1) * n + row];
+    }
+    const float g = finish_objective(objective, a, b, dim);
+    f_out[row] = raw ? g : fmaf(lambda, g, bias);
+}
+
+// Composition (cf_cal, CEC2014.cc:1481-1519) in FP32: one wave per row.
+// Weights in the log domain, lw_c = -log(d_c)/2 - d_c / (2 D sigma_c^2),
+// normalised by their maximum, so no weight underflows before the ratio is
+// formed; a component with d_c == 0 takes all the weight (cf_cal's INF).
+__global__ void k_comp_combine(const CompParams *comps, int ncomp, int dim, const float *X,
+                               int ld, int n, const float *g, float *f_out) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= n) return;
+    const float *x = X + (size_t)wave * ld;
+    float lw[kMaxComp];
+    int zero_c = -1;
+    float lwmax = -INFINITY;
+    for (int c = 0; c < ncomp; ++c) {
+        float d = 0.f;
+        for (int j = lane; j < dim; j += 64) {
+            const float v = x[j] - comps[c].shift32[j];
+            d = fmaf(v, v, d);
+        }
+        for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+        if (d == 0.f && zero_c < 0) zero_c = c;
+        const float s = (float)comps[c].sigma;
+        lw[c] = -0.5f * logf(d) - d / (2.0f * (float)dim * s * s);
+        lwmax = fmaxf(lwmax, lw[c]);
+    }
+    if (lane == 0) {
+        float wsum = 0.f, acc = 0.f;
+        for (int c = 0; c < ncomp; ++c) {
+            const float w = zero_c >= 0 ? (c == zero_c ? 1.f : 0.f) : expf(lw[c] - lwmax);
+            const float fit = fmaf((float)comps[c].lambda, g[(size_t)c * n + wave], (float)comps[c].bias);
+            wsum += w;
+            acc = fmaf(w, fit, acc);
+        }
+        f_out[wave] = acc / wsum;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_eval_f64(const CompParams *comps, int ncomp, int dim,
+                                                  const float *X, int ld, int n, double *f_out) {
+    extern __shared__ __align__(16) char lds_raw[];
+    const int row = blockIdx.x;
+    if (row >= n) return;
+    const double f = eval_row_f64(comps, ncomp, dim, X + (size_t)row * ld,
+                                  reinterpret_cast<double *>(lds_raw));
+    if (threadIdx.x == 0) f_out[row] = f;
+}
+
+// ------------------------------------------------------------------ launch
+
+template <int RT, int PREC>
+static int launch_rt_prec(const EvalLaunch &p, int n, int nchunks, hipStream_t s) {
+    static std::once_flag once;
+    constexpr int lds = gemm_lds_bytes<RT, PREC>();
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval<RT, PREC>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    });
+    dim3 grid((n + RT - 1) / RT, nchunks);
+    hipLaunchKernelGGL((k_eval<RT, PREC>), grid, dim3(256), lds, s, p);
+    return check_launch("k_eval");
+}
+
+template <int RT>
+static int launch_rt(const EvalLaunch &p, int precision, int n, int nchunks, hipStream_t s) {
+    switch (precision) {
+    case SPEO_PREC_FP32: return launch_rt_prec<RT, SPEO_PREC_FP32>(p, n, nchunks, s);
+    case SPEO_PREC_BF16: return launch_rt_prec<RT, SPEO_PREC_BF16>(p, n, nchunks, s);
+    case SPEO_PREC_FP16: return launch_rt_prec<RT, SPEO_PREC_FP16>(p, n, nchunks, s);
+    case SPEO_PREC_BF16X3: return launch_rt_prec<RT, SPEO_PREC_BF16X3>(p, n, nchunks, s);
+    case SPEO_PREC_FP16X3: return launch_rt_prec<RT, SPEO_PREC_FP16X3>(p, n, nchunks, s);
+    case SPEO_PREC_BF16X6: return launch_rt_prec<RT, SPEO_PREC_BF16X6>(p, n, nchunks, s);
+    default: return set_error(SPEO_ERR_ARG, "unknown precision mode");
+    }
+}
+
+int eval_component(const CompDev &comp, const float *X, int ld, int n, int dim, float *f_out,
+                   float *part, float lambda, float bias, int raw, int precision,
+                   int rows_per_block, hipStream_t s) {
+    if (n <= 0) return SPEO_OK;
+    const int nchunks = (dim + kNC - 1) / kNC;
+    if (nchunks > 1 && part == nullptr) return set_error(SPEO_ERR_ARG, "eval: missing partial buffer");
+    EvalLaunch p{comp, X, ld, n, dim, f_out, part, nchunks, lambda, bias, raw};
+    int rc;
+    switch (rows_per_block) {
+    case 16: rc = launch_rt<16>(p, precision, n, nchunks, s); break;
+    case 32: rc = launch_rt<32>(p, precision, n, nchunks, s); break;
+    case 64: rc = launch_rt<64>(p, precision, n, nchunks, s); break;
+    default: return set_error(SPEO_ERR_ARG, "rows_per_block must be 16, 32 or 64");
+    }
+    if (rc != SPEO_OK || nchunks == 1) return rc;
+    hipLaunchKernelGGL(k_eval_finalize, dim3((n + 255) / 256), dim3(256), 0, s, part, nchunks, n,
+                       dim, comp.objective, lambda, bias, raw, f_out);
+    return check_launch("k_eval_finalize");
+}
+
+size_t eval_scratch_floats(int ncomp, int dim, int n) {
+    const int nchunks = (dim + kNC - 1) / kNC;
+    size_t part = nchunks > 1 ? (size_t)nchunks * 2 * n : 0;
+    size_t comp = ncomp > 1 ? (size_t)ncomp * n : 0;
+    return part + comp;
+}
+
+int eval_problem(const ProblemHost &P, const float *X, int ld, int n, float *f_out,
+                 float *scratch, int precision, int rows_per_block, hipStream_t s) {
+    if (n <= 0) return SPEO_OK;
+    const int nchunks = (P.dim + kNC - 1) / kNC;
+    float *part = nchunks > 1 ? scratch : nullptr;
+    float *gbuf = scratch + (nchunks > 1 ? (size_t)nchunks * 2 * n : 0);
+    if (P.ncomp == 1) {
+        const CompHost &c = P.comp[0];
+        return eval_component(P.view(0, precision), X, ld, n, P.dim, f_out, part,
+                              (float)c.lambda, (float)c.bias, 0, precision, rows_per_block, s);
+    }
+    for (int c = 0; c < P.ncomp; ++c)
+        SPEO_TRY(eval_component(P.view(c, precision), X, ld, n, P.dim, gbuf + (size_t)c * n, part,
+                                1.f, 0.f, 1, precision, rows_per_block, s));
+    const int waves_per_block = 4;
+    hipLaunchKernelGGL(k_comp_combine, dim3((n + waves_per_block - 1) / waves_per_block),
+                       dim3(64 * waves_per_block), 0, s, P.params_dev, P.ncomp, P.dim, X, ld, n,
+                       gbuf, f_out);
+    return check_launch("k_comp_combine");
+}
+
+int eval_problem_f64(const ProblemHost &P, const float *X, int ld, int n, double *f_out,
+                     hipStream_t s) {
+    if (n <= 0) return SPEO_OK;
+    const size_t lds = sizeof(double) * ((size_t)P.dim + 16);
+    if (lds > 64 * 1024) {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval_f64),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        });
+    }
+    hipLaunchKernelGGL(k_eval_f64, dim3(n), dim3(256), lds, s, P.params_dev, P.ncomp, P.dim, X,
+                       ld, n, f_out);
+    return check_launch("k_eval_f64");
+}
+
+int default_rows_per_block(int n, int dim) {
+    // Enough blocks to cover the 256 CUs; larger tiles share B staging.
+    const int nchunks = (dim + kNC - 1) / kNC;
+    if ((long)((n + 63) / 64) * nchunks >= 512) return 64;
+    if ((long)((n + 31) / 32) * nchunks >= 256) return 32;
+    return 16;
+}
+
+// ------------------------------------------------- drop-in API_evaluateFitness
+
+namespace {
+struct DimTables {
+    float *colw = nullptr;
+    float *identity = nullptr;
+};
+std::mutex g_tables_mu;
+std::unordered_map<int, DimTables> g_tables;
+
+int dim_tables(int dim, DimTables &out) {
+    std::lock_guard<std::mutex> lk(g_tables_mu);
+    auto it = g_tables.find(dim);
+    if (it != g_tables.end()) {
+        out = it->second;
+        return SPEO_OK;
+    }
+    DimTables t;
+    std::vector<float> w((size_t)dim), I((size_t)dim * dim, 0.f);
+    for (int j = 0; j < dim; ++j) {
+        w[(size_t)j] = (float)pow(10.0, 6.0 * j / (dim > 1 ? dim - 1 : 1));
+        I[(size_t)j * dim + j] = 1.f;
+    }
+    SPEO_TRY_HIP(hipMalloc(&t.colw, sizeof(float) * dim));
+    SPEO_TRY_HIP(hipMalloc(&t.identity, sizeof(float) * dim * dim));
+    SPEO_TRY_HIP(hipMemcpy(t.colw, w.data(), sizeof(float) * dim, hipMemcpyHostToDevice));
+    SPEO_TRY_HIP(hipMemcpy(t.identity, I.data(), sizeof(float) * dim * dim, hipMemcpyHostToDevice));
+    g_tables[dim] = t;
+    out = t;
+    return SPEO_OK;
+}
+}  // namespace
+
+}  // namespace speo
+
+using namespace speo;
+
+extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real *d_original_elements,
+                                    real *d_shift_data, real *d_rotated_data, int *, real *, real *,
+                                    real *, int pop_size, int problem_dim, real, int function_ID) {
+    if (pop_size <= 0 || problem_dim < 2 || !d_fitness_value || !d_original_elements ||
+        !d_shift_data) {
+        defer_error(SPEO_ERR_ARG, "API_evaluateFitness: invalid arguments");
+        return;
+    }
+    CompDev c{};
+    bool rotate = true;
+    switch (function_ID) {
+    case 1: c.objective = SPEO_OBJ_ELLIPTIC; c.rate = 1.0f; break;       // F1, kernel_CEC2014_CUDA.cu:1105
+    case 5: c.objective = SPEO_OBJ_ACKLEY; c.rate = 1.0f; break;         // F5, :1130
+    case 8: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; rotate = false; break;  // F8, :1150
+    case 9: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; break;   // F9, :1156
+    default:
+        defer_error(SPEO_ERR_UNSUPPORTED, "API_evaluateFitness: function_ID " +
+                                              std::to_string(function_ID) + " is not built");
+        return;
+    }
+    if (rotate && !d_rotated_data) {
+        defer_error(SPEO_ERR_ARG, "API_evaluateFitness: missing rotation matrix");
+        return;
+    }
+    DimTables t;
+    if (dim_tables(problem_dim, t) != SPEO_OK) {
+        defer_error(SPEO_ERR_HIP, "API_evaluateFitness: table allocation failed");
+        return;
+    }
+    c.shift = d_shift_data;
+    c.B32 = rotate ? d_rotated_data : t.identity;
+    c.ldb32 = problem_dim;  // reference layout [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255)
+    c.colw = t.colw;
+    if (problem_dim > kNC) {
+        defer_error(SPEO_ERR_UNSUPPORTED, "API_evaluateFitness: dim > 128 needs a workspace; "
+                                          "use speo_evaluate");
+        return;
+    }
+    const int ld = next_pow2_dim(problem_dim);
+    const int rc = eval_component(c, d_original_elements, ld, pop_size, problem_dim,
+                                  d_fitness_value, nullptr, 1.f, 0.f, 0, SPEO_PREC_FP32,
+                                  default_rows_per_block(pop_size, problem_dim), nullptr);
+    if (rc != SPEO_OK) defer_error(rc, speo_last_error());
+}

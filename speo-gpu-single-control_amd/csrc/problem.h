@@ -1,0 +1,53 @@
+// problem.h -- device-resident shifted-rotated problem (objective host wrapper).
+// Counterpart of the reference's CEC2014_CUDA (CUDA_EA_library/CEC2014_CUDA.h:
+// 38-82): owns the shift vectors and rotation matrices on the device, in every
+// layout the kernels use, instead of loading CEC text files.
+#pragma once
+
+#include <vector>
+
+#include "eval_core.h"
+
+namespace speo {
+
+constexpr int kMaxComp = 8;
+
+struct CompHost {
+    int objective = SPEO_OBJ_RASTRIGIN;
+    float rate = 1.0f;
+    double sigma = 10.0, bias = 0.0, lambda = 1.0;
+    float *shift32 = nullptr;   // [dim]
+    double *shift64 = nullptr;  // [dim]
+    float *B32 = nullptr;       // [dim][ldb32]: B[k][c] = M[c][k]
+    uint16_t *Mbf[3] = {nullptr, nullptr, nullptr};  // BF16 parts of M, [dim][ldm16]
+    uint16_t *Mh[3] = {nullptr, nullptr, nullptr};   // FP16 parts of 2^14 M
+    double *M64 = nullptr;      // [dim][dim]
+};
+
+// Per-component parameters of the FP64 path and of the composition combine,
+// stored on the device as an array.
+struct CompParams {
+    int objective;
+    float rate;
+    double sigma, bias, lambda;
+    const float *shift32;
+    const double *shift64;
+    const double *M64;
+};
+
+struct ProblemHost {
+    int dim = 0, ncomp = 0;
+    int ldb32 = 0, ldm16 = 0;
+    float *colw = nullptr;          // [dim]
+    CompParams *params_dev = nullptr;  // [ncomp]
+    CompHost comp[kMaxComp];
+    CompDev view(int c, int precision) const;
+};
+
+}  // namespace speo
+
+struct speo_problem_s {
+    speo::ProblemHost h;
+    float *scratch = nullptr;  // eval_problem workspace of speo_evaluate
+    size_t scratch_cap = 0;
+};

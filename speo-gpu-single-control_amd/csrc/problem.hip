@@ -1,0 +1,304 @@
+// problem.hip -- problem objects and the seeded problem generator.
+//
+// The reference reads o and M from CEC text files and keeps M transposed and
+// row-padded on the device (CUDA_EA_library/CEC2014_CUDA.cc:160-289).  Here a
+// problem is created from caller-provided FP64 data (or from the seeded
+// generator below) and stored in every layout the kernels read:
+//   shift32/shift64, B32 = M^T (FP32 GEMM operand), BF16 and 2^14-scaled FP16
+//   hi/lo/lo2 splits of M (16-bit MFMA operands), M64 (FP64 re-verification),
+//   and the elliptic column weights.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "eval.h"
+
+namespace speo {
+
+static uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static double bf16_val(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+static uint16_t f16_rne(float f) {
+    _Float16 h = (_Float16)f;
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+static double f16_val(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return (double)(float)h;
+}
+
+CompDev ProblemHost::view(int c, int precision) const {
+    const CompHost &h = comp[c];
+    const bool half = precision == SPEO_PREC_FP16 || precision == SPEO_PREC_FP16X3;
+    CompDev d{};
+    d.objective = h.objective;
+    d.rate = h.rate;
+    d.shift = h.shift32;
+    d.B32 = h.B32;
+    d.ldb32 = ldb32;
+    for (int p = 0; p < 3; ++p) d.M16[p] = half ? h.Mh[p] : h.Mbf[p];
+    d.ldm16 = ldm16;
+    d.is_half = half ? 1 : 0;
+    d.colw = colw;
+    return d;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &src) {
+    SPEO_TRY_HIP(hipMalloc(reinterpret_cast<void **>(dst), sizeof(T) * src.size()));
+    SPEO_TRY_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return SPEO_OK;
+}
+
+static void free_problem(ProblemHost &P) {
+    for (int c = 0; c < kMaxComp; ++c) {
+        CompHost &h = P.comp[c];
+        (void)hipFree(h.shift32);
+        (void)hipFree(h.shift64);
+        (void)hipFree(h.B32);
+        (void)hipFree(h.M64);
+        for (int p = 0; p < 3; ++p) {
+            (void)hipFree(h.Mbf[p]);
+            (void)hipFree(h.Mh[p]);
+        }
+        h = CompHost{};
+    }
+    (void)hipFree(P.colw);
+    (void)hipFree(P.params_dev);
+    P.colw = nullptr;
+    P.params_dev = nullptr;
+}
+
+static int create_problem(int dim, int ncomp, const int *objectives, const double *shift,
+                          const double *rotation, const double *sigma, const double *bias,
+                          const double *lambda, const double *rate, ProblemHost &P) {
+    if (dim < 2 || ncomp < 1 || ncomp > kMaxComp || !objectives || !shift)
+        return set_error(SPEO_ERR_ARG, "speo_problem_create: need dim >= 2, 1 <= ncomp <= 8, "
+                                       "objectives and shift");
+    for (int c = 0; c < ncomp; ++c)
+        if (objectives[c] < SPEO_OBJ_ELLIPTIC || objectives[c] > SPEO_OBJ_ACKLEY)
+            return set_error(SPEO_ERR_ARG, "speo_problem_create: unknown objective");
+    P.dim = dim;
+    P.ncomp = ncomp;
+    P.ldb32 = round_up(dim, 4);
+    P.ldm16 = round_up(dim, 8);
+    const size_t D = (size_t)dim;
+    std::vector<float> w(D);
+    for (size_t j = 0; j < D; ++j) w[j] = (float)std::pow(10.0, 6.0 * (double)j / (dim - 1));
+    SPEO_TRY(upload(&P.colw, w));
+    std::vector<CompParams> params((size_t)ncomp);
+    for (int c = 0; c < ncomp; ++c) {
+        CompHost &h = P.comp[c];
+        h.objective = objectives[c];
+        h.rate = rate ? (float)rate[c] : 1.0f;
+        h.sigma = sigma ? sigma[c] : 10.0;
+        h.bias = bias ? bias[c] : 0.0;
+        h.lambda = lambda ? lambda[c] : 1.0;
+        std::vector<double> M(D * D, 0.0);
+        if (rotation)
+            std::memcpy(M.data(), rotation + (size_t)c * D * D, sizeof(double) * D * D);
+        else
+            for (size_t i = 0; i < D; ++i) M[i * D + i] = 1.0;
+        std::vector<double> o(shift + (size_t)c * D, shift + (size_t)(c + 1) * D);
+        std::vector<float> o32(D);
+        for (size_t j = 0; j < D; ++j) o32[j] = (float)o[j];
+        SPEO_TRY(upload(&h.shift32, o32));
+        SPEO_TRY(upload(&h.shift64, o));
+        SPEO_TRY(upload(&h.M64, M));
+        std::vector<float> B((size_t)dim * P.ldb32, 0.f);
+        for (size_t k = 0; k < D; ++k)
+            for (size_t cc = 0; cc < D; ++cc) B[k * P.ldb32 + cc] = (float)M[cc * D + k];
+        SPEO_TRY(upload(&h.B32, B));
+        std::vector<uint16_t> pb[3], ph[3];
+        for (int p = 0; p < 3; ++p) {
+            pb[p].assign((size_t)dim * P.ldm16, 0);
+            ph[p].assign((size_t)dim * P.ldm16, 0);
+        }
+        for (size_t r = 0; r < D; ++r)
+            for (size_t k = 0; k < D; ++k) {
+                double rb = M[r * D + k], rh = M[r * D + k] * (double)kF16ScaleB;
+                for (int p = 0; p < 3; ++p) {
+                    const uint16_t b = bf16_rne((float)rb);
+                    const uint16_t hh = f16_rne((float)rh);
+                    pb[p][r * P.ldm16 + k] = b;
+                    ph[p][r * P.ldm16 + k] = hh;
+                    rb -= bf16_val(b);
+                    rh -= f16_val(hh);
+                }
+            }
+        for (int p = 0; p < 3; ++p) {
+            SPEO_TRY(upload(&h.Mbf[p], pb[p]));
+            SPEO_TRY(upload(&h.Mh[p], ph[p]));
+        }
+        params[(size_t)c] = CompParams{h.objective, h.rate, h.sigma, h.bias, h.lambda,
+                                       h.shift32, h.shift64, h.M64};
+    }
+    SPEO_TRY(upload(&P.params_dev, params));
+    return SPEO_OK;
+}
+
+// ------------------------------------------------------------- generator
+
+static double u53(uint32_t a, uint32_t b) {
+    const uint64_t k = (uint64_t)(a >> 5) * (1ull << 26) + (uint64_t)(b >> 6) + 1ull;
+    return (double)k * 0x1p-53;
+}
+
+// Householder QR of the row-major D x D matrix G; returns Q (row-major) with
+// columns sign-fixed so that diag(R) > 0, which makes Q Haar-distributed.
+static void haar_from_gauss(const std::vector<double> &G, int D, std::vector<double> &Q_out) {
+    const size_t n = (size_t)D;
+    std::vector<double> A(n * n), Q(n * n, 0.0), V(n * n, 0.0), rdiag(n);
+    for (size_t i = 0; i < n; ++i)  // column-major copy: A[col*n + row]
+        for (size_t j = 0; j < n; ++j) A[j * n + i] = G[i * n + j];
+    for (size_t k = 0; k < n; ++k) {
+        double *a = &A[k * n];
+        if (k + 1 == n) {
+            rdiag[k] = a[k];
+            break;
+        }
+        double norm = 0.0;
+        for (size_t i = k; i < n; ++i) norm += a[i] * a[i];
+        norm = std::sqrt(norm);
+        const double alpha = a[k] >= 0.0 ? -norm : norm;
+        double *v = &V[k * n];
+        for (size_t i = k; i < n; ++i) v[i] = a[i];
+        v[k] -= alpha;
+        double vn = 0.0;
+        for (size_t i = k; i < n; ++i) vn += v[i] * v[i];
+        vn = std::sqrt(vn);
+        rdiag[k] = alpha;
+        if (vn == 0.0) continue;
+        for (size_t i = k; i < n; ++i) v[i] /= vn;
+        for (size_t j = k; j < n; ++j) {
+            double *col = &A[j * n];
+            double dot = 0.0;
+            for (size_t i = k; i < n; ++i) dot += v[i] * col[i];
+            dot *= 2.0;
+            for (size_t i = k; i < n; ++i) col[i] -= dot * v[i];
+        }
+    }
+    for (size_t i = 0; i < n; ++i) Q[i * n + i] = 1.0;  // column-major identity
+    for (size_t kk = n - 1; kk-- > 0;) {
+        const double *v = &V[kk * n];
+        for (size_t j = 0; j < n; ++j) {
+            double *col = &Q[j * n];
+            double dot = 0.0;
+            for (size_t i = kk; i < n; ++i) dot += v[i] * col[i];
+            dot *= 2.0;
+            for (size_t i = kk; i < n; ++i) col[i] -= dot * v[i];
+        }
+    }
+    Q_out.assign(n * n, 0.0);
+    for (size_t j = 0; j < n; ++j) {
+        const double s = rdiag[j] < 0.0 ? -1.0 : 1.0;
+        for (size_t i = 0; i < n; ++i) Q_out[i * n + j] = s * Q[j * n + i];
+    }
+}
+
+}  // namespace speo
+
+using namespace speo;
+
+extern "C" {
+
+uint64_t speo_problem_seed(uint64_t base_seed, uint32_t index) {
+    const U4 w = philox_block(base_seed, index, 0, 0, kPurposeProblem);
+    return (uint64_t)w.x | ((uint64_t)w.y << 32);
+}
+
+int speo_generate_problem_data(uint64_t seed, int dim, double *shift_out, double *rotation_out) {
+    if (dim < 1 || (!shift_out && !rotation_out))
+        return set_error(SPEO_ERR_ARG, "speo_generate_problem_data: bad arguments");
+    if (shift_out) {
+        for (int j = 0; j < dim; ++j) {
+            const U4 w = philox_block(seed, (uint32_t)(j / 2), (uint32_t)dim, 0, kPurposeShift);
+            const double u = (j % 2 == 0) ? u53(w.x, w.y) : u53(w.z, w.w);
+            shift_out[j] = (double)(float)(-80.0 + 160.0 * u);
+        }
+    }
+    if (rotation_out) {
+        const size_t n = (size_t)dim;
+        std::vector<double> G(n * n), Q;
+        const double two_pi = 6.283185307179586476925286766559;
+        for (size_t e = 0; e < n * n; ++e) {
+            const U4 w = philox_block(seed, (uint32_t)(e / 2), (uint32_t)dim, 0, kPurposeGauss);
+            const double r = std::sqrt(-2.0 * std::log(u53(w.x, w.y)));
+            const double th = two_pi * u53(w.z, w.w);
+            G[e] = (e % 2 == 0) ? r * std::cos(th) : r * std::sin(th);
+        }
+        haar_from_gauss(G, dim, Q);
+        std::memcpy(rotation_out, Q.data(), sizeof(double) * n * n);
+    }
+    return SPEO_OK;
+}
+
+int speo_problem_create(int dim, int ncomp, const int *objectives, const double *shift,
+                        const double *rotation, const double *sigma, const double *bias,
+                        const double *lambda, const double *rate, speo_problem *out) {
+    if (!out) return set_error(SPEO_ERR_ARG, "speo_problem_create: null output");
+    *out = nullptr;
+    auto *p = new (std::nothrow) speo_problem_s;
+    if (!p) return set_error(SPEO_ERR_ALLOC, "speo_problem_create: out of host memory");
+    const int rc = create_problem(dim, ncomp, objectives, shift, rotation, sigma, bias, lambda,
+                                  rate, p->h);
+    if (rc != SPEO_OK) {
+        free_problem(p->h);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return SPEO_OK;
+}
+
+int speo_problem_destroy(speo_problem p) {
+    if (!p) return SPEO_OK;
+    (void)hipDeviceSynchronize();
+    free_problem(p->h);
+    (void)hipFree(p->scratch);
+    delete p;
+    return SPEO_OK;
+}
+
+int speo_problem_dim(speo_problem p) { return p ? p->h.dim : 0; }
+
+int speo_evaluate(speo_problem p, const float *X, int ld, int n, float *f_out, int precision,
+                  speo_stream_t stream) {
+    if (!p || !X || !f_out || n < 0 || ld < p->h.dim)
+        return set_error(SPEO_ERR_ARG, "speo_evaluate: bad arguments");
+    if (precision < SPEO_PREC_FP32 || precision > SPEO_PREC_BF16X6)
+        return set_error(SPEO_ERR_ARG, "speo_evaluate: unknown precision");
+    const size_t need = eval_scratch_floats(p->h.ncomp, p->h.dim, n);
+    if (need > p->scratch_cap) {
+        SPEO_TRY_HIP(hipStreamSynchronize(as_stream(stream)));
+        (void)hipFree(p->scratch);
+        p->scratch = nullptr;
+        p->scratch_cap = 0;
+        SPEO_TRY_HIP(hipMalloc(&p->scratch, sizeof(float) * need));
+        p->scratch_cap = need;
+    }
+    return eval_problem(p->h, X, ld, n, f_out, p->scratch, precision,
+                        default_rows_per_block(n, p->h.dim), as_stream(stream));
+}
+
+int speo_evaluate_f64(speo_problem p, const float *X, int ld, int n, double *f_out,
+                      speo_stream_t stream) {
+    if (!p || !X || !f_out || n < 0 || ld < p->h.dim)
+        return set_error(SPEO_ERR_ARG, "speo_evaluate_f64: bad arguments");
+    return eval_problem_f64(p->h, X, ld, n, f_out, as_stream(stream));
+}
+
+}  // extern "C"

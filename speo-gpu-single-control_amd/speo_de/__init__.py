@@ -1,0 +1,250 @@
+"""Python host mirror of the MI355X island-model DE engine.
+
+Thin wrappers over the C ABI (include/speo_de.h, libspeo_de.so).  Device
+buffers are torch tensors on ``cuda`` (PyTorch is plumbing here: memory and
+streams); every computation runs in the library's HIP kernels.  The classes
+follow the reference's host roles:
+
+* ``Problem``  -- CEC2014_CUDA (CUDA_EA_library/CEC2014_CUDA.h:38-82): a
+  shifted-rotated objective held on the device.
+* ``Engine``   -- EA_CUDA / DE_CUDA + IslandEA (EA_CUDA.h:46-103,
+  src/island_EA.cc:78-113): islands in HBM, generations, migration, best.
+* module functions ``init_population``, ``de_generate``, ``de_select``,
+  ``evaluate`` -- the kernel-level entry points (API_Initilize,
+  API_DE_GenerateNewPopulation, API_DE_SelectSurvival, API_evaluateFitness).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_double, c_float, c_int, c_void_p
+
+import numpy as np
+
+from ._lib import (DEInfo, EngineConfig, EngineStats, GPU_IslandInfo, GPU_Population,
+                   MODE_FUSED, MODE_PLAIN, OBJ_ACKLEY, OBJ_ELLIPTIC, OBJ_RASTRIGIN, PREC_BF16,
+                   PREC_BF16X3, PREC_BF16X6, PREC_FP16, PREC_FP16X3, PREC_FP32, PREC_NAMES,
+                   ProblemInfo, SPEO_RandState, SpeoError, check, last_error, lib)
+
+__all__ = [
+    "Problem", "Engine", "generate_problem_data", "problem_seed", "init_population",
+    "de_generate", "de_select", "evaluate", "evaluate_f64", "segment_argmin", "SpeoError",
+    "OBJ_ELLIPTIC", "OBJ_RASTRIGIN", "OBJ_ACKLEY", "PREC_FP32", "PREC_BF16", "PREC_FP16",
+    "PREC_BF16X3", "PREC_FP16X3", "PREC_BF16X6", "PREC_NAMES", "MODE_PLAIN", "MODE_FUSED",
+    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
+]
+
+
+def _p(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else c_void_p(t.data_ptr())
+
+
+def _np_ptr(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return c_void_p(stream.cuda_stream)
+
+
+# ------------------------------------------------------------------ problems
+
+def problem_seed(base_seed: int, index: int) -> int:
+    return int(lib().speo_problem_seed(base_seed, index))
+
+
+def generate_problem_data(seed: int, dim: int):
+    """Seeded o ~ U[-80, 80]^D (FP32-exact) and Haar orthogonal M (FP64)."""
+    o = np.zeros(dim, np.float64)
+    M = np.zeros((dim, dim), np.float64)
+    check(lib().speo_generate_problem_data(seed, dim, _np_ptr(o), _np_ptr(M)))
+    return o, M
+
+
+class Problem:
+    """A shifted-rotated problem (single component or cf_cal composition)."""
+
+    def __init__(self, objectives, shifts, rotations=None, sigma=None, bias=None, lam=None,
+                 rate=None):
+        objectives = np.atleast_1d(np.asarray(objectives, np.int32))
+        nc = objectives.size
+        shifts = np.ascontiguousarray(np.asarray(shifts, np.float64).reshape(nc, -1))
+        self.dim = shifts.shape[1]
+        self.ncomp = nc
+        rot = None
+        if rotations is not None:
+            rot = np.ascontiguousarray(np.asarray(rotations, np.float64).reshape(nc, self.dim,
+                                                                                self.dim))
+        arrs = [None if v is None else np.ascontiguousarray(np.asarray(v, np.float64).reshape(nc))
+                for v in (sigma, bias, lam, rate)]
+        h = c_void_p()
+        check(lib().speo_problem_create(self.dim, nc, _np_ptr(objectives), _np_ptr(shifts),
+                                        _np_ptr(rot), *[_np_ptr(a) for a in arrs], byref(h)))
+        self.handle = h
+        self.objectives, self.shifts, self.rotations = objectives, shifts, rot
+        self.sigma, self.bias, self.lam, self.rate = arrs
+
+    @classmethod
+    def seeded(cls, objective: int, dim: int, seed: int):
+        o, M = generate_problem_data(seed, dim)
+        return cls([objective], o[None, :], M[None, :, :])
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().speo_problem_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def evaluate(problem: Problem, X, f_out, precision=PREC_FP32, ld=None, n=None, stream=None):
+    """f_out[i] = f(X[i]) (FP32 values, rotation GEMM in `precision`)."""
+    ld = X.shape[1] if ld is None else ld
+    n = X.shape[0] if n is None else n
+    check(lib().speo_evaluate(problem.handle, _p(X), ld, n, _p(f_out), precision,
+                              _stream(stream)))
+
+
+def evaluate_f64(problem: Problem, X, f_out, ld=None, n=None, stream=None):
+    ld = X.shape[1] if ld is None else ld
+    n = X.shape[0] if n is None else n
+    check(lib().speo_evaluate_f64(problem.handle, _p(X), ld, n, _p(f_out), _stream(stream)))
+
+
+# ------------------------------------------------------------- DE operators
+
+def init_population(X, dim, lo, hi, seed, row_base=0, n=None, stream=None):
+    n = X.shape[0] if n is None else n
+    check(lib().speo_init_population(_p(X), X.shape[1], n, dim, lo, hi, seed, row_base,
+                                     _stream(stream)))
+
+
+def de_generate(U, X, dim, de_info, subisland_num=1, perm=None, best=None, lo=-100.0, hi=100.0,
+                seed=0, gen=0, row_base=0, inj_r=None, inj_cr=None, r_out=None, jrand_out=None,
+                n=None, stream=None):
+    """DE generation X -> U (device tensors; de_info a device DEInfo array)."""
+    n = X.shape[0] if n is None else n
+    rng = SPEO_RandState(seed, gen, row_base)
+    n_inj = 0 if inj_r is None else inj_r.shape[1]
+    check(lib().speo_de_generate(_p(U), _p(X), X.shape[1], n, dim, subisland_num, _p(perm),
+                                 _p(de_info), _p(best), lo, hi, rng, _p(inj_r), n_inj,
+                                 _p(inj_cr), _p(r_out), _p(jrand_out), _stream(stream)))
+
+
+def de_select(X, fX, U, fU, dim, accept_out=None, n=None, stream=None):
+    n = X.shape[0] if n is None else n
+    check(lib().speo_de_select(_p(X), _p(fX), _p(U), _p(fU), X.shape[1], n, dim,
+                               _p(accept_out), _stream(stream)))
+
+
+def segment_argmin(f, off, best_out, stream=None):
+    check(lib().speo_segment_argmin(_p(f), _p(off), off.numel() - 1, _p(best_out),
+                                    _stream(stream)))
+
+
+def de_info_bytes(cr, f, strategy=4, pop_size=-1) -> bytes:
+    """Packed DEInfo array (EA_CUDA.h:33-39) for upload to the device."""
+    cr = np.atleast_1d(np.asarray(cr, np.float32))
+    f = np.broadcast_to(np.asarray(f, np.float32), cr.shape)
+    st = np.broadcast_to(np.asarray(strategy, np.int32), cr.shape)
+    arr = (DEInfo * cr.size)()
+    for i in range(cr.size):
+        arr[i] = DEInfo(float(cr[i]), float(f[i]), int(st[i]), pop_size)
+    return bytes(arr)
+
+
+# ------------------------------------------------------------------- engine
+
+class Engine:
+    """Islands of DE/rand/1/bin in one GPU's HBM (see speo_engine_* in the header)."""
+
+    def __init__(self, problems, island_size, island_problem, F=0.5, CR=0.9, lo=-100.0,
+                 hi=100.0, seed=1, interval=0, migration_size=0, precision=PREC_FP32,
+                 mode=MODE_FUSED, use_graph=False, rows_per_block=0, island_F=None,
+                 island_CR=None):
+        self.problems = list(problems)
+        self.island_size = np.ascontiguousarray(island_size, np.int32)
+        self.island_problem = np.ascontiguousarray(island_problem, np.int32)
+        self._probs = (c_void_p * len(self.problems))(*[p.handle.value for p in self.problems])
+        self._iF = None if island_F is None else np.ascontiguousarray(island_F, np.float32)
+        self._iCR = None if island_CR is None else np.ascontiguousarray(island_CR, np.float32)
+        cfg = EngineConfig()
+        cfg.num_islands = self.island_size.size
+        cfg.island_size = self.island_size.ctypes.data_as(POINTER(c_int))
+        cfg.island_problem = self.island_problem.ctypes.data_as(POINTER(c_int))
+        cfg.num_problems = len(self.problems)
+        cfg.problems = ctypes.cast(self._probs, POINTER(c_void_p))
+        cfg.island_F = None if self._iF is None else self._iF.ctypes.data_as(POINTER(c_float))
+        cfg.island_CR = None if self._iCR is None else self._iCR.ctypes.data_as(POINTER(c_float))
+        cfg.F, cfg.CR, cfg.min_bound, cfg.max_bound = F, CR, lo, hi
+        cfg.seed = seed
+        cfg.interval, cfg.migration_size = interval, migration_size
+        cfg.precision, cfg.mode, cfg.use_graph = precision, mode, int(use_graph)
+        cfg.rows_per_block = rows_per_block
+        self.cfg = cfg
+        h = c_void_p()
+        check(lib().speo_engine_create(byref(cfg), None, byref(h)))
+        self.handle = h
+        self.total_rows = lib().speo_engine_total_rows(h)
+        self.max_dim = lib().speo_engine_max_dim(h)
+        self.num_islands = int(self.island_size.size)
+        self.row_off = np.concatenate([[0], np.cumsum(self.island_size)]).astype(np.int64)
+
+    def run(self, generations: int):
+        check(lib().speo_engine_run(self.handle, generations))
+
+    def sync(self):
+        check(lib().speo_engine_sync(self.handle))
+
+    def best(self):
+        I = self.num_islands
+        rows = np.zeros(I, np.int32)
+        f32 = np.zeros(I, np.float32)
+        f64 = np.zeros(I, np.float64)
+        check(lib().speo_engine_best(self.handle, _np_ptr(rows), _np_ptr(f32), _np_ptr(f64)))
+        return rows, f32, f64
+
+    def read(self):
+        X = np.zeros((self.total_rows, self.max_dim), np.float32)
+        f = np.zeros(self.total_rows, np.float32)
+        check(lib().speo_engine_read(self.handle, _np_ptr(X), _np_ptr(f)))
+        return X, f
+
+    def read_trials(self):
+        U = np.zeros((self.total_rows, self.max_dim), np.float32)
+        f = np.zeros(self.total_rows, np.float32)
+        check(lib().speo_engine_read_trials(self.handle, _np_ptr(U), _np_ptr(f)))
+        return U, f
+
+    def write(self, X=None, f=None):
+        Xa = None if X is None else np.ascontiguousarray(X, np.float32)
+        fa = None if f is None else np.ascontiguousarray(f, np.float32)
+        check(lib().speo_engine_write(self.handle, _np_ptr(Xa), _np_ptr(fa)))
+
+    def profile(self, generations: int):
+        """Average kernel ms per role: generate, evaluate, select, fused."""
+        out = np.zeros(4, np.float64)
+        check(lib().speo_engine_profile(self.handle, generations, _np_ptr(out)))
+        return dict(zip(("generate", "evaluate", "select", "fused"), out.tolist()))
+
+    def stats(self):
+        s = EngineStats()
+        check(lib().speo_engine_get_stats(self.handle, byref(s)))
+        return {k: getattr(s, k) for k, _ in EngineStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().speo_engine_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

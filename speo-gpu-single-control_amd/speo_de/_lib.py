@@ -1,0 +1,150 @@
+"""ctypes binding of libspeo_de.so (the C ABI declared in include/speo_de.h).
+
+The library is built in-tree by ``make -C speo-gpu-single-control_amd`` (or
+``__graft_entry__.build()``).  There is no fallback: when the library is
+missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int64, c_uint8,
+                    c_uint32, c_uint64, c_void_p)
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "libspeo_de.so"
+HEADER = PKG_DIR.parent / "include" / "speo_de.h"
+
+SPEO_OK, SPEO_ERR_ARG, SPEO_ERR_HIP, SPEO_ERR_UNSUPPORTED, SPEO_ERR_NO_DEVICE, SPEO_ERR_ALLOC = (
+    0, -1, -2, -3, -4, -5)
+OBJ_ELLIPTIC, OBJ_RASTRIGIN, OBJ_ACKLEY = 1, 2, 3
+PREC_FP32, PREC_BF16, PREC_FP16, PREC_BF16X3, PREC_FP16X3, PREC_BF16X6 = range(6)
+PREC_NAMES = {PREC_FP32: "fp32", PREC_BF16: "bf16", PREC_FP16: "fp16", PREC_BF16X3: "bf16x3",
+              PREC_FP16X3: "fp16x3", PREC_BF16X6: "bf16x6"}
+MODE_PLAIN, MODE_FUSED = 0, 1
+
+
+class SpeoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"speo error {code}: {msg}")
+        self.code = code
+
+
+# ---- drop-in structs (CUDA_EA_library/EA_CUDA.h:21-39, config.h:48-59) ----
+
+class GPU_Population(Structure):
+    _fields_ = [("elements", c_void_p), ("fitness_value", c_void_p)]
+
+
+class GPU_IslandInfo(Structure):
+    _fields_ = [("island_size", c_int), ("subisland_num", c_int), ("permutated_index", c_void_p)]
+
+
+class DEInfo(Structure):
+    _fields_ = [("CR", c_float), ("F", c_float), ("strategy_ID", c_int), ("pop_size", c_int)]
+
+
+class ProblemInfo(Structure):
+    _fields_ = [("dim", c_int), ("function_ID", c_int), ("run_ID", c_int),
+                ("max_base_FEs", c_int), ("seed", c_int), ("running_time", c_int),
+                ("computing_time", c_int), ("max_bound", c_float), ("min_bound", c_float)]
+
+
+class SPEO_RandState(Structure):
+    _fields_ = [("seed", c_uint64), ("generation", c_uint32), ("row_base", c_uint32)]
+
+
+class EngineConfig(Structure):
+    _fields_ = [("num_islands", c_int), ("island_size", POINTER(c_int)),
+                ("island_problem", POINTER(c_int)), ("num_problems", c_int),
+                ("problems", POINTER(c_void_p)), ("island_F", POINTER(c_float)),
+                ("island_CR", POINTER(c_float)), ("F", c_float), ("CR", c_float),
+                ("min_bound", c_float), ("max_bound", c_float), ("seed", c_uint64),
+                ("interval", c_int), ("migration_size", c_int), ("precision", c_int),
+                ("mode", c_int), ("use_graph", c_int), ("rows_per_block", c_int)]
+
+
+class EngineStats(Structure):
+    _fields_ = [("generations", c_int64), ("evaluations", c_int64), ("migrations", c_int64),
+                ("graph_launches", c_int64), ("rows_per_block", c_int)]
+
+
+_SIGS = {
+    "SPEO_NextPow2Dim": (c_int, [c_int]),
+    "API_Initilize": (c_int, [GPU_Population, POINTER(SPEO_RandState), GPU_IslandInfo,
+                              ProblemInfo]),
+    "API_CheckCUDAError": (c_int, []),
+    "API_DE_GenerateNewPopulation": (c_int, [GPU_Population, GPU_Population, c_void_p,
+                                             GPU_IslandInfo, c_void_p, POINTER(SPEO_RandState),
+                                             ProblemInfo]),
+    "API_DE_SelectSurvival": (c_int, [GPU_Population, GPU_Population, GPU_IslandInfo,
+                                      ProblemInfo]),
+    "API_evaluateFitness": (None, [c_void_p] * 10 + [c_int, c_int, c_float, c_int]),
+    "speo_last_error": (c_char_p, []),
+    "speo_device_check": (c_int, []),
+    "speo_problem_create": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
+    "speo_problem_destroy": (c_int, [c_void_p]),
+    "speo_problem_dim": (c_int, [c_void_p]),
+    "speo_generate_problem_data": (c_int, [c_uint64, c_int, c_void_p, c_void_p]),
+    "speo_problem_seed": (c_uint64, [c_uint64, c_uint32]),
+    "speo_evaluate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
+    "speo_evaluate_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "speo_init_population": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_float, c_uint64,
+                                     c_uint32, c_void_p]),
+    "speo_de_generate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                 c_void_p, c_void_p, c_float, c_float, SPEO_RandState, c_void_p,
+                                 c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "speo_de_select": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                               c_void_p, c_void_p]),
+    "speo_segment_argmin": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "speo_engine_create": (c_int, [POINTER(EngineConfig), c_void_p, POINTER(c_void_p)]),
+    "speo_engine_destroy": (c_int, [c_void_p]),
+    "speo_engine_run": (c_int, [c_void_p, c_int]),
+    "speo_engine_sync": (c_int, [c_void_p]),
+    "speo_engine_best": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "speo_engine_read": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "speo_engine_read_trials": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "speo_engine_write": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "speo_engine_get_stats": (c_int, [c_void_p, POINTER(EngineStats)]),
+    "speo_engine_profile": (c_int, [c_void_p, c_int, c_void_p]),
+    "speo_engine_total_rows": (c_int, [c_void_p]),
+    "speo_engine_max_dim": (c_int, [c_void_p]),
+    "speo_engine_buffers": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), c_void_p,
+                                    c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libspeo_de.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built: run `make -C {PKG_DIR}` "
+                               "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().speo_last_error().decode()
+
+
+def check(rc: int) -> None:
+    if rc != SPEO_OK:
+        raise SpeoError(rc, last_error())
+
+
+def header_symbols() -> list[str]:
+    """Every SPEO_EXPORT function declared in include/speo_de.h."""
+    import re
+    text = HEADER.read_text()
+    names = re.findall(r"^SPEO_EXPORT\s+[\w\s\*]+?\**\s*(\w+)\s*\(", text, re.M)
+    return sorted(set(names))

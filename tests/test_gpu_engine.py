@@ -1,0 +1,173 @@
+"""GPU parity of the island engine (IslandEA::Execute / DE_CUDA::Run semantics,
+src/island_EA.cc:78-113, CUDA_EA_library/DE_CUDA.cc:200-216) against the
+oracle: initial population, trial vectors and selection bit-exact, fitness
+within 1e-5 of FP64, ring migration bit-exact, the fused one-launch generation
+equal to the plain three-launch one, and graph replays equal to eager runs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x5EED
+
+
+def _problems(speo, oracle, dims_objs):
+    probs = []
+    for k, (D, obj) in enumerate(dims_objs):
+        o, M = oracle.gen_problem(1000 + k, D)
+        probs.append(speo.Problem([obj], o[None, :], M[None, :, :]))
+    return probs
+
+
+def _engine(speo, probs, sizes, pidx, **kw):
+    args = dict(F=0.5, CR=0.9, seed=SEED, rows_per_block=32)
+    args.update(kw)
+    return speo.Engine(probs, sizes, pidx, **args)
+
+
+LAYOUT = dict(sizes=[16, 24, 40, 20], pidx=[0, 0, 1, 0])
+DIMS = [(10, 2), (30, 3)]
+
+
+def test_plain_generation_matches_oracle(cuda, speo, oracle):
+    probs = _problems(speo, oracle, DIMS)
+    e = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], mode=speo.MODE_PLAIN)
+    X0, f0 = e.read()
+    e.run(1)
+    U, fU = e.read_trials()
+    X1, f1 = e.read()
+    off = e.row_off
+    for i, (np_i, p) in enumerate(zip(LAYOUT["sizes"], LAYOUT["pidx"])):
+        D = DIMS[p][0]
+        obj = DIMS[p][1]
+        rows = slice(off[i], off[i + 1])
+        ld = e.max_dim
+        Xi = np.ascontiguousarray(X0[rows])
+        ref0 = oracle.init_population(np_i, D, ld, -100.0, 100.0, SEED, int(off[i]))
+        np.testing.assert_array_equal(Xi[:, :D], ref0[:, :D])
+        o = probs[p].shifts[0]
+        M = probs[p].rotations[0]
+        np.testing.assert_allclose(f0[rows], oracle.evaluate(obj, Xi[:, :D], o, M), rtol=1e-5)
+        Uref, *_ = oracle.de_generate(Xi, ld, np_i, D, seed=SEED, gen=0, row_base=int(off[i]),
+                                      U=np.zeros_like(Xi))
+        np.testing.assert_array_equal(U[rows][:, :D], Uref[:, :D])
+        np.testing.assert_allclose(fU[rows], oracle.evaluate(obj, Uref[:, :D], o, M), rtol=1e-5)
+        X2, f2, _ = oracle.de_select(Xi, f0[rows], np.ascontiguousarray(U[rows]), fU[rows], ld,
+                                     np_i, D)
+        np.testing.assert_array_equal(X1[rows], X2)
+        np.testing.assert_array_equal(f1[rows], f2)
+    st = e.stats()
+    assert st["generations"] == 1 and st["evaluations"] == 2 * sum(LAYOUT["sizes"])
+
+
+@pytest.mark.parametrize("rt", [16, 32, 64])
+def test_fused_equals_plain(cuda, speo, oracle, rt):
+    probs = _problems(speo, oracle, DIMS)
+    a = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], mode=speo.MODE_PLAIN,
+                rows_per_block=rt)
+    b = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], mode=speo.MODE_FUSED,
+                rows_per_block=rt)
+    for chunk in (1, 2, 5):
+        a.run(chunk)
+        b.run(chunk)
+    Ua, fUa = a.read_trials()
+    Ub, fUb = b.read_trials()
+    np.testing.assert_array_equal(Ua, Ub)
+    np.testing.assert_array_equal(fUa, fUb)
+    Xa, fa = a.read()
+    Xb, fb = b.read()
+    np.testing.assert_array_equal(Xa, Xb)
+    np.testing.assert_array_equal(fa, fb)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_migration_matches_oracle(cuda, speo, oracle, mode):
+    probs = _problems(speo, oracle, DIMS)
+    k = 2
+    base = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], mode=mode)
+    mig = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], mode=mode, interval=3,
+                  migration_size=k)
+    base.run(1)      # generation 0, then migration (0 % 3 == 0) in `mig`
+    mig.run(1)
+    X, f = base.read()
+    Xm, fm = mig.read()
+    off = base.row_off.astype(np.int32)
+    dims = np.array([DIMS[p][0] for p in LAYOUT["pidx"]], np.int32)
+    # ring per problem: 0 -> 1 -> 3 -> 0 (problem 0), island 2 alone
+    dst = np.array([1, 3, -1, 0], np.int32)
+    Xo, fo = oracle.migrate_ring(X, f, X.shape[1], off, dims, dst, k)
+    np.testing.assert_array_equal(Xm, Xo)
+    np.testing.assert_array_equal(fm, fo)
+    assert mig.stats()["migrations"] == 1
+    # generations 1, 2 do not migrate; 3 does
+    base.run(3)
+    mig.run(2)
+    assert mig.stats()["migrations"] == 1
+    mig.run(1)
+    assert mig.stats()["migrations"] == 2
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_graph_replay_equals_eager(cuda, speo, oracle, mode):
+    probs = _problems(speo, oracle, DIMS)
+    kw = dict(mode=mode, interval=7, migration_size=2)
+    eager = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], use_graph=False, **kw)
+    graph = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], use_graph=True, **kw)
+    eager.run(40)
+    for chunk in (1, 13, 7, 19):          # several replays of the same graphs
+        graph.run(chunk)
+        _ = graph.stats()
+    Xe, fe = eager.read()
+    Xg, fg = graph.read()
+    np.testing.assert_array_equal(Xe, Xg)
+    np.testing.assert_array_equal(fe, fg)
+    st = graph.stats()
+    assert st["graph_launches"] >= 6 and st["generations"] == 40
+    assert st["migrations"] == eager.stats()["migrations"] == 6   # after 0, 7, ..., 35
+
+
+def test_best_reverified_in_fp64(cuda, speo, oracle):
+    probs = _problems(speo, oracle, DIMS)
+    e = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], mode=speo.MODE_FUSED)
+    e.run(25)
+    rows, f32, f64 = e.best()
+    X, f = e.read()
+    for i, p in enumerate(LAYOUT["pidx"]):
+        seg = f[e.row_off[i]:e.row_off[i + 1]]
+        assert rows[i] == e.row_off[i] + int(np.argmin(seg))
+        assert f32[i] == seg.min()
+        D, obj = DIMS[p]
+        ref = oracle.evaluate(obj, X[rows[i], :D][None, :], probs[p].shifts[0],
+                              probs[p].rotations[0])[0]
+        assert f64[i] == pytest.approx(ref, rel=1e-12)
+        assert f32[i] == pytest.approx(ref, rel=1e-5)
+
+
+def test_uneven_islands_and_precisions(cuda, speo, oracle):
+    dims_objs = [(10, 1), (30, 2), (50, 3), (100, 2)]
+    probs = _problems(speo, oracle, dims_objs)
+    rng = np.random.default_rng(4)
+    sizes = rng.integers(50, 200, 9).tolist()
+    pidx = [i % 4 for i in range(9)]
+    for prec in (speo.PREC_FP32, speo.PREC_BF16X3, speo.PREC_BF16X6):
+        e = _engine(speo, probs, sizes, pidx, precision=prec, interval=5, migration_size=2,
+                    use_graph=True, rows_per_block=0)
+        _, f0, _ = e.best()
+        e.run(30)
+        rows, f32, f64 = e.best()
+        # the FP32 best never increases; its FP64 value may move by the FP32
+        # evaluation error of the split precisions
+        assert (f64 <= f0 * (1 + 1e-3) + 1e-9).all()
+        assert np.isfinite(f64).all()
+        st = e.stats()
+        assert st["evaluations"] == sum(sizes) * 31
+        assert st["rows_per_block"] in (16, 32, 64)
+
+
+def test_convergence_on_small_problem(cuda, speo, oracle):
+    probs = _problems(speo, oracle, [(10, 1)])
+    e = _engine(speo, probs, [64, 64], [0, 0], interval=20, migration_size=2, use_graph=True)
+    _, _, f_start = e.best()
+    e.run(600)
+    _, _, f_end = e.best()
+    assert f_end.min() < 1e-4 * f_start.min()
