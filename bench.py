@@ -49,6 +49,103 @@ def build_engine(S, args):
     return eng, problems, data
 
 
+TTE_VARIANTS = {
+    # name: (mode, precision, graph, surrogate candidates)
+    "plain_fp32": ("plain", "fp32", False, 0),
+    "batched_fp32": ("fused", "fp32", True, 0),
+    "batched_bf16x3": ("fused", "bf16x3", True, 0),
+    "surrogate_bf16x3": ("plain", "bf16x3", False, 8),
+}
+
+
+def run_to_targets(S, problems, objs, islands, island_size, variant, targets, max_gens,
+                   check_every):
+    """Runs one engine variant, checking the FP64-verified best of each
+    objective every `check_every` generations; returns the trajectory and the
+    wall time at which each objective first reached its target."""
+    mode, prec, graph, cands = TTE_VARIANTS[variant]
+    precision = {v: k for k, v in S.PREC_NAMES.items()}[prec]
+    pidx = [i % len(problems) for i in range(islands)]
+    extra = {}
+    if cands:
+        extra = dict(surrogate_candidates=cands, surrogate_train_interval=10,
+                     surrogate_archive=65536, surrogate_steps=40, surrogate_batch=1024,
+                     surrogate_lr=1e-3, surrogate_rho_min=0.5)
+    eng = S.Engine(problems, [island_size] * islands, pidx, F=0.5, CR=0.9, seed=BASE_SEED + 7,
+                   interval=50, migration_size=2, precision=precision,
+                   mode=S.MODE_FUSED if mode == "fused" else S.MODE_PLAIN, use_graph=graph,
+                   rows_per_block=0, **extra)
+    eng.run(1)          # graph capture / first launches outside the clock
+    eng.best()
+    pidx = np.array(pidx)
+    traj, reached = [], {}
+    gens = 0
+    t0 = time.perf_counter()
+    while gens < max_gens:
+        eng.run(check_every)
+        gens += check_every
+        _, _, f64 = eng.best()
+        t = time.perf_counter() - t0
+        errs = {objs[k]: float(f64[pidx == k].min()) for k in range(len(problems))}
+        traj.append((gens, t, errs))
+        for name, e in errs.items():
+            if targets is not None and name not in reached and e <= targets[name]:
+                reached[name] = (t, gens)
+        if targets is not None and len(reached) == len(targets):
+            break
+    out = {"generations": gens, "seconds": time.perf_counter() - t0,
+           "final_error": traj[-1][2], "reached": reached}
+    if cands:
+        rho, act, loss = eng.surrogate_stats()
+        out["surrogate_rho"] = [float(r) for r in rho]
+        out["surrogate_active"] = [int(a) for a in act]
+        out["surrogate_predictions"] = eng.stats()["surrogate_predictions"]
+    eng.close()
+    return traj, out
+
+
+def time_to_error(S, dim, islands, island_size, max_gens, check_every):
+    """Time-to-error protocol: the plain FP32 engine runs max_gens generations;
+    each objective's target is 1e-2 if plain reaches it, else plain's final
+    error; every variant is timed to those targets (time includes the periodic
+    FP64-verified best checks, identical for all variants)."""
+    names = {S.OBJ_ELLIPTIC: "elliptic", S.OBJ_RASTRIGIN: "rastrigin"}
+    problems, objs = [], []
+    for k, obj in enumerate((S.OBJ_RASTRIGIN, S.OBJ_ELLIPTIC)):
+        o, M = S.generate_problem_data(S.problem_seed(BASE_SEED + 100, k), dim)
+        problems.append(S.Problem([obj], o[None, :], M[None, :, :]))
+        objs.append(names[obj])
+    traj, plain = run_to_targets(S, problems, objs, islands, island_size, "plain_fp32", None,
+                                 max_gens, check_every)
+    final = traj[-1][2]
+    targets = {n: (1e-2 if final[n] <= 1e-2 else final[n]) for n in objs}
+    plain_t = {}
+    for n in objs:
+        for g, t, errs in traj:
+            if errs[n] <= targets[n]:
+                plain_t[n] = (t, g)
+                break
+    result = {"workload": f"{islands} islands x NP={island_size}, D={dim}, Rastrigin/elliptic "
+                          f"round-robin, ring best-2 every 50; best checked every {check_every} "
+                          "generations (FP64 re-verified)",
+              "targets": targets, "plain_fp32": {"reached": plain_t, **plain}, "variants": {}}
+    for v in ("batched_fp32", "batched_bf16x3", "surrogate_bf16x3"):
+        print(f"# tte variant {v}", file=sys.stderr, flush=True)
+        _, res = run_to_targets(S, problems, objs, islands, island_size, v, targets,
+                                max_gens * 4, check_every)
+        sp = {}
+        for n in objs:
+            if n in res["reached"] and n in plain_t:
+                sp[n] = plain_t[n][0] / res["reached"][n][0]
+        if len(res["reached"]) == len(objs):
+            sp["all"] = max(t for t, _ in plain_t.values()) / max(t for t, _ in res["reached"].values())
+        res["speedup_vs_plain"] = sp
+        result["variants"][v] = res
+    for p in problems:
+        p.close()
+    return result
+
+
 def cpu_baseline(data, seconds, D, NP):
     """Single-thread CPU trial evals/s: objective evaluation + DE step."""
     import oracle as O
@@ -120,6 +217,10 @@ def main():
     ap.add_argument("--profile-gens", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=4.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tte", action="store_true", help="skip the time-to-error runs")
+    ap.add_argument("--tte-gens", type=int, default=2000)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run a device copy of this many MiB (scales rocprof byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -132,6 +233,16 @@ def main():
     import speo_de as S
 
     S.check(S.lib().speo_device_check())
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: 3 x {args.calibrate_copy_mib} MiB read + written",
+              file=sys.stderr, flush=True)
+        del src, dst
     eng, problems, data = build_engine(S, args)
     rows = eng.total_rows
     D = args.dim
@@ -209,9 +320,12 @@ def main():
         "final_error_f64": final,
         "generations_total": eng.stats()["generations"],
     }
+    eng.close()
+    if not args.no_tte:
+        print("# time-to-error runs", file=sys.stderr, flush=True)
+        out["time_to_error"] = time_to_error(S, D, 32, 256, args.tte_gens, 10)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(data, args.cpu_seconds, D, args.island_size)
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

@@ -259,6 +259,20 @@ typedef struct {
     int mode;            /* SPEO_MODE_*                                             */
     int use_graph;       /* capture the generation loop between migrations in a hipGraph */
     int rows_per_block;  /* 0 = autotuned; else 16, 32 or 64                        */
+    /* Learned surrogate (no counterpart in the reference).  0 candidates = off.
+     * When on, every target gets `surrogate_candidates` trials (candidate 0 is
+     * the plain DE trial); a per-problem MLP trained on the archive of true
+     * evaluations ranks them and only the best-ranked one is truly evaluated.
+     * Until the MLP is trained, and whenever its Spearman rho on held-out
+     * archive points is below surrogate_rho_min, candidate 0 is used (plain
+     * DE).  All islands must then share one dimension. */
+    int surrogate_candidates;     /* e.g. 8                                       */
+    int surrogate_train_interval; /* generations between training rounds (10)    */
+    int surrogate_archive;        /* ring archive rows per problem (65536)        */
+    int surrogate_steps;          /* Adam steps per training round                */
+    int surrogate_batch;          /* minibatch rows                               */
+    float surrogate_lr;           /* Adam learning rate (1e-3)                    */
+    float surrogate_rho_min;      /* minimum held-out Spearman rho (0.5)          */
 } speo_engine_config;
 
 typedef struct speo_engine_s *speo_engine;
@@ -269,6 +283,8 @@ typedef struct {
     int64_t migrations;    /* migration events                        */
     int64_t graph_launches;
     int rows_per_block;    /* tile in use                            */
+    int64_t surrogate_predictions; /* candidates scored by the surrogate */
+    int64_t surrogate_rounds;      /* training rounds run               */
 } speo_engine_stats;
 
 /* Allocates the populations, initialises them (Philox) and evaluates them
@@ -310,9 +326,45 @@ SPEO_EXPORT int speo_engine_max_dim(speo_engine e);
  * Migration follows the same schedule as speo_engine_run.  Synchronous. */
 SPEO_EXPORT int speo_engine_profile(speo_engine e, int generations, double *avg_ms_out);
 
+/* Per problem: the surrogate's last held-out Spearman rho (-1 before the
+ * first training round), whether it currently screens candidates, and its
+ * last training loss.  Host outputs [num_problems], each may be NULL.
+ * Synchronous. */
+SPEO_EXPORT int speo_engine_surrogate_stats(speo_engine e, float *rho_out, int *active_out,
+                                            float *loss_out);
+
 /* Device pointers of the engine's buffers (for profiling and tests). */
 SPEO_EXPORT int speo_engine_buffers(speo_engine e, float **X, float **fX, int64_t *elem_off,
                                     int *row_off, int *ld);
+
+/* ---- the surrogate MLP ------------------------------------------------ *
+ * No counterpart in the reference (new in this build).  An MLP
+ * dim -> 256 -> 256 -> 1 with SiLU on inputs x / 100, BF16 MFMA operands with
+ * FP32 accumulation and FP32 master weights.  Flat parameter layout:
+ * W1 [256][round_up(dim, 32)] (padding columns 0), b1 [256], W2 [256][256],
+ * b2 [256], w3 [256], b3 [4] (one used).  These entry points expose one
+ * network for tests and tools; the engine owns its own networks. */
+typedef struct speo_mlp_s *speo_mlp;
+
+SPEO_EXPORT int speo_mlp_create(int dim, int max_batch, uint64_t seed, speo_mlp *out);
+SPEO_EXPORT int speo_mlp_destroy(speo_mlp m);
+SPEO_EXPORT int64_t speo_mlp_num_params(speo_mlp m);
+/* Host copies of the flat parameters: get_out receives them, set_in (if not
+ * NULL) replaces them first and resets the Adam state.  Synchronous. */
+SPEO_EXPORT int speo_mlp_params(speo_mlp m, float *get_out, const float *set_in);
+/* pred[i] = MLP(X[i]) for n rows (device pointers, stride ld >= dim). */
+SPEO_EXPORT int speo_mlp_forward(speo_mlp m, const float *X, int ld, int n, float *pred,
+                                 speo_stream_t stream);
+/* Loss 0.5 mean((y - t')^2) with t' = (t - t_mean) / t_std over n <= max_batch
+ * rows (device X, t) and its gradient (host, flat layout).  Synchronous. */
+SPEO_EXPORT int speo_mlp_loss_grad(speo_mlp m, const float *X, int ld, int n, const float *t,
+                                   float t_mean, float t_std, float *loss_out, float *grad_out);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the last gradient.  Synchronous. */
+SPEO_EXPORT int speo_mlp_adam_step(speo_mlp m, float lr);
+/* Spearman rank correlation (average ranks) of device arrays a, b of n <=
+ * 4096 values into *rho_out (device). */
+SPEO_EXPORT int speo_spearman(const float *a, const float *b, int n, float *rho_out,
+                              speo_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -36,16 +36,16 @@ struct GenArgs {
 // {k/4, row, gen, purpose 0}; the first n_inj draws may be injected.
 struct DonorStream {
     uint64_t seed;
-    uint32_t row, gen, k, cached;
+    uint32_t row, gen, k, cached, purpose;
     U4 cw;
-    __device__ DonorStream(uint64_t s, uint32_t r, uint32_t g)
-        : seed(s), row(r), gen(g), k(0), cached(0xffffffffu), cw{0, 0, 0, 0} {}
+    __device__ DonorStream(uint64_t s, uint32_t r, uint32_t g, uint32_t pur = kPurposeDonor)
+        : seed(s), row(r), gen(g), k(0), cached(0xffffffffu), purpose(pur), cw{0, 0, 0, 0} {}
     __device__ __forceinline__ float next(const float *inj, int n_inj) {
         const uint32_t kk = k++;
         if (inj && (int)kk < n_inj) return inj[kk];
         const uint32_t blk = kk >> 2;
         if (blk != cached) {
-            cw = philox_block(seed, blk, row, gen, kPurposeDonor);
+            cw = philox_block(seed, blk, row, gen, purpose);
             cached = blk;
         }
         return u01(word(cw, kk & 3u));

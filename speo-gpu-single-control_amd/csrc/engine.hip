@@ -20,7 +20,9 @@
 #pragma clang fp contract(off)
 
 #include <chrono>
+#include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -28,6 +30,7 @@
 #include "de.h"
 #include "eval.h"
 #include "eval_f64.h"
+#include "surrogate.h"
 
 namespace speo {
 
@@ -321,6 +324,24 @@ struct Engine {
 
     int rt = 32;
     int64_t gens = 0, fes = 0, migrations = 0, graph_launches = 0;
+
+    // learned surrogate (config.surrogate_candidates > 0)
+    struct SurrProblem {
+        Mlp net;
+        int qoff = 0, rows = 0, head = 0;
+        int64_t appended = 0;
+        bool trained = false;
+    };
+    int C = 0, Kp = 0, Q = 0;
+    std::vector<std::unique_ptr<SurrProblem>> surr;
+    SurrRowDesc *srows_dev = nullptr;
+    float *cand = nullptr, *pred = nullptr, *rho_dev = nullptr, *AX = nullptr, *Ay = nullptr;
+    float *Xb = nullptr, *tb = nullptr, *Xh = nullptr, *yh = nullptr, *predh = nullptr;
+    bool any_trained = false;
+    int64_t surr_pred = 0, surr_rounds = 0;
+    int surrogate_setup();
+    int surrogate_generation(int gen_offset, hipStream_t s);
+    int surrogate_train(hipStream_t s);
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
     bool graph_failed = false;
 
@@ -347,7 +368,8 @@ Engine::~Engine() {
     for (auto &kv : tiles_dev) (void)hipFree(kv.second);
     void *bufs[] = {isl_dev, row_off_dev, dst_dev, best_buf, worst_buf, best_row_dev,
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
-                    X, fX[0], fX[1], U[0], U[1], fU[0], fU[1]};
+                    X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
+                    AX, Ay, Xb, tb, Xh, yh, predh};
     for (void *b : bufs) (void)hipFree(b);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -496,8 +518,13 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
         hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, fU[b], total_rows, INFINITY);
         SPEO_TRY(check_launch("k_fill"));
     }
+    if (c.surrogate_candidates > 0) {
+        cfg.mode = SPEO_MODE_PLAIN;  // the screened trial replaces generate; selection each generation
+        cfg.use_graph = 0;
+    }
     rt = c.rows_per_block;
     if (rt == 0) SPEO_TRY(autotune());
+    if (c.surrogate_candidates > 0) SPEO_TRY(surrogate_setup());
     SPEO_TRY(build_tiles(rt));
     // Initial evaluation (island_EA.cc:81 counts it as island_size FEs).
     GenerationArgs a{};
@@ -511,6 +538,107 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     SPEO_TRY(check_launch("k_fill"));
     fes = total_rows;
     SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    return SPEO_OK;
+}
+
+int Engine::surrogate_setup() {
+    C = cfg.surrogate_candidates;
+    const int D = isl[0].dim;
+    for (const auto &d : isl)
+        if (d.dim != D) return set_error(SPEO_ERR_ARG, "surrogate: all islands must share one dimension");
+    if (C > 32 || cfg.num_problems > kMaxSurrProblems || cfg.surrogate_archive < 1024 ||
+        cfg.surrogate_batch < 16 || cfg.surrogate_steps < 0 || cfg.surrogate_train_interval < 1 ||
+        !(cfg.surrogate_lr > 0.f))
+        return set_error(SPEO_ERR_ARG, "surrogate: need <= 32 candidates, <= 16 problems, archive "
+                                       ">= 1024, batch >= 16, train interval >= 1, lr > 0");
+    Kp = round_up(D, 32);
+    std::vector<SurrRowDesc> rows;
+    for (int p = 0; p < cfg.num_problems; ++p) {
+        auto sp = std::make_unique<SurrProblem>();
+        sp->qoff = (int)rows.size();
+        for (int i = 0; i < I; ++i)
+            if (isl[(size_t)i].problem == p)
+                for (int li = 0; li < isl[(size_t)i].np; ++li)
+                    rows.push_back(SurrRowDesc{isl[(size_t)i].row_off + li, i, p, li});
+        sp->rows = (int)rows.size() - sp->qoff;
+        if (sp->rows > 0) {
+            const int mb = std::max({cfg.surrogate_batch, sp->rows * C, 1024});
+            SPEO_TRY(sp->net.alloc(D, mb));
+            SPEO_TRY(sp->net.init(cfg.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(p + 1)), stream));
+        }
+        surr.push_back(std::move(sp));
+    }
+    Q = (int)rows.size();
+    SPEO_TRY(upload_vec(&srows_dev, rows));
+    const size_t P = (size_t)cfg.num_problems, cap = (size_t)cfg.surrogate_archive;
+    SPEO_TRY_HIP(hipMalloc(&cand, sizeof(float) * (size_t)Q * C * Kp));
+    SPEO_TRY_HIP(hipMalloc(&pred, sizeof(float) * (size_t)Q * C));
+    SPEO_TRY(upload_vec(&rho_dev, std::vector<float>(P, -1.0f)));
+    SPEO_TRY_HIP(hipMalloc(&AX, sizeof(float) * P * cap * Kp));
+    SPEO_TRY_HIP(hipMalloc(&Ay, sizeof(float) * P * cap));
+    SPEO_TRY_HIP(hipMalloc(&Xb, sizeof(float) * (size_t)cfg.surrogate_batch * Kp));
+    SPEO_TRY_HIP(hipMalloc(&tb, sizeof(float) * (size_t)cfg.surrogate_batch));
+    SPEO_TRY_HIP(hipMalloc(&Xh, sizeof(float) * 1024 * (size_t)Kp));
+    SPEO_TRY_HIP(hipMalloc(&yh, sizeof(float) * 1024));
+    SPEO_TRY_HIP(hipMalloc(&predh, sizeof(float) * 1024));
+    return SPEO_OK;
+}
+
+// One surrogate-screened generation: C candidates per target (only candidate
+// 0 before the first training round), MLP scores, pick, true evaluation of
+// the picked trial, archive append, selection.
+int Engine::surrogate_generation(int gen_offset, hipStream_t s) {
+    const int Ce = any_trained ? C : 1;
+    SPEO_TRY(gen_candidates(isl_dev, srows_dev, Q, Ce, X, cfg.min_bound, cfg.max_bound, cfg.seed,
+                            gen_base, (uint32_t)gen_offset, cand, Kp, s));
+    if (any_trained) {
+        for (auto &sp : surr)
+            if (sp->rows > 0)
+                SPEO_TRY(sp->net.forward(cand + (size_t)sp->qoff * Ce * Kp, Kp, sp->rows * Ce,
+                                         pred + (size_t)sp->qoff * Ce, s));
+        surr_pred += (int64_t)Q * Ce;
+    }
+    SPEO_TRY(pick_candidates(isl_dev, srows_dev, Q, Ce, cand, Kp, pred, rho_dev,
+                             any_trained ? cfg.surrogate_rho_min : INFINITY, U[0], s));
+    SPEO_TRY(launch(kModeEval, gen_offset, rt, s));
+    SurrHeads h{};
+    for (size_t p = 0; p < surr.size(); ++p) {
+        h.head[p] = surr[p]->head;
+        h.qoff[p] = surr[p]->qoff;
+    }
+    SPEO_TRY(archive_append(isl_dev, srows_dev, Q, U[0], fU[0], AX, Ay, cfg.surrogate_archive, Kp,
+                            h, s));
+    for (auto &sp : surr) {
+        sp->head = (int)(((int64_t)sp->head + sp->rows) % cfg.surrogate_archive);
+        sp->appended += sp->rows;
+    }
+    return launch(kModeSelect, gen_offset, rt, s);
+}
+
+int Engine::surrogate_train(hipStream_t s) {
+    const size_t cap = (size_t)cfg.surrogate_archive;
+    for (size_t p = 0; p < surr.size(); ++p) {
+        SurrProblem &sp = *surr[p];
+        if (sp.rows == 0) continue;
+        const int n_valid = (int)std::min<int64_t>(sp.appended, (int64_t)cap);
+        if (n_valid < std::max(cfg.surrogate_batch, 512)) continue;
+        const float *ax = AX + p * cap * Kp, *ay = Ay + p * cap;
+        SPEO_TRY(mean_std(ay, n_valid, sp.net.stats, s));
+        for (int step = 0; step < cfg.surrogate_steps; ++step) {
+            SPEO_TRY(gather_batch(ax, ay, n_valid, Kp, cfg.seed ^ (0xD1B54A32D192ED03ull * (p + 1)),
+                                  (uint32_t)surr_rounds, (uint32_t)step, cfg.surrogate_batch, Xb,
+                                  tb, s));
+            SPEO_TRY(sp.net.loss_grad(Xb, Kp, cfg.surrogate_batch, tb, s));
+            SPEO_TRY(sp.net.adam(cfg.surrogate_lr, s));
+        }
+        const int H = std::min(1024, n_valid / 8);
+        SPEO_TRY(gather_holdout(ax, ay, H, Kp, Xh, yh, s));
+        SPEO_TRY(sp.net.forward(Xh, Kp, H, predh, s));
+        SPEO_TRY(spearman(predh, yh, H, rho_dev + p, s));
+        sp.trained = true;
+        any_trained = true;
+    }
+    ++surr_rounds;
     return SPEO_OK;
 }
 
@@ -577,7 +705,9 @@ int Engine::migrate(hipStream_t s) {
 // `p` and `pending` advance on the host as the launches are enqueued.
 int Engine::enqueue_chunk(int L, bool do_migrate, hipStream_t s) {
     for (int g = 0; g < L; ++g) {
-        if (cfg.mode == SPEO_MODE_FUSED) {
+        if (C > 0) {
+            SPEO_TRY(surrogate_generation(g, s));
+        } else if (cfg.mode == SPEO_MODE_FUSED) {
             SPEO_TRY(launch(kModeFused, g, rt, s));
             p = 1 - p;
             pending = true;
@@ -593,6 +723,17 @@ int Engine::enqueue_chunk(int L, bool do_migrate, hipStream_t s) {
 }
 
 int Engine::run(int generations) {
+    if (C > 0) {
+        for (int g = 0; g < generations; ++g) {
+            const bool mig = cfg.interval > 0 && cfg.migration_size > 0 && gens % cfg.interval == 0;
+            SPEO_TRY(enqueue_chunk(1, mig, stream));
+            ++gens;
+            fes += total_rows;
+            if (mig) ++migrations;
+            if (gens % cfg.surrogate_train_interval == 0) SPEO_TRY(surrogate_train(stream));
+        }
+        return SPEO_OK;
+    }
     int left = generations;
     while (left > 0) {
         // Generations are counted from 0; migration follows generation g when
@@ -933,6 +1074,30 @@ int speo_engine_get_stats(speo_engine e, speo_engine_stats *out) {
     out->migrations = e->e.migrations;
     out->graph_launches = e->e.graph_launches;
     out->rows_per_block = e->e.rt;
+    out->surrogate_predictions = e->e.surr_pred;
+    out->surrogate_rounds = e->e.surr_rounds;
+    return SPEO_OK;
+}
+
+int speo_engine_surrogate_stats(speo_engine e, float *rho_out, int *active_out, float *loss_out) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_surrogate_stats: null engine");
+    Engine &E = e->e;
+    const int P = E.cfg.num_problems;
+    std::vector<float> rho((size_t)P, -1.f), loss((size_t)P, 0.f);
+    if (E.C > 0) {
+        SPEO_TRY_HIP(hipStreamSynchronize(E.stream));
+        SPEO_TRY_HIP(hipMemcpy(rho.data(), E.rho_dev, sizeof(float) * P, hipMemcpyDeviceToHost));
+        for (int p = 0; p < P; ++p)
+            if (E.surr[(size_t)p]->rows > 0 && E.surr[(size_t)p]->trained)
+                SPEO_TRY_HIP(hipMemcpy(&loss[(size_t)p], E.surr[(size_t)p]->net.stats, sizeof(float),
+                                       hipMemcpyDeviceToHost));
+    }
+    for (int p = 0; p < P; ++p) {
+        if (rho_out) rho_out[p] = rho[(size_t)p];
+        if (active_out)
+            active_out[p] = (E.C > 0 && E.surr[(size_t)p]->trained && rho[(size_t)p] >= E.cfg.surrogate_rho_min) ? 1 : 0;
+        if (loss_out) loss_out[p] = loss[(size_t)p];
+    }
     return SPEO_OK;
 }
 

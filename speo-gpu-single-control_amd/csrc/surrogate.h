@@ -1,0 +1,84 @@
+// surrogate.h -- the learned surrogate: an MLP D -> 256 -> 256 -> 1 with SiLU,
+// BF16 MFMA operands with FP32 accumulation and FP32 master weights, trained
+// online with Adam on a ring archive of true evaluations y = log(1 + f).
+// No counterpart in the reference; new in this build.
+#pragma once
+
+#include "common.h"
+
+namespace speo {
+
+constexpr int kHidden = 256;
+constexpr float kXScale = 0.01f;  // inputs in [-100, 100] -> [-1, 1]
+
+// Flat parameter layout (floats): W1 [256][Kp], b1 [256], W2 [256][256],
+// b2 [256], w3 [256], b3 [4 (1 used)].
+struct MlpLayout {
+    int D = 0, Kp = 0;
+    size_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, ow3 = 0, ob3 = 0, n = 0;
+    void set(int dim) {
+        D = dim;
+        Kp = round_up(dim, 32);
+        oW1 = 0;
+        ob1 = oW1 + (size_t)kHidden * Kp;
+        oW2 = ob1 + kHidden;
+        ob2 = oW2 + (size_t)kHidden * kHidden;
+        ow3 = ob2 + kHidden;
+        ob3 = ow3 + kHidden;
+        n = ob3 + 4;
+    }
+};
+
+// Device buffers of one MLP with its training workspace.
+struct Mlp {
+    MlpLayout L;
+    float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr;  // [n]
+    int max_batch = 0;
+    float *A1 = nullptr, *H1 = nullptr, *A2 = nullptr, *H2 = nullptr;  // [max_batch][256]
+    float *dH1 = nullptr, *dH2 = nullptr;                              // [max_batch][256]
+    float *y = nullptr, *dy = nullptr;                                 // [max_batch]
+    float *stats = nullptr;  // [0] loss, [1] target mean, [2] target std, [3] rho
+    int64_t adam_t = 0;
+    ~Mlp();
+    int alloc(int dim, int batch);
+    int init(uint64_t seed, hipStream_t s);
+    // y = MLP(X) for n rows of X (stride ld >= D, raw inputs), batched through
+    // the workspace; pred receives n values.
+    int forward(const float *X, int ld, int n, float *pred, hipStream_t s);
+    // Loss (0.5 mean (y - t)^2) and gradients into G for n <= max_batch rows.
+    int loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s);
+    int adam(float lr, hipStream_t s);
+};
+
+// One target row in surrogate mode, rows ordered problem by problem.
+struct SurrRowDesc {
+    int gr;       // global row
+    int island;   // island index
+    int problem;  // problem index
+    int li;       // row within its island
+};
+
+constexpr int kMaxSurrProblems = 16;
+struct SurrHeads {
+    int head[kMaxSurrProblems];  // next archive slot per problem
+    int qoff[kMaxSurrProblems];  // first q of each problem
+};
+
+struct IslandDesc;
+int gen_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C, const float *X,
+                   float lo, float hi, uint64_t seed, const uint32_t *gen_base,
+                   uint32_t gen_offset, float *cand, int Kp, hipStream_t s);
+int pick_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C,
+                    const float *cand, int Kp, const float *pred, const float *rho,
+                    float rho_min, float *U, hipStream_t s);
+int archive_append(const IslandDesc *isl, const SurrRowDesc *rows, int Q, const float *U,
+                   const float *fU, float *AX, float *Ay, int cap, int Kp, SurrHeads heads,
+                   hipStream_t s);
+int gather_batch(const float *AX, const float *Ay, int n_valid, int Kp, uint64_t seed,
+                 uint32_t round, uint32_t step, int batch, float *Xb, float *tb, hipStream_t s);
+int gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh, float *yh,
+                   hipStream_t s);
+int mean_std(const float *y, int n, float *stats, hipStream_t s);
+int spearman(const float *a, const float *b, int n, float *rho_out, hipStream_t s);
+
+}  // namespace speo

@@ -1,0 +1,644 @@
+// surrogate.hip -- HIP kernels of the surrogate MLP (forward, backward, Adam),
+// its archive and rank statistics, and the candidate-screening kernels.
+//
+// GEMMs run on v_mfma_f32_16x16x32_bf16: FP32 operands are rounded to BF16
+// (round to nearest even) while staged to LDS, accumulation is FP32.  The
+// output layer (256 -> 1), the loss, the SiLU derivatives and Adam run in
+// FP32 on the vector ALUs.
+#include "surrogate.h"
+
+#include "de.h"
+#include "eval_core.h"
+
+namespace speo {
+
+enum { kEpiStore = 0, kEpiBiasSilu = 1, kEpiMulDsilu = 2 };
+
+struct GemmArgs {
+    int M, N, K;
+    const float *A;
+    long a_m, a_k;  // A(m, k) = A[m * a_m + k * a_k]
+    const float *B;
+    long b_k, b_n;  // B(k, n) = B[k * b_k + n * b_n]
+    float a_scale, b_scale;
+    float *C;
+    long c_m;
+    const float *bias;  // [N]
+    float *P;           // pre-activations: written by kEpiBiasSilu, read by kEpiMulDsilu
+    long p_m;
+    int epi;
+};
+
+__device__ __forceinline__ float sigmoidf_(float a) { return 1.0f / (1.0f + expf(-a)); }
+__device__ __forceinline__ float siluf_(float a) { return a * sigmoidf_(a); }
+__device__ __forceinline__ float dsiluf_(float a) {
+    const float s = sigmoidf_(a);
+    return s * (1.0f + a * (1.0f - s));
+}
+
+// 64 x 64 tile, 4 waves as 2 x 2, each wave 32 x 32 = 2 x 2 MFMA 16x16 tiles,
+// K staged 32 at a time.
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
+    constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+    __shared__ __align__(16) uint16_t As[TM * LDK];
+    __shared__ __align__(16) uint16_t Bs[TN * LDK];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1, lr = lane & 15, lk = lane >> 4;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += TK) {
+        __syncthreads();
+        for (int e = tid; e < TM * TK; e += 256) {
+            const int r = e / TK, k = e % TK;
+            const int m = m0 + r, kk = k0 + k;
+            float v = 0.f;
+            if (m < g.M && kk < g.K) v = g.A[(long)m * g.a_m + (long)kk * g.a_k] * g.a_scale;
+            As[r * LDK + k] = to_bits(v, false);
+        }
+        for (int e = tid; e < TN * TK; e += 256) {
+            const int n = e / TK, k = e % TK;
+            const int nn = n0 + n, kk = k0 + k;
+            float v = 0.f;
+            if (nn < g.N && kk < g.K) v = g.B[(long)kk * g.b_k + (long)nn * g.b_n] * g.b_scale;
+            Bs[n * LDK + k] = to_bits(v, false);
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * 32 + i * 16 + lr) * LDK + lk * 8]);
+            b[i] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * 32 + i * 16 + lr) * LDK + lk * 8]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = m0 + wm * 32 + i * 16 + lk * 4 + q;
+                const int n = n0 + wn * 32 + j * 16 + lr;
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[i][j][q];
+                if (g.epi == kEpiBiasSilu) {
+                    v += g.bias ? g.bias[n] : 0.f;
+                    if (g.P) g.P[(long)m * g.p_m + n] = v;
+                    v = siluf_(v);
+                } else if (g.epi == kEpiMulDsilu) {
+                    v *= dsiluf_(g.P[(long)m * g.p_m + n]);
+                }
+                g.C[(long)m * g.c_m + n] = v;
+            }
+}
+
+static int gemm(const GemmArgs &g, hipStream_t s) {
+    if (g.M <= 0 || g.N <= 0) return SPEO_OK;
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
+    return check_launch("k_gemm_bf16");
+}
+
+// y[b] = h2[b] . w3 + b3 (one wave per row); optional dy = (y - t') / B with
+// t' = (t - mean) / std from stats[1..2], and the loss into stats[0].
+__global__ void k_mlp_out(const float *H2, const float *w3, const float *b3, int n, float *y,
+                          const float *t, float *dy, float *stats) {
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    float acc = 0.f;
+    for (int j = lane; j < kHidden; j += 64) acc = fmaf(H2[(long)row * kHidden + j], w3[j], acc);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) {
+        const float yy = acc + b3[0];
+        y[row] = yy;
+        if (t) {
+            const float tt = (t[row] - stats[1]) / stats[2];
+            dy[row] = (yy - tt) / (float)n;
+            atomicAdd(&stats[0], 0.5f * (yy - tt) * (yy - tt) / (float)n);
+        }
+    }
+}
+
+// dH2[b][j] = dy[b] w3[j] silu'(A2[b][j])
+__global__ void k_mlp_dh2(const float *dy, const float *w3, const float *A2, int n, float *dH2) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * kHidden) return;
+    const int b = (int)(i / kHidden), j = (int)(i % kHidden);
+    dH2[i] = dy[b] * w3[j] * dsiluf_(A2[i]);
+}
+
+// out[j] = sum_b M[b][j] (* w[b] if w): one thread per column, fixed order.
+__global__ void k_colsum(const float *Mx, const float *w, int n, int cols, float *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    float acc = 0.f;
+    for (int b = 0; b < n; ++b) acc = fmaf(Mx[(long)b * cols + j], w ? w[b] : 1.0f, acc);
+    out[j] = acc;
+}
+
+__global__ void k_sum(const float *v, int n, float *out) {
+    __shared__ float sh[256];
+    float acc = 0.f;
+    for (int i = threadIdx.x; i < n; i += 256) acc += v[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// Adam with bias correction (Kingma & Ba, 2015).
+__global__ void k_adam(float *P, const float *G, float *m, float *v, size_t n, float lr, float c1,
+                       float c2) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = G[i];
+    const float mi = 0.9f * m[i] + 0.1f * g;
+    const float vi = 0.999f * v[i] + 0.001f * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// He-uniform init from Philox: W1 ~ U(+-sqrt(6/D)), W2 ~ U(+-sqrt(6/256)),
+// w3 ~ 0.1 U(+-sqrt(6/256)), biases 0; W1 padding columns 0.
+__global__ void k_mlp_init(float *P, MlpLayout L, uint64_t seed) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const U4 w = philox_block(seed, (uint32_t)(i >> 2), 0u, 0u, kPurposeSurrogate);
+    const float u = 2.0f * u01(word(w, (uint32_t)(i & 3))) - 1.0f;
+    float v = 0.f;
+    if (i < L.ob1) {
+        const int col = (int)(i % L.Kp);
+        v = col < L.D ? u * sqrtf(6.0f / (float)L.D) : 0.f;
+    } else if (i >= L.oW2 && i < L.ob2) {
+        v = u * sqrtf(6.0f / (float)kHidden);
+    } else if (i >= L.ow3 && i < L.ob3) {
+        v = 0.1f * u * sqrtf(6.0f / (float)kHidden);
+    }
+    P[i] = v;
+}
+
+__global__ void k_zero(float *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+static int zero(float *p, size_t n, hipStream_t s) {
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, p, n);
+    return check_launch("k_zero");
+}
+
+// ------------------------------------------------------------------ Mlp
+
+Mlp::~Mlp() {
+    void *b[] = {P, G, m, v, A1, H1, A2, H2, dH1, dH2, y, dy, stats};
+    for (void *p : b) (void)hipFree(p);
+}
+
+int Mlp::alloc(int dim, int batch) {
+    L.set(dim);
+    max_batch = batch;
+    for (float **p : {&P, &G, &m, &v}) SPEO_TRY_HIP(hipMalloc(p, sizeof(float) * L.n));
+    for (float **p : {&A1, &H1, &A2, &H2, &dH1, &dH2})
+        SPEO_TRY_HIP(hipMalloc(p, sizeof(float) * (size_t)batch * kHidden));
+    SPEO_TRY_HIP(hipMalloc(&y, sizeof(float) * batch));
+    SPEO_TRY_HIP(hipMalloc(&dy, sizeof(float) * batch));
+    SPEO_TRY_HIP(hipMalloc(&stats, sizeof(float) * 8));
+    return SPEO_OK;
+}
+
+int Mlp::init(uint64_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_init, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, P, L, seed);
+    SPEO_TRY(check_launch("k_mlp_init"));
+    SPEO_TRY_HIP(hipMemsetAsync(m, 0, sizeof(float) * L.n, s));
+    SPEO_TRY_HIP(hipMemsetAsync(v, 0, sizeof(float) * L.n, s));
+    const float st[8] = {0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    SPEO_TRY_HIP(hipMemcpyAsync(stats, st, sizeof(st), hipMemcpyHostToDevice, s));
+    SPEO_TRY_HIP(hipStreamSynchronize(s));
+    adam_t = 0;
+    return SPEO_OK;
+}
+
+static GemmArgs layer1(const Mlp &N, const float *X, int ld, int n) {
+    GemmArgs g{};
+    g.M = n; g.N = kHidden; g.K = N.L.D;
+    g.A = X; g.a_m = ld; g.a_k = 1; g.a_scale = kXScale;
+    g.B = N.P + N.L.oW1; g.b_k = 1; g.b_n = N.L.Kp; g.b_scale = 1.f;
+    g.C = N.H1; g.c_m = kHidden; g.bias = N.P + N.L.ob1; g.P = N.A1; g.p_m = kHidden;
+    g.epi = kEpiBiasSilu;
+    return g;
+}
+
+static GemmArgs layer2(const Mlp &N, int n) {
+    GemmArgs g{};
+    g.M = n; g.N = kHidden; g.K = kHidden;
+    g.A = N.H1; g.a_m = kHidden; g.a_k = 1; g.a_scale = 1.f;
+    g.B = N.P + N.L.oW2; g.b_k = 1; g.b_n = kHidden; g.b_scale = 1.f;
+    g.C = N.H2; g.c_m = kHidden; g.bias = N.P + N.L.ob2; g.P = N.A2; g.p_m = kHidden;
+    g.epi = kEpiBiasSilu;
+    return g;
+}
+
+int Mlp::forward(const float *X, int ld, int n, float *pred, hipStream_t s) {
+    for (int r0 = 0; r0 < n; r0 += max_batch) {
+        const int nb = std::min(max_batch, n - r0);
+        SPEO_TRY(gemm(layer1(*this, X + (size_t)r0 * ld, ld, nb), s));
+        SPEO_TRY(gemm(layer2(*this, nb), s));
+        hipLaunchKernelGGL(k_mlp_out, dim3((nb + 3) / 4), dim3(256), 0, s, H2, P + L.ow3,
+                           P + L.ob3, nb, pred + r0, nullptr, nullptr, nullptr);
+        SPEO_TRY(check_launch("k_mlp_out"));
+    }
+    return SPEO_OK;
+}
+
+int Mlp::loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s) {
+    if (n > max_batch) return set_error(SPEO_ERR_ARG, "mlp: batch larger than the workspace");
+    SPEO_TRY(gemm(layer1(*this, X, ld, n), s));
+    SPEO_TRY(gemm(layer2(*this, n), s));
+    SPEO_TRY(zero(stats, 1, s));
+    hipLaunchKernelGGL(k_mlp_out, dim3((n + 3) / 4), dim3(256), 0, s, H2, P + L.ow3, P + L.ob3, n,
+                       y, t, dy, stats);
+    SPEO_TRY(check_launch("k_mlp_out"));
+    hipLaunchKernelGGL(k_mlp_dh2, dim3((unsigned)(((size_t)n * kHidden + 255) / 256)), dim3(256),
+                       0, s, dy, P + L.ow3, A2, n, dH2);
+    SPEO_TRY(check_launch("k_mlp_dh2"));
+    // g_w3 = H2^T dy, g_b3 = sum dy, g_b2 = colsum dH2
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, H2, dy, n, kHidden, G + L.ow3);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, dy, n, G + L.ob3);
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, dH2, nullptr, n, kHidden, G + L.ob2);
+    SPEO_TRY(check_launch("k_colsum"));
+    // g_W2 = dH2^T H1
+    GemmArgs g{};
+    g.M = kHidden; g.N = kHidden; g.K = n;
+    g.A = dH2; g.a_m = 1; g.a_k = kHidden; g.a_scale = 1.f;
+    g.B = H1; g.b_k = kHidden; g.b_n = 1; g.b_scale = 1.f;
+    g.C = G + L.oW2; g.c_m = kHidden; g.epi = kEpiStore;
+    SPEO_TRY(gemm(g, s));
+    // dH1 = (dH2 W2) * silu'(A1)
+    g = GemmArgs{};
+    g.M = n; g.N = kHidden; g.K = kHidden;
+    g.A = dH2; g.a_m = kHidden; g.a_k = 1; g.a_scale = 1.f;
+    g.B = P + L.oW2; g.b_k = kHidden; g.b_n = 1; g.b_scale = 1.f;
+    g.C = dH1; g.c_m = kHidden; g.P = A1; g.p_m = kHidden; g.epi = kEpiMulDsilu;
+    SPEO_TRY(gemm(g, s));
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, dH1, nullptr, n, kHidden, G + L.ob1);
+    SPEO_TRY(check_launch("k_colsum"));
+    // g_W1 = dH1^T (X / 100), columns >= D stay 0
+    SPEO_TRY(zero(G + L.oW1, (size_t)kHidden * L.Kp, s));
+    g = GemmArgs{};
+    g.M = kHidden; g.N = L.D; g.K = n;
+    g.A = dH1; g.a_m = 1; g.a_k = kHidden; g.a_scale = 1.f;
+    g.B = X; g.b_k = ld; g.b_n = 1; g.b_scale = kXScale;
+    g.C = G + L.oW1; g.c_m = L.Kp; g.epi = kEpiStore;
+    return gemm(g, s);
+}
+
+int Mlp::adam(float lr, hipStream_t s) {
+    ++adam_t;
+    const float c1 = 1.0f - powf(0.9f, (float)adam_t);
+    const float c2 = 1.0f - powf(0.999f, (float)adam_t);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, P, G, m, v,
+                       L.n, lr, c1, c2);
+    return check_launch("k_adam");
+}
+
+// ------------------------------------------------------------- statistics
+
+// Spearman rank correlation of a[0..n) and b[0..n) (average ranks for ties),
+// one block; O(n^2) ranks, n <= 4096.
+__global__ __launch_bounds__(256) void k_spearman(const float *a, const float *b, int n,
+                                                  float *rho_out) {
+    __shared__ double sh[3][256];
+    double sab = 0.0, saa = 0.0, sbb = 0.0;
+    const double mean = 0.5 * (n - 1);
+    for (int i = threadIdx.x; i < n; i += 256) {
+        int la = 0, ea = 0, lb = 0, eb = 0;
+        const float ai = a[i], bi = b[i];
+        for (int j = 0; j < n; ++j) {
+            la += a[j] < ai;
+            ea += a[j] == ai;
+            lb += b[j] < bi;
+            eb += b[j] == bi;
+        }
+        const double ra = la + 0.5 * (ea - 1) - mean, rb = lb + 0.5 * (eb - 1) - mean;
+        sab += ra * rb;
+        saa += ra * ra;
+        sbb += rb * rb;
+    }
+    sh[0][threadIdx.x] = sab;
+    sh[1][threadIdx.x] = saa;
+    sh[2][threadIdx.x] = sbb;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double den = sqrt(sh[1][0] * sh[2][0]);
+        *rho_out = den > 0 ? (float)(sh[0][0] / den) : 0.f;
+    }
+}
+
+int spearman(const float *a, const float *b, int n, float *rho_out, hipStream_t s) {
+    if (n < 2 || n > 4096) return set_error(SPEO_ERR_ARG, "spearman: need 2 <= n <= 4096");
+    hipLaunchKernelGGL(k_spearman, dim3(1), dim3(256), 0, s, a, b, n, rho_out);
+    return check_launch("k_spearman");
+}
+
+// ------------------------------------------------------ candidate screening
+
+__device__ __forceinline__ float4 ld4s(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ float pick4(const float4 &v, int q) {
+    return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
+}
+
+// 16 lanes per (target q, candidate c).  Candidate 0 uses the plain DE
+// streams (purposes 0 / 1), so it is exactly the plain engine's trial;
+// candidate c >= 1 uses purposes 64 + 2c / 65 + 2c.
+__global__ __launch_bounds__(256) void k_gen_candidates(const IslandDesc *isl,
+                                                        const SurrRowDesc *rows, int Q, int C,
+                                                        const float *X, float lo, float hi,
+                                                        uint64_t seed, const uint32_t *gen_base,
+                                                        uint32_t gen_offset, float *cand, int Kp) {
+#pragma clang fp contract(off)
+    const int item = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int l = threadIdx.x & 15;
+    if (item >= Q * C) return;
+    const int q = item / C, c = item % C;
+    const SurrRowDesc rd = rows[q];
+    const IslandDesc I = isl[rd.island];
+    const uint32_t gen = (gen_base ? *gen_base : 0u) + gen_offset;
+    const uint32_t pd = c == 0 ? (uint32_t)kPurposeDonor : (uint32_t)(kPurposeCandidate + 2 * c);
+    const uint32_t pc = c == 0 ? (uint32_t)kPurposeCross : (uint32_t)(kPurposeCandidate + 2 * c + 1);
+    DonorStream ds(seed, (uint32_t)rd.gr, gen, pd);
+    int r[kMaxR], jr;
+    draw_donors(ds, nullptr, 0, I.np, I.dim, r, jr);
+    const float *base = X + I.elem_off;
+    const float *p0 = base + (size_t)r[0] * I.ld, *p1 = base + (size_t)r[1] * I.ld;
+    const float *p2 = base + (size_t)r[2] * I.ld, *own = base + (size_t)rd.li * I.ld;
+    float *out = cand + (size_t)item * Kp;
+    const int ngroups = (I.dim + 3) >> 2;
+    for (int g = l; g < ngroups; g += 16) {
+        const U4 w = philox_block(seed, (uint32_t)g, (uint32_t)rd.gr, gen, pc);
+        const float4 e0 = ld4s(p0 + 4 * g), e1 = ld4s(p1 + 4 * g), e2 = ld4s(p2 + 4 * g);
+        const float4 xo = ld4s(own + 4 * g);
+        float v[4];
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            const int j = 4 * g + qq;
+            if (j < I.dim) {
+                const bool take = (j == jr) || (u01(word(w, qq)) < I.CR);
+                const float m = take ? fmaf(I.F, pick4(e1, qq) - pick4(e2, qq), pick4(e0, qq))
+                                     : pick4(xo, qq);
+                v[qq] = check_bound(m, lo, hi);
+            } else {
+                v[qq] = 0.f;
+            }
+        }
+        *reinterpret_cast<float4 *>(out + 4 * g) = float4{v[0], v[1], v[2], v[3]};
+    }
+    for (int g = ngroups + l; g < (Kp >> 2); g += 16)
+        *reinterpret_cast<float4 *>(out + 4 * g) = float4{0.f, 0.f, 0.f, 0.f};
+}
+
+int gen_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C, const float *X,
+                   float lo, float hi, uint64_t seed, const uint32_t *gen_base,
+                   uint32_t gen_offset, float *cand, int Kp, hipStream_t s) {
+    const int items = Q * C;
+    if (items <= 0) return SPEO_OK;
+    hipLaunchKernelGGL(k_gen_candidates, dim3((items + 15) / 16), dim3(256), 0, s, isl, rows, Q,
+                       C, X, lo, hi, seed, gen_base, gen_offset, cand, Kp);
+    return check_launch("k_gen_candidates");
+}
+
+// 16 lanes per target: the candidate with the lowest prediction (ties to the
+// lower index, NaN predictions skipped) while the problem's surrogate is
+// active, else candidate 0; copied into the target's trial row of U.
+__global__ void k_pick_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C,
+                                  const float *cand, int Kp, const float *pred, const float *rho,
+                                  float rho_min, float *U) {
+    const int q = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int l = threadIdx.x & 15;
+    if (q >= Q) return;
+    const SurrRowDesc rd = rows[q];
+    const IslandDesc I = isl[rd.island];
+    int best = 0;
+    if (rho[rd.problem] >= rho_min) {
+        float bv = pred[(size_t)q * C];
+        for (int c = 1; c < C; ++c) {
+            const float v = pred[(size_t)q * C + c];
+            if (v < bv || (isnan(bv) && !isnan(v))) {
+                bv = v;
+                best = c;
+            }
+        }
+    }
+    const float *src = cand + ((size_t)q * C + best) * Kp;
+    float *dst = U + I.elem_off + (size_t)rd.li * I.ld;
+    for (int g = l; g < (I.ld >> 2); g += 16)
+        *reinterpret_cast<float4 *>(dst + 4 * g) = *reinterpret_cast<const float4 *>(src + 4 * g);
+}
+
+int pick_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C,
+                    const float *cand, int Kp, const float *pred, const float *rho,
+                    float rho_min, float *U, hipStream_t s) {
+    if (Q <= 0) return SPEO_OK;
+    hipLaunchKernelGGL(k_pick_candidates, dim3((Q + 15) / 16), dim3(256), 0, s, isl, rows, Q, C,
+                       cand, Kp, pred, rho, rho_min, U);
+    return check_launch("k_pick_candidates");
+}
+
+// Archive slot of target q: (head[p] + q - qoff[p]) % cap of its problem p;
+// x is stored with Kp columns (zero padded) and y = log(1 + f).
+__global__ void k_archive_append(const IslandDesc *isl, const SurrRowDesc *rows, int Q,
+                                 const float *U, const float *fU, float *AX, float *Ay, int cap,
+                                 int Kp, SurrHeads heads) {
+    const int q = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int l = threadIdx.x & 15;
+    if (q >= Q) return;
+    const SurrRowDesc rd = rows[q];
+    const IslandDesc I = isl[rd.island];
+    const int p = rd.problem;
+    const int slot = (int)(((int64_t)heads.head[p] + q - heads.qoff[p]) % cap);
+    float *dst = AX + ((size_t)p * cap + slot) * Kp;
+    const float *src = U + I.elem_off + (size_t)rd.li * I.ld;
+    for (int j = l; j < Kp; j += 16) dst[j] = j < I.dim ? src[j] : 0.f;
+    if (l == 0) Ay[(size_t)p * cap + slot] = log1pf(fmaxf(fU[rd.gr], 0.f));
+}
+
+int archive_append(const IslandDesc *isl, const SurrRowDesc *rows, int Q, const float *U,
+                   const float *fU, float *AX, float *Ay, int cap, int Kp, SurrHeads heads,
+                   hipStream_t s) {
+    if (Q <= 0) return SPEO_OK;
+    hipLaunchKernelGGL(k_archive_append, dim3((Q + 15) / 16), dim3(256), 0, s, isl, rows, Q, U, fU,
+                       AX, Ay, cap, Kp, heads);
+    return check_launch("k_archive_append");
+}
+
+// Minibatch of the valid archive slots [0, n_valid), skipping the held-out
+// slots (slot % 8 == 7, mapped to slot - 1).
+__global__ void k_gather_batch(const float *AX, const float *Ay, int n_valid, int Kp,
+                               uint64_t seed, uint32_t round, uint32_t step, int batch, float *Xb,
+                               float *tb) {
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= batch) return;
+    const U4 w = philox_block(seed, (uint32_t)(b >> 2), round, step, kPurposeSurrBatch);
+    int slot = (int)(u01(word(w, (uint32_t)(b & 3))) * (float)n_valid);
+    slot = slot >= n_valid ? n_valid - 1 : slot;
+    if (slot % 8 == 7) slot -= 1;
+    for (int j = lane; j < Kp; j += 64) Xb[(size_t)b * Kp + j] = AX[(size_t)slot * Kp + j];
+    if (lane == 0) tb[b] = Ay[slot];
+}
+
+int gather_batch(const float *AX, const float *Ay, int n_valid, int Kp, uint64_t seed,
+                 uint32_t round, uint32_t step, int batch, float *Xb, float *tb, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_batch, dim3((batch + 3) / 4), dim3(256), 0, s, AX, Ay, n_valid, Kp,
+                       seed, round, step, batch, Xb, tb);
+    return check_launch("k_gather_batch");
+}
+
+// Held-out points: slots 8h + 7, h < H.
+__global__ void k_gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh,
+                                 float *yh) {
+    const int h = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (h >= H) return;
+    const size_t slot = (size_t)h * 8 + 7;
+    for (int j = lane; j < Kp; j += 64) Xh[(size_t)h * Kp + j] = AX[slot * Kp + j];
+    if (lane == 0) yh[h] = Ay[slot];
+}
+
+int gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh, float *yh,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_holdout, dim3((H + 3) / 4), dim3(256), 0, s, AX, Ay, H, Kp, Xh, yh);
+    return check_launch("k_gather_holdout");
+}
+
+// stats[1] = mean(y), stats[2] = max(std(y), 1e-3) over n values (one block,
+// FP64 accumulation).
+__global__ void k_mean_std(const float *y, int n, float *stats) {
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += y[i];
+        b += (double)y[i] * y[i];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+            s2[threadIdx.x] += s2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = s1[0] / n;
+        const double var = fmax(s2[0] / n - m * m, 0.0);
+        stats[1] = (float)m;
+        stats[2] = (float)fmax(sqrt(var), 1e-3);
+    }
+}
+
+int mean_std(const float *y, int n, float *stats, hipStream_t s) {
+    hipLaunchKernelGGL(k_mean_std, dim3(1), dim3(256), 0, s, y, n, stats);
+    return check_launch("k_mean_std");
+}
+
+}  // namespace speo
+
+// ======================================================================
+//  C ABI (test and tooling access to one MLP)
+// ======================================================================
+using namespace speo;
+
+struct speo_mlp_s {
+    Mlp m;
+};
+
+extern "C" {
+
+int speo_mlp_create(int dim, int max_batch, uint64_t seed, speo_mlp *out) {
+    if (!out || dim < 1 || max_batch < 1) return set_error(SPEO_ERR_ARG, "speo_mlp_create: bad arguments");
+    *out = nullptr;
+    auto *h = new (std::nothrow) speo_mlp_s;
+    if (!h) return set_error(SPEO_ERR_ALLOC, "speo_mlp_create: out of host memory");
+    int rc = h->m.alloc(dim, max_batch);
+    if (rc == SPEO_OK) rc = h->m.init(seed, nullptr);
+    if (rc != SPEO_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return SPEO_OK;
+}
+
+int speo_mlp_destroy(speo_mlp h) {
+    delete h;
+    return SPEO_OK;
+}
+
+int64_t speo_mlp_num_params(speo_mlp h) { return h ? (int64_t)h->m.L.n : 0; }
+
+int speo_mlp_params(speo_mlp h, float *get_out, const float *set_in) {
+    if (!h) return set_error(SPEO_ERR_ARG, "speo_mlp_params: null handle");
+    if (set_in) {
+        SPEO_TRY_HIP(hipMemcpy(h->m.P, set_in, sizeof(float) * h->m.L.n, hipMemcpyHostToDevice));
+        SPEO_TRY_HIP(hipMemset(h->m.m, 0, sizeof(float) * h->m.L.n));
+        SPEO_TRY_HIP(hipMemset(h->m.v, 0, sizeof(float) * h->m.L.n));
+        h->m.adam_t = 0;
+    }
+    if (get_out) SPEO_TRY_HIP(hipMemcpy(get_out, h->m.P, sizeof(float) * h->m.L.n, hipMemcpyDeviceToHost));
+    return SPEO_OK;
+}
+
+int speo_mlp_forward(speo_mlp h, const float *X, int ld, int n, float *pred,
+                     speo_stream_t stream) {
+    if (!h || !X || !pred || n < 0 || ld < h->m.L.D)
+        return set_error(SPEO_ERR_ARG, "speo_mlp_forward: bad arguments");
+    return h->m.forward(X, ld, n, pred, as_stream(stream));
+}
+
+int speo_mlp_loss_grad(speo_mlp h, const float *X, int ld, int n, const float *t,
+                       float t_mean, float t_std, float *loss_out, float *grad_out) {
+    if (!h || !X || !t || n < 1 || ld < h->m.L.D || !(t_std > 0))
+        return set_error(SPEO_ERR_ARG, "speo_mlp_loss_grad: bad arguments");
+    const float st[3] = {0.f, t_mean, t_std};
+    SPEO_TRY_HIP(hipMemcpy(h->m.stats, st, sizeof(st), hipMemcpyHostToDevice));
+    SPEO_TRY(h->m.loss_grad(X, ld, n, t, nullptr));
+    SPEO_TRY_HIP(hipDeviceSynchronize());
+    if (loss_out) SPEO_TRY_HIP(hipMemcpy(loss_out, h->m.stats, sizeof(float), hipMemcpyDeviceToHost));
+    if (grad_out)
+        SPEO_TRY_HIP(hipMemcpy(grad_out, h->m.G, sizeof(float) * h->m.L.n, hipMemcpyDeviceToHost));
+    return SPEO_OK;
+}
+
+int speo_mlp_adam_step(speo_mlp h, float lr) {
+    if (!h) return set_error(SPEO_ERR_ARG, "speo_mlp_adam_step: null handle");
+    SPEO_TRY(h->m.adam(lr, nullptr));
+    return check_hip(hipDeviceSynchronize(), "speo_mlp_adam_step");
+}
+
+int speo_spearman(const float *a, const float *b, int n, float *rho_out, speo_stream_t stream) {
+    if (!a || !b || !rho_out) return set_error(SPEO_ERR_ARG, "speo_spearman: null argument");
+    return spearman(a, b, n, rho_out, as_stream(stream));
+}
+
+}  // extern "C"

@@ -167,7 +167,9 @@ class Engine:
     def __init__(self, problems, island_size, island_problem, F=0.5, CR=0.9, lo=-100.0,
                  hi=100.0, seed=1, interval=0, migration_size=0, precision=PREC_FP32,
                  mode=MODE_FUSED, use_graph=False, rows_per_block=0, island_F=None,
-                 island_CR=None):
+                 island_CR=None, surrogate_candidates=0, surrogate_train_interval=10,
+                 surrogate_archive=65536, surrogate_steps=40, surrogate_batch=1024,
+                 surrogate_lr=1e-3, surrogate_rho_min=0.5):
         self.problems = list(problems)
         self.island_size = np.ascontiguousarray(island_size, np.int32)
         self.island_problem = np.ascontiguousarray(island_problem, np.int32)
@@ -187,6 +189,13 @@ class Engine:
         cfg.interval, cfg.migration_size = interval, migration_size
         cfg.precision, cfg.mode, cfg.use_graph = precision, mode, int(use_graph)
         cfg.rows_per_block = rows_per_block
+        cfg.surrogate_candidates = surrogate_candidates
+        cfg.surrogate_train_interval = surrogate_train_interval
+        cfg.surrogate_archive = surrogate_archive
+        cfg.surrogate_steps = surrogate_steps
+        cfg.surrogate_batch = surrogate_batch
+        cfg.surrogate_lr = surrogate_lr
+        cfg.surrogate_rho_min = surrogate_rho_min
         self.cfg = cfg
         h = c_void_p()
         check(lib().speo_engine_create(byref(cfg), None, byref(h)))
@@ -232,6 +241,15 @@ class Engine:
         out = np.zeros(4, np.float64)
         check(lib().speo_engine_profile(self.handle, generations, _np_ptr(out)))
         return dict(zip(("generate", "evaluate", "select", "fused"), out.tolist()))
+
+    def surrogate_stats(self):
+        n = len(self.problems)
+        rho = np.zeros(n, np.float32)
+        act = np.zeros(n, np.int32)
+        loss = np.zeros(n, np.float32)
+        check(lib().speo_engine_surrogate_stats(self.handle, _np_ptr(rho), _np_ptr(act),
+                                                _np_ptr(loss)))
+        return rho, act, loss
 
     def stats(self):
         s = EngineStats()

@@ -61,12 +61,17 @@ class EngineConfig(Structure):
                 ("island_CR", POINTER(c_float)), ("F", c_float), ("CR", c_float),
                 ("min_bound", c_float), ("max_bound", c_float), ("seed", c_uint64),
                 ("interval", c_int), ("migration_size", c_int), ("precision", c_int),
-                ("mode", c_int), ("use_graph", c_int), ("rows_per_block", c_int)]
+                ("mode", c_int), ("use_graph", c_int), ("rows_per_block", c_int),
+                ("surrogate_candidates", c_int), ("surrogate_train_interval", c_int),
+                ("surrogate_archive", c_int), ("surrogate_steps", c_int),
+                ("surrogate_batch", c_int), ("surrogate_lr", c_float),
+                ("surrogate_rho_min", c_float)]
 
 
 class EngineStats(Structure):
     _fields_ = [("generations", c_int64), ("evaluations", c_int64), ("migrations", c_int64),
-                ("graph_launches", c_int64), ("rows_per_block", c_int)]
+                ("graph_launches", c_int64), ("rows_per_block", c_int),
+                ("surrogate_predictions", c_int64), ("surrogate_rounds", c_int64)]
 
 
 _SIGS = {
@@ -108,10 +113,20 @@ _SIGS = {
     "speo_engine_write": (c_int, [c_void_p, c_void_p, c_void_p]),
     "speo_engine_get_stats": (c_int, [c_void_p, POINTER(EngineStats)]),
     "speo_engine_profile": (c_int, [c_void_p, c_int, c_void_p]),
+    "speo_engine_surrogate_stats": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "speo_engine_total_rows": (c_int, [c_void_p]),
     "speo_engine_max_dim": (c_int, [c_void_p]),
     "speo_engine_buffers": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), c_void_p,
                                     c_void_p, c_void_p]),
+    "speo_mlp_create": (c_int, [c_int, c_int, c_uint64, POINTER(c_void_p)]),
+    "speo_mlp_destroy": (c_int, [c_void_p]),
+    "speo_mlp_num_params": (c_int64, [c_void_p]),
+    "speo_mlp_params": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "speo_mlp_forward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "speo_mlp_loss_grad": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_float, c_float,
+                                   c_void_p, c_void_p]),
+    "speo_mlp_adam_step": (c_int, [c_void_p, c_float]),
+    "speo_spearman": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
 }
 
 _lib = None

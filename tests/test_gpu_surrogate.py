@@ -1,0 +1,199 @@
+"""GPU parity of the surrogate MLP kernels against the CPU oracle
+(oracle/surrogate.py, pinned by finite differences): forward and gradients
+with the same BF16 operand rounding, Adam, Spearman; the engine's screening
+path; and what the trained network gains over plain DE at an equal number of
+true evaluations."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = 256
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+class Net:
+    def __init__(self, speo, D, batch, seed=3):
+        self.L = speo.lib()
+        self.h = ctypes.c_void_p()
+        speo.check(self.L.speo_mlp_create(D, batch, seed, ctypes.byref(self.h)))
+        self.D, self.Kp = D, (D + 31) // 32 * 32
+        self.n = int(self.L.speo_mlp_num_params(self.h))
+        assert self.n == H * self.Kp + H + H * H + H + H + 4
+
+    def flat(self):
+        out = np.zeros(self.n, np.float32)
+        self.L.speo_mlp_params(self.h, out.ctypes.data_as(ctypes.c_void_p), None)
+        return out
+
+    def set_flat(self, f):
+        f = np.ascontiguousarray(f, np.float32)
+        self.L.speo_mlp_params(self.h, None, f.ctypes.data_as(ctypes.c_void_p))
+
+    def to_dict(self, f):
+        D, Kp = self.D, self.Kp
+        o = 0
+        W1 = f[o:o + H * Kp].reshape(H, Kp)[:, :D].astype(np.float64); o += H * Kp
+        b1 = f[o:o + H].astype(np.float64); o += H
+        W2 = f[o:o + H * H].reshape(H, H).astype(np.float64); o += H * H
+        b2 = f[o:o + H].astype(np.float64); o += H
+        w3 = f[o:o + H].astype(np.float64); o += H
+        b3 = f[o:o + 1].astype(np.float64)
+        return {"W1": W1, "b1": b1, "W2": W2, "b2": b2, "w3": w3, "b3": b3}
+
+    def close(self):
+        self.L.speo_mlp_destroy(self.h)
+
+
+def _random_params(net, rng):
+    f = net.flat()
+    f = f + rng.normal(0, 0.05, f.size).astype(np.float32)
+    f[: H * net.Kp].reshape(H, net.Kp)[:, net.D:] = 0.0
+    return f
+
+
+def test_mlp_forward_matches_oracle(cuda, speo):
+    import surrogate as SO
+    import torch
+    rng = np.random.default_rng(0)
+    D, n = 100, 300
+    net = Net(speo, D, 512)
+    f = _random_params(net, rng)
+    net.set_flat(f)
+    X = rng.uniform(-100, 100, (n, 104)).astype(np.float32)
+    y = torch.zeros(n, dtype=torch.float32, device=cuda)
+    speo.check(net.L.speo_mlp_forward(net.h, ctypes.c_void_p(_t(X, cuda).data_ptr()), 104, n,
+                                      ctypes.c_void_p(y.data_ptr()), None))
+    torch.cuda.synchronize()
+    ref, _ = SO.forward(net.to_dict(f), X[:, :D].astype(np.float64), round_bf16=True)
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(y.cpu().numpy(), ref, atol=2e-4 * scale)
+    net.close()
+
+
+def test_mlp_gradients_match_oracle(cuda, speo):
+    import surrogate as SO
+    rng = np.random.default_rng(1)
+    D, n = 100, 256
+    net = Net(speo, D, 512)
+    f = _random_params(net, rng)
+    net.set_flat(f)
+    X = rng.uniform(-100, 100, (n, D)).astype(np.float32)
+    t = rng.normal(3.0, 2.0, n).astype(np.float32)
+    mu, sd = 3.0, 2.0
+    loss = np.zeros(1, np.float32)
+    g = np.zeros(net.n, np.float32)
+    Xd, td = _t(X, cuda), _t(t, cuda)
+    speo.check(net.L.speo_mlp_loss_grad(net.h, ctypes.c_void_p(Xd.data_ptr()), D, n,
+                                        ctypes.c_void_p(td.data_ptr()), mu, sd,
+                                        loss.ctypes.data_as(ctypes.c_void_p),
+                                        g.ctypes.data_as(ctypes.c_void_p)))
+    p = net.to_dict(f)
+    L, gr = SO.loss_and_grads(p, X.astype(np.float64), (t.astype(np.float64) - mu) / sd,
+                              round_bf16=True)
+    assert loss[0] == pytest.approx(L, rel=1e-3)
+    gd = net.to_dict(g)
+    for k in ("W1", "b1", "W2", "b2", "w3", "b3"):
+        scale = np.abs(gr[k]).max()
+        np.testing.assert_allclose(gd[k], gr[k], atol=3e-3 * scale, err_msg=k)
+    net.close()
+
+
+def test_adam_step_matches_oracle(cuda, speo):
+    import surrogate as SO
+    rng = np.random.default_rng(2)
+    D, n = 30, 128
+    net = Net(speo, D, 256)
+    f = _random_params(net, rng)
+    net.set_flat(f)
+    X = rng.uniform(-100, 100, (n, D)).astype(np.float32)
+    t = rng.normal(0, 1, n).astype(np.float32)
+    g = np.zeros(net.n, np.float32)
+    Xd, td = _t(X, cuda), _t(t, cuda)
+    for step in range(1, 4):
+        before = net.flat().astype(np.float64)
+        speo.check(net.L.speo_mlp_loss_grad(net.h, ctypes.c_void_p(Xd.data_ptr()), D, n,
+                                            ctypes.c_void_p(td.data_ptr()), 0.0, 1.0, None,
+                                            g.ctypes.data_as(ctypes.c_void_p)))
+        if step == 1:
+            m = {"a": np.zeros(net.n)}
+            v = {"a": np.zeros(net.n)}
+        p2, m, v = SO.adam_step({"a": before}, {"a": g.astype(np.float64)}, m, v, step, lr=1e-3)
+        speo.check(net.L.speo_mlp_adam_step(net.h, 1e-3))
+        after = net.flat()
+        np.testing.assert_allclose(after, p2["a"], rtol=1e-5, atol=1e-7)
+    net.close()
+
+
+def test_spearman_kernel(cuda, speo):
+    import surrogate as SO
+    import torch
+    rng = np.random.default_rng(4)
+    for n in (2, 17, 1024, 4096):
+        a = rng.normal(size=n).astype(np.float32)
+        b = (a + rng.normal(scale=1.0, size=n)).astype(np.float32)
+        a[::9] = a[0]
+        out = torch.zeros(1, dtype=torch.float32, device=cuda)
+        speo.check(speo.lib().speo_spearman(ctypes.c_void_p(_t(a, cuda).data_ptr()),
+                                            ctypes.c_void_p(_t(b, cuda).data_ptr()), n,
+                                            ctypes.c_void_p(out.data_ptr()), None))
+        torch.cuda.synchronize()
+        assert out.item() == pytest.approx(SO.spearman(a, b), abs=1e-5)
+
+
+def _problems(speo, oracle, D, objs):
+    out = []
+    for k, obj in enumerate(objs):
+        o, M = oracle.gen_problem(500 + k, D)
+        out.append(speo.Problem([obj], o[None, :], M[None, :, :]))
+    return out
+
+
+def test_inactive_surrogate_equals_plain(cuda, speo, oracle):
+    # rho_min above any correlation: candidate 0 (the plain trial) every time,
+    # while candidates are still generated, scored and the network trained.
+    probs = _problems(speo, oracle, 20, [1, 2])
+    kw = dict(seed=9, rows_per_block=32, interval=10, migration_size=2)
+    plain = speo.Engine(probs, [64] * 4, [0, 1, 0, 1], mode=speo.MODE_PLAIN, **kw)
+    surr = speo.Engine(probs, [64] * 4, [0, 1, 0, 1], mode=speo.MODE_PLAIN,
+                       surrogate_candidates=8, surrogate_train_interval=4, surrogate_archive=2048,
+                       surrogate_steps=5, surrogate_batch=256, surrogate_rho_min=2.0, **kw)
+    plain.run(24)
+    surr.run(24)
+    Xp, fp = plain.read()
+    Xs, fs = surr.read()
+    np.testing.assert_array_equal(Xp, Xs)
+    np.testing.assert_array_equal(fp, fs)
+    rho, act, loss = surr.surrogate_stats()
+    assert (act == 0).all()
+    assert surr.stats()["surrogate_rounds"] >= 5
+    assert surr.stats()["evaluations"] == plain.stats()["evaluations"]
+
+
+def test_surrogate_gain_over_plain(cuda, speo, oracle):
+    """At an equal number of true evaluations, screening 8 candidates with the
+    trained network reaches a lower error than plain DE (elliptic, D=30)."""
+    probs = _problems(speo, oracle, 30, [1])
+    kw = dict(seed=21, rows_per_block=32, interval=20, migration_size=2)
+    sizes, pidx = [128] * 4, [0] * 4
+    plain = speo.Engine(probs, sizes, pidx, mode=speo.MODE_PLAIN, **kw)
+    surr = speo.Engine(probs, sizes, pidx, mode=speo.MODE_PLAIN, surrogate_candidates=8,
+                       surrogate_train_interval=10, surrogate_archive=16384, surrogate_steps=60,
+                       surrogate_batch=512, surrogate_lr=1e-3, **kw)
+    G = 200
+    plain.run(G)
+    surr.run(G)
+    _, _, fp = plain.best()
+    _, _, fs = surr.best()
+    rho, act, loss = surr.surrogate_stats()
+    print(f"\nplain best {fp.min():.4e}  surrogate best {fs.min():.4e}  rho {rho[0]:.3f} "
+          f"active {act[0]} loss {loss[0]:.4f}")
+    assert plain.stats()["evaluations"] == surr.stats()["evaluations"]
+    assert rho[0] > 0.5
+    assert fs.min() < fp.min()
