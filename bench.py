@@ -146,6 +146,92 @@ def time_to_error(S, dim, islands, island_size, max_gens, check_every):
     return result
 
 
+def uneven_batch(S, islands=256, gens=500, interval=25, precision="fp32", seed=BASE_SEED + 300):
+    """Config 3: 256 islands with D drawn from {10, 30, 50, 100} and NP uniform
+    in [50, 512], objectives round-robin, one fused launch per generation over
+    all islands, generations captured in hipGraphs, autotuned row tile,
+    migration every 25 generations.  Returns throughput and kernel time."""
+    rng = np.random.default_rng(seed)
+    dims = rng.choice([10, 30, 50, 100], islands)
+    sizes = rng.integers(50, 513, islands)
+    objs = [S.OBJ_ELLIPTIC, S.OBJ_RASTRIGIN, S.OBJ_ACKLEY]
+    keys = sorted({(int(d), i % 3) for i, d in enumerate(dims)})
+    problems, index = [], {}
+    for k, (d, o_i) in enumerate(keys):
+        o, M = S.generate_problem_data(S.problem_seed(seed, k), d)
+        index[(d, o_i)] = len(problems)
+        problems.append(S.Problem([objs[o_i]], o[None, :], M[None, :, :]))
+    pidx = [index[(int(d), i % 3)] for i, d in enumerate(dims)]
+    prec = {v: k for k, v in S.PREC_NAMES.items()}[precision]
+    eng = S.Engine(problems, sizes.tolist(), pidx, F=0.5, CR=0.9, seed=seed, interval=interval,
+                   migration_size=2, precision=prec, mode=S.MODE_FUSED, use_graph=True,
+                   rows_per_block=0)
+    eng.run(25)
+    eng.sync()
+    rows = eng.total_rows
+    t0 = time.perf_counter()
+    eng.run(gens)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    kms = eng.profile(20)
+    flops = float(sum(2.0 * int(d) * int(d) * int(n) for d, n in zip(dims, sizes)))
+    out = {"islands": islands, "total_rows": rows, "generations": gens,
+           "trial_evals_per_s": gens * rows / dt, "ms_per_generation": dt / gens * 1e3,
+           "fused_kernel_ms": kms["fused"], "rows_per_block": eng.stats()["rows_per_block"],
+           "mfma_TFLOPs": flops / (kms["fused"] * 1e-3) / 1e12,
+           "graph_launches": eng.stats()["graph_launches"], "precision": precision}
+    eng.close()
+    for p in problems:
+        p.close()
+    return out
+
+
+def precision_study(S, n_points=10000, D=1000, reps=5):
+    """Config 4: error of each rotation precision against the FP64 device
+    evaluation (pinned to the CPU oracle at 1e-12 by the tests) on n random
+    points, for the 3-component composition (sigma 10/20/30, bias 0/100/200)
+    and each single objective; plus the evaluation rate of each precision."""
+    import torch
+    objs = [S.OBJ_ELLIPTIC, S.OBJ_RASTRIGIN, S.OBJ_ACKLEY]
+    shifts, rots = [], []
+    for k in range(3):
+        o, M = S.generate_problem_data(S.problem_seed(BASE_SEED + 200, k), D)
+        shifts.append(o)
+        rots.append(M)
+    probs = {"composition": S.Problem(objs, np.stack(shifts), np.stack(rots),
+                                      sigma=[10.0, 20.0, 30.0], bias=[0.0, 100.0, 200.0])}
+    for k, name in enumerate(("elliptic", "rastrigin", "ackley")):
+        probs[name] = S.Problem([objs[k]], shifts[k][None, :], rots[k][None, :, :])
+    rng = np.random.default_rng(11)
+    X = torch.from_numpy(rng.uniform(-100, 100, (n_points, D)).astype(np.float32)).cuda()
+    out = {"points": n_points, "dim": D, "errors": {}, "eval_rate": {}}
+    for name, prob in probs.items():
+        f64 = torch.zeros(n_points, dtype=torch.float64, device="cuda")
+        S.evaluate_f64(prob, X, f64)
+        torch.cuda.synchronize()
+        ref = f64.cpu().numpy()
+        out["errors"][name] = {}
+        for prec, pname in S.PREC_NAMES.items():
+            f = torch.zeros(n_points, dtype=torch.float32, device="cuda")
+            S.evaluate(prob, X, f, precision=prec)
+            torch.cuda.synchronize()
+            rel = np.abs(f.cpu().numpy() - ref) / np.abs(ref)
+            out["errors"][name][pname] = {"max_rel": float(rel.max()),
+                                          "median_rel": float(np.median(rel))}
+            if name == "composition":
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    S.evaluate(prob, X, f, precision=prec)
+                torch.cuda.synchronize()
+                dt = (time.perf_counter() - t0) / reps
+                flops = 3 * 2.0 * D * D * n_points
+                out["eval_rate"][pname] = {"evals_per_s": n_points / dt,
+                                           "effective_TFLOPs": flops / dt / 1e12}
+    for p in probs.values():
+        p.close()
+    return out
+
+
 def cpu_baseline(data, seconds, D, NP):
     """Single-thread CPU trial evals/s: objective evaluation + DE step."""
     import oracle as O
@@ -219,6 +305,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-error runs")
     ap.add_argument("--tte-gens", type=int, default=2000)
+    ap.add_argument("--uneven", action="store_true",
+                    help="also run the uneven 256-island batch (D in {10,30,50,100}, NP 50..512)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="also run the D=1000 precision study (10k points)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (scales rocprof byte counters)")
     args = ap.parse_args()
@@ -321,6 +411,12 @@ def main():
         "generations_total": eng.stats()["generations"],
     }
     eng.close()
+    if args.uneven:
+        print("# uneven batch", file=sys.stderr, flush=True)
+        out["uneven_batch"] = {p: uneven_batch(S, precision=p) for p in ("fp32", "bf16x3")}
+    if args.precision_study:
+        print("# precision study", file=sys.stderr, flush=True)
+        out["precision_study"] = precision_study(S)
     if not args.no_tte:
         print("# time-to-error runs", file=sys.stderr, flush=True)
         out["time_to_error"] = time_to_error(S, D, 32, 256, args.tte_gens, 10)

@@ -355,6 +355,10 @@ SPEO_EXPORT int speo_mlp_params(speo_mlp m, float *get_out, const float *set_in)
 /* pred[i] = MLP(X[i]) for n rows (device pointers, stride ld >= dim). */
 SPEO_EXPORT int speo_mlp_forward(speo_mlp m, const float *X, int ld, int n, float *pred,
                                  speo_stream_t stream);
+/* Same prediction through the fused single-kernel inference path (BF16 weight
+ * copies refreshed from the master weights first). */
+SPEO_EXPORT int speo_mlp_infer(speo_mlp m, const float *X, int ld, int n, float *pred,
+                               speo_stream_t stream);
 /* Loss 0.5 mean((y - t')^2) with t' = (t - t_mean) / t_std over n <= max_batch
  * rows (device X, t) and its gradient (host, flat layout).  Synchronous. */
 SPEO_EXPORT int speo_mlp_loss_grad(speo_mlp m, const float *X, int ld, int n, const float *t,

@@ -594,8 +594,8 @@ int Engine::surrogate_generation(int gen_offset, hipStream_t s) {
     if (any_trained) {
         for (auto &sp : surr)
             if (sp->rows > 0)
-                SPEO_TRY(sp->net.forward(cand + (size_t)sp->qoff * Ce * Kp, Kp, sp->rows * Ce,
-                                         pred + (size_t)sp->qoff * Ce, s));
+                SPEO_TRY(sp->net.infer(cand + (size_t)sp->qoff * Ce * Kp, Kp, sp->rows * Ce,
+                                       pred + (size_t)sp->qoff * Ce, s));
         surr_pred += (int64_t)Q * Ce;
     }
     SPEO_TRY(pick_candidates(isl_dev, srows_dev, Q, Ce, cand, Kp, pred, rho_dev,
@@ -631,9 +631,10 @@ int Engine::surrogate_train(hipStream_t s) {
             SPEO_TRY(sp.net.loss_grad(Xb, Kp, cfg.surrogate_batch, tb, s));
             SPEO_TRY(sp.net.adam(cfg.surrogate_lr, s));
         }
+        SPEO_TRY(sp.net.refresh_bf16(s));
         const int H = std::min(1024, n_valid / 8);
         SPEO_TRY(gather_holdout(ax, ay, H, Kp, Xh, yh, s));
-        SPEO_TRY(sp.net.forward(Xh, Kp, H, predh, s));
+        SPEO_TRY(sp.net.infer(Xh, Kp, H, predh, s));
         SPEO_TRY(spearman(predh, yh, H, rho_dev + p, s));
         sp.trained = true;
         any_trained = true;
@@ -937,12 +938,10 @@ int Engine::best(int *row_out, float *f32_out, double *f64_out) {
     SPEO_TRY(finalize_pending(stream));
     const int cur = cfg.mode == SPEO_MODE_FUSED ? p : 0;
     SPEO_TRY(segment_argmin(fX[cur], row_off_dev, I, best_row_dev, stream));
-    size_t lds = sizeof(double) * ((size_t)max_dim + 16);
-    static std::once_flag once;
-    std::call_once(once, [] {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval_f64_rows),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    });
+    const size_t lds = sizeof(double) * ((size_t)max_dim + 16);
+    if (lds > 48 * 1024)
+        SPEO_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval_f64_rows),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_eval_f64_rows, dim3(I), dim3(256), lds, stream, params_tab_dev,
                        ncomp_dev, dims_dev, isl_dev, best_row_dev, X, best_f64_dev);
     SPEO_TRY(check_launch("k_eval_f64_rows"));

@@ -178,13 +178,9 @@ int eval_problem_f64(const ProblemHost &P, const float *X, int ld, int n, double
                      hipStream_t s) {
     if (n <= 0) return SPEO_OK;
     const size_t lds = sizeof(double) * ((size_t)P.dim + 16);
-    if (lds > 64 * 1024) {
-        static std::once_flag once;
-        std::call_once(once, [] {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval_f64),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        });
-    }
+    if (lds > 48 * 1024)
+        SPEO_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval_f64),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_eval_f64, dim3(n), dim3(256), lds, s, P.params_dev, P.ncomp, P.dim, X,
                        ld, n, f_out);
     return check_launch("k_eval_f64");

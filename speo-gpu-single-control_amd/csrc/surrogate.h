@@ -38,6 +38,7 @@ struct Mlp {
     float *dH1 = nullptr, *dH2 = nullptr;                              // [max_batch][256]
     float *y = nullptr, *dy = nullptr;                                 // [max_batch]
     float *stats = nullptr;  // [0] loss, [1] target mean, [2] target std, [3] rho
+    uint16_t *W1b = nullptr, *W2b = nullptr;  // BF16 copies for the fused inference kernel
     int64_t adam_t = 0;
     ~Mlp();
     int alloc(int dim, int batch);
@@ -48,6 +49,10 @@ struct Mlp {
     // Loss (0.5 mean (y - t)^2) and gradients into G for n <= max_batch rows.
     int loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s);
     int adam(float lr, hipStream_t s);
+    // Refreshes the BF16 weight copies from the FP32 master weights.
+    int refresh_bf16(hipStream_t s);
+    // Fused inference (one kernel): pred[i] = MLP(X[i]); uses the BF16 copies.
+    int infer(const float *X, int ld, int n, float *pred, hipStream_t s);
 };
 
 // One target row in surrogate mode, rows ordered problem by problem.

@@ -205,7 +205,7 @@ static int zero(float *p, size_t n, hipStream_t s) {
 // ------------------------------------------------------------------ Mlp
 
 Mlp::~Mlp() {
-    void *b[] = {P, G, m, v, A1, H1, A2, H2, dH1, dH2, y, dy, stats};
+    void *b[] = {P, G, m, v, A1, H1, A2, H2, dH1, dH2, y, dy, stats, W1b, W2b};
     for (void *p : b) (void)hipFree(p);
 }
 
@@ -314,6 +314,139 @@ int Mlp::adam(float lr, hipStream_t s) {
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, P, G, m, v,
                        L.n, lr, c1, c2);
     return check_launch("k_adam");
+}
+
+// ------------------------------------------------------ fused inference
+
+__global__ void k_to_bf16(const float *src, uint16_t *dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = to_bits(src[i], false);
+}
+
+// 64 rows per block, 4 waves; wave w owns hidden columns [64w, 64w + 64) of
+// both layers for all 64 rows (4 x 4 MFMA 16x16 tiles).  X (scaled, BF16) and
+// H1 (BF16) live in LDS; weights are read as BF16 from L2 (16 bytes per lane
+// per fragment); the output layer is reduced across lanes and waves in LDS.
+// Same rounding points as the unfused path: BF16 operands, FP32 accumulation,
+// FP32 SiLU and output layer.
+template <int KP>
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *X, int ld, int n, int D,
+                                                   const uint16_t *W1b, const float *b1,
+                                                   const uint16_t *W2b, const float *b2,
+                                                   const float *w3, const float *b3,
+                                                   float *pred) {
+    constexpr int TM = 64, LX = KP + 8, LH = kHidden + 8;
+    __shared__ __align__(16) uint16_t Xs[TM * LX];
+    __shared__ __align__(16) uint16_t Hs[TM * LH];
+    __shared__ float red[4][TM];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int row0 = blockIdx.x * TM;
+    for (int e = tid; e < TM * KP; e += 256) {
+        const int r = e / KP, k = e % KP;
+        float v = 0.f;
+        if (row0 + r < n && k < D) v = X[(size_t)(row0 + r) * ld + k] * kXScale;
+        Xs[r * LX + k] = to_bits(v, false);
+    }
+    __syncthreads();
+    const int c0 = wave * 64;
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < KP / 32; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&Xs[(i * 16 + lr) * LX + kb]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            b[j] = *reinterpret_cast<const bf16x8 *>(&W1b[(size_t)(c0 + j * 16 + lr) * KP + kb]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = c0 + j * 16 + lr;
+        const float bias = b1[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = i * 16 + lk * 4 + q;
+                Hs[r * LH + col] = to_bits(siluf_(acc[i][j][q] + bias), false);
+                acc[i][j][q] = 0.f;
+            }
+    }
+    __syncthreads();
+    for (int ks = 0; ks < kHidden / 32; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&Hs[(i * 16 + lr) * LH + kb]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            b[j] = *reinterpret_cast<const bf16x8 *>(&W2b[(size_t)(c0 + j * 16 + lr) * kHidden + kb]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    float part[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[i][q] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = c0 + j * 16 + lr;
+        const float bias = b2[col], w = w3[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) part[i][q] = fmaf(siluf_(acc[i][j][q] + bias), w, part[i][q]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float v = sum16(part[i][q]);
+            if (lr == 0) red[wave][i * 16 + lk * 4 + q] = v;
+        }
+    __syncthreads();
+    if (tid < TM && row0 + tid < n)
+        pred[row0 + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid] + b3[0];
+}
+
+int Mlp::refresh_bf16(hipStream_t s) {
+    if (!W1b) {
+        SPEO_TRY_HIP(hipMalloc(&W1b, sizeof(uint16_t) * (size_t)kHidden * L.Kp));
+        SPEO_TRY_HIP(hipMalloc(&W2b, sizeof(uint16_t) * (size_t)kHidden * kHidden));
+    }
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, s, P + L.oW1, W1b, (size_t)kHidden * L.Kp);
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, s, P + L.oW2, W2b, (size_t)kHidden * kHidden);
+    return check_launch("k_to_bf16");
+}
+
+int Mlp::infer(const float *X, int ld, int n, float *pred, hipStream_t s) {
+    if (n <= 0) return SPEO_OK;
+    if (!W1b) SPEO_TRY(refresh_bf16(s));
+    const dim3 grid((n + 63) / 64), block(256);
+    const float *b1 = P + L.ob1, *b2 = P + L.ob2, *w3 = P + L.ow3, *b3 = P + L.ob3;
+    switch (L.Kp) {
+    case 32: hipLaunchKernelGGL(k_mlp_infer<32>, grid, block, 0, s, X, ld, n, L.D, W1b, b1, W2b, b2, w3, b3, pred); break;
+    case 64: hipLaunchKernelGGL(k_mlp_infer<64>, grid, block, 0, s, X, ld, n, L.D, W1b, b1, W2b, b2, w3, b3, pred); break;
+    case 96: hipLaunchKernelGGL(k_mlp_infer<96>, grid, block, 0, s, X, ld, n, L.D, W1b, b1, W2b, b2, w3, b3, pred); break;
+    case 128: hipLaunchKernelGGL(k_mlp_infer<128>, grid, block, 0, s, X, ld, n, L.D, W1b, b1, W2b, b2, w3, b3, pred); break;
+    default: return forward(X, ld, n, pred, s);  // wide inputs: the unfused path
+    }
+    return check_launch("k_mlp_infer");
 }
 
 // ------------------------------------------------------------- statistics
@@ -614,6 +747,14 @@ int speo_mlp_forward(speo_mlp h, const float *X, int ld, int n, float *pred,
     if (!h || !X || !pred || n < 0 || ld < h->m.L.D)
         return set_error(SPEO_ERR_ARG, "speo_mlp_forward: bad arguments");
     return h->m.forward(X, ld, n, pred, as_stream(stream));
+}
+
+int speo_mlp_infer(speo_mlp h, const float *X, int ld, int n, float *pred,
+                   speo_stream_t stream) {
+    if (!h || !X || !pred || n < 0 || ld < h->m.L.D)
+        return set_error(SPEO_ERR_ARG, "speo_mlp_infer: bad arguments");
+    SPEO_TRY(h->m.refresh_bf16(as_stream(stream)));
+    return h->m.infer(X, ld, n, pred, as_stream(stream));
 }
 
 int speo_mlp_loss_grad(speo_mlp h, const float *X, int ld, int n, const float *t,

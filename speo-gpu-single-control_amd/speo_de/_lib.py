@@ -123,6 +123,7 @@ _SIGS = {
     "speo_mlp_num_params": (c_int64, [c_void_p]),
     "speo_mlp_params": (c_int, [c_void_p, c_void_p, c_void_p]),
     "speo_mlp_forward": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "speo_mlp_infer": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "speo_mlp_loss_grad": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_float, c_float,
                                    c_void_p, c_void_p]),
     "speo_mlp_adam_step": (c_int, [c_void_p, c_float]),
@@ -136,6 +137,14 @@ def lib():
     """Load libspeo_de.so; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # libspeo_de.so and torch both need libamdhip64.so.7; the first one
+        # loaded serves the process.  Load torch's first so that the library
+        # and torch share torch's HIP runtime (the one that matches the
+        # driver on the GPU boxes).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is not built: run `make -C {PKG_DIR}` "
                                "(or __graft_entry__.build()); there is no CPU fallback")

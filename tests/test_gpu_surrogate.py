@@ -68,12 +68,39 @@ def test_mlp_forward_matches_oracle(cuda, speo):
     net.set_flat(f)
     X = rng.uniform(-100, 100, (n, 104)).astype(np.float32)
     y = torch.zeros(n, dtype=torch.float32, device=cuda)
-    speo.check(net.L.speo_mlp_forward(net.h, ctypes.c_void_p(_t(X, cuda).data_ptr()), 104, n,
+    Xd = _t(X, cuda)          # keep a reference: a freed temporary may be reused
+    speo.check(net.L.speo_mlp_forward(net.h, ctypes.c_void_p(Xd.data_ptr()), 104, n,
                                       ctypes.c_void_p(y.data_ptr()), None))
     torch.cuda.synchronize()
     ref, _ = SO.forward(net.to_dict(f), X[:, :D].astype(np.float64), round_bf16=True)
     scale = np.abs(ref).max()
     np.testing.assert_allclose(y.cpu().numpy(), ref, atol=2e-4 * scale)
+    # the fused single-kernel inference path gives the same predictions
+    y2 = torch.zeros(n, dtype=torch.float32, device=cuda)
+    speo.check(net.L.speo_mlp_infer(net.h, ctypes.c_void_p(Xd.data_ptr()), 104, n,
+                                    ctypes.c_void_p(y2.data_ptr()), None))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(y2.cpu().numpy(), ref, atol=2e-4 * scale)
+    net.close()
+
+
+@pytest.mark.parametrize("D", [10, 30, 64, 100, 200])
+def test_mlp_infer_dims(cuda, speo, D):
+    import surrogate as SO
+    import torch
+    rng = np.random.default_rng(D)
+    n = 130
+    net = Net(speo, D, 256)
+    f = _random_params(net, rng)
+    net.set_flat(f)
+    X = rng.uniform(-100, 100, (n, D)).astype(np.float32)
+    y = torch.zeros(n, dtype=torch.float32, device=cuda)
+    Xd = _t(X, cuda)
+    speo.check(net.L.speo_mlp_infer(net.h, ctypes.c_void_p(Xd.data_ptr()), D, n,
+                                    ctypes.c_void_p(y.data_ptr()), None))
+    torch.cuda.synchronize()
+    ref, _ = SO.forward(net.to_dict(f), X.astype(np.float64), round_bf16=True)
+    np.testing.assert_allclose(y.cpu().numpy(), ref, atol=2e-4 * np.abs(ref).max())
     net.close()
 
 
@@ -140,8 +167,9 @@ def test_spearman_kernel(cuda, speo):
         b = (a + rng.normal(scale=1.0, size=n)).astype(np.float32)
         a[::9] = a[0]
         out = torch.zeros(1, dtype=torch.float32, device=cuda)
-        speo.check(speo.lib().speo_spearman(ctypes.c_void_p(_t(a, cuda).data_ptr()),
-                                            ctypes.c_void_p(_t(b, cuda).data_ptr()), n,
+        ad, bd = _t(a, cuda), _t(b, cuda)   # both alive while the kernel is enqueued
+        speo.check(speo.lib().speo_spearman(ctypes.c_void_p(ad.data_ptr()),
+                                            ctypes.c_void_p(bd.data_ptr()), n,
                                             ctypes.c_void_p(out.data_ptr()), None))
         torch.cuda.synchronize()
         assert out.item() == pytest.approx(SO.spearman(a, b), abs=1e-5)
