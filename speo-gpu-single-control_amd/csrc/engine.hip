@@ -67,11 +67,16 @@ struct GenerationArgs {
     int clear_trials;
 };
 
-constexpr int kTLD = 128;  // LDS row stride of the trial tile (dim <= 128)
+constexpr int kTLD = kSmallD;  // widest dimension whose rows stay in LDS
 
+// LDS of one block: [trial rows RT x kTS] [gval kMaxComp x RT] [GEMM region];
+// with rows in LDS the GEMM region is the small-D path's (scratch + row
+// reduction), otherwise the K-chunked path's.
 template <int RT, int PREC>
 constexpr int generation_lds_bytes(bool trial_in_lds) {
-    return (trial_in_lds ? RT * kTLD * 4 : 0) + kMaxComp * RT * 4 + gemm_lds_bytes<RT, PREC>();
+    return trial_in_lds ? RT * kTS * 4 + kMaxComp * RT * 4 +
+                              small_scratch_bytes<RT>(Prec<PREC>::parts) + 10 * RT * 4
+                        : kMaxComp * RT * 4 + gemm_lds_bytes<RT, PREC>();
 }
 
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
@@ -108,9 +113,9 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
         return;
     }
 
-    const bool lds_trial = (MODE == kModeFused) && a.trial_in_lds;
+    const bool lds_trial = (MODE == kModeFused || MODE == kModeEval) && a.trial_in_lds;
     float *trial = reinterpret_cast<float *>(lds);
-    float *gval = trial + (lds_trial ? RT * kTLD : 0);
+    float *gval = trial + (lds_trial ? RT * kTS : 0);
     char *gemm_lds = reinterpret_cast<char *>(gval + kMaxComp * RT);
 
     if constexpr (MODE == kModeGen || MODE == kModeFused) {
@@ -118,7 +123,11 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
         const int ngroups = (I.dim + 3) >> 2;
         for (int pass = 0; pass < RT / 16; ++pass) {
             const int lr = pass * 16 + (tid >> 4), l = tid & 15;
-            if (lr >= t.nrows) continue;
+            if (lr >= t.nrows) {
+                if (lds_trial)
+                    for (int g = l; g < ngroups; g += 16) st4(trial + lr * kTS + 4 * g, float4{0.f, 0.f, 0.f, 0.f});
+                continue;
+            }
             const int li = t.row0 + lr;
             const uint32_t gr = (uint32_t)(I.row_off + li);
             DonorStream ds(a.seed, gr, gen);
@@ -162,7 +171,7 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
                 }
                 const float4 out{v[0], v[1], v[2], v[3]};
                 st4(urow + 4 * g, out);
-                if (lds_trial) st4(trial + lr * kTLD + 4 * g, out);
+                if (lds_trial) st4(trial + lr * kTS + 4 * g, out);
                 if (MODE == kModeFused && own_acc) st4(xrow + 4 * g, xo);
             }
             if (MODE == kModeFused && l == 0) a.fX_new[gr] = own_acc ? fu_own : fx_own;
@@ -170,24 +179,43 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
     }
 
     if constexpr (MODE == kModeEval || MODE == kModeFused) {
-        __syncthreads();
         const FusedProblem *P = a.probs + I.problem;
         const int ncomp = P->ncomp, dim = I.dim;
+        if (MODE == kModeEval && lds_trial) {
+            // stage the rows to evaluate into the LDS tile (zero rows past nrows)
+            const float *srcp = a.eval_src + I.elem_off + (size_t)t.row0 * I.ld;
+            const int ngroups = (dim + 3) >> 2;
+            for (int e = tid; e < RT * ngroups; e += 256) {
+                const int r = e / ngroups, g = e % ngroups;
+                st4(trial + r * kTS + 4 * g,
+                    r < t.nrows ? ld4(srcp + (size_t)r * I.ld + 4 * g) : float4{0.f, 0.f, 0.f, 0.f});
+            }
+        }
+        __syncthreads();
         RowSource src;
-        if (MODE == kModeEval)
+        if (lds_trial)
+            src = RowSource{trial, kTS, t.nrows};
+        else if (MODE == kModeEval)
             src = RowSource{a.eval_src + I.elem_off + (size_t)t.row0 * I.ld, I.ld, t.nrows};
-        else if (lds_trial)
-            src = RowSource{trial, kTLD, t.nrows};
         else
             src = RowSource{a.U_new + I.elem_off + (size_t)t.row0 * I.ld, I.ld, t.nrows};
-        float *red = reinterpret_cast<float *>(gemm_lds);
-        float *o1 = red + 8 * RT, *o2 = o1 + RT;
         for (int c = 0; c < ncomp; ++c) {
             const CompDev comp = P->comp[c];
-            float s1[4] = {0.f, 0.f, 0.f, 0.f}, s2[4] = {0.f, 0.f, 0.f, 0.f};
-            for (int n0 = 0; n0 < dim; n0 += kNC) gemm_chunk<RT, PREC>(comp, src, dim, n0, gemm_lds, s1, s2);
-            reduce_rows<RT>(s1, s2, red, o1, o2);
-            if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);
+            if (lds_trial) {
+                float *red = reinterpret_cast<float *>(gemm_lds + small_scratch_bytes<RT>(Prec<PREC>::parts));
+                float *o1 = red + 8 * RT, *o2 = o1 + RT;
+                float s1[RT / 16][4], s2[RT / 16][4];
+                gemm_small<RT, PREC>(comp, trial, dim, gemm_lds, s1, s2);
+                reduce_rows_small<RT>(s1, s2, red, o1, o2);
+                if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);
+            } else {
+                float *red = reinterpret_cast<float *>(gemm_lds);
+                float *o1 = red + 8 * RT, *o2 = o1 + RT;
+                float s1[4] = {0.f, 0.f, 0.f, 0.f}, s2[4] = {0.f, 0.f, 0.f, 0.f};
+                for (int n0 = 0; n0 < dim; n0 += kNC) gemm_chunk<RT, PREC>(comp, src, dim, n0, gemm_lds, s1, s2);
+                reduce_rows<RT>(s1, s2, red, o1, o2);
+                if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);
+            }
         }
         if (tid < t.nrows) {
             float f;
@@ -253,7 +281,8 @@ static int launch_generation(const GenerationArgs &a, int ntiles, hipStream_t s)
     std::call_once(once, [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_generation<RT, PREC, MODE>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  generation_lds_bytes<RT, PREC>(true));
+                                  std::max(generation_lds_bytes<RT, PREC>(true),
+                                           generation_lds_bytes<RT, PREC>(false)));
     });
     hipLaunchKernelGGL((k_generation<RT, PREC, MODE>), dim3(ntiles), dim3(256), lds, s, a);
     return check_launch("k_generation");
@@ -533,6 +562,7 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     a.probs = probs_dev;
     a.eval_src = X;
     a.eval_out = fX[0];
+    a.trial_in_lds = all_small ? 1 : 0;
     SPEO_TRY(launch_any(rt, cfg.precision, kModeEval, a, ntiles[rt], stream));
     hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, fX[1], total_rows, INFINITY);
     SPEO_TRY(check_launch("k_fill"));
@@ -662,6 +692,7 @@ int Engine::launch(int mode, int gen_offset, int rt_, hipStream_t s, int clear) 
     case kModeEval:
         a.eval_src = U[0];
         a.eval_out = fU[0];
+        a.trial_in_lds = all_small ? 1 : 0;
         break;
     case kModeSelect:
         if (cfg.mode == SPEO_MODE_FUSED) {  // apply the pending selection

@@ -14,6 +14,7 @@ struct EvalLaunch {
     int nchunks;
     float lambda, bias;
     int raw;        // 1: write g (composition component), 0: lambda * g + bias
+    int small;      // dim <= 128 with the padded layouts: register-resident-B path
 };
 
 // One component over n rows; `part` is needed when dim > 128.

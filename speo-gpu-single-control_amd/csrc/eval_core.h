@@ -74,7 +74,14 @@ struct CompDev {
     int ldm16;
     int is_half;             // M16 holds FP16 (scaled by 2^14) instead of BF16
     const float *colw;       // elliptic column weights 10^(6j/(D-1)), [dim]
+    // Small-D layouts (dim <= 128), zero padded, or null:
+    const float *B32s;          // [kp4][128]: B[k][c] = M[c][k]
+    const uint16_t *M16s[3];    // [128][kp32]: 16-bit parts of M (FP16 scaled by 2^14)
+    int kp32;                   // round_up(dim, 32)
 };
+
+constexpr int kSmallD = 128;  // widest dimension of the register-resident-B path
+constexpr int kTS = 132;      // LDS row stride (floats) of staged rows, 16-byte aligned
 
 // LDS footprint (bytes) of the GEMM tile for RT rows.
 template <int RT, int PREC> constexpr int gemm_lds_bytes() {
@@ -282,6 +289,180 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
             for (int i = 0; i < 4; ++i) add_terms(c.objective, acc[t][i] * unscale, w, s1[i], s2[i]);
         }
     }
+}
+
+// ------------------------------------------------------- small-D path
+//
+// dim <= 128.  The tile's RT rows are in LDS (rows[r * kTS + k], FP32,
+// unshifted; rows >= nrows and columns >= dim hold zeros).  Wave w owns the
+// 16-column tiles w and w + 4 for all RT rows; its B fragments come straight
+// from the padded global layouts into registers (no LDS staging, no barrier
+// inside the K loop).  `scratch` holds the shifted rows' 16-bit parts for the
+// 16-bit precisions: P * RT * (kp32 + 8) uint16, and the shift: 128 floats.
+// Output: s1[i][q], s2[i][q] for row 16 i + 4 (lane / 16) + q of the tile,
+// summed over this wave's columns and the lane's 16-lane group.
+template <int RT> constexpr int small_scratch_bytes(int parts) {
+    return 128 * 4 + parts * RT * (kSmallD + 8) * 2;
+}
+
+template <int RT, int PREC>
+__device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, int dim,
+                                           char *scratch, float (&s1)[RT / 16][4],
+                                           float (&s2)[RT / 16][4]) {
+    constexpr int NRT = RT / 16;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    float *sh = reinterpret_cast<float *>(scratch);  // shift, then rate-scaled in place
+    floatx4 acc[NRT][2];
+#pragma unroll
+    for (int i = 0; i < NRT; ++i) {
+        acc[i][0] = floatx4{0.f, 0.f, 0.f, 0.f};
+        acc[i][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+    const int t0 = wave, t1 = wave + 4;
+    const bool v0 = t0 * 16 < dim, v1 = t1 * 16 < dim;
+
+    if constexpr (Prec<PREC>::parts == 0) {
+        const int kp4 = (dim + 3) & ~3, KS = kp4 >> 2;
+        // B fragments into registers first (their latency overlaps the shift staging)
+        float b0[32], b1[32];
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks) {
+            b0[ks] = 0.f;
+            b1[ks] = 0.f;
+            if (ks < KS) {
+                const float *bp = c.B32s + (size_t)(ks * 4 + lk) * kSmallD + lr;
+                if (v0) b0[ks] = bp[t0 * 16];
+                if (v1) b1[ks] = bp[t1 * 16];
+            }
+        }
+        for (int k = tid; k < kSmallD; k += 256) sh[k] = k < dim ? c.shift[k] : 0.f;
+        __syncthreads();
+        const float rate = c.rate;
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks) {
+            if (ks < KS) {
+                const int k = ks * 4 + lk;
+                const float o = sh[k];
+#pragma unroll
+                for (int i = 0; i < NRT; ++i) {
+                    const float a = (rows[(i * 16 + lr) * kTS + k] - o) * rate;
+                    if (v0) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0[ks], acc[i][0], 0, 0, 0);
+                    if (v1) acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1[ks], acc[i][1], 0, 0, 0);
+                }
+            }
+        }
+    } else {
+        constexpr int P = Prec<PREC>::parts;
+        constexpr bool H = Prec<PREC>::half;
+        typedef typename std::conditional<H, halfx8, bf16x8>::type vec8;
+        const int kp32 = c.kp32, KS = kp32 >> 5, LDA = kSmallD + 8;
+        uint16_t *As = reinterpret_cast<uint16_t *>(sh + kSmallD);  // [P][RT][LDA]
+        for (int k = tid; k < kSmallD; k += 256) sh[k] = k < dim ? c.shift[k] : 0.f;
+        __syncthreads();
+        const float scale = c.rate * (H ? kF16ScaleA : 1.0f);
+        for (int e = tid; e < RT * kp32; e += 256) {
+            const int r = e / kp32, k = e % kp32;
+            float rem = k < dim ? (rows[r * kTS + k] - sh[k]) * scale : 0.f;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const uint16_t b = to_bits(rem, H);
+                As[(p * RT + r) * LDA + k] = b;
+                rem -= from_bits(b, H);
+            }
+        }
+        __syncthreads();
+        for (int ks = 0; ks < KS; ++ks) {
+            const int kb = ks * 32 + lk * 8;
+            vec8 b[2][P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                b[0][p] = v0 ? *reinterpret_cast<const vec8 *>(c.M16s[p] + (size_t)(t0 * 16 + lr) * kp32 + kb) : vec8{};
+                b[1][p] = v1 ? *reinterpret_cast<const vec8 *>(c.M16s[p] + (size_t)(t1 * 16 + lr) * kp32 + kb) : vec8{};
+            }
+#pragma unroll
+            for (int i = 0; i < NRT; ++i) {
+                vec8 a[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    a[p] = *reinterpret_cast<const vec8 *>(&As[(p * RT + i * 16 + lr) * LDA + kb]);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    if (t == 0 ? !v0 : !v1) continue;
+#pragma unroll
+                    for (int sum = P - 1; sum >= 0; --sum) {
+#pragma unroll
+                        for (int pa = 0; pa <= sum; ++pa) {
+                            const int pb = sum - pa;
+                            if (pa < P && pb < P) {
+                                if constexpr (H)
+                                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[pa], b[t][pb], acc[i][t], 0, 0, 0);
+                                else
+                                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[pa], b[t][pb], acc[i][t], 0, 0, 0);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const float unscale = Prec<PREC>::half ? kF16Unscale : 1.0f;
+#pragma unroll
+    for (int i = 0; i < NRT; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s1[i][q] = 0.f;
+            s2[i][q] = 0.f;
+        }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (t == 0 ? t0 : t1) * 16 + lr;
+        if (col < dim) {
+            const float w = (c.objective == SPEO_OBJ_ELLIPTIC) ? c.colw[col] : 1.0f;
+#pragma unroll
+            for (int i = 0; i < NRT; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) add_terms(c.objective, acc[i][t][q] * unscale, w, s1[i][q], s2[i][q]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NRT; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s1[i][q] = sum16(s1[i][q]);
+            s2[i][q] = sum16(s2[i][q]);
+        }
+}
+
+// Per-row sums of gemm_small's per-wave partials (fixed order over waves).
+// red: 2 * 4 * RT floats of LDS; out1/out2: RT floats each.
+template <int RT>
+__device__ __forceinline__ void reduce_rows_small(const float (&s1)[RT / 16][4],
+                                                  const float (&s2)[RT / 16][4], float *red,
+                                                  float *out1, float *out2) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    __syncthreads();
+    if ((lane & 15) == 0) {
+#pragma unroll
+        for (int i = 0; i < RT / 16; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = i * 16 + (lane >> 4) * 4 + q;
+                red[(wave * RT + r) * 2 + 0] = s1[i][q];
+                red[(wave * RT + r) * 2 + 1] = s2[i][q];
+            }
+    }
+    __syncthreads();
+    if (tid < RT) {
+        float a = 0.f, b = 0.f;
+        for (int w = 0; w < 4; ++w) {
+            a += red[(w * RT + tid) * 2 + 0];
+            b += red[(w * RT + tid) * 2 + 1];
+        }
+        out1[tid] = a;
+        out2[tid] = b;
+    }
+    __syncthreads();
 }
 
 // Reduces s1/s2 of all waves of the block into per-row sums:

@@ -4,6 +4,7 @@
 // Replaces the reference's per-objective launch chains (API_evaluateFitness,
 // kernel_CEC2014_CUDA.cu:1672-1797: ShiftAndRotation -> global_fN, plus
 // CalW/CombineFitness for compositions) by one kernel per N chunk of Z.
+#include <algorithm>
 #include <mutex>
 #include <unordered_map>
 
@@ -14,17 +15,41 @@ namespace speo {
 
 // ------------------------------------------------------------------ kernels
 
+template <int RT, int PREC> constexpr int eval_small_lds_bytes() {
+    return RT * kTS * 4 + small_scratch_bytes<RT>(Prec<PREC>::parts) + 10 * RT * 4;
+}
+
 template <int RT, int PREC>
 __global__ __launch_bounds__(256) void k_eval(EvalLaunch p) {
     extern __shared__ __align__(16) char lds[];
     const int row0 = blockIdx.x * RT;
     const int chunk = blockIdx.y;
-    RowSource src{p.X + (size_t)row0 * p.ld, p.ld, min(RT, p.n - row0)};
-    float s1[4] = {0.f, 0.f, 0.f, 0.f}, s2[4] = {0.f, 0.f, 0.f, 0.f};
-    gemm_chunk<RT, PREC>(p.comp, src, p.dim, chunk * kNC, lds, s1, s2);
-    float *red = reinterpret_cast<float *>(lds);
-    float *o1 = red + 4 * RT * 2, *o2 = o1 + RT;
-    reduce_rows<RT>(s1, s2, red, o1, o2);
+    const int nrows = min(RT, p.n - row0);
+    float *o1, *o2;
+    if (p.small) {
+        float *rows = reinterpret_cast<float *>(lds);
+        char *scratch = lds + RT * kTS * 4;
+        float *red = reinterpret_cast<float *>(scratch + small_scratch_bytes<RT>(Prec<PREC>::parts));
+        o1 = red + 8 * RT;
+        o2 = o1 + RT;
+        const int kp4 = (p.dim + 3) & ~3;
+        for (int e = threadIdx.x; e < RT * kp4; e += 256) {
+            const int r = e / kp4, k = e % kp4;
+            rows[r * kTS + k] = (r < nrows && k < p.dim) ? p.X[(size_t)(row0 + r) * p.ld + k] : 0.f;
+        }
+        __syncthreads();
+        float s1[RT / 16][4], s2[RT / 16][4];
+        gemm_small<RT, PREC>(p.comp, rows, p.dim, scratch, s1, s2);
+        reduce_rows_small<RT>(s1, s2, red, o1, o2);
+    } else {
+        RowSource src{p.X + (size_t)row0 * p.ld, p.ld, nrows};
+        float s1[4] = {0.f, 0.f, 0.f, 0.f}, s2[4] = {0.f, 0.f, 0.f, 0.f};
+        gemm_chunk<RT, PREC>(p.comp, src, p.dim, chunk * kNC, lds, s1, s2);
+        float *red = reinterpret_cast<float *>(lds);
+        o1 = red + 4 * RT * 2;
+        o2 = o1 + RT;
+        reduce_rows<RT>(s1, s2, red, o1, o2);
+    }
     const int t = threadIdx.x;
     if (t < RT && row0 + t < p.n) {
         const int row = row0 + t;
@@ -103,11 +128,12 @@ __global__ __launch_bounds__(256) void k_eval_f64(const CompParams *comps, int n
 template <int RT, int PREC>
 static int launch_rt_prec(const EvalLaunch &p, int n, int nchunks, hipStream_t s) {
     static std::once_flag once;
-    constexpr int lds = gemm_lds_bytes<RT, PREC>();
+    constexpr int lds_max = std::max(gemm_lds_bytes<RT, PREC>(), eval_small_lds_bytes<RT, PREC>());
     std::call_once(once, [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval<RT, PREC>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
     });
+    const int lds = p.small ? eval_small_lds_bytes<RT, PREC>() : gemm_lds_bytes<RT, PREC>();
     dim3 grid((n + RT - 1) / RT, nchunks);
     hipLaunchKernelGGL((k_eval<RT, PREC>), grid, dim3(256), lds, s, p);
     return check_launch("k_eval");
@@ -132,7 +158,8 @@ int eval_component(const CompDev &comp, const float *X, int ld, int n, int dim, 
     if (n <= 0) return SPEO_OK;
     const int nchunks = (dim + kNC - 1) / kNC;
     if (nchunks > 1 && part == nullptr) return set_error(SPEO_ERR_ARG, "eval: missing partial buffer");
-    EvalLaunch p{comp, X, ld, n, dim, f_out, part, nchunks, lambda, bias, raw};
+    const int small = (dim <= kSmallD && comp.B32s != nullptr) ? 1 : 0;
+    EvalLaunch p{comp, X, ld, n, dim, f_out, part, nchunks, lambda, bias, raw, small};
     int rc;
     switch (rows_per_block) {
     case 16: rc = launch_rt<16>(p, precision, n, nchunks, s); break;

@@ -22,6 +22,10 @@ struct CompHost {
     uint16_t *Mbf[3] = {nullptr, nullptr, nullptr};  // BF16 parts of M, [dim][ldm16]
     uint16_t *Mh[3] = {nullptr, nullptr, nullptr};   // FP16 parts of 2^14 M
     double *M64 = nullptr;      // [dim][dim]
+    // dim <= 128 only: zero-padded layouts of the register-resident-B path
+    float *B32s = nullptr;                                   // [round_up(dim,4)][128]
+    uint16_t *Mbfs[3] = {nullptr, nullptr, nullptr};         // [128][round_up(dim,32)]
+    uint16_t *Mhs[3] = {nullptr, nullptr, nullptr};
 };
 
 // Per-component parameters of the FP64 path and of the composition combine,

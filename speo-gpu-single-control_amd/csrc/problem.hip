@@ -53,6 +53,9 @@ CompDev ProblemHost::view(int c, int precision) const {
     d.ldm16 = ldm16;
     d.is_half = half ? 1 : 0;
     d.colw = colw;
+    d.B32s = h.B32s;
+    for (int p = 0; p < 3; ++p) d.M16s[p] = half ? h.Mhs[p] : h.Mbfs[p];
+    d.kp32 = round_up(dim, 32);
     return d;
 }
 
@@ -70,9 +73,12 @@ static void free_problem(ProblemHost &P) {
         (void)hipFree(h.shift64);
         (void)hipFree(h.B32);
         (void)hipFree(h.M64);
+        (void)hipFree(h.B32s);
         for (int p = 0; p < 3; ++p) {
             (void)hipFree(h.Mbf[p]);
             (void)hipFree(h.Mh[p]);
+            (void)hipFree(h.Mbfs[p]);
+            (void)hipFree(h.Mhs[p]);
         }
         h = CompHost{};
     }
@@ -142,6 +148,23 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
         for (int p = 0; p < 3; ++p) {
             SPEO_TRY(upload(&h.Mbf[p], pb[p]));
             SPEO_TRY(upload(&h.Mh[p], ph[p]));
+        }
+        if (dim <= kSmallD) {
+            const size_t kp4 = (size_t)round_up(dim, 4), kp32 = (size_t)round_up(dim, 32);
+            std::vector<float> Bs(kp4 * kSmallD, 0.f);
+            for (size_t k = 0; k < D; ++k)
+                for (size_t cc = 0; cc < D; ++cc) Bs[k * kSmallD + cc] = (float)M[cc * D + k];
+            SPEO_TRY(upload(&h.B32s, Bs));
+            for (int p = 0; p < 3; ++p) {
+                std::vector<uint16_t> sb((size_t)kSmallD * kp32, 0), sh((size_t)kSmallD * kp32, 0);
+                for (size_t r = 0; r < D; ++r)
+                    for (size_t k = 0; k < D; ++k) {
+                        sb[r * kp32 + k] = pb[p][r * P.ldm16 + k];
+                        sh[r * kp32 + k] = ph[p][r * P.ldm16 + k];
+                    }
+                SPEO_TRY(upload(&h.Mbfs[p], sb));
+                SPEO_TRY(upload(&h.Mhs[p], sh));
+            }
         }
         params[(size_t)c] = CompParams{h.objective, h.rate, h.sigma, h.bias, h.lambda,
                                        h.shift32, h.shift64, h.M64};
