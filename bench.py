@@ -42,7 +42,7 @@ def build_engine(S, args):
         problems.append(S.Problem([obj], o[None, :], M[None, :, :]))
         data.append((obj, o, M))
     prec = {v: k for k, v in S.PREC_NAMES.items()}[args.precision]
-    mode = S.MODE_FUSED if args.mode == "fused" else S.MODE_PLAIN
+    mode = {"fused": S.MODE_FUSED, "plain": S.MODE_PLAIN, "epoch": S.MODE_EPOCH}[args.mode]
     eng = S.Engine(problems, [NP] * I, [i % 3 for i in range(I)], F=0.5, CR=0.9, seed=BASE_SEED,
                    interval=50, migration_size=2, precision=prec, mode=mode,
                    use_graph=not args.no_graph, rows_per_block=args.rows_per_block)
@@ -52,17 +52,14 @@ def build_engine(S, args):
 TTE_VARIANTS = {
     # name: (mode, precision, graph, surrogate candidates)
     "plain_fp32": ("plain", "fp32", False, 0),
-    "batched_fp32": ("fused", "fp32", True, 0),
-    "batched_bf16x3": ("fused", "bf16x3", True, 0),
+    "fused_fp32": ("fused", "fp32", True, 0),
+    "batched_fp32": ("epoch", "fp32", True, 0),
+    "batched_bf16x3": ("epoch", "bf16x3", True, 0),
     "surrogate_bf16x3": ("plain", "bf16x3", False, 8),
 }
 
 
-def run_to_targets(S, problems, objs, islands, island_size, variant, targets, max_gens,
-                   check_every):
-    """Runs one engine variant, checking the FP64-verified best of each
-    objective every `check_every` generations; returns the trajectory and the
-    wall time at which each objective first reached its target."""
+def tte_engine(S, problems, islands, island_size, variant):
     mode, prec, graph, cands = TTE_VARIANTS[variant]
     precision = {v: k for k, v in S.PREC_NAMES.items()}[prec]
     pidx = [i % len(problems) for i in range(islands)]
@@ -73,72 +70,103 @@ def run_to_targets(S, problems, objs, islands, island_size, variant, targets, ma
                      surrogate_lr=1e-3, surrogate_rho_min=0.5)
     eng = S.Engine(problems, [island_size] * islands, pidx, F=0.5, CR=0.9, seed=BASE_SEED + 7,
                    interval=50, migration_size=2, precision=precision,
-                   mode=S.MODE_FUSED if mode == "fused" else S.MODE_PLAIN, use_graph=graph,
-                   rows_per_block=0, **extra)
-    eng.run(1)          # graph capture / first launches outside the clock
-    eng.best()
-    pidx = np.array(pidx)
-    traj, reached = [], {}
-    gens = 0
-    t0 = time.perf_counter()
+                   mode={"fused": S.MODE_FUSED, "plain": S.MODE_PLAIN, "epoch": S.MODE_EPOCH}[mode],
+                   use_graph=graph, rows_per_block=0, **extra)
+    return eng, np.array(pidx)
+
+
+def trajectory(S, problems, objs, islands, island_size, variant, targets, max_gens, check_every):
+    """Pass 1 (untimed): the FP64-verified best error of each objective every
+    `check_every` generations, until every target is reached or max_gens.
+    Returns the trajectory and, per objective, the first checked generation at
+    which it reached its target."""
+    eng, pidx = tte_engine(S, problems, islands, island_size, variant)
+    traj, reached, gens = [], {}, 0
     while gens < max_gens:
         eng.run(check_every)
         gens += check_every
         _, _, f64 = eng.best()
-        t = time.perf_counter() - t0
         errs = {objs[k]: float(f64[pidx == k].min()) for k in range(len(problems))}
-        traj.append((gens, t, errs))
+        traj.append((gens, errs))
         for name, e in errs.items():
             if targets is not None and name not in reached and e <= targets[name]:
-                reached[name] = (t, gens)
+                reached[name] = gens
         if targets is not None and len(reached) == len(targets):
             break
-    out = {"generations": gens, "seconds": time.perf_counter() - t0,
-           "final_error": traj[-1][2], "reached": reached}
-    if cands:
+    out = {"generations_run": gens, "final_error": traj[-1][1]}
+    if TTE_VARIANTS[variant][3]:
         rho, act, loss = eng.surrogate_stats()
         out["surrogate_rho"] = [float(r) for r in rho]
         out["surrogate_active"] = [int(a) for a in act]
         out["surrogate_predictions"] = eng.stats()["surrogate_predictions"]
     eng.close()
-    return traj, out
+    return traj, reached, out
+
+
+def timed_run_to(S, problems, objs, islands, island_size, variant, stops, targets):
+    """Pass 2 (timed): a fresh engine with the same seed runs to each stop
+    generation with no instrumentation (the engine is deterministic, so it
+    retraces pass 1); the clock runs from the first launch to the
+    synchronisation at each stop and pauses while the stop's FP64-verified
+    best is read to confirm the target."""
+    eng, pidx = tte_engine(S, problems, islands, island_size, variant)
+    eng.sync()
+    out, elapsed, gens = {}, 0.0, 0
+    for name, g in sorted(stops.items(), key=lambda kv: kv[1]):
+        if g > gens:
+            t0 = time.perf_counter()
+            eng.run(g - gens)
+            eng.sync()
+            elapsed += time.perf_counter() - t0
+            gens = g
+        _, _, f64 = eng.best()
+        k = objs.index(name)
+        err = float(f64[pidx == k].min())
+        out[name] = {"seconds": elapsed, "generations": g, "error": err,
+                     "confirmed": bool(err <= targets[name])}
+    eng.close()
+    return out
 
 
 def time_to_error(S, dim, islands, island_size, max_gens, check_every):
-    """Time-to-error protocol: the plain FP32 engine runs max_gens generations;
-    each objective's target is 1e-2 if plain reaches it, else plain's final
-    error; every variant is timed to those targets (time includes the periodic
-    FP64-verified best checks, identical for all variants)."""
+    """Time-to-error protocol.  The plain FP32 engine (three launches per
+    generation, no graph) runs max_gens generations; each objective's target is
+    1e-2 if plain reaches it, else plain's final error.  For every variant,
+    pass 1 finds the first generation (checked every `check_every`) at which
+    each objective reaches its target, and pass 2 times a fresh, uninstrumented
+    run to exactly those generations.  Speed-up = plain's time / variant's time
+    per objective, "all" = time until both targets are reached."""
     names = {S.OBJ_ELLIPTIC: "elliptic", S.OBJ_RASTRIGIN: "rastrigin"}
     problems, objs = [], []
     for k, obj in enumerate((S.OBJ_RASTRIGIN, S.OBJ_ELLIPTIC)):
         o, M = S.generate_problem_data(S.problem_seed(BASE_SEED + 100, k), dim)
         problems.append(S.Problem([obj], o[None, :], M[None, :, :]))
         objs.append(names[obj])
-    traj, plain = run_to_targets(S, problems, objs, islands, island_size, "plain_fp32", None,
-                                 max_gens, check_every)
-    final = traj[-1][2]
+    print("# tte plain_fp32 trajectory", file=sys.stderr, flush=True)
+    traj, _, plain = trajectory(S, problems, objs, islands, island_size, "plain_fp32", None,
+                                max_gens, check_every)
+    final = traj[-1][1]
     targets = {n: (1e-2 if final[n] <= 1e-2 else final[n]) for n in objs}
-    plain_t = {}
-    for n in objs:
-        for g, t, errs in traj:
-            if errs[n] <= targets[n]:
-                plain_t[n] = (t, g)
-                break
+    plain_g = {n: next(g for g, errs in traj if errs[n] <= targets[n]) for n in objs}
+    plain_t = timed_run_to(S, problems, objs, islands, island_size, "plain_fp32", plain_g, targets)
     result = {"workload": f"{islands} islands x NP={island_size}, D={dim}, Rastrigin/elliptic "
-                          f"round-robin, ring best-2 every 50; best checked every {check_every} "
-                          "generations (FP64 re-verified)",
-              "targets": targets, "plain_fp32": {"reached": plain_t, **plain}, "variants": {}}
-    for v in ("batched_fp32", "batched_bf16x3", "surrogate_bf16x3"):
+                          f"round-robin, ring best-2 every 50; targets checked every "
+                          f"{check_every} generations (FP64 re-verified), then timed on an "
+                          "uninstrumented re-run",
+              "targets": targets, "plain_fp32": {"to_target": plain_t, **plain}, "variants": {}}
+    for v in ("fused_fp32", "batched_fp32", "batched_bf16x3", "surrogate_bf16x3"):
         print(f"# tte variant {v}", file=sys.stderr, flush=True)
-        _, res = run_to_targets(S, problems, objs, islands, island_size, v, targets,
-                                max_gens * 4, check_every)
+        _, reached, res = trajectory(S, problems, objs, islands, island_size, v, targets,
+                                     max_gens * 4, check_every)
+        res["to_target"] = timed_run_to(S, problems, objs, islands, island_size, v, reached,
+                                        targets) if reached else {}
         sp = {}
         for n in objs:
-            if n in res["reached"] and n in plain_t:
-                sp[n] = plain_t[n][0] / res["reached"][n][0]
-        if len(res["reached"]) == len(objs):
-            sp["all"] = max(t for t, _ in plain_t.values()) / max(t for t, _ in res["reached"].values())
+            if n in res["to_target"]:
+                sp[n] = plain_t[n]["seconds"] / res["to_target"][n]["seconds"]
+        if len(res["to_target"]) == len(objs):
+            sp["all"] = (max(r["seconds"] for r in plain_t.values()) /
+                         max(r["seconds"] for r in res["to_target"].values()))
         res["speedup_vs_plain"] = sp
         result["variants"][v] = res
     for p in problems:
@@ -146,11 +174,13 @@ def time_to_error(S, dim, islands, island_size, max_gens, check_every):
     return result
 
 
-def uneven_batch(S, islands=256, gens=500, interval=25, precision="fp32", seed=BASE_SEED + 300):
+def uneven_batch(S, islands=256, gens=500, interval=25, precision="fp32", seed=BASE_SEED + 300,
+                 mode="epoch"):
     """Config 3: 256 islands with D drawn from {10, 30, 50, 100} and NP uniform
-    in [50, 512], objectives round-robin, one fused launch per generation over
-    all islands, generations captured in hipGraphs, autotuned row tile,
-    migration every 25 generations.  Returns throughput and kernel time."""
+    in [50, 512], objectives round-robin, all islands in one launch (one epoch
+    launch per 25 generations, or one fused launch per generation), launches
+    captured in hipGraphs, migration every 25 generations.  Returns throughput
+    and kernel time."""
     rng = np.random.default_rng(seed)
     dims = rng.choice([10, 30, 50, 100], islands)
     sizes = rng.integers(50, 513, islands)
@@ -164,7 +194,8 @@ def uneven_batch(S, islands=256, gens=500, interval=25, precision="fp32", seed=B
     pidx = [index[(int(d), i % 3)] for i, d in enumerate(dims)]
     prec = {v: k for k, v in S.PREC_NAMES.items()}[precision]
     eng = S.Engine(problems, sizes.tolist(), pidx, F=0.5, CR=0.9, seed=seed, interval=interval,
-                   migration_size=2, precision=prec, mode=S.MODE_FUSED, use_graph=True,
+                   migration_size=2, precision=prec,
+                   mode=S.MODE_EPOCH if mode == "epoch" else S.MODE_FUSED, use_graph=True,
                    rows_per_block=0)
     eng.run(25)
     eng.sync()
@@ -173,12 +204,13 @@ def uneven_batch(S, islands=256, gens=500, interval=25, precision="fp32", seed=B
     eng.run(gens)
     eng.sync()
     dt = time.perf_counter() - t0
-    kms = eng.profile(20)
+    kms = eng.profile(50)
+    kernel_ms = kms["epoch" if mode == "epoch" else "fused"]
     flops = float(sum(2.0 * int(d) * int(d) * int(n) for d, n in zip(dims, sizes)))
-    out = {"islands": islands, "total_rows": rows, "generations": gens,
+    out = {"islands": islands, "total_rows": rows, "generations": gens, "mode": mode,
            "trial_evals_per_s": gens * rows / dt, "ms_per_generation": dt / gens * 1e3,
-           "fused_kernel_ms": kms["fused"], "rows_per_block": eng.stats()["rows_per_block"],
-           "mfma_TFLOPs": flops / (kms["fused"] * 1e-3) / 1e12,
+           "kernel_ms_per_generation": kernel_ms, "rows_per_block": eng.stats()["rows_per_block"],
+           "mfma_TFLOPs": flops / (kernel_ms * 1e-3) / 1e12,
            "graph_launches": eng.stats()["graph_launches"], "precision": precision}
     eng.close()
     for p in problems:
@@ -292,7 +324,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000, help="timed generations")
     ap.add_argument("--warmup", type=int, default=50, help="untimed generations")
-    ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--mode", choices=["epoch", "fused", "plain"], default="epoch")
     ap.add_argument("--precision", default="fp32",
                     choices=["fp32", "bf16", "fp16", "bf16x3", "fp16x3", "bf16x6"])
     ap.add_argument("--no-graph", action="store_true")
@@ -358,8 +390,10 @@ def main():
     final = {names[k]: float(f64[pidx == k].min()) for k in range(3) if (pidx == k).any()}
 
     kms = eng.profile(args.profile_gens)
-    dominant = "fused" if args.mode == "fused" else "evaluate"
+    dominant = {"epoch": "epoch", "fused": "fused", "plain": "evaluate"}[args.mode]
     kernel_ms = kms[dominant]
+    kernel_name = {"epoch": "k_island_epoch (per generation)", "fused": "k_generation (fused)",
+                   "plain": "k_generation (evaluate)"}[args.mode]
     flops = 2.0 * D * D * rows
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     de_bytes = 4.0 * (3 * D + 3) * rows
@@ -390,7 +424,7 @@ def main():
             "rows_per_block": eng.stats()["rows_per_block"],
         },
         "roofline": {
-            "kernel": "k_generation (" + dominant + ")",
+            "kernel": kernel_name,
             "bound": "mfma",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
@@ -413,7 +447,9 @@ def main():
     eng.close()
     if args.uneven:
         print("# uneven batch", file=sys.stderr, flush=True)
-        out["uneven_batch"] = {p: uneven_batch(S, precision=p) for p in ("fp32", "bf16x3")}
+        out["uneven_batch"] = {f"{m}_{p}": uneven_batch(S, precision=p, mode=m)
+                               for m, p in (("epoch", "fp32"), ("fused", "fp32"),
+                                            ("epoch", "bf16x3"))}
     if args.precision_study:
         print("# precision study", file=sys.stderr, flush=True)
         out["precision_study"] = precision_study(S)

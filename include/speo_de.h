@@ -239,7 +239,10 @@ SPEO_EXPORT int speo_segment_argmin(const float *f, const int *off, int nseg, in
 
 enum {
     SPEO_MODE_PLAIN = 0, /* generate, evaluate, select: three launches per generation */
-    SPEO_MODE_FUSED = 1  /* one launch per generation: select(g-1)+generate(g)+evaluate(g) */
+    SPEO_MODE_FUSED = 1, /* one launch per generation: select(g-1)+generate(g)+evaluate(g) */
+    SPEO_MODE_EPOCH = 2  /* one launch per epoch (the generations between migrations):
+                          * one workgroup per island keeps its population on chip.
+                          * Needs single-component objectives with dim <= 128.      */
 };
 
 typedef struct {
@@ -321,9 +324,11 @@ SPEO_EXPORT int speo_engine_max_dim(speo_engine e);
 
 /* Runs `generations` generations eagerly with HIP events recorded on the
  * engine's stream around every launch and returns the average duration (ms)
- * of each per-generation kernel role: [0] generate, [1] evaluate, [2] select
- * (plain mode) and [3] fused generation (fused mode); unused roles are 0.
- * Migration follows the same schedule as speo_engine_run.  Synchronous. */
+ * of each kernel role per generation: [0] generate, [1] evaluate, [2] select
+ * (plain mode), [3] fused generation (fused mode) and [4] the epoch launch
+ * divided by its generations (epoch mode); unused roles are 0.  avg_ms_out
+ * holds 5 doubles.  Migration follows the same schedule as speo_engine_run.
+ * Synchronous. */
 SPEO_EXPORT int speo_engine_profile(speo_engine e, int generations, double *avg_ms_out);
 
 /* Per problem: the surrogate's last held-out Spearman rho (-1 before the

@@ -90,18 +90,25 @@ __host__ __device__ __forceinline__ uint32_t word(const U4 &v, uint32_t i) {
 }
 
 // Reflective bound (kernel_DE_CUDA.cu:160-170), capped at 64 reflections then
-// clamped; NaN passes through.
-__host__ __device__ __forceinline__ float check_bound(float v, float lo, float hi) {
+// clamped; NaN passes through.  The reflection loop sits behind the in-bound
+// test so that a wavefront whose lanes are all in bounds skips it.
+__host__ __device__ __forceinline__ float reflect_bound(float v, float lo, float hi) {
     int it = 0;
-    while ((v < lo) || (v > hi)) {
+#pragma clang loop unroll(disable)
+    do {
         if (v < lo) v = lo + (lo - v);
         if (v > hi) v = hi - (v - hi);
         if (++it >= 64) {
             v = v < lo ? lo : (v > hi ? hi : v);
             break;
         }
-    }
+    } while ((v < lo) || (v > hi));
     return v;
+}
+
+__host__ __device__ __forceinline__ float check_bound(float v, float lo, float hi) {
+    if (!((v < lo) || (v > hi))) return v;
+    return reflect_bound(v, lo, hi);
 }
 
 inline int next_pow2_dim(int dim) {

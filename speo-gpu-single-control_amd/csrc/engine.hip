@@ -27,7 +27,7 @@
 #include <tuple>
 #include <vector>
 
-#include "de.h"
+#include "engine.h"
 #include "eval.h"
 #include "eval_f64.h"
 #include "surrogate.h"
@@ -38,12 +38,6 @@ enum GenMode { kModeGen = 0, kModeEval = 1, kModeSelect = 2, kModeFused = 3 };
 
 struct TileDesc {
     int island, row0, nrows, pad;
-};
-
-struct FusedProblem {
-    int ncomp, dim;
-    float lambda[kMaxComp], bias[kMaxComp], sigma[kMaxComp];
-    CompDev comp[kMaxComp];
 };
 
 struct GenerationArgs {
@@ -374,6 +368,13 @@ struct Engine {
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
     bool graph_failed = false;
 
+    // epoch mode: per-island plans, block order, LDS of a launch
+    EpochPlan *plan_dev = nullptr;
+    int *order_dev = nullptr;
+    int epoch_lds = 0;
+    int epoch_setup();
+    int launch_epoch_chunk(int L, hipStream_t s);
+
     ~Engine();
     int create(const speo_engine_config &c, hipStream_t s);
     int upload_tables();
@@ -384,6 +385,8 @@ struct Engine {
     int migrate(hipStream_t s);
     int run(int generations);
     int profile(int generations, double *avg_ms);
+    int profile_epoch(int generations, double *avg_ms);
+    int next_chunk(int left, bool *mig) const;
     int autotune();
     int best(int *row_out, float *f32_out, double *f64_out);
     int read(float *Xo, float *fo);
@@ -397,6 +400,7 @@ Engine::~Engine() {
     for (auto &kv : tiles_dev) (void)hipFree(kv.second);
     void *bufs[] = {isl_dev, row_off_dev, dst_dev, best_buf, worst_buf, best_row_dev,
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
+                    plan_dev, order_dev,
                     X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
                     AX, Ay, Xb, tb, Xh, yh, predh};
     for (void *b : bufs) (void)hipFree(b);
@@ -466,7 +470,7 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     if (c.num_islands < 1 || !c.island_size || !c.island_problem || c.num_problems < 1 ||
         !c.problems)
         return set_error(SPEO_ERR_ARG, "speo_engine_create: islands and problems are required");
-    if (c.mode != SPEO_MODE_PLAIN && c.mode != SPEO_MODE_FUSED)
+    if (c.mode != SPEO_MODE_PLAIN && c.mode != SPEO_MODE_FUSED && c.mode != SPEO_MODE_EPOCH)
         return set_error(SPEO_ERR_ARG, "speo_engine_create: unknown mode");
     if (c.precision < SPEO_PREC_FP32 || c.precision > SPEO_PREC_BF16X6)
         return set_error(SPEO_ERR_ARG, "speo_engine_create: unknown precision");
@@ -551,6 +555,7 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
         cfg.mode = SPEO_MODE_PLAIN;  // the screened trial replaces generate; selection each generation
         cfg.use_graph = 0;
     }
+    if (cfg.mode == SPEO_MODE_EPOCH) SPEO_TRY(epoch_setup());
     rt = c.rows_per_block;
     if (rt == 0) SPEO_TRY(autotune());
     if (c.surrogate_candidates > 0) SPEO_TRY(surrogate_setup());
@@ -569,6 +574,52 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     fes = total_rows;
     SPEO_TRY_HIP(hipStreamSynchronize(stream));
     return SPEO_OK;
+}
+
+// Plans the epoch kernel: every island's LDS layout, and the block order
+// (largest islands first, so that they start first).
+int Engine::epoch_setup() {
+    std::vector<EpochPlan> plans((size_t)I);
+    for (int i = 0; i < I; ++i) {
+        const ProblemHost &P = problems[(size_t)isl[(size_t)i].problem]->h;
+        if (P.ncomp != 1 || P.dim > kSmallD)
+            return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode needs single-component objectives "
+                                                   "with dim <= 128");
+        if (isl[(size_t)i].np > 8192)
+            return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: islands of at most 8192 rows");
+        plans[(size_t)i] = epoch_plan(isl[(size_t)i].np, P.dim);
+        if (plans[(size_t)i].lds_bytes > kEpochLdsMax)
+            return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: island too large for one workgroup");
+        epoch_lds = std::max(epoch_lds, plans[(size_t)i].lds_bytes);
+    }
+    std::vector<int> order((size_t)I);
+    for (int i = 0; i < I; ++i) order[(size_t)i] = i;
+    auto cost = [&](int i) {
+        return (int64_t)isl[(size_t)i].np * ((isl[(size_t)i].dim + 15) / 16) * (isl[(size_t)i].dim + 3);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    SPEO_TRY(upload_vec(&plan_dev, plans));
+    SPEO_TRY(upload_vec(&order_dev, order));
+    return SPEO_OK;
+}
+
+int Engine::launch_epoch_chunk(int L, hipStream_t s) {
+    EpochArgs a{};
+    a.isl = isl_dev;
+    a.plan = plan_dev;
+    a.order = order_dev;
+    a.probs = probs_dev;
+    a.X = X;
+    a.fX = fX[0];
+    a.Xalt = U[1];
+    a.U = U[0];
+    a.fU = fU[0];
+    a.lo = cfg.min_bound;
+    a.hi = cfg.max_bound;
+    a.seed = cfg.seed;
+    a.gen_base = gen_base;
+    a.L = L;
+    return launch_epoch(cfg.precision, a, I, epoch_lds, s);
 }
 
 int Engine::surrogate_setup() {
@@ -736,22 +787,43 @@ int Engine::migrate(hipStream_t s) {
 // L generations, then (optionally) migration, then gen_base += L.  The parity
 // `p` and `pending` advance on the host as the launches are enqueued.
 int Engine::enqueue_chunk(int L, bool do_migrate, hipStream_t s) {
-    for (int g = 0; g < L; ++g) {
-        if (C > 0) {
-            SPEO_TRY(surrogate_generation(g, s));
-        } else if (cfg.mode == SPEO_MODE_FUSED) {
-            SPEO_TRY(launch(kModeFused, g, rt, s));
-            p = 1 - p;
-            pending = true;
-        } else {
-            SPEO_TRY(launch(kModeGen, g, rt, s));
-            SPEO_TRY(launch(kModeEval, g, rt, s));
-            SPEO_TRY(launch(kModeSelect, g, rt, s));
+    if (cfg.mode == SPEO_MODE_EPOCH) {
+        SPEO_TRY(launch_epoch_chunk(L, s));
+    } else {
+        for (int g = 0; g < L; ++g) {
+            if (C > 0) {
+                SPEO_TRY(surrogate_generation(g, s));
+            } else if (cfg.mode == SPEO_MODE_FUSED) {
+                SPEO_TRY(launch(kModeFused, g, rt, s));
+                p = 1 - p;
+                pending = true;
+            } else {
+                SPEO_TRY(launch(kModeGen, g, rt, s));
+                SPEO_TRY(launch(kModeEval, g, rt, s));
+                SPEO_TRY(launch(kModeSelect, g, rt, s));
+            }
         }
     }
     if (do_migrate) SPEO_TRY(migrate(s));
     hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, s, gen_base, (uint32_t)L);
     return check_launch("k_add_u32");
+}
+
+// Generations are counted from 0; migration follows generation g when
+// g % interval == 0 (island_EA.cc:93-99).  Returns the generations up to and
+// including the next migration point, or `left` when it lies beyond.
+int Engine::next_chunk(int left, bool *mig) const {
+    *mig = false;
+    if (cfg.interval > 0 && cfg.migration_size > 0) {
+        const int64_t g0 = gens;
+        const int64_t next = (g0 % cfg.interval == 0) ? g0 : (g0 / cfg.interval + 1) * cfg.interval;
+        const int64_t to_mig = next - g0 + 1;
+        if (to_mig <= left) {
+            *mig = true;
+            return (int)to_mig;
+        }
+    }
+    return left;
 }
 
 int Engine::run(int generations) {
@@ -768,19 +840,8 @@ int Engine::run(int generations) {
     }
     int left = generations;
     while (left > 0) {
-        // Generations are counted from 0; migration follows generation g when
-        // g % interval == 0 (island_EA.cc:93-99).
-        int L = left;
         bool mig = false;
-        if (cfg.interval > 0 && cfg.migration_size > 0) {
-            const int64_t g0 = gens;
-            const int64_t next = (g0 % cfg.interval == 0) ? g0 : (g0 / cfg.interval + 1) * cfg.interval;
-            const int64_t to_mig = next - g0 + 1;  // generations up to and including `next`
-            if (to_mig <= left) {
-                L = (int)to_mig;
-                mig = true;
-            }
-        }
+        const int L = next_chunk(left, &mig);
         const bool use_graph = cfg.use_graph && !graph_failed;
         if (use_graph) {
             const auto key = std::make_tuple(L, cfg.mode == SPEO_MODE_FUSED ? p : 0,
@@ -828,7 +889,8 @@ int Engine::run(int generations) {
 }
 
 int Engine::profile(int generations, double *avg_ms) {
-    const int roles = 4;
+    if (cfg.mode == SPEO_MODE_EPOCH) return profile_epoch(generations, avg_ms);
+    const int roles = 5;
     std::vector<hipEvent_t> ev;
     std::vector<int> role;
     auto record = [&](int r) -> int {
@@ -867,8 +929,8 @@ int Engine::profile(int generations, double *avg_ms) {
         if (mig) ++migrations;
     }
     SPEO_TRY_HIP(hipStreamSynchronize(stream));
-    double sum[roles] = {0, 0, 0, 0};
-    int cnt[roles] = {0, 0, 0, 0};
+    double sum[roles] = {0, 0, 0, 0, 0};
+    int cnt[roles] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < role.size(); ++i) {
         float ms = 0.f;
         SPEO_TRY_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
@@ -878,6 +940,42 @@ int Engine::profile(int generations, double *avg_ms) {
     for (auto e : ev) (void)hipEventDestroy(e);
     for (int r = 0; r < roles; ++r) avg_ms[r] = cnt[r] ? sum[r] / cnt[r] : 0.0;
     return SPEO_OK;
+}
+
+// Epoch mode: epochs on the run schedule, HIP events around every epoch
+// launch; role 4 is the launch duration per generation.
+int Engine::profile_epoch(int generations, double *avg_ms) {
+    for (int r = 0; r < 5; ++r) avg_ms[r] = 0.0;
+    hipEvent_t e0, e1;
+    SPEO_TRY_HIP(hipEventCreate(&e0));
+    SPEO_TRY_HIP(hipEventCreate(&e1));
+    double total_ms = 0.0;
+    int64_t total_gens = 0;
+    int left = generations, rc = SPEO_OK;
+    while (left > 0 && rc == SPEO_OK) {
+        bool mig = false;
+        const int L = next_chunk(left, &mig);
+        float ms = 0.f;
+        if ((rc = check_hip(hipEventRecord(e0, stream), "hipEventRecord")) != SPEO_OK) break;
+        if ((rc = launch_epoch_chunk(L, stream)) != SPEO_OK) break;
+        if ((rc = check_hip(hipEventRecord(e1, stream), "hipEventRecord")) != SPEO_OK) break;
+        if (mig && (rc = migrate(stream)) != SPEO_OK) break;
+        hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, stream, gen_base, (uint32_t)L);
+        if ((rc = check_launch("k_add_u32")) != SPEO_OK) break;
+        if ((rc = check_hip(hipEventSynchronize(e1), "hipEventSynchronize")) != SPEO_OK) break;
+        if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != SPEO_OK) break;
+        total_ms += ms;
+        total_gens += L;
+        gens += L;
+        fes += (int64_t)L * total_rows;
+        if (mig) ++migrations;
+        left -= L;
+    }
+    if (rc == SPEO_OK) rc = check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (total_gens > 0) avg_ms[4] = total_ms / (double)total_gens;
+    return rc;
 }
 
 // Times the generation kernel with each row tile on scratch copies of the

@@ -1,0 +1,459 @@
+// epoch.hip -- the epoch kernel: one workgroup per island runs all the
+// generations between two migrations without leaving the chip.
+//
+// Reference counterpart: the generation loop of IslandEA::Execute
+// (src/island_EA.cc:84-92) around DE_CUDA::Run (CUDA_EA_library/DE_CUDA.cc:
+// 200-216), whose three launches per generation (generate, evaluate, select)
+// and their HBM round trips become phases of one workgroup separated by
+// barriers.  Per generation and island:
+//   1. donors: one lane per row draws the five donors and j_rand
+//      (kernel_DE_CUDA.cu:187-222) into LDS;
+//   2. trials: (row, 4-coordinate group) items, DE/rand/1/bin with the
+//      crossover draws of Philox block {group, row, generation, 1}
+//      (kernel_DE_CUDA.cu:139-158), written to the row's free slot;
+//   3. evaluation: z = rate M (x - o) on MFMA with the rotation's B fragments
+//      held in registers for the whole epoch, per-coordinate terms reduced by
+//      wave shuffles (kernel_CEC2014_CUDA.cu:147-181, 254-286, 389-442,
+//      557-587);
+//   4. selection: strict "<" (kernel_DE_CUDA.cu:124-136) flips the row's slot
+//      bit.
+// The arithmetic of every step is the per-generation kernels' (the column
+// split of the rotation over four waves and the order of every sum are the
+// same), so an epoch launch reproduces the plain engine bit for bit.
+#pragma clang fp contract(off)
+
+#include <mutex>
+
+#include "engine.h"
+
+namespace speo {
+
+namespace {
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+__device__ __forceinline__ float comp4(const float4 &v, int q) {
+    return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
+}
+
+constexpr int kFixedLds = 128 * 4 + 128 * 4;  // shift vector, zero pad after the rows
+constexpr int kPassRowBytes = 16 + 8 + 4 + 32;  // donor offsets, j_rand + trial slot, A row, partials
+
+int lds_row_stride(int dim) {
+    int S = round_up(dim, 4);
+    if (S % 32 == 0) S += 4;  // consecutive rows start in different LDS banks
+    return S;
+}
+
+}  // namespace
+
+EpochPlan epoch_plan(int np, int dim) {
+    EpochPlan p{};
+    p.S = lds_row_stride(dim);
+    const int NPp = round_up(np, 16);
+    const int per_np = NPp * 8 + round_up(NPp, 16) + kFixedLds;  // fX, fU, flags, shift, pad
+    // Donors are drawn for up to four generations at once (all lanes busy).
+    for (int ga = std::max(1, std::min(4, kEpochThreads / NPp)); ga >= 1; --ga) {
+        const int fixed = per_np + ga * NPp * 16;
+        const int in_lds = fixed + NPp * kPassRowBytes + 2 * NPp * p.S * 4;
+        if (in_lds <= kEpochLdsMax) {
+            p.in_lds = 1;
+            p.pass_rows = NPp;
+            p.gens_ahead = ga;
+            p.lds_bytes = in_lds;
+            return p;
+        }
+    }
+    const int fixed = per_np + NPp * 16;
+    int pr = (kEpochLdsMax - fixed) / (kPassRowBytes + 4 * p.S);
+    pr = std::min(pr / 16 * 16, NPp);
+    p.in_lds = 0;
+    p.pass_rows = pr;
+    p.gens_ahead = 1;
+    p.lds_bytes = pr >= 16 ? fixed + pr * (kPassRowBytes + 4 * p.S) : kEpochLdsMax + 1;
+    return p;
+}
+
+// Per-coordinate terms of one 16-row tile's two column tiles, summed over the
+// 16 lanes of each row (gemm_small's epilogue with the objective fixed).
+template <int OBJ>
+__device__ __forceinline__ void tile_terms(const floatx4 &acc0, const floatx4 &acc1, bool c0, bool c1,
+                                           float w0, float w1, float unscale, float (&s1)[4],
+                                           float (&s2)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        s1[q] = 0.f;
+        s2[q] = 0.f;
+    }
+    if (c0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc0[q] * unscale, w0, s1[q], s2[q]);
+    }
+    if (c1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc1[q] * unscale, w1, s1[q], s2[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        s1[q] = sum16(s1[q]);
+        if (OBJ == SPEO_OBJ_ACKLEY) s2[q] = sum16(s2[q]);
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    constexpr int P = Prec<PREC>::parts;
+    constexpr bool H = Prec<PREC>::half;
+    const int island = a.order[blockIdx.x];
+    const IslandDesc I = a.isl[island];
+    const EpochPlan pl = a.plan[island];
+    const FusedProblem *prob = a.probs + I.problem;
+    const CompDev c = prob->comp[0];
+    const float lambda = prob->lambda[0], bias = prob->bias[0];
+    const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, PR = pl.pass_rows;
+    const int GA = pl.gens_ahead;
+    const int ng = (D + 3) >> 2;                        // float4 groups per row
+    // it / ng as umulhi(it, ceil(2^32 / ng)): exact for it < 2^32 / ng (it < 2^18 here)
+    const uint32_t inv_ng = ng > 1 ? (uint32_t)((0x100000000ull + ng - 1) / ng) : 0u;
+    auto div_ng = [&](int it) -> int { return ng > 1 ? (int)__umulhi((uint32_t)it, inv_ng) : it; };
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    const int wq = wave & 3, grp = wave >> 2, lr = lane & 15, lk = lane >> 4;
+    const bool in_lds = pl.in_lds != 0;
+
+    // LDS: fX[NPp] fU[NPp] flags[NPp] shift[128] donors[GA][NPp] doff[PR] tinfo[PR]
+    //      aoff[PR] part[8][PR] rows (+128 zero floats)
+    float *fXs = reinterpret_cast<float *>(lds);
+    float *fUs = fXs + NPp;
+    uint8_t *flags = reinterpret_cast<uint8_t *>(fUs + NPp);  // bit 0: current slot, bit 1: last trial's
+    float *sh = reinterpret_cast<float *>(flags + ((NPp + 15) & ~15));
+    int4 *dnq = reinterpret_cast<int4 *>(sh + 128);      // r0, r1, r2, j_rand
+    int4 *doff = dnq + GA * NPp;                           // rows of e0, e1, e2 and the target
+    int2 *tinfo = reinterpret_cast<int2 *>(doff + PR);     // j_rand, trial slot row
+    int *aoff = reinterpret_cast<int *>(tinfo + PR);       // LDS row of the trial to evaluate
+    float *part = reinterpret_cast<float *>(aoff + PR);
+    float *rows = part + 8 * PR;  // LDS plan: slots [2][NPp][S]; else the pass's trials [PR][S]
+    const int nrows_f = (in_lds ? 2 * NPp : PR) * S + 128;
+
+    // Row addresses: in LDS a float offset into `rows`; in HBM 2 * (r * ld) + slot.
+    float *gX = a.X + I.elem_off, *gA = a.Xalt + I.elem_off;
+    auto row_code = [&](int slot, int r) -> int {
+        return in_lds ? (slot * NPp + r) * S : 2 * r * I.ld + slot;
+    };
+    auto grow = [&](int code) -> float * { return ((code & 1) ? gA : gX) + (code >> 1); };
+
+    for (int k = tid; k < 128; k += kEpochThreads) sh[k] = k < D ? c.shift[k] : 0.f;
+    for (int r = tid; r < NPp; r += kEpochThreads) {
+        fXs[r] = r < NP ? a.fX[I.row_off + r] : INFINITY;
+        flags[r] = 0;
+    }
+    // zero rows (A reads run up to 127 floats past a row start), then slot 0
+    for (int e = tid; e < nrows_f; e += kEpochThreads) rows[e] = 0.f;
+    __syncthreads();
+    if (in_lds) {
+        for (int it = tid; it < NP * ng; it += kEpochThreads) {
+            const int r = div_ng(it), q4 = it - r * ng;
+            float4 v = ld4(gX + (size_t)r * I.ld + 4 * q4);
+            const int j = 4 * q4;
+            if (j + 1 >= D) v.y = 0.f;
+            if (j + 2 >= D) v.z = 0.f;
+            if (j + 3 >= D) v.w = 0.f;
+            st4(rows + (size_t)r * S + 4 * q4, v);
+        }
+    }
+
+    // The rotation's fragments of this wave's column tiles t0, t0 + 4, held
+    // for the whole epoch (the four waves of a group split the columns).
+    const int t0 = wq, t1 = wq + 4;
+    const bool v0 = t0 * 16 < D, v1 = t1 * 16 < D;
+    const bool c0 = t0 * 16 + lr < D, c1 = t1 * 16 + lr < D;
+    const bool ell = c.objective == SPEO_OBJ_ELLIPTIC;
+    const float w0 = (ell && c0) ? c.colw[t0 * 16 + lr] : 1.0f;
+    const float w1 = (ell && c1) ? c.colw[t1 * 16 + lr] : 1.0f;
+    const int KS = P == 0 ? ng : (c.kp32 >> 5);
+    typedef typename std::conditional<H, halfx8, bf16x8>::type vec8;
+    typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+    constexpr int NB32 = P == 0 ? 32 : 1;
+    constexpr int NB16 = P == 0 ? 1 : P;
+    float b0[NB32], b1[NB32], shv[NB32];
+    vec8 bb[2][4][NB16];
+    if constexpr (P == 0) {
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks) {
+            b0[ks] = 0.f;
+            b1[ks] = 0.f;
+            shv[ks] = ks * 4 + lk < D ? c.shift[ks * 4 + lk] : 0.f;
+            if (ks < KS) {
+                const float *bp = c.B32s + (size_t)(ks * 4 + lk) * kSmallD + lr;
+                if (v0) b0[ks] = bp[t0 * 16];
+                if (v1) b1[ks] = bp[t1 * 16];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const bool ok = (t == 0 ? v0 : v1) && ks < KS;
+                    bb[t][ks][p] = ok ? *reinterpret_cast<const vec8 *>(
+                                            c.M16s[p] + (size_t)((t == 0 ? t0 : t1) * 16 + lr) * c.kp32 +
+                                            ks * 32 + lk * 8)
+                                      : vec8{};
+                }
+    }
+    const uint32_t gen0 = *a.gen_base;
+    __syncthreads();
+
+    for (int g = 0; g < a.L; ++g) {
+        const uint32_t gen = gen0 + (uint32_t)g;
+        // 1. donors and j_rand of the next GA generations, one (generation, row) per lane
+        if (g % GA == 0) {
+            for (int it = tid; it < GA * NP; it += kEpochThreads) {
+                const int gi = it / NP, r = it - gi * NP;
+                if (g + gi < a.L) {
+                    DonorStream ds(a.seed, (uint32_t)(I.row_off + r), gen + (uint32_t)gi);
+                    int rr[kMaxR], jr;
+                    draw_donors(ds, nullptr, 0, NP, D, rr, jr);
+                    dnq[gi * NPp + r] = int4{rr[0], rr[1], rr[2], jr};
+                }
+            }
+            __syncthreads();
+        }
+        const int4 *dn = dnq + (g % GA) * NPp;
+        for (int p0 = 0; p0 < NP; p0 += PR) {
+            const int nr = min(PR, NP - p0);
+            // 2. the pass's rows: current slots of the donors and the target, trial slot
+            for (int rl = tid; rl < nr; rl += kEpochThreads) {
+                const int r = p0 + rl;
+                const int4 d = dn[r];
+                const int cur = flags[r] & 1;
+                doff[rl] = int4{row_code(flags[d.x] & 1, d.x), row_code(flags[d.y] & 1, d.y),
+                                row_code(flags[d.z] & 1, d.z), row_code(cur, r)};
+                tinfo[rl] = int2{d.w, row_code(cur ^ 1, r)};
+                aoff[rl] = in_lds ? row_code(cur ^ 1, r) : rl * S;
+            }
+            __syncthreads();
+            // 3. trials (the LDS plan and the HBM plan as separate loops, so that
+            // each compiles to plain LDS or global accesses)
+            auto trials = [&](auto row_of, auto store) {
+                for (int it = tid; it < nr * ng; it += kEpochThreads) {
+                    const int rl = div_ng(it), q4 = it - rl * ng;
+                    const int4 o = doff[rl];
+                    const int2 ti = tinfo[rl];
+                    const U4 w = philox_block(a.seed, (uint32_t)q4, (uint32_t)(I.row_off + p0 + rl), gen,
+                                              kPurposeCross);
+                    const float4 e0 = ld4(row_of(o.x) + 4 * q4);
+                    const float4 e1 = ld4(row_of(o.y) + 4 * q4);
+                    const float4 e2 = ld4(row_of(o.z) + 4 * q4);
+                    const float4 xo = ld4(row_of(o.w) + 4 * q4);
+                    float v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int j = 4 * q4 + q;
+                        if (j < D) {
+                            const bool take = (j == ti.x) || (u01(word(w, q)) < I.CR);
+                            const float m = take ? fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q))
+                                                 : comp4(xo, q);
+                            v[q] = check_bound(m, a.lo, a.hi);
+                        } else {
+                            v[q] = 0.f;
+                        }
+                    }
+                    store(rl, ti.y, q4, float4{v[0], v[1], v[2], v[3]});
+                }
+            };
+            if (in_lds) {
+                trials([&](int code) -> const float * { return rows + code; },
+                       [&](int, int code, int q4, float4 v) { st4(rows + code + 4 * q4, v); });
+            } else {
+                trials([&](int code) -> const float * { return grow(code); },
+                       [&](int rl, int code, int q4, float4 v) {
+                           st4(rows + (size_t)rl * S + 4 * q4, v);
+                           st4(grow(code) + 4 * q4, v);
+                       });
+            }
+            __syncthreads();
+            // 4. evaluation of the pass's trials, 16-row tiles, two tiles at a time
+            const int ntile = (nr + 15) >> 4;
+            for (int tile = grp; tile < ntile; tile += 2) {
+                const float *arow = rows + aoff[min(tile * 16 + lr, nr - 1)];
+                floatx4 acc0 = floatx4{0.f, 0.f, 0.f, 0.f}, acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+                if constexpr (P == 0) {
+                    // k steps in blocks of four (A read at fixed offsets; the
+                    // steps of the last block past KS have B = 0 in every lane
+                    // and add exact zeros).
+                    const float rate = c.rate;
+                    const float *ap = arow + lk;
+                    float av[32];
+#pragma unroll
+                    for (int kb = 0; kb < 32; kb += 4) {
+                        if (kb >= KS) break;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) av[kb + u] = (ap[(kb + u) * 4] - shv[kb + u]) * rate;
+                    }
+                    if (v1) {
+#pragma unroll
+                        for (int kb = 0; kb < 32; kb += 4) {
+                            if (kb >= KS) break;
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kb + u], b0[kb + u], acc0, 0, 0, 0);
+                                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kb + u], b1[kb + u], acc1, 0, 0, 0);
+                            }
+                        }
+                    } else if (v0) {
+#pragma unroll
+                        for (int kb = 0; kb < 32; kb += 4) {
+                            if (kb >= KS) break;
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kb + u], b0[kb + u], acc0, 0, 0, 0);
+                        }
+                    }
+                } else {
+                    const float scale = c.rate * (H ? kF16ScaleA : 1.0f);
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        if (ks < KS) {
+                            const int kb = ks * 32 + lk * 8;
+                            float xv[8];
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                const float4 x4 = ld4(arow + kb + 4 * h);
+                                xv[4 * h + 0] = x4.x;
+                                xv[4 * h + 1] = x4.y;
+                                xv[4 * h + 2] = x4.z;
+                                xv[4 * h + 3] = x4.w;
+                            }
+                            u16x8 pb[NB16];
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const int k = kb + e;
+                                float rem = k < D ? (xv[e] - sh[k]) * scale : 0.f;
+#pragma unroll
+                                for (int p = 0; p < P; ++p) {
+                                    const uint16_t bits = to_bits(rem, H);
+                                    pb[p][e] = bits;
+                                    rem -= from_bits(bits, H);
+                                }
+                            }
+                            vec8 av[NB16];
+#pragma unroll
+                            for (int p = 0; p < P; ++p) av[p] = __builtin_bit_cast(vec8, pb[p]);
+#pragma unroll
+                            for (int t = 0; t < 2; ++t) {
+                                if (t == 0 ? !v0 : !v1) continue;
+                                floatx4 &acc = t == 0 ? acc0 : acc1;
+#pragma unroll
+                                for (int sum = P - 1; sum >= 0; --sum) {
+#pragma unroll
+                                    for (int pa = 0; pa <= sum; ++pa) {
+                                        const int pbi = sum - pa;
+                                        if (pa < P && pbi < P) {
+                                            if constexpr (H)
+                                                acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                                    av[pa], bb[t][ks][pbi], acc, 0, 0, 0);
+                                            else
+                                                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                                    av[pa], bb[t][ks][pbi], acc, 0, 0, 0);
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                const float unscale = H ? kF16Unscale : 1.0f;
+                float s1[4], s2[4];
+                switch (c.objective) {
+                case SPEO_OBJ_ELLIPTIC:
+                    tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, acc1, c0, c1, w0, w1, unscale, s1, s2);
+                    break;
+                case SPEO_OBJ_RASTRIGIN:
+                    tile_terms<SPEO_OBJ_RASTRIGIN>(acc0, acc1, c0, c1, w0, w1, unscale, s1, s2);
+                    break;
+                default:
+                    tile_terms<SPEO_OBJ_ACKLEY>(acc0, acc1, c0, c1, w0, w1, unscale, s1, s2);
+                    break;
+                }
+                if (lr == 0) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int rl = tile * 16 + lk * 4 + q;
+                        part[wq * PR + rl] = s1[q];
+                        part[(4 + wq) * PR + rl] = s2[q];
+                    }
+                }
+            }
+            __syncthreads();
+            // the four column groups' partial sums, in column-group order
+            for (int rl = tid; rl < nr; rl += kEpochThreads) {
+                float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    s1 += part[w * PR + rl];
+                    s2 += part[(4 + w) * PR + rl];
+                }
+                fUs[p0 + rl] = fmaf(lambda, finish_objective(c.objective, s1, s2, D), bias);
+            }
+            __syncthreads();
+        }
+        // 5. selection: an accepted trial's slot becomes the row's current one
+        for (int r = tid; r < NP; r += kEpochThreads) {
+            const float fu = fUs[r];
+            const int cur = flags[r] & 1, ts = cur ^ 1;
+            const bool acc = fu < fXs[r];
+            if (acc) fXs[r] = fu;
+            flags[r] = (uint8_t)((acc ? ts : cur) | (ts << 1));
+        }
+        __syncthreads();
+    }
+
+    // population and the last generation's trials back to HBM
+    float *gU = a.U + I.elem_off;
+    for (int it = tid; it < NP * ng; it += kEpochThreads) {
+        const int r = div_ng(it), q4 = it - r * ng;
+        const int fl = flags[r];
+        const int cc = row_code(fl & 1, r), tc = row_code((fl >> 1) & 1, r);
+        const float4 cur = ld4((in_lds ? rows + cc : grow(cc)) + 4 * q4);
+        const float4 tri = ld4((in_lds ? rows + tc : grow(tc)) + 4 * q4);
+        st4(gX + (size_t)r * I.ld + 4 * q4, cur);
+        st4(gU + (size_t)r * I.ld + 4 * q4, tri);
+    }
+    for (int r = tid; r < NP; r += kEpochThreads) {
+        a.fX[I.row_off + r] = fXs[r];
+        a.fU[I.row_off + r] = fUs[r];
+    }
+}
+
+template <int PREC>
+static int launch_epoch_t(const EpochArgs &a, int nblocks, int lds, hipStream_t s) {
+    static std::once_flag once;
+    static hipError_t attr = hipSuccess;
+    std::call_once(once, [] {
+        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_epoch<PREC>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kEpochLdsMax);
+    });
+    if (attr != hipSuccess) return check_hip(attr, "hipFuncSetAttribute(k_island_epoch)");
+    hipLaunchKernelGGL((k_island_epoch<PREC>), dim3(nblocks), dim3(kEpochThreads), lds, s, a);
+    return check_launch("k_island_epoch");
+}
+
+int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, hipStream_t s) {
+    if (a.L < 1) return set_error(SPEO_ERR_ARG, "epoch launch needs at least one generation");
+    if (lds_bytes > kEpochLdsMax) return set_error(SPEO_ERR_ARG, "epoch launch: LDS plan too large");
+    switch (precision) {
+    case SPEO_PREC_FP32: return launch_epoch_t<SPEO_PREC_FP32>(a, nblocks, lds_bytes, s);
+    case SPEO_PREC_BF16: return launch_epoch_t<SPEO_PREC_BF16>(a, nblocks, lds_bytes, s);
+    case SPEO_PREC_FP16: return launch_epoch_t<SPEO_PREC_FP16>(a, nblocks, lds_bytes, s);
+    case SPEO_PREC_BF16X3: return launch_epoch_t<SPEO_PREC_BF16X3>(a, nblocks, lds_bytes, s);
+    case SPEO_PREC_FP16X3: return launch_epoch_t<SPEO_PREC_FP16X3>(a, nblocks, lds_bytes, s);
+    case SPEO_PREC_BF16X6: return launch_epoch_t<SPEO_PREC_BF16X6>(a, nblocks, lds_bytes, s);
+    default: return set_error(SPEO_ERR_ARG, "unknown precision");
+    }
+}
+
+}  // namespace speo

@@ -103,12 +103,23 @@ __device__ __forceinline__ float from_bits(uint16_t v, bool half) {
     return (float)__builtin_bit_cast(__bf16, v);
 }
 
-// Sum over the 16 lanes holding one row (lanes differing in bits 0..3).
+// DPP lane permutation within rows of 16 lanes (GCN/CDNA DPP controls:
+// quad_perm 0x00-0xFF, row_mirror 0x140, row_half_mirror 0x141).
+template <int CTRL> __device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                                 0xF, 0xF, false));
+}
+
+// Sum over the 16 lanes holding one row (lanes differing in bits 0..3); every
+// lane of the row receives it.  The butterfly pairs lanes i^1, i^2 (quad
+// permutations), then i and 7-i within each half row, then i and 15-i: after
+// each step all lanes of the group hold one value, so every lane gets the
+// xor-butterfly sum ((v0+v1)+(v2+v3)) + ... bit for bit.
 __device__ __forceinline__ float sum16(float v) {
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
-    v += __shfl_xor(v, 8, 64);
+    v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp<0x141>(v);  // row_half_mirror
+    v += dpp<0x140>(v);  // row_mirror
     return v;
 }
 

@@ -21,7 +21,7 @@ from ctypes import POINTER, byref, c_double, c_float, c_int, c_void_p
 import numpy as np
 
 from ._lib import (DEInfo, EngineConfig, EngineStats, GPU_IslandInfo, GPU_Population,
-                   MODE_FUSED, MODE_PLAIN, OBJ_ACKLEY, OBJ_ELLIPTIC, OBJ_RASTRIGIN, PREC_BF16,
+                   MODE_EPOCH, MODE_FUSED, MODE_PLAIN, OBJ_ACKLEY, OBJ_ELLIPTIC, OBJ_RASTRIGIN, PREC_BF16,
                    PREC_BF16X3, PREC_BF16X6, PREC_FP16, PREC_FP16X3, PREC_FP32, PREC_NAMES,
                    ProblemInfo, SPEO_RandState, SpeoError, check, last_error, lib)
 
@@ -30,7 +30,7 @@ __all__ = [
     "de_generate", "de_select", "evaluate", "evaluate_f64", "segment_argmin", "SpeoError",
     "OBJ_ELLIPTIC", "OBJ_RASTRIGIN", "OBJ_ACKLEY", "PREC_FP32", "PREC_BF16", "PREC_FP16",
     "PREC_BF16X3", "PREC_FP16X3", "PREC_BF16X6", "PREC_NAMES", "MODE_PLAIN", "MODE_FUSED",
-    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
+    "MODE_EPOCH",    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
 ]
 
 
@@ -237,10 +237,11 @@ class Engine:
         check(lib().speo_engine_write(self.handle, _np_ptr(Xa), _np_ptr(fa)))
 
     def profile(self, generations: int):
-        """Average kernel ms per role: generate, evaluate, select, fused."""
-        out = np.zeros(4, np.float64)
+        """Average kernel ms per generation and role: generate, evaluate, select
+        (plain mode), fused (fused mode), epoch (epoch mode: launch / generations)."""
+        out = np.zeros(5, np.float64)
         check(lib().speo_engine_profile(self.handle, generations, _np_ptr(out)))
-        return dict(zip(("generate", "evaluate", "select", "fused"), out.tolist()))
+        return dict(zip(("generate", "evaluate", "select", "fused", "epoch"), out.tolist()))
 
     def surrogate_stats(self):
         n = len(self.problems)

@@ -21,7 +21,7 @@ OBJ_ELLIPTIC, OBJ_RASTRIGIN, OBJ_ACKLEY = 1, 2, 3
 PREC_FP32, PREC_BF16, PREC_FP16, PREC_BF16X3, PREC_FP16X3, PREC_BF16X6 = range(6)
 PREC_NAMES = {PREC_FP32: "fp32", PREC_BF16: "bf16", PREC_FP16: "fp16", PREC_BF16X3: "bf16x3",
               PREC_FP16X3: "fp16x3", PREC_BF16X6: "bf16x6"}
-MODE_PLAIN, MODE_FUSED = 0, 1
+MODE_PLAIN, MODE_FUSED, MODE_EPOCH = 0, 1, 2
 
 
 class SpeoError(RuntimeError):
