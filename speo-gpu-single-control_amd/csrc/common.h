@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -78,6 +79,19 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t
 // u32 -> float in (0,1]: ((x >> 8) + 1) * 2^-24, exact in FP32.
 __host__ __device__ __forceinline__ float u01(uint32_t x) {
     return (float)((x >> 8) + 1u) * 0x1p-24f;
+}
+
+// The crossover test u01(x) < CR as an integer compare: with n = x >> 8,
+// u01(x) = (n + 1) 2^-24 exactly, and for an integer m and real t,
+// m < t <=> m < ceil(t); so u01(x) < CR <=> n < ceil(CR 2^24) - 1.
+// NaN or CR <= 0 never takes (threshold 0), CR >= 1 + 2^-24 always does.
+__host__ __device__ inline uint32_t cr_threshold(float CR) {
+    if (!(CR > 0.0f)) return 0u;
+    const double t = ceil((double)CR * 16777216.0) - 1.0;
+    return t >= 16777216.0 ? 16777216u : (uint32_t)t;
+}
+__host__ __device__ __forceinline__ bool cross_take(uint32_t x, uint32_t threshold) {
+    return (x >> 8) < threshold;
 }
 
 __host__ __device__ __forceinline__ U4 philox_block(uint64_t seed, uint32_t block,

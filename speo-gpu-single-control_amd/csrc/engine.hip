@@ -115,6 +115,7 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
     if constexpr (MODE == kModeGen || MODE == kModeFused) {
         const uint32_t gen = (a.gen_base ? *a.gen_base : 0u) + a.gen_offset;
         const int ngroups = (I.dim + 3) >> 2;
+        const uint32_t crt = cr_threshold(I.CR);
         for (int pass = 0; pass < RT / 16; ++pass) {
             const int lr = pass * 16 + (tid >> 4), l = tid & 15;
             if (lr >= t.nrows) {
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
                 for (int q = 0; q < 4; ++q) {
                     const int j = 4 * g + q;
                     if (j < I.dim) {
-                        const bool take = (j == jr) || (u01(word(w, q)) < I.CR);
+                        const bool take = (j == jr) || cross_take(word(w, q), crt);
                         const float m = take ? fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q))
                                              : comp4(xo, q);
                         v[q] = check_bound(m, a.lo, a.hi);
@@ -709,7 +710,8 @@ int Engine::surrogate_train(hipStream_t s) {
             SPEO_TRY(gather_batch(ax, ay, n_valid, Kp, cfg.seed ^ (0xD1B54A32D192ED03ull * (p + 1)),
                                   (uint32_t)surr_rounds, (uint32_t)step, cfg.surrogate_batch, Xb,
                                   tb, s));
-            SPEO_TRY(sp.net.loss_grad(Xb, Kp, cfg.surrogate_batch, tb, s));
+            SPEO_TRY(sp.net.loss_grad(Xb, Kp, cfg.surrogate_batch, tb, s,
+                                      step == cfg.surrogate_steps - 1));
             SPEO_TRY(sp.net.adam(cfg.surrogate_lr, s));
         }
         SPEO_TRY(sp.net.refresh_bf16(s));

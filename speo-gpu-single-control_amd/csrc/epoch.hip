@@ -121,6 +121,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int wq = wave & 3, grp = wave >> 2, lr = lane & 15, lk = lane >> 4;
     const bool in_lds = pl.in_lds != 0;
+    const uint32_t crt = cr_threshold(I.CR);
 
     // LDS: fX[NPp] fU[NPp] flags[NPp] shift[128] donors[GA][NPp] doff[PR] tinfo[PR]
     //      aoff[PR] part[8][PR] rows (+128 zero floats)
@@ -238,31 +239,50 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
             __syncthreads();
             // 3. trials (the LDS plan and the HBM plan as separate loops, so that
             // each compiles to plain LDS or global accesses)
+            // Two items per lane and iteration: their Philox chains are
+            // independent, which doubles the instruction-level parallelism of
+            // the dominant integer work.
             auto trials = [&](auto row_of, auto store) {
-                for (int it = tid; it < nr * ng; it += kEpochThreads) {
+                const int n_items = nr * ng;
+                auto one = [&](int it, const U4 &w) {
                     const int rl = div_ng(it), q4 = it - rl * ng;
                     const int4 o = doff[rl];
                     const int2 ti = tinfo[rl];
-                    const U4 w = philox_block(a.seed, (uint32_t)q4, (uint32_t)(I.row_off + p0 + rl), gen,
-                                              kPurposeCross);
                     const float4 e0 = ld4(row_of(o.x) + 4 * q4);
                     const float4 e1 = ld4(row_of(o.y) + 4 * q4);
                     const float4 e2 = ld4(row_of(o.z) + 4 * q4);
                     const float4 xo = ld4(row_of(o.w) + 4 * q4);
+                    // the four coordinates branch-free; the reflective bound
+                    // only when a lane holds a coordinate out of bounds
                     float v[4];
+                    bool out = false;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int j = 4 * q4 + q;
-                        if (j < D) {
-                            const bool take = (j == ti.x) || (u01(word(w, q)) < I.CR);
-                            const float m = take ? fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q))
-                                                 : comp4(xo, q);
-                            v[q] = check_bound(m, a.lo, a.hi);
-                        } else {
-                            v[q] = 0.f;
-                        }
+                        const bool take = (j == ti.x) || cross_take(word(w, q), crt);
+                        const float mt = fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q));
+                        v[q] = j < D ? (take ? mt : comp4(xo, q)) : 0.f;
+                        out |= (j < D) && ((v[q] < a.lo) || (v[q] > a.hi));
+                    }
+                    if (out) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (4 * q4 + q < D) v[q] = check_bound(v[q], a.lo, a.hi);
                     }
                     store(rl, ti.y, q4, float4{v[0], v[1], v[2], v[3]});
+                };
+                auto draw = [&](int it) {
+                    const int rl = div_ng(it), q4 = it - rl * ng;
+                    return philox_block(a.seed, (uint32_t)q4, (uint32_t)(I.row_off + p0 + rl), gen,
+                                        kPurposeCross);
+                };
+                for (int it = tid; it < n_items; it += 2 * kEpochThreads) {
+                    const int it2 = it + kEpochThreads;
+                    const bool two = it2 < n_items;
+                    const U4 w1 = draw(it);
+                    const U4 w2 = draw(two ? it2 : it);
+                    one(it, w1);
+                    if (two) one(it2, w2);
                 }
             };
             if (in_lds) {

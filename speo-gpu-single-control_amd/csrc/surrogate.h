@@ -13,6 +13,8 @@ constexpr float kXScale = 0.01f;  // inputs in [-100, 100] -> [-1, 1]
 
 // Flat parameter layout (floats): W1 [256][Kp], b1 [256], W2 [256][256],
 // b2 [256], w3 [256], b3 [4 (1 used)].
+constexpr int kMaxSplitK = 8;
+
 struct MlpLayout {
     int D = 0, Kp = 0;
     size_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, ow3 = 0, ob3 = 0, n = 0;
@@ -39,6 +41,8 @@ struct Mlp {
     float *y = nullptr, *dy = nullptr;                                 // [max_batch]
     float *stats = nullptr;  // [0] loss, [1] target mean, [2] target std, [3] rho
     uint16_t *W1b = nullptr, *W2b = nullptr;  // BF16 copies for the fused inference kernel
+    float *part = nullptr;                    // split-K partial products of the weight gradients
+    float *lterm = nullptr;                   // [max_batch] per-row loss terms
     int64_t adam_t = 0;
     ~Mlp();
     int alloc(int dim, int batch);
@@ -47,7 +51,8 @@ struct Mlp {
     // the workspace; pred receives n values.
     int forward(const float *X, int ld, int n, float *pred, hipStream_t s);
     // Loss (0.5 mean (y - t)^2) and gradients into G for n <= max_batch rows.
-    int loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s);
+    int loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s,
+                  bool want_loss = true);
     int adam(float lr, hipStream_t s);
     // Refreshes the BF16 weight copies from the FP32 master weights.
     int refresh_bf16(hipStream_t s);

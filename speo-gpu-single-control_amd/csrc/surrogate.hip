@@ -27,7 +27,16 @@ struct GemmArgs {
     float *P;           // pre-activations: written by kEpiBiasSilu, read by kEpiMulDsilu
     long p_m;
     int epi;
+    int kc;             // K per split (blockIdx.z); C of split z at C + z * c_split
+    long c_split;
+    int vec_a, vec_b;   // 16-byte loads along the unit-stride index are aligned
 };
+
+// Whether 16-byte loads along the unit-stride index of a matrix are aligned.
+static int vec_ok(const float *p, long s_row, long s_k) {
+    const long other = s_k == 1 ? s_row : s_k;
+    return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && (other % 4 == 0) && (s_row == 1 || s_k == 1);
+}
 
 __device__ __forceinline__ float sigmoidf_(float a) { return 1.0f / (1.0f + expf(-a)); }
 __device__ __forceinline__ float siluf_(float a) { return a * sigmoidf_(a); }
@@ -37,47 +46,81 @@ __device__ __forceinline__ float dsiluf_(float a) {
 }
 
 // 64 x 64 tile, 4 waves as 2 x 2, each wave 32 x 32 = 2 x 2 MFMA 16x16 tiles,
-// K staged 32 at a time.
+// K staged 128 at a time (one or two stages for the MLP's K <= 256, so the
+// global-load latency is paid once or twice per tile); split z of the K range [z kc, (z + 1) kc) writes
+// its own C.  Operand tiles are staged with the unit-stride index fastest,
+// so the global reads coalesce for either layout.
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
-    constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+    constexpr int TM = 64, TN = 64, TK = 128, LDK = TK + 8;
     __shared__ __align__(16) uint16_t As[TM * LDK];
     __shared__ __align__(16) uint16_t Bs[TN * LDK];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1, lr = lane & 15, lk = lane >> 4;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kbeg = blockIdx.z * g.kc, kend = min(g.K, kbeg + g.kc);
+    float *C = g.C + (long)blockIdx.z * g.c_split;
+    const bool a_kfast = g.a_k == 1, b_kfast = g.b_k == 1;
     floatx4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += TK) {
+    // One operand tile: `rows` x TK (K fast) or TK x `rows` (rows fast) from a
+    // global matrix whose unit-stride index is the fast one; 16-byte loads
+    // when the tile is aligned (vec), element loads at the matrix edges.
+    auto stage = [&](const float *src, long s_row, long s_k, bool kfast, bool vec, int row0,
+                     int nrows_total, int k0, float scale, uint16_t *dst) {
+        constexpr int NV = TM * TK / 4;  // float4 groups per tile
+#pragma unroll
+        for (int e4 = tid; e4 < NV; e4 += 256) {
+            int r, k;
+            if (kfast) {
+                r = e4 / (TK / 4);
+                k = (e4 % (TK / 4)) * 4;
+            } else {
+                k = e4 / (TM / 4);
+                r = (e4 % (TM / 4)) * 4;
+            }
+            float v[4];
+            const int gr = row0 + r, gk = k0 + k;
+            const bool full = kfast ? (gr < nrows_total && gk + 3 < kend)
+                                    : (gk < kend && gr + 3 < nrows_total);
+            if (vec && full) {
+                const float4 x = *reinterpret_cast<const float4 *>(src + (long)gr * s_row + (long)gk * s_k);
+                v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = kfast ? gr : gr + q, kk = kfast ? gk + q : gk;
+                    v[q] = (rr < nrows_total && kk < kend) ? src[(long)rr * s_row + (long)kk * s_k] : 0.f;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = kfast ? r : r + q, kk = kfast ? k + q : k;
+                dst[rr * LDK + kk] = to_bits(v[q] * scale, false);
+            }
+        }
+    };
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
         __syncthreads();
-        for (int e = tid; e < TM * TK; e += 256) {
-            const int r = e / TK, k = e % TK;
-            const int m = m0 + r, kk = k0 + k;
-            float v = 0.f;
-            if (m < g.M && kk < g.K) v = g.A[(long)m * g.a_m + (long)kk * g.a_k] * g.a_scale;
-            As[r * LDK + k] = to_bits(v, false);
-        }
-        for (int e = tid; e < TN * TK; e += 256) {
-            const int n = e / TK, k = e % TK;
-            const int nn = n0 + n, kk = k0 + k;
-            float v = 0.f;
-            if (nn < g.N && kk < g.K) v = g.B[(long)kk * g.b_k + (long)nn * g.b_n] * g.b_scale;
-            Bs[n * LDK + k] = to_bits(v, false);
-        }
+        stage(g.A, g.a_m, g.a_k, a_kfast, g.vec_a != 0, m0, g.M, k0, g.a_scale, As);
+        stage(g.B, g.b_n, g.b_k, b_kfast, g.vec_b != 0, n0, g.N, k0, g.b_scale, Bs);
         __syncthreads();
-        bf16x8 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * 32 + i * 16 + lr) * LDK + lk * 8]);
-            b[i] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * 32 + i * 16 + lr) * LDK + lk * 8]);
+        for (int ks = 0; ks < TK; ks += 32) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * 32 + i * 16 + lr) * LDK + ks + lk * 8]);
+                b[i] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * 32 + i * 16 + lr) * LDK + ks + lk * 8]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -96,21 +139,59 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
                 } else if (g.epi == kEpiMulDsilu) {
                     v *= dsiluf_(g.P[(long)m * g.p_m + n]);
                 }
-                g.C[(long)m * g.c_m + n] = v;
+                C[(long)m * g.c_m + n] = v;
             }
 }
 
-static int gemm(const GemmArgs &g, hipStream_t s) {
-    if (g.M <= 0 || g.N <= 0) return SPEO_OK;
+// C[m][n] = sum over the splits of part[z][m][n], in split order.
+__global__ void k_splitk_reduce(const float *part, int splits, int M, int N, float *C, long c_m) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)M * N) return;
+    float v = 0.f;
+    for (int z = 0; z < splits; ++z) v += part[(long)z * M * N + i];
+    C[(i / N) * c_m + i % N] = v;
+}
+
+static int gemm(const GemmArgs &g0, hipStream_t s) {
+    if (g0.M <= 0 || g0.N <= 0) return SPEO_OK;
+    GemmArgs g = g0;
+    g.kc = g.K;
+    g.c_split = 0;
+    g.vec_a = vec_ok(g.A, g.a_m, g.a_k);
+    g.vec_b = vec_ok(g.B, g.b_n, g.b_k);
     dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
     hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
     return check_launch("k_gemm_bf16");
 }
 
+// K split into chunks of 128 (plain store epilogue only), partial products
+// reduced in a fixed order: the weight gradients, whose K is the batch.
+static int gemm_splitk(const GemmArgs &g0, float *part, hipStream_t s) {
+    if (g0.M <= 0 || g0.N <= 0) return SPEO_OK;
+    const int splits = std::min(kMaxSplitK, (g0.K + 127) / 128);
+    if (splits <= 1) return gemm(g0, s);
+    GemmArgs g = g0;
+    g.vec_a = vec_ok(g.A, g.a_m, g.a_k);
+    g.vec_b = vec_ok(g.B, g.b_n, g.b_k);
+    g.kc = (g.K + splits - 1) / splits;
+    g.kc = (g.kc + 31) / 32 * 32;
+    g.C = part;
+    g.c_m = g.N;
+    g.c_split = (long)g.M * g.N;
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, splits);
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
+    SPEO_TRY(check_launch("k_gemm_bf16"));
+    const long total = (long)g.M * g.N;
+    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, part,
+                       splits, g.M, g.N, g0.C, g0.c_m);
+    return check_launch("k_splitk_reduce");
+}
+
 // y[b] = h2[b] . w3 + b3 (one wave per row); optional dy = (y - t') / B with
-// t' = (t - mean) / std from stats[1..2], and the loss into stats[0].
+// t' = (t - mean) / std from stats[1..2], and the per-row loss terms
+// 0.5 (y - t')^2 / B into lterm (summed in a fixed order when the loss is wanted).
 __global__ void k_mlp_out(const float *H2, const float *w3, const float *b3, int n, float *y,
-                          const float *t, float *dy, float *stats) {
+                          const float *t, float *dy, const float *stats, float *lterm) {
     const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
@@ -123,7 +204,7 @@ __global__ void k_mlp_out(const float *H2, const float *w3, const float *b3, int
         if (t) {
             const float tt = (t[row] - stats[1]) / stats[2];
             dy[row] = (yy - tt) / (float)n;
-            atomicAdd(&stats[0], 0.5f * (yy - tt) * (yy - tt) / (float)n);
+            lterm[row] = 0.5f * (yy - tt) * (yy - tt) / (float)n;
         }
     }
 }
@@ -136,13 +217,29 @@ __global__ void k_mlp_dh2(const float *dy, const float *w3, const float *A2, int
     dH2[i] = dy[b] * w3[j] * dsiluf_(A2[i]);
 }
 
-// out[j] = sum_b M[b][j] (* w[b] if w): one thread per column, fixed order.
-__global__ void k_colsum(const float *Mx, const float *w, int n, int cols, float *out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cols) return;
+// out[j] = sum_b M[b][j] (* w[b] if w): 64 columns per block, 16 row groups
+// (rows g, g + 16, ...) summed in row order, then the groups in order.
+__global__ __launch_bounds__(1024) void k_colsum(const float *Mx, const float *w, int n, int cols,
+                                                float *out) {
+    __shared__ float red[16][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), grp = threadIdx.x >> 6;
     float acc = 0.f;
-    for (int b = 0; b < n; ++b) acc = fmaf(Mx[(long)b * cols + j], w ? w[b] : 1.0f, acc);
-    out[j] = acc;
+    if (c < cols) {
+#pragma unroll 8
+        for (int b = grp; b < n; b += 16) acc = fmaf(Mx[(long)b * cols + c], w ? w[b] : 1.0f, acc);
+    }
+    red[grp][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (grp == 0 && c < cols) {
+        float v = 0.f;
+        for (int q = 0; q < 16; ++q) v += red[q][threadIdx.x];
+        out[c] = v;
+    }
+}
+
+static int colsum(const float *Mx, const float *w, int n, int cols, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_colsum, dim3((cols + 63) / 64), dim3(1024), 0, s, Mx, w, n, cols, out);
+    return check_launch("k_colsum");
 }
 
 __global__ void k_sum(const float *v, int n, float *out) {
@@ -205,7 +302,7 @@ static int zero(float *p, size_t n, hipStream_t s) {
 // ------------------------------------------------------------------ Mlp
 
 Mlp::~Mlp() {
-    void *b[] = {P, G, m, v, A1, H1, A2, H2, dH1, dH2, y, dy, stats, W1b, W2b};
+    void *b[] = {P, G, m, v, A1, H1, A2, H2, dH1, dH2, y, dy, stats, W1b, W2b, part, lterm};
     for (void *p : b) (void)hipFree(p);
 }
 
@@ -217,7 +314,13 @@ int Mlp::alloc(int dim, int batch) {
         SPEO_TRY_HIP(hipMalloc(p, sizeof(float) * (size_t)batch * kHidden));
     SPEO_TRY_HIP(hipMalloc(&y, sizeof(float) * batch));
     SPEO_TRY_HIP(hipMalloc(&dy, sizeof(float) * batch));
+    SPEO_TRY_HIP(hipMalloc(&lterm, sizeof(float) * batch));
     SPEO_TRY_HIP(hipMalloc(&stats, sizeof(float) * 8));
+    SPEO_TRY_HIP(hipMemset(G, 0, sizeof(float) * L.n));
+    SPEO_TRY_HIP(hipMemset(stats, 0, sizeof(float) * 8));
+    const int splits = std::min(kMaxSplitK, (batch + 127) / 128);
+    SPEO_TRY_HIP(hipMalloc(&part, sizeof(float) * (size_t)std::max(splits, 1) * kHidden *
+                                      (size_t)std::max(kHidden, L.Kp)));
     return SPEO_OK;
 }
 
@@ -259,35 +362,38 @@ int Mlp::forward(const float *X, int ld, int n, float *pred, hipStream_t s) {
         SPEO_TRY(gemm(layer1(*this, X + (size_t)r0 * ld, ld, nb), s));
         SPEO_TRY(gemm(layer2(*this, nb), s));
         hipLaunchKernelGGL(k_mlp_out, dim3((nb + 3) / 4), dim3(256), 0, s, H2, P + L.ow3,
-                           P + L.ob3, nb, pred + r0, nullptr, nullptr, nullptr);
+                           P + L.ob3, nb, pred + r0, nullptr, nullptr, nullptr, nullptr);
         SPEO_TRY(check_launch("k_mlp_out"));
     }
     return SPEO_OK;
 }
 
-int Mlp::loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s) {
+int Mlp::loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s, bool want_loss) {
     if (n > max_batch) return set_error(SPEO_ERR_ARG, "mlp: batch larger than the workspace");
     SPEO_TRY(gemm(layer1(*this, X, ld, n), s));
     SPEO_TRY(gemm(layer2(*this, n), s));
-    SPEO_TRY(zero(stats, 1, s));
     hipLaunchKernelGGL(k_mlp_out, dim3((n + 3) / 4), dim3(256), 0, s, H2, P + L.ow3, P + L.ob3, n,
-                       y, t, dy, stats);
+                       y, t, dy, stats, lterm);
     SPEO_TRY(check_launch("k_mlp_out"));
+    if (want_loss) {
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, lterm, n, stats);
+        SPEO_TRY(check_launch("k_sum"));
+    }
     hipLaunchKernelGGL(k_mlp_dh2, dim3((unsigned)(((size_t)n * kHidden + 255) / 256)), dim3(256),
                        0, s, dy, P + L.ow3, A2, n, dH2);
     SPEO_TRY(check_launch("k_mlp_dh2"));
     // g_w3 = H2^T dy, g_b3 = sum dy, g_b2 = colsum dH2
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, H2, dy, n, kHidden, G + L.ow3);
+    SPEO_TRY(colsum(H2, dy, n, kHidden, G + L.ow3, s));
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, dy, n, G + L.ob3);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, dH2, nullptr, n, kHidden, G + L.ob2);
-    SPEO_TRY(check_launch("k_colsum"));
+    SPEO_TRY(check_launch("k_sum"));
+    SPEO_TRY(colsum(dH2, nullptr, n, kHidden, G + L.ob2, s));
     // g_W2 = dH2^T H1
     GemmArgs g{};
     g.M = kHidden; g.N = kHidden; g.K = n;
     g.A = dH2; g.a_m = 1; g.a_k = kHidden; g.a_scale = 1.f;
     g.B = H1; g.b_k = kHidden; g.b_n = 1; g.b_scale = 1.f;
     g.C = G + L.oW2; g.c_m = kHidden; g.epi = kEpiStore;
-    SPEO_TRY(gemm(g, s));
+    SPEO_TRY(gemm_splitk(g, part, s));
     // dH1 = (dH2 W2) * silu'(A1)
     g = GemmArgs{};
     g.M = n; g.N = kHidden; g.K = kHidden;
@@ -295,16 +401,14 @@ int Mlp::loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s)
     g.B = P + L.oW2; g.b_k = kHidden; g.b_n = 1; g.b_scale = 1.f;
     g.C = dH1; g.c_m = kHidden; g.P = A1; g.p_m = kHidden; g.epi = kEpiMulDsilu;
     SPEO_TRY(gemm(g, s));
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, dH1, nullptr, n, kHidden, G + L.ob1);
-    SPEO_TRY(check_launch("k_colsum"));
-    // g_W1 = dH1^T (X / 100), columns >= D stay 0
-    SPEO_TRY(zero(G + L.oW1, (size_t)kHidden * L.Kp, s));
+    SPEO_TRY(colsum(dH1, nullptr, n, kHidden, G + L.ob1, s));
+    // g_W1 = dH1^T (X / 100); columns >= D are never written (zero since alloc)
     g = GemmArgs{};
     g.M = kHidden; g.N = L.D; g.K = n;
     g.A = dH1; g.a_m = 1; g.a_k = kHidden; g.a_scale = 1.f;
     g.B = X; g.b_k = ld; g.b_n = 1; g.b_scale = kXScale;
     g.C = G + L.oW1; g.c_m = L.Kp; g.epi = kEpiStore;
-    return gemm(g, s);
+    return gemm_splitk(g, part, s);
 }
 
 int Mlp::adam(float lr, hipStream_t s) {
@@ -452,20 +556,27 @@ int Mlp::infer(const float *X, int ld, int n, float *pred, hipStream_t s) {
 // ------------------------------------------------------------- statistics
 
 // Spearman rank correlation of a[0..n) and b[0..n) (average ranks for ties),
-// one block; O(n^2) ranks, n <= 4096.
-__global__ __launch_bounds__(256) void k_spearman(const float *a, const float *b, int n,
-                                                  float *rho_out) {
-    __shared__ double sh[3][256];
+// one block of 1024 threads with both inputs in LDS; O(n^2) ranks, n <= 4096.
+__global__ __launch_bounds__(1024) void k_spearman(const float *a, const float *b, int n,
+                                                   float *rho_out) {
+    __shared__ double sh[3][1024];
+    __shared__ float as[4096], bs[4096];
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        as[i] = a[i];
+        bs[i] = b[i];
+    }
+    __syncthreads();
     double sab = 0.0, saa = 0.0, sbb = 0.0;
     const double mean = 0.5 * (n - 1);
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = threadIdx.x; i < n; i += 1024) {
         int la = 0, ea = 0, lb = 0, eb = 0;
-        const float ai = a[i], bi = b[i];
+        const float ai = as[i], bi = bs[i];
         for (int j = 0; j < n; ++j) {
-            la += a[j] < ai;
-            ea += a[j] == ai;
-            lb += b[j] < bi;
-            eb += b[j] == bi;
+            const float aj = as[j], bj = bs[j];
+            la += aj < ai;
+            ea += aj == ai;
+            lb += bj < bi;
+            eb += bj == bi;
         }
         const double ra = la + 0.5 * (ea - 1) - mean, rb = lb + 0.5 * (eb - 1) - mean;
         sab += ra * rb;
@@ -476,7 +587,7 @@ __global__ __launch_bounds__(256) void k_spearman(const float *a, const float *b
     sh[1][threadIdx.x] = saa;
     sh[2][threadIdx.x] = sbb;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s)
             for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
         __syncthreads();
@@ -489,7 +600,7 @@ __global__ __launch_bounds__(256) void k_spearman(const float *a, const float *b
 
 int spearman(const float *a, const float *b, int n, float *rho_out, hipStream_t s) {
     if (n < 2 || n > 4096) return set_error(SPEO_ERR_ARG, "spearman: need 2 <= n <= 4096");
-    hipLaunchKernelGGL(k_spearman, dim3(1), dim3(256), 0, s, a, b, n, rho_out);
+    hipLaunchKernelGGL(k_spearman, dim3(1), dim3(1024), 0, s, a, b, n, rho_out);
     return check_launch("k_spearman");
 }
 
@@ -665,17 +776,17 @@ int gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh, f
 
 // stats[1] = mean(y), stats[2] = max(std(y), 1e-3) over n values (one block,
 // FP64 accumulation).
-__global__ void k_mean_std(const float *y, int n, float *stats) {
-    __shared__ double s1[256], s2[256];
+__global__ __launch_bounds__(1024) void k_mean_std(const float *y, int n, float *stats) {
+    __shared__ double s1[1024], s2[1024];
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = threadIdx.x; i < n; i += 1024) {
         a += y[i];
         b += (double)y[i] * y[i];
     }
     s1[threadIdx.x] = a;
     s2[threadIdx.x] = b;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             s1[threadIdx.x] += s1[threadIdx.x + s];
             s2[threadIdx.x] += s2[threadIdx.x + s];
@@ -691,7 +802,7 @@ __global__ void k_mean_std(const float *y, int n, float *stats) {
 }
 
 int mean_std(const float *y, int n, float *stats, hipStream_t s) {
-    hipLaunchKernelGGL(k_mean_std, dim3(1), dim3(256), 0, s, y, n, stats);
+    hipLaunchKernelGGL(k_mean_std, dim3(1), dim3(1024), 0, s, y, n, stats);
     return check_launch("k_mean_std");
 }
 
@@ -763,7 +874,7 @@ int speo_mlp_loss_grad(speo_mlp h, const float *X, int ld, int n, const float *t
         return set_error(SPEO_ERR_ARG, "speo_mlp_loss_grad: bad arguments");
     const float st[3] = {0.f, t_mean, t_std};
     SPEO_TRY_HIP(hipMemcpy(h->m.stats, st, sizeof(st), hipMemcpyHostToDevice));
-    SPEO_TRY(h->m.loss_grad(X, ld, n, t, nullptr));
+    SPEO_TRY(h->m.loss_grad(X, ld, n, t, nullptr, loss_out != nullptr));
     SPEO_TRY_HIP(hipDeviceSynchronize());
     if (loss_out) SPEO_TRY_HIP(hipMemcpy(loss_out, h->m.stats, sizeof(float), hipMemcpyDeviceToHost));
     if (grad_out)

@@ -3,7 +3,10 @@
 // oracle on the CPU.  Input lines:
 //   D <seed> <row> <gen> <S> <D> <n_inj> <inj_0> ... <inj_{n_inj-1}>
 //   B <v as u32 bits> <lo> <hi>
-// Output per line: "r0 r1 r2 r3 r4 jr draws" or the bound's u32 bits.
+//   T <CR as u32 bits>          (crossover threshold)
+//   S <z as u32 bits>           (sin^2(pi z) of the objective terms)
+// Output per line: "r0 r1 r2 r3 r4 jr draws", or one u32 (bits of a float, or
+// the threshold).
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +16,7 @@
 #include <vector>
 
 #include "de.h"
+#include "eval_core.h"
 
 int main() {
     std::string line;
@@ -35,6 +39,20 @@ int main() {
             int r[speo::kMaxR], jr;
             speo::draw_donors(ds, n ? inj.data() : nullptr, n, S, D, r, jr);
             std::printf("%d %d %d %d %d %d %u\n", r[0], r[1], r[2], r[3], r[4], jr, ds.k);
+        } else if (kind == "T") {
+            uint32_t bits;
+            in >> bits;
+            float cr;
+            std::memcpy(&cr, &bits, 4);
+            std::printf("%u\n", speo::cr_threshold(cr));
+        } else if (kind == "S") {
+            uint32_t bits;
+            in >> bits;
+            float z;
+            std::memcpy(&z, &bits, 4);
+            const float o = speo::sin2pi(z);
+            std::memcpy(&bits, &o, 4);
+            std::printf("%u\n", bits);
         } else if (kind == "B") {
             uint32_t bits;
             float lo, hi;

@@ -4,7 +4,10 @@ csrc/de.h, common.h), built for the host, against the oracle
 kernel_DE_CUDA.cu:160-170): the same draws, donors, j_rand and draw count for
 random rows, for tiny sub-islands where redraws are frequent (down to S = 1,
 which runs into the 4096-draw cap), and for injected sequences; and the same
-bits of the bound for in-range, reflected, far-out, infinite and NaN values."""
+bits of the bound for in-range, reflected, far-out, infinite and NaN values.
+Also the integer form of the crossover test (exhaustive over all 2^24 draw
+values for a set of CR) and the accuracy of the objective's sin^2(pi z)
+polynomial against FP64."""
 import shutil
 import struct
 import subprocess
@@ -79,3 +82,33 @@ def test_check_bound_matches_oracle(exe, oracle):
     ref = [_bits(oracle.check_bound(float(v), -100.0, 100.0)) for v in vals] + \
           [_bits(oracle.check_bound(float(v), -5.0, 17.5)) for v in vals[:100]]
     assert got == ref
+
+
+def _f32(bits):
+    return struct.unpack("<f", struct.pack("<I", bits))[0]
+
+
+def test_crossover_threshold_exhaustive(exe):
+    crs = [0.9, 0.5, 0.1, 1.0, 0.0, -0.5, 1e-9, 2.0 ** -24, 3 * 2.0 ** -25, 0.999999,
+           1.0 + 2.0 ** -23, 1.5, float("inf"), float("nan"), 0.3333333]
+    got = [int(x) for x in _run(exe, [f"T {_bits(c)}" for c in crs])]
+    n = np.arange(1 << 24, dtype=np.uint32)
+    u = (n + np.uint32(1)).astype(np.float32) * np.float32(2.0 ** -24)   # u01 of x = n << 8
+    for cr, thr in zip(crs, got):
+        want = u < np.float32(cr)
+        assert np.array_equal(n < thr, want), cr
+
+
+def test_sin2pi_accuracy(exe):
+    rng = np.random.default_rng(9)
+    z = np.concatenate([rng.uniform(-3, 3, 20000), rng.uniform(-200, 200, 20000),
+                        rng.normal(0, 1e-3, 2000), np.arange(-8, 8.5, 0.5),
+                        [1e-30, -1e-20, 2.0 ** 23, 2.0 ** 23 + 2, 1e7 + 0.25]]).astype(np.float32)
+    got = np.array([_f32(int(x)) for x in _run(exe, [f"S {_bits(float(v))}" for v in z])],
+                   np.float64)
+    zd = z.astype(np.float64)
+    ref = np.sin(np.pi * (zd - np.rint(zd))) ** 2
+    tiny = ref < 1e-30
+    assert np.all(got[tiny] <= 1e-30)
+    rel = np.abs(got[~tiny] - ref[~tiny]) / ref[~tiny]
+    assert rel.max() < 3e-7, rel.max()
