@@ -276,6 +276,11 @@ typedef struct {
     int surrogate_batch;          /* minibatch rows                               */
     float surrogate_lr;           /* Adam learning rate (1e-3)                    */
     float surrogate_rho_min;      /* minimum held-out Spearman rho (0.5)          */
+    /* Epoch mode: workgroups per island (0 = automatic: islands are split while
+     * there are fewer islands than compute units; 1, 2, 4 or 8 = fixed).  Split
+     * islands exchange their rows through HBM every generation, inside one
+     * cooperative launch per epoch (not captured in graphs). */
+    int epoch_parts;
 } speo_engine_config;
 
 typedef struct speo_engine_s *speo_engine;
@@ -288,6 +293,7 @@ typedef struct {
     int rows_per_block;    /* tile in use                            */
     int64_t surrogate_predictions; /* candidates scored by the surrogate */
     int64_t surrogate_rounds;      /* training rounds run               */
+    int epoch_parts;               /* workgroups per island (epoch mode) */
 } speo_engine_stats;
 
 /* Allocates the populations, initialises them (Philox) and evaluates them

@@ -15,42 +15,62 @@ struct FusedProblem {
 
 // ------------------------------------------------------------ epoch kernel
 //
-// One workgroup per island runs a whole epoch (the generations between two
-// migrations).  Every row has two slots, the current individual and the
-// slot the next trial is written to; an accepted trial flips the row's slot
-// bit instead of being copied.  The slots live in LDS when the island fits
-// ("LDS plan"), else in HBM (slot 0 = X, slot 1 = a scratch population) with
-// the trials of a pass of rows staged in LDS for the evaluation.
+// A workgroup runs a whole epoch (the generations between two migrations) of
+// one island, or of its share of the rows when an island is split over
+// `parts` workgroups.
+//   LDS plan: the island's current rows and the trial rows of the workgroup's
+//     share live in LDS; an accepted trial is copied over its row.  With
+//     parts > 1 a launch runs one generation: every workgroup loads all rows
+//     of its island from the input population, generates, evaluates and
+//     selects its share and writes the share's rows to the output population
+//     (the next launch's input), so the kernel boundary is the only
+//     synchronisation between the shares.
+//   HBM plan (one workgroup per island, islands too large for LDS): every row
+//     has two slots in HBM (slot 0 = X, slot 1 = a scratch population), an
+//     accepted trial flips the row's slot bit, and the trials of a pass of
+//     rows are staged in LDS for the evaluation.
 
 constexpr int kEpochThreads = 512;
 constexpr int kEpochLdsMax = 160 * 1024;
+constexpr int kEpochMaxParts = 8;
 
 struct EpochPlan {
     int S;          // row stride (floats) of rows in LDS
-    int in_lds;     // 1: both slots in LDS; 0: slots in HBM, trials staged per pass
+    int in_lds;     // 1: LDS plan; 0: HBM plan
     int pass_rows;  // rows per generate/evaluate pass (multiple of 16)
-    int gens_ahead; // generations whose donors are drawn together
+    int parts;      // workgroups sharing the island (LDS plan only)
+    int own_rows;   // rows per share (multiple of 16; the last share may hold fewer)
     int lds_bytes;  // LDS the island needs
 };
 
 struct EpochArgs {
     const IslandDesc *isl;
     const EpochPlan *plan;
-    const int *order;          // block -> island
+    const int2 *blocks;        // workgroup -> (island, share)
     const FusedProblem *probs;
-    float *X, *fX;             // population and fitness (in, out)
-    float *Xalt;               // HBM slot 1 of islands without the LDS plan
+    const float *Xin, *fXin;   // population and fitness at the start of the launch
+    float *Xout, *fXout;       // at the end (the same buffers when parts == 1)
+    float *Xalt;               // HBM slot 1 of islands on the HBM plan
     float *U, *fU;             // the last generation's trials and their fitness (out)
     float lo, hi;
     uint64_t seed;
-    const uint32_t *gen_base;  // generation of the epoch's first generation
-    int L;                     // generations in this launch
+    const uint32_t *gen_base;  // generation counter; the first generation is *gen_base + gen_offset
+    uint32_t gen_offset;
+    int L;                     // generations in this launch (1 when parts > 1)
+    const int4 *donors;        // [generation][row] donors and j_rand (k_epoch_donors)
+    int rows;                  // rows of all islands
+    int donor_gen;             // generation g of the launch reads donors[donor_gen + g]
 };
 
-// Plan of one island (dim <= 128, single component); lds_bytes > kEpochLdsMax
-// means the island cannot run in the epoch kernel.
-EpochPlan epoch_plan(int np, int dim);
+// Plan of one island (dim <= 128, single component) split over `parts`
+// workgroups; lds_bytes > kEpochLdsMax means it does not fit.
+EpochPlan epoch_plan(int np, int dim, int parts);
 
 int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, hipStream_t s);
+
+// Donors and j_rand of generations *gen_base + gen_offset + [0, L) of every
+// row into out[g * rows + row]; row_island maps a row to its island.
+int epoch_donors(const IslandDesc *isl, const int *row_island, int rows, int L, uint64_t seed,
+                 const uint32_t *gen_base, uint32_t gen_offset, int4 *out, hipStream_t s);
 
 }  // namespace speo

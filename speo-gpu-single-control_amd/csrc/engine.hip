@@ -371,10 +371,17 @@ struct Engine {
 
     // epoch mode: per-island plans, block order, LDS of a launch
     EpochPlan *plan_dev = nullptr;
-    int *order_dev = nullptr;
-    int epoch_lds = 0;
+    int2 *blocks_dev = nullptr;
+    int epoch_lds = 0, epoch_parts = 1, epoch_blocks = 0;
+    float *Xnext = nullptr;  // split islands: the population of the next generation
+    int ep_par = 0;          // split islands: buffers swapped an odd number of times
+    int4 *donor_buf = nullptr;    // [donor_gens][rows] donors drawn ahead of the epoch launches
+    int *row_island_dev = nullptr;
+    int donor_gens = 0;
     int epoch_setup();
+    int epoch_plan_all(int parts, std::vector<EpochPlan> *plans, int *lds);
     int launch_epoch_chunk(int L, hipStream_t s);
+    void swap_epoch_buffers();
 
     ~Engine();
     int create(const speo_engine_config &c, hipStream_t s);
@@ -401,7 +408,7 @@ Engine::~Engine() {
     for (auto &kv : tiles_dev) (void)hipFree(kv.second);
     void *bufs[] = {isl_dev, row_off_dev, dst_dev, best_buf, worst_buf, best_row_dev,
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
-                    plan_dev, order_dev,
+                    plan_dev, blocks_dev, Xnext, donor_buf, row_island_dev,
                     X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
                     AX, Ay, Xb, tb, Xh, yh, predh};
     for (void *b : bufs) (void)hipFree(b);
@@ -577,10 +584,29 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     return SPEO_OK;
 }
 
-// Plans the epoch kernel: every island's LDS layout, and the block order
-// (largest islands first, so that they start first).
+// Plans of all islands for `parts` workgroups per island; fails when one
+// does not fit.
+int Engine::epoch_plan_all(int parts, std::vector<EpochPlan> *plans, int *lds) {
+    plans->assign((size_t)I, EpochPlan{});
+    *lds = 0;
+    for (int i = 0; i < I; ++i) {
+        const IslandDesc &d = isl[(size_t)i];
+        const EpochPlan p = epoch_plan(d.np, d.dim, parts);
+        if (p.lds_bytes > kEpochLdsMax) return SPEO_ERR_UNSUPPORTED;
+        if (parts > 1 && d.np < 16 * parts) return SPEO_ERR_UNSUPPORTED;
+        (*plans)[(size_t)i] = p;
+        *lds = std::max(*lds, p.lds_bytes);
+    }
+    return SPEO_OK;
+}
+
+// Plans the epoch kernel.  With fewer islands than compute units an island
+// is split over `parts` workgroups, one generation per launch
+// (cfg.epoch_parts = 0: the largest power of two up to 8 with islands x parts
+// <= compute units whose plans fit in LDS; else as configured); otherwise one
+// workgroup per island runs whole epochs.  Workgroup b serves island b % I,
+// share b / I.
 int Engine::epoch_setup() {
-    std::vector<EpochPlan> plans((size_t)I);
     for (int i = 0; i < I; ++i) {
         const ProblemHost &P = problems[(size_t)isl[(size_t)i].problem]->h;
         if (P.ncomp != 1 || P.dim > kSmallD)
@@ -588,30 +614,62 @@ int Engine::epoch_setup() {
                                                    "with dim <= 128");
         if (isl[(size_t)i].np > 8192)
             return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: islands of at most 8192 rows");
-        plans[(size_t)i] = epoch_plan(isl[(size_t)i].np, P.dim);
-        if (plans[(size_t)i].lds_bytes > kEpochLdsMax)
-            return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: island too large for one workgroup");
-        epoch_lds = std::max(epoch_lds, plans[(size_t)i].lds_bytes);
     }
-    std::vector<int> order((size_t)I);
-    for (int i = 0; i < I; ++i) order[(size_t)i] = i;
-    auto cost = [&](int i) {
-        return (int64_t)isl[(size_t)i].np * ((isl[(size_t)i].dim + 15) / 16) * (isl[(size_t)i].dim + 3);
-    };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    int dev = 0, cus = 0;
+    SPEO_TRY_HIP(hipGetDevice(&dev));
+    SPEO_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cfg.epoch_parts < 0 || cfg.epoch_parts > kEpochMaxParts ||
+        (cfg.epoch_parts & (cfg.epoch_parts - 1)) != 0)
+        return set_error(SPEO_ERR_ARG, "epoch_parts must be 0 (automatic), 1, 2, 4 or 8");
+    int parts = cfg.epoch_parts;
+    if (parts == 0) {
+        parts = 1;
+        while (parts < kEpochMaxParts && (int64_t)I * parts * 2 <= cus) parts *= 2;
+    }
+    std::vector<EpochPlan> plans;
+    int lds = 0;
+    while (epoch_plan_all(parts, &plans, &lds) != SPEO_OK) {
+        if (parts == 1) return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: island too large for one workgroup");
+        if (cfg.epoch_parts > 1)
+            return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: islands cannot be split that far");
+        parts /= 2;
+    }
+    epoch_parts = parts;
+    epoch_lds = lds;
+    epoch_blocks = I * parts;
+    std::vector<int2> blocks((size_t)epoch_blocks);
+    for (int b = 0; b < epoch_blocks; ++b) blocks[(size_t)b] = int2{b % I, b / I};
     SPEO_TRY(upload_vec(&plan_dev, plans));
-    SPEO_TRY(upload_vec(&order_dev, order));
+    SPEO_TRY(upload_vec(&blocks_dev, blocks));
+    std::vector<int> row_island((size_t)total_rows);
+    for (int i = 0; i < I; ++i)
+        for (int r = 0; r < isl[(size_t)i].np; ++r) row_island[(size_t)(isl[(size_t)i].row_off + r)] = i;
+    SPEO_TRY(upload_vec(&row_island_dev, row_island));
+    donor_gens = (int)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t(1) << 26) / total_rows));
+    SPEO_TRY_HIP(hipMalloc(&donor_buf, sizeof(int4) * (size_t)donor_gens * (size_t)total_rows));
+    if (parts > 1) {
+        SPEO_TRY_HIP(hipMalloc(&Xnext, sizeof(float) * (size_t)total_elems));
+        SPEO_TRY_HIP(hipMemcpy(Xnext, X, sizeof(float) * (size_t)total_elems, hipMemcpyDeviceToDevice));
+    }
     return SPEO_OK;
 }
 
+void Engine::swap_epoch_buffers() {
+    std::swap(X, Xnext);
+    std::swap(fX[0], fX[1]);
+    ep_par ^= 1;
+}
+
+// Chunks of at most donor_gens generations: the donors of the chunk are drawn
+// first, then one launch runs the chunk (one workgroup per island) or one
+// launch per generation runs it (split islands, swapping the population
+// buffers).
 int Engine::launch_epoch_chunk(int L, hipStream_t s) {
     EpochArgs a{};
     a.isl = isl_dev;
     a.plan = plan_dev;
-    a.order = order_dev;
+    a.blocks = blocks_dev;
     a.probs = probs_dev;
-    a.X = X;
-    a.fX = fX[0];
     a.Xalt = U[1];
     a.U = U[0];
     a.fU = fU[0];
@@ -619,8 +677,36 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
     a.hi = cfg.max_bound;
     a.seed = cfg.seed;
     a.gen_base = gen_base;
-    a.L = L;
-    return launch_epoch(cfg.precision, a, I, epoch_lds, s);
+    a.donors = donor_buf;
+    a.rows = total_rows;
+    for (int g0 = 0; g0 < L; g0 += donor_gens) {
+        const int Ls = std::min(donor_gens, L - g0);
+        SPEO_TRY(epoch_donors(isl_dev, row_island_dev, total_rows, Ls, cfg.seed, gen_base,
+                              (uint32_t)g0, donor_buf, s));
+        if (epoch_parts == 1) {
+            a.Xin = X;
+            a.Xout = X;
+            a.fXin = fX[0];
+            a.fXout = fX[0];
+            a.gen_offset = (uint32_t)g0;
+            a.donor_gen = 0;
+            a.L = Ls;
+            SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, s));
+            continue;
+        }
+        for (int g = 0; g < Ls; ++g) {
+            a.Xin = X;
+            a.Xout = Xnext;
+            a.fXin = fX[0];
+            a.fXout = fX[1];
+            a.gen_offset = (uint32_t)(g0 + g);
+            a.donor_gen = g;
+            a.L = 1;
+            SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, s));
+            swap_epoch_buffers();
+        }
+    }
+    return SPEO_OK;
 }
 
 int Engine::surrogate_setup() {
@@ -846,11 +932,12 @@ int Engine::run(int generations) {
         const int L = next_chunk(left, &mig);
         const bool use_graph = cfg.use_graph && !graph_failed;
         if (use_graph) {
-            const auto key = std::make_tuple(L, cfg.mode == SPEO_MODE_FUSED ? p : 0,
+            const auto key = std::make_tuple(L, cfg.mode == SPEO_MODE_FUSED ? p : ep_par,
                                              (mig ? 1 : 0) | (pending ? 2 : 0));
             auto it = graphs.find(key);
             const int p_before = p;
             const bool pend_before = pending;
+            const int ep_before = ep_par;
             if (it == graphs.end()) {
                 hipGraph_t g = nullptr;
                 hipGraphExec_t ge = nullptr;
@@ -864,6 +951,7 @@ int Engine::run(int generations) {
                     graph_failed = true;  // fall back to eager launches
                     p = p_before;
                     pending = pend_before;
+                    if (ep_par != ep_before) swap_epoch_buffers();
                     continue;
                 }
                 (void)hipGraphDestroy(g);
@@ -871,6 +959,7 @@ int Engine::run(int generations) {
                 // enqueue_chunk advanced the host-side parity during capture
                 p = p_before;
                 pending = pend_before;
+                if (ep_par != ep_before) swap_epoch_buffers();
             }
             SPEO_TRY_HIP(hipGraphLaunch(it->second, stream));
             ++graph_launches;
@@ -879,6 +968,7 @@ int Engine::run(int generations) {
                 pending = L > 0 ? true : pending;
                 if (mig && cfg.migration_size > 0) pending = false;
             }
+            if (cfg.mode == SPEO_MODE_EPOCH && epoch_parts > 1 && L % 2) swap_epoch_buffers();
         } else {
             SPEO_TRY(enqueue_chunk(L, mig, stream));
         }
@@ -1206,6 +1296,7 @@ int speo_engine_get_stats(speo_engine e, speo_engine_stats *out) {
     out->rows_per_block = e->e.rt;
     out->surrogate_predictions = e->e.surr_pred;
     out->surrogate_rounds = e->e.surr_rounds;
+    out->epoch_parts = e->e.cfg.mode == SPEO_MODE_EPOCH ? e->e.epoch_parts : 0;
     return SPEO_OK;
 }
 

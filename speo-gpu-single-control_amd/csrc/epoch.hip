@@ -37,7 +37,8 @@ __device__ __forceinline__ float comp4(const float4 &v, int q) {
 }
 
 constexpr int kFixedLds = 128 * 4 + 128 * 4;  // shift vector, zero pad after the rows
-constexpr int kPassRowBytes = 16 + 8 + 4 + 32;  // donor offsets, j_rand + trial slot, A row, partials
+constexpr int kPassRowBytes = 16 + 8 + 4 + 32;  // donor offsets, j_rand + trial row, A row, partials
+
 
 int lds_row_stride(int dim) {
     int S = round_up(dim, 4);
@@ -47,31 +48,60 @@ int lds_row_stride(int dim) {
 
 }  // namespace
 
-EpochPlan epoch_plan(int np, int dim) {
+EpochPlan epoch_plan(int np, int dim, int parts) {
     EpochPlan p{};
     p.S = lds_row_stride(dim);
+    p.parts = parts;
     const int NPp = round_up(np, 16);
-    const int per_np = NPp * 8 + round_up(NPp, 16) + kFixedLds;  // fX, fU, flags, shift, pad
-    // Donors are drawn for up to four generations at once (all lanes busy).
-    for (int ga = std::max(1, std::min(4, kEpochThreads / NPp)); ga >= 1; --ga) {
-        const int fixed = per_np + ga * NPp * 16;
-        const int in_lds = fixed + NPp * kPassRowBytes + 2 * NPp * p.S * 4;
-        if (in_lds <= kEpochLdsMax) {
-            p.in_lds = 1;
-            p.pass_rows = NPp;
-            p.gens_ahead = ga;
-            p.lds_bytes = in_lds;
-            return p;
-        }
+    const int own = parts > 1 ? round_up((np + parts - 1) / parts, 16) : NPp;
+    p.own_rows = own;
+    const int fixed = NPp * 8 + round_up(NPp, 16) + kFixedLds;  // fX, fU, flags, shift, pad
+    const int in_lds = fixed + own * kPassRowBytes + (NPp + own) * p.S * 4;
+    if (in_lds <= kEpochLdsMax) {
+        p.in_lds = 1;
+        p.pass_rows = own;
+        p.lds_bytes = in_lds;
+        return p;
     }
-    const int fixed = per_np + NPp * 16;
+    if (parts > 1) {
+        p.lds_bytes = kEpochLdsMax + 1;
+        return p;
+    }
     int pr = (kEpochLdsMax - fixed) / (kPassRowBytes + 4 * p.S);
     pr = std::min(pr / 16 * 16, NPp);
     p.in_lds = 0;
     p.pass_rows = pr;
-    p.gens_ahead = 1;
     p.lds_bytes = pr >= 16 ? fixed + pr * (kPassRowBytes + 4 * p.S) : kEpochLdsMax + 1;
     return p;
+}
+
+// Donors and j_rand of generations [gen0, gen0 + L) for every row: one
+// (generation, row) per lane, out[g * rows + row] = {r0, r1, r2, j_rand}
+// (island-local).  They do not depend on the population, so a whole chunk is
+// drawn ahead of the epoch launches.
+__global__ void k_epoch_donors(const IslandDesc *isl, const int *row_island, int rows, int L,
+                               uint64_t seed, const uint32_t *gen_base, uint32_t gen_offset,
+                               int4 *out) {
+    const int64_t total = (int64_t)rows * L;
+    const uint32_t gen0 = *gen_base + gen_offset;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int g = (int)(t / rows), row = (int)(t - (int64_t)g * rows);
+        const IslandDesc I = isl[row_island[row]];
+        DonorStream ds(seed, (uint32_t)row, gen0 + (uint32_t)g);
+        int r[kMaxR], jr;
+        draw_donors(ds, nullptr, 0, I.np, I.dim, r, jr);
+        out[t] = int4{r[0], r[1], r[2], jr};
+    }
+}
+
+int epoch_donors(const IslandDesc *isl, const int *row_island, int rows, int L, uint64_t seed,
+                 const uint32_t *gen_base, uint32_t gen_offset, int4 *out, hipStream_t s) {
+    const int64_t total = (int64_t)rows * L;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_epoch_donors, dim3(blocks), dim3(256), 0, s, isl, row_island, rows, L, seed,
+                       gen_base, gen_offset, out);
+    return check_launch("k_epoch_donors");
 }
 
 // Per-coordinate terms of one 16-row tile's two column tiles, summed over the
@@ -105,14 +135,19 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     extern __shared__ __align__(16) char lds[];
     constexpr int P = Prec<PREC>::parts;
     constexpr bool H = Prec<PREC>::half;
-    const int island = a.order[blockIdx.x];
+    const int2 bk = a.blocks[blockIdx.x];
+    const int island = bk.x, share = bk.y;
     const IslandDesc I = a.isl[island];
     const EpochPlan pl = a.plan[island];
     const FusedProblem *prob = a.probs + I.problem;
     const CompDev c = prob->comp[0];
     const float lambda = prob->lambda[0], bias = prob->bias[0];
     const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, PR = pl.pass_rows;
-    const int GA = pl.gens_ahead;
+    const bool in_lds = pl.in_lds != 0;
+    // the workgroup's share of the rows: [own0, own0 + nown)
+    const int OWN = in_lds ? pl.own_rows : NPp;
+    const int own0 = in_lds ? min(NP, share * OWN) : 0;
+    const int nown = in_lds ? min(NP, own0 + OWN) - own0 : NP;
     const int ng = (D + 3) >> 2;                        // float4 groups per row
     // it / ng as umulhi(it, ceil(2^32 / ng)): exact for it < 2^32 / ng (it < 2^18 here)
     const uint32_t inv_ng = ng > 1 ? (uint32_t)((0x100000000ull + ng - 1) / ng) : 0u;
@@ -120,47 +155,69 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int wq = wave & 3, grp = wave >> 2, lr = lane & 15, lk = lane >> 4;
-    const bool in_lds = pl.in_lds != 0;
     const uint32_t crt = cr_threshold(I.CR);
 
-    // LDS: fX[NPp] fU[NPp] flags[NPp] shift[128] donors[GA][NPp] doff[PR] tinfo[PR]
-    //      aoff[PR] part[8][PR] rows (+128 zero floats)
+    // LDS: fX[NPp] fU[NPp] flags[NPp] shift[128] doff[PR] tinfo[PR] aoff[PR]
+    //      part[8][PR] rows (+128 zero floats)
     float *fXs = reinterpret_cast<float *>(lds);
     float *fUs = fXs + NPp;
-    uint8_t *flags = reinterpret_cast<uint8_t *>(fUs + NPp);  // bit 0: current slot, bit 1: last trial's
+    uint8_t *flags = reinterpret_cast<uint8_t *>(fUs + NPp);  // HBM plan: bit 0 current slot, bit 1
+                                                              // last trial's; LDS plan: accepted
     float *sh = reinterpret_cast<float *>(flags + ((NPp + 15) & ~15));
-    int4 *dnq = reinterpret_cast<int4 *>(sh + 128);      // r0, r1, r2, j_rand
-    int4 *doff = dnq + GA * NPp;                           // rows of e0, e1, e2 and the target
-    int2 *tinfo = reinterpret_cast<int2 *>(doff + PR);     // j_rand, trial slot row
+    int4 *doff = reinterpret_cast<int4 *>(sh + 128);     // rows of e0, e1, e2 and the target
+    int2 *tinfo = reinterpret_cast<int2 *>(doff + PR);     // j_rand, trial row
     int *aoff = reinterpret_cast<int *>(tinfo + PR);       // LDS row of the trial to evaluate
     float *part = reinterpret_cast<float *>(aoff + PR);
-    float *rows = part + 8 * PR;  // LDS plan: slots [2][NPp][S]; else the pass's trials [PR][S]
-    const int nrows_f = (in_lds ? 2 * NPp : PR) * S + 128;
+    // LDS plan: current rows [NPp][S], then the share's trial rows [OWN][S];
+    // HBM plan: the pass's trials [PR][S]
+    float *rows = part + 8 * PR;
+    const int trial0 = NPp * S;
+    const int nrows_f = (in_lds ? (NPp + OWN) : PR) * S + 128;
 
     // Row addresses: in LDS a float offset into `rows`; in HBM 2 * (r * ld) + slot.
-    float *gX = a.X + I.elem_off, *gA = a.Xalt + I.elem_off;
+    // HBM plan (one share, Xin == Xout): slot 0 is the population itself
+    float *gX = a.Xout + I.elem_off, *gA = a.Xalt + I.elem_off;
+    const float *gXin = a.Xin + I.elem_off;
     auto row_code = [&](int slot, int r) -> int {
-        return in_lds ? (slot * NPp + r) * S : 2 * r * I.ld + slot;
+        return in_lds ? (slot ? trial0 + (r - own0) * S : r * S) : 2 * r * I.ld + slot;
     };
     auto grow = [&](int code) -> float * { return ((code & 1) ? gA : gX) + (code >> 1); };
 
     for (int k = tid; k < 128; k += kEpochThreads) sh[k] = k < D ? c.shift[k] : 0.f;
     for (int r = tid; r < NPp; r += kEpochThreads) {
-        fXs[r] = r < NP ? a.fX[I.row_off + r] : INFINITY;
+        fXs[r] = r < NP ? a.fXin[I.row_off + r] : INFINITY;
+        fUs[r] = INFINITY;
         flags[r] = 0;
     }
-    // zero rows (A reads run up to 127 floats past a row start), then slot 0
+    // zero rows (A reads run up to 127 floats past a row start), then the population
     for (int e = tid; e < nrows_f; e += kEpochThreads) rows[e] = 0.f;
     __syncthreads();
     if (in_lds) {
-        for (int it = tid; it < NP * ng; it += kEpochThreads) {
-            const int r = div_ng(it), q4 = it - r * ng;
-            float4 v = ld4(gX + (size_t)r * I.ld + 4 * q4);
-            const int j = 4 * q4;
-            if (j + 1 >= D) v.y = 0.f;
-            if (j + 2 >= D) v.z = 0.f;
-            if (j + 3 >= D) v.w = 0.f;
-            st4(rows + (size_t)r * S + 4 * q4, v);
+        // eight 16-byte loads in flight per lane before their LDS stores
+        constexpr int kBatch = 8;
+        const int total = NP * ng;
+        for (int base = tid; base < total; base += kBatch * kEpochThreads) {
+            float4 v[kBatch];
+#pragma unroll
+            for (int b = 0; b < kBatch; ++b) {
+                const int it = base + b * kEpochThreads;
+                if (it < total) {
+                    const int r = div_ng(it), q4 = it - r * ng;
+                    v[b] = ld4(gXin + (size_t)r * I.ld + 4 * q4);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kBatch; ++b) {
+                const int it = base + b * kEpochThreads;
+                if (it < total) {
+                    const int r = div_ng(it), q4 = it - r * ng, j = 4 * q4;
+                    float4 x = v[b];
+                    if (j + 1 >= D) x.y = 0.f;
+                    if (j + 2 >= D) x.z = 0.f;
+                    if (j + 3 >= D) x.w = 0.f;
+                    st4(rows + (size_t)r * S + 4 * q4, x);
+                }
+            }
         }
     }
 
@@ -205,36 +262,30 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                                       : vec8{};
                 }
     }
-    const uint32_t gen0 = *a.gen_base;
+    const uint32_t gen0 = *a.gen_base + a.gen_offset;
     __syncthreads();
 
     for (int g = 0; g < a.L; ++g) {
         const uint32_t gen = gen0 + (uint32_t)g;
-        // 1. donors and j_rand of the next GA generations, one (generation, row) per lane
-        if (g % GA == 0) {
-            for (int it = tid; it < GA * NP; it += kEpochThreads) {
-                const int gi = it / NP, r = it - gi * NP;
-                if (g + gi < a.L) {
-                    DonorStream ds(a.seed, (uint32_t)(I.row_off + r), gen + (uint32_t)gi);
-                    int rr[kMaxR], jr;
-                    draw_donors(ds, nullptr, 0, NP, D, rr, jr);
-                    dnq[gi * NPp + r] = int4{rr[0], rr[1], rr[2], jr};
-                }
-            }
-            __syncthreads();
-        }
-        const int4 *dn = dnq + (g % GA) * NPp;
-        for (int p0 = 0; p0 < NP; p0 += PR) {
-            const int nr = min(PR, NP - p0);
-            // 2. the pass's rows: current slots of the donors and the target, trial slot
+        // 1. donors and j_rand of this generation, drawn ahead of the launch
+        const int4 *dn = a.donors + (size_t)(a.donor_gen + g) * a.rows + I.row_off;
+        for (int p0 = own0; p0 < own0 + nown; p0 += PR) {
+            const int nr = min(PR, own0 + nown - p0);
+            // 2. the pass's rows: rows of the donors and the target, trial row
             for (int rl = tid; rl < nr; rl += kEpochThreads) {
                 const int r = p0 + rl;
                 const int4 d = dn[r];
-                const int cur = flags[r] & 1;
-                doff[rl] = int4{row_code(flags[d.x] & 1, d.x), row_code(flags[d.y] & 1, d.y),
-                                row_code(flags[d.z] & 1, d.z), row_code(cur, r)};
-                tinfo[rl] = int2{d.w, row_code(cur ^ 1, r)};
-                aoff[rl] = in_lds ? row_code(cur ^ 1, r) : rl * S;
+                if (in_lds) {
+                    doff[rl] = int4{d.x * S, d.y * S, d.z * S, r * S};
+                    tinfo[rl] = int2{d.w, trial0 + (r - own0) * S};
+                    aoff[rl] = trial0 + (r - own0) * S;
+                } else {
+                    const int cur = flags[r] & 1;
+                    doff[rl] = int4{row_code(flags[d.x] & 1, d.x), row_code(flags[d.y] & 1, d.y),
+                                    row_code(flags[d.z] & 1, d.z), row_code(cur, r)};
+                    tinfo[rl] = int2{d.w, row_code(cur ^ 1, r)};
+                    aoff[rl] = rl * S;
+                }
             }
             __syncthreads();
             // 3. trials (the LDS plan and the HBM plan as separate loops, so that
@@ -421,30 +472,51 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
             }
             __syncthreads();
         }
-        // 5. selection: an accepted trial's slot becomes the row's current one
-        for (int r = tid; r < NP; r += kEpochThreads) {
-            const float fu = fUs[r];
-            const int cur = flags[r] & 1, ts = cur ^ 1;
-            const bool acc = fu < fXs[r];
-            if (acc) fXs[r] = fu;
-            flags[r] = (uint8_t)((acc ? ts : cur) | (ts << 1));
+        // 5. selection, strict "<"
+        if (in_lds) {
+            // the share's accepted trials are copied over their rows
+            for (int r = own0 + tid; r < own0 + nown; r += kEpochThreads) {
+                const bool acc = fUs[r] < fXs[r];
+                if (acc) fXs[r] = fUs[r];
+                flags[r] = acc ? 1 : 0;
+            }
+            __syncthreads();
+            for (int it = tid; it < nown * ng; it += kEpochThreads) {
+                const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
+                if (flags[r]) st4(rows + r * S + 4 * q4, ld4(rows + trial0 + rl * S + 4 * q4));
+            }
+            __syncthreads();
+        } else {
+            // an accepted trial's slot becomes the row's current one
+            for (int r = tid; r < NP; r += kEpochThreads) {
+                const float fu = fUs[r];
+                const int cur = flags[r] & 1, ts = cur ^ 1;
+                const bool acc = fu < fXs[r];
+                if (acc) fXs[r] = fu;
+                flags[r] = (uint8_t)((acc ? ts : cur) | (ts << 1));
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
-    // population and the last generation's trials back to HBM
+    // the share's rows and last trials back to HBM
     float *gU = a.U + I.elem_off;
-    for (int it = tid; it < NP * ng; it += kEpochThreads) {
-        const int r = div_ng(it), q4 = it - r * ng;
-        const int fl = flags[r];
-        const int cc = row_code(fl & 1, r), tc = row_code((fl >> 1) & 1, r);
-        const float4 cur = ld4((in_lds ? rows + cc : grow(cc)) + 4 * q4);
-        const float4 tri = ld4((in_lds ? rows + tc : grow(tc)) + 4 * q4);
+    for (int it = tid; it < nown * ng; it += kEpochThreads) {
+        const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
+        float4 cur, tri;
+        if (in_lds) {
+            cur = ld4(rows + r * S + 4 * q4);
+            tri = ld4(rows + trial0 + rl * S + 4 * q4);
+        } else {
+            const int fl = flags[r];
+            cur = ld4(grow(row_code(fl & 1, r)) + 4 * q4);
+            tri = ld4(grow(row_code((fl >> 1) & 1, r)) + 4 * q4);
+        }
         st4(gX + (size_t)r * I.ld + 4 * q4, cur);
         st4(gU + (size_t)r * I.ld + 4 * q4, tri);
     }
-    for (int r = tid; r < NP; r += kEpochThreads) {
-        a.fX[I.row_off + r] = fXs[r];
+    for (int r = own0 + tid; r < own0 + nown; r += kEpochThreads) {
+        a.fXout[I.row_off + r] = fXs[r];
         a.fU[I.row_off + r] = fUs[r];
     }
 }

@@ -169,7 +169,7 @@ class Engine:
                  mode=MODE_FUSED, use_graph=False, rows_per_block=0, island_F=None,
                  island_CR=None, surrogate_candidates=0, surrogate_train_interval=10,
                  surrogate_archive=65536, surrogate_steps=40, surrogate_batch=1024,
-                 surrogate_lr=1e-3, surrogate_rho_min=0.5):
+                 surrogate_lr=1e-3, surrogate_rho_min=0.5, epoch_parts=0):
         self.problems = list(problems)
         self.island_size = np.ascontiguousarray(island_size, np.int32)
         self.island_problem = np.ascontiguousarray(island_problem, np.int32)
@@ -196,6 +196,7 @@ class Engine:
         cfg.surrogate_batch = surrogate_batch
         cfg.surrogate_lr = surrogate_lr
         cfg.surrogate_rho_min = surrogate_rho_min
+        cfg.epoch_parts = epoch_parts
         self.cfg = cfg
         h = c_void_p()
         check(lib().speo_engine_create(byref(cfg), None, byref(h)))

@@ -65,13 +65,14 @@ class EngineConfig(Structure):
                 ("surrogate_candidates", c_int), ("surrogate_train_interval", c_int),
                 ("surrogate_archive", c_int), ("surrogate_steps", c_int),
                 ("surrogate_batch", c_int), ("surrogate_lr", c_float),
-                ("surrogate_rho_min", c_float)]
+                ("surrogate_rho_min", c_float), ("epoch_parts", c_int)]
 
 
 class EngineStats(Structure):
     _fields_ = [("generations", c_int64), ("evaluations", c_int64), ("migrations", c_int64),
                 ("graph_launches", c_int64), ("rows_per_block", c_int),
-                ("surrogate_predictions", c_int64), ("surrogate_rounds", c_int64)]
+                ("surrogate_predictions", c_int64), ("surrogate_rounds", c_int64),
+                ("epoch_parts", c_int)]
 
 
 _SIGS = {

@@ -94,16 +94,17 @@ def test_epoch_precisions_equal_plain(cuda, speo, oracle, prec):
 def test_epoch_graph_replay_equals_eager(cuda, speo, oracle):
     probs = _problems(speo, oracle, [(30, 3), (100, 2)], base=2400)
     sizes, pidx = [40, 72, 40, 72], [0, 1, 0, 1]
-    kw = dict(F=0.5, CR=0.9, seed=SEED, rows_per_block=32, interval=6, migration_size=2,
-              mode=speo.MODE_EPOCH)
-    eager = speo.Engine(probs, sizes, pidx, use_graph=False, **kw)
-    graph = speo.Engine(probs, sizes, pidx, use_graph=True, **kw)
-    eager.run(37)
-    for chunk in (1, 11, 6, 19):
-        graph.run(chunk)
-        _ = graph.stats()
-    _assert_same(eager, graph)
-    assert graph.stats()["graph_launches"] >= 6
+    for parts in (1, 2):   # whole epochs per launch; split islands, a launch per generation
+        kw = dict(F=0.5, CR=0.9, seed=SEED, rows_per_block=32, interval=6, migration_size=2,
+                  mode=speo.MODE_EPOCH, epoch_parts=parts)
+        eager = speo.Engine(probs, sizes, pidx, use_graph=False, **kw)
+        graph = speo.Engine(probs, sizes, pidx, use_graph=True, **kw)
+        eager.run(37)
+        for chunk in (1, 11, 6, 19):
+            graph.run(chunk)
+            _ = graph.stats()
+        _assert_same(eager, graph)
+        assert graph.stats()["graph_launches"] >= 6
 
 
 def test_epoch_profile_reports_epoch_time(cuda, speo, oracle):
@@ -120,3 +121,33 @@ def test_epoch_rejects_unsupported(cuda, speo, oracle):
     big = speo.Problem([2], o[None, :], M[None, :, :])
     with pytest.raises(speo.SpeoError, match="dim <= 128"):
         speo.Engine([big], [32, 32], [0, 0], mode=speo.MODE_EPOCH)
+
+
+@pytest.mark.parametrize("parts", [2, 4, 8])
+def test_epoch_split_islands_equal_plain(cuda, speo, oracle, parts):
+    # islands split over `parts` workgroups, one generation per launch, each
+    # workgroup loading the whole island and writing its share of the rows;
+    # NP=136 leaves a short share (and, at 8 parts, empty ones)
+    probs = _problems(speo, oracle, [(100, 2), (30, 3), (64, 1)], base=2700)
+    sizes, pidx = [128, 136, 160, 144, 128], [0, 1, 2, 0, 1]
+    kw = dict(F=0.5, CR=0.9, seed=SEED, rows_per_block=32, interval=6, migration_size=2)
+    plain = speo.Engine(probs, sizes, pidx, mode=speo.MODE_PLAIN, **kw)
+    split = speo.Engine(probs, sizes, pidx, mode=speo.MODE_EPOCH, epoch_parts=parts, **kw)
+    assert split.stats()["epoch_parts"] == parts
+    for chunk in (1, 7, 13):
+        print(f"parts {parts} chunk {chunk}", flush=True)
+        plain.run(chunk)
+        split.run(chunk)
+    _assert_same(plain, split)
+
+
+def test_epoch_automatic_split(cuda, speo, oracle):
+    probs = _problems(speo, oracle, [(100, 1)], base=2800)
+    e = speo.Engine(probs, [128] * 8, [0] * 8, mode=speo.MODE_EPOCH, interval=10, migration_size=2,
+                    seed=SEED)
+    assert e.stats()["epoch_parts"] == 8     # 8 islands on 256 compute units
+    one = speo.Engine(probs, [128] * 8, [0] * 8, mode=speo.MODE_EPOCH, interval=10,
+                      migration_size=2, seed=SEED, epoch_parts=1)
+    e.run(25)
+    one.run(25)
+    _assert_same(one, e)
