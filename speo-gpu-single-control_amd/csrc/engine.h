@@ -43,11 +43,19 @@ struct EpochPlan {
     int lds_bytes;  // LDS the island needs
 };
 
+// Everything a workgroup needs to know about its island, in one record.
+struct EpochBlock {
+    IslandDesc I;
+    EpochPlan pl;
+    int share;                 // this workgroup's share of the island's rows
+    int objective, kp32;
+    float rate, lambda, bias;
+    const float *shift, *B32s, *colw;
+    const uint16_t *M16s[3];
+};
+
 struct EpochArgs {
-    const IslandDesc *isl;
-    const EpochPlan *plan;
-    const int2 *blocks;        // workgroup -> (island, share)
-    const FusedProblem *probs;
+    const EpochBlock *blocks;  // [workgroups]
     const float *Xin, *fXin;   // population and fitness at the start of the launch
     float *Xout, *fXout;       // at the end (the same buffers when parts == 1)
     float *Xalt;               // HBM slot 1 of islands on the HBM plan

@@ -370,8 +370,7 @@ struct Engine {
     bool graph_failed = false;
 
     // epoch mode: per-island plans, block order, LDS of a launch
-    EpochPlan *plan_dev = nullptr;
-    int2 *blocks_dev = nullptr;
+    EpochBlock *blocks_dev = nullptr;
     int epoch_lds = 0, epoch_parts = 1, epoch_blocks = 0;
     float *Xnext = nullptr;  // split islands: the population of the next generation
     int ep_par = 0;          // split islands: buffers swapped an odd number of times
@@ -408,7 +407,7 @@ Engine::~Engine() {
     for (auto &kv : tiles_dev) (void)hipFree(kv.second);
     void *bufs[] = {isl_dev, row_off_dev, dst_dev, best_buf, worst_buf, best_row_dev,
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
-                    plan_dev, blocks_dev, Xnext, donor_buf, row_island_dev,
+                    blocks_dev, Xnext, donor_buf, row_island_dev,
                     X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
                     AX, Ay, Xb, tb, Xh, yh, predh};
     for (void *b : bufs) (void)hipFree(b);
@@ -637,9 +636,26 @@ int Engine::epoch_setup() {
     epoch_parts = parts;
     epoch_lds = lds;
     epoch_blocks = I * parts;
-    std::vector<int2> blocks((size_t)epoch_blocks);
-    for (int b = 0; b < epoch_blocks; ++b) blocks[(size_t)b] = int2{b % I, b / I};
-    SPEO_TRY(upload_vec(&plan_dev, plans));
+    std::vector<EpochBlock> blocks((size_t)epoch_blocks);
+    for (int b = 0; b < epoch_blocks; ++b) {
+        const int i = b % I;
+        const IslandDesc &d = isl[(size_t)i];
+        const ProblemHost &P = problems[(size_t)d.problem]->h;
+        const CompDev c = P.view(0, cfg.precision);
+        EpochBlock &e = blocks[(size_t)b];
+        e.I = d;
+        e.pl = plans[(size_t)i];
+        e.share = b / I;
+        e.objective = c.objective;
+        e.kp32 = c.kp32;
+        e.rate = c.rate;
+        e.lambda = (float)P.comp[0].lambda;
+        e.bias = (float)P.comp[0].bias;
+        e.shift = c.shift;
+        e.B32s = c.B32s;
+        e.colw = c.colw;
+        for (int q = 0; q < 3; ++q) e.M16s[q] = c.M16s[q];
+    }
     SPEO_TRY(upload_vec(&blocks_dev, blocks));
     std::vector<int> row_island((size_t)total_rows);
     for (int i = 0; i < I; ++i)
@@ -666,10 +682,7 @@ void Engine::swap_epoch_buffers() {
 // buffers).
 int Engine::launch_epoch_chunk(int L, hipStream_t s) {
     EpochArgs a{};
-    a.isl = isl_dev;
-    a.plan = plan_dev;
     a.blocks = blocks_dev;
-    a.probs = probs_dev;
     a.Xalt = U[1];
     a.U = U[0];
     a.fU = fU[0];

@@ -135,13 +135,11 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     extern __shared__ __align__(16) char lds[];
     constexpr int P = Prec<PREC>::parts;
     constexpr bool H = Prec<PREC>::half;
-    const int2 bk = a.blocks[blockIdx.x];
-    const int island = bk.x, share = bk.y;
-    const IslandDesc I = a.isl[island];
-    const EpochPlan pl = a.plan[island];
-    const FusedProblem *prob = a.probs + I.problem;
-    const CompDev c = prob->comp[0];
-    const float lambda = prob->lambda[0], bias = prob->bias[0];
+    const EpochBlock blk = a.blocks[blockIdx.x];
+    const IslandDesc &I = blk.I;
+    const EpochPlan &pl = blk.pl;
+    const int share = blk.share;
+    const float lambda = blk.lambda, bias = blk.bias;
     const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, PR = pl.pass_rows;
     const bool in_lds = pl.in_lds != 0;
     // the workgroup's share of the rows: [own0, own0 + nown)
@@ -183,53 +181,18 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     };
     auto grow = [&](int code) -> float * { return ((code & 1) ? gA : gX) + (code >> 1); };
 
-    for (int k = tid; k < 128; k += kEpochThreads) sh[k] = k < D ? c.shift[k] : 0.f;
-    for (int r = tid; r < NPp; r += kEpochThreads) {
-        fXs[r] = r < NP ? a.fXin[I.row_off + r] : INFINITY;
-        fUs[r] = INFINITY;
-        flags[r] = 0;
-    }
-    // zero rows (A reads run up to 127 floats past a row start), then the population
-    for (int e = tid; e < nrows_f; e += kEpochThreads) rows[e] = 0.f;
-    __syncthreads();
-    if (in_lds) {
-        // eight 16-byte loads in flight per lane before their LDS stores
-        constexpr int kBatch = 8;
-        const int total = NP * ng;
-        for (int base = tid; base < total; base += kBatch * kEpochThreads) {
-            float4 v[kBatch];
-#pragma unroll
-            for (int b = 0; b < kBatch; ++b) {
-                const int it = base + b * kEpochThreads;
-                if (it < total) {
-                    const int r = div_ng(it), q4 = it - r * ng;
-                    v[b] = ld4(gXin + (size_t)r * I.ld + 4 * q4);
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < kBatch; ++b) {
-                const int it = base + b * kEpochThreads;
-                if (it < total) {
-                    const int r = div_ng(it), q4 = it - r * ng, j = 4 * q4;
-                    float4 x = v[b];
-                    if (j + 1 >= D) x.y = 0.f;
-                    if (j + 2 >= D) x.z = 0.f;
-                    if (j + 3 >= D) x.w = 0.f;
-                    st4(rows + (size_t)r * S + 4 * q4, x);
-                }
-            }
-        }
-    }
-
+    // Prologue: every global load is issued before the first barrier (the
+    // rotation's register fragments, a first batch of the population, the
+    // shift, the fitness), LDS is zeroed meanwhile, and the stores follow.
+    //
     // The rotation's fragments of this wave's column tiles t0, t0 + 4, held
-    // for the whole epoch (the four waves of a group split the columns).
+    // for the whole epoch (the four waves of a group split the columns); B32s
+    // is zero padded to 128 x 128, so the FP32 loads need no bounds.
     const int t0 = wq, t1 = wq + 4;
     const bool v0 = t0 * 16 < D, v1 = t1 * 16 < D;
     const bool c0 = t0 * 16 + lr < D, c1 = t1 * 16 + lr < D;
-    const bool ell = c.objective == SPEO_OBJ_ELLIPTIC;
-    const float w0 = (ell && c0) ? c.colw[t0 * 16 + lr] : 1.0f;
-    const float w1 = (ell && c1) ? c.colw[t1 * 16 + lr] : 1.0f;
-    const int KS = P == 0 ? ng : (c.kp32 >> 5);
+    const bool ell = blk.objective == SPEO_OBJ_ELLIPTIC;
+    const int KS = P == 0 ? ng : (blk.kp32 >> 5);
     typedef typename std::conditional<H, halfx8, bf16x8>::type vec8;
     typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
     constexpr int NB32 = P == 0 ? 32 : 1;
@@ -237,16 +200,11 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     float b0[NB32], b1[NB32], shv[NB32];
     vec8 bb[2][4][NB16];
     if constexpr (P == 0) {
+        const float *bp = blk.B32s + (size_t)lk * kSmallD + lr;
 #pragma unroll
         for (int ks = 0; ks < 32; ++ks) {
-            b0[ks] = 0.f;
-            b1[ks] = 0.f;
-            shv[ks] = ks * 4 + lk < D ? c.shift[ks * 4 + lk] : 0.f;
-            if (ks < KS) {
-                const float *bp = c.B32s + (size_t)(ks * 4 + lk) * kSmallD + lr;
-                if (v0) b0[ks] = bp[t0 * 16];
-                if (v1) b1[ks] = bp[t1 * 16];
-            }
+            b0[ks] = bp[(size_t)ks * 4 * kSmallD + t0 * 16];
+            b1[ks] = bp[(size_t)ks * 4 * kSmallD + t1 * 16];
         }
     } else {
 #pragma unroll
@@ -257,10 +215,54 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                 for (int p = 0; p < P; ++p) {
                     const bool ok = (t == 0 ? v0 : v1) && ks < KS;
                     bb[t][ks][p] = ok ? *reinterpret_cast<const vec8 *>(
-                                            c.M16s[p] + (size_t)((t == 0 ? t0 : t1) * 16 + lr) * c.kp32 +
+                                            blk.M16s[p] + (size_t)((t == 0 ? t0 : t1) * 16 + lr) * blk.kp32 +
                                             ks * 32 + lk * 8)
                                       : vec8{};
                 }
+    }
+    const float w0 = (ell && c0) ? blk.colw[t0 * 16 + lr] : 1.0f;
+    const float w1 = (ell && c1) ? blk.colw[t1 * 16 + lr] : 1.0f;
+    constexpr int kBatch = 16;
+    const int total = in_lds ? NP * ng : 0;
+    float4 pre[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+        const int it = tid + b * kEpochThreads;
+        if (it < total) {
+            const int r = div_ng(it), q4 = it - r * ng;
+            pre[b] = ld4(gXin + (size_t)r * I.ld + 4 * q4);
+        }
+    }
+    const float shift_k = tid < 128 && tid < D ? blk.shift[tid] : 0.f;
+    const float fx_r = tid < NP ? a.fXin[I.row_off + tid] : INFINITY;
+    for (int e = tid; e < nrows_f; e += kEpochThreads) rows[e] = 0.f;
+    __syncthreads();
+    if (tid < 128) sh[tid] = shift_k;
+    for (int r = tid; r < NPp; r += kEpochThreads) {
+        fXs[r] = r == tid ? fx_r : (r < NP ? a.fXin[I.row_off + r] : INFINITY);
+        fUs[r] = INFINITY;
+        flags[r] = 0;
+    }
+    auto put_row_group = [&](int it, float4 x) {
+        const int r = div_ng(it), q4 = it - r * ng, j = 4 * q4;
+        if (j + 1 >= D) x.y = 0.f;
+        if (j + 2 >= D) x.z = 0.f;
+        if (j + 3 >= D) x.w = 0.f;
+        st4(rows + (size_t)r * S + 4 * q4, x);
+    };
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+        const int it = tid + b * kEpochThreads;
+        if (it < total) put_row_group(it, pre[b]);
+    }
+    for (int it = tid + kBatch * kEpochThreads; it < total; it += kEpochThreads) {
+        const int r = div_ng(it), q4 = it - r * ng;
+        put_row_group(it, ld4(gXin + (size_t)r * I.ld + 4 * q4));
+    }
+    __syncthreads();
+    if constexpr (P == 0) {
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks) shv[ks] = sh[ks * 4 + lk];
     }
     const uint32_t gen0 = *a.gen_base + a.gen_offset;
     __syncthreads();
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                     // k steps in blocks of four (A read at fixed offsets; the
                     // steps of the last block past KS have B = 0 in every lane
                     // and add exact zeros).
-                    const float rate = c.rate;
+                    const float rate = blk.rate;
                     const float *ap = arow + lk;
                     float av[32];
 #pragma unroll
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                         }
                     }
                 } else {
-                    const float scale = c.rate * (H ? kF16ScaleA : 1.0f);
+                    const float scale = blk.rate * (H ? kF16ScaleA : 1.0f);
 #pragma unroll
                     for (int ks = 0; ks < 4; ++ks) {
                         if (ks < KS) {
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                 }
                 const float unscale = H ? kF16Unscale : 1.0f;
                 float s1[4], s2[4];
-                switch (c.objective) {
+                switch (blk.objective) {
                 case SPEO_OBJ_ELLIPTIC:
                     tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, acc1, c0, c1, w0, w1, unscale, s1, s2);
                     break;
@@ -468,7 +470,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                     s1 += part[w * PR + rl];
                     s2 += part[(4 + w) * PR + rl];
                 }
-                fUs[p0 + rl] = fmaf(lambda, finish_objective(c.objective, s1, s2, D), bias);
+                fUs[p0 + rl] = fmaf(lambda, finish_objective(blk.objective, s1, s2, D), bias);
             }
             __syncthreads();
         }

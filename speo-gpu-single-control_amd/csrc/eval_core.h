@@ -75,7 +75,7 @@ struct CompDev {
     int is_half;             // M16 holds FP16 (scaled by 2^14) instead of BF16
     const float *colw;       // elliptic column weights 10^(6j/(D-1)), [dim]
     // Small-D layouts (dim <= 128), zero padded, or null:
-    const float *B32s;          // [kp4][128]: B[k][c] = M[c][k]
+    const float *B32s;          // [128][128]: B[k][c] = M[c][k], zero padded
     const uint16_t *M16s[3];    // [128][kp32]: 16-bit parts of M (FP16 scaled by 2^14)
     int kp32;                   // round_up(dim, 32)
 };

@@ -150,8 +150,8 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
             SPEO_TRY(upload(&h.Mh[p], ph[p]));
         }
         if (dim <= kSmallD) {
-            const size_t kp4 = (size_t)round_up(dim, 4), kp32 = (size_t)round_up(dim, 32);
-            std::vector<float> Bs(kp4 * kSmallD, 0.f);
+            const size_t kp32 = (size_t)round_up(dim, 32);
+            std::vector<float> Bs((size_t)kSmallD * kSmallD, 0.f);  // all 128 rows: fragment loads need no bounds
             for (size_t k = 0; k < D; ++k)
                 for (size_t cc = 0; cc < D; ++cc) Bs[k * kSmallD + cc] = (float)M[cc * D + k];
             SPEO_TRY(upload(&h.B32s, Bs));
