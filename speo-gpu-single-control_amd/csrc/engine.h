@@ -68,6 +68,7 @@ struct EpochArgs {
     const int4 *donors;        // [generation][row] donors and j_rand (k_epoch_donors)
     int rows;                  // rows of all islands
     int donor_gen;             // generation g of the launch reads donors[donor_gen + g]
+    int write_trials;          // write U / fU (the chunk's last generation)
 };
 
 // Plan of one island (dim <= 128, single component) split over `parts`

@@ -704,6 +704,7 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
             a.gen_offset = (uint32_t)g0;
             a.donor_gen = 0;
             a.L = Ls;
+            a.write_trials = 1;
             SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, s));
             continue;
         }
@@ -715,6 +716,7 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
             a.gen_offset = (uint32_t)(g0 + g);
             a.donor_gen = g;
             a.L = 1;
+            a.write_trials = g0 + g == L - 1;
             SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, s));
             swap_epoch_buffers();
         }

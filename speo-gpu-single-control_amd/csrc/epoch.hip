@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         }
     }
 
-    // the share's rows and last trials back to HBM
+    // the share's rows (and, at the end of a chunk, its last trials) back to HBM
     float *gU = a.U + I.elem_off;
     for (int it = tid; it < nown * ng; it += kEpochThreads) {
         const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
@@ -515,11 +515,11 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
             tri = ld4(grow(row_code((fl >> 1) & 1, r)) + 4 * q4);
         }
         st4(gX + (size_t)r * I.ld + 4 * q4, cur);
-        st4(gU + (size_t)r * I.ld + 4 * q4, tri);
+        if (a.write_trials) st4(gU + (size_t)r * I.ld + 4 * q4, tri);
     }
     for (int r = own0 + tid; r < own0 + nown; r += kEpochThreads) {
         a.fXout[I.row_off + r] = fXs[r];
-        a.fU[I.row_off + r] = fUs[r];
+        if (a.write_trials) a.fU[I.row_off + r] = fUs[r];
     }
 }
 
