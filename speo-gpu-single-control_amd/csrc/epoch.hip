@@ -186,8 +186,9 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     // shift, the fitness), LDS is zeroed meanwhile, and the stores follow.
     //
     // The rotation's fragments of this wave's column tiles t0, t0 + 4, held
-    // for the whole epoch (the four waves of a group split the columns); B32s
-    // is zero padded to 128 x 128, so the FP32 loads need no bounds.
+    // for the whole epoch (the four waves of a group split the columns); the
+    // FP32 ones come from the fragment-ordered B32s as 16-byte loads, so that
+    // with the population batch they stay within a wave's outstanding loads.
     const int t0 = wq, t1 = wq + 4;
     const bool v0 = t0 * 16 < D, v1 = t1 * 16 < D;
     const bool c0 = t0 * 16 + lr < D, c1 = t1 * 16 + lr < D;
@@ -200,12 +201,8 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     float b0[NB32], b1[NB32], shv[NB32];
     vec8 bb[2][4][NB16];
     if constexpr (P == 0) {
-        const float *bp = blk.B32s + (size_t)lk * kSmallD + lr;
-#pragma unroll
-        for (int ks = 0; ks < 32; ++ks) {
-            b0[ks] = bp[(size_t)ks * 4 * kSmallD + t0 * 16];
-            b1[ks] = bp[(size_t)ks * 4 * kSmallD + t1 * 16];
-        }
+        load_b_frag(blk.B32s, t0, v0, KS, lane, b0);
+        load_b_frag(blk.B32s, t1, v1, KS, lane, b1);
     } else {
 #pragma unroll
         for (int t = 0; t < 2; ++t)
@@ -214,14 +211,14 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
                     const bool ok = (t == 0 ? v0 : v1) && ks < KS;
-                    bb[t][ks][p] = ok ? *reinterpret_cast<const vec8 *>(
+                    bb[t][ks][p] = ok ? gld(reinterpret_cast<const vec8 *>(
                                             blk.M16s[p] + (size_t)((t == 0 ? t0 : t1) * 16 + lr) * blk.kp32 +
-                                            ks * 32 + lk * 8)
+                                            ks * 32 + lk * 8))
                                       : vec8{};
                 }
     }
-    const float w0 = (ell && c0) ? blk.colw[t0 * 16 + lr] : 1.0f;
-    const float w1 = (ell && c1) ? blk.colw[t1 * 16 + lr] : 1.0f;
+    const float w0 = (ell && c0) ? gld(blk.colw + t0 * 16 + lr) : 1.0f;
+    const float w1 = (ell && c1) ? gld(blk.colw + t1 * 16 + lr) : 1.0f;
     constexpr int kBatch = 16;
     const int total = in_lds ? NP * ng : 0;
     float4 pre[kBatch];
@@ -230,11 +227,15 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         const int it = tid + b * kEpochThreads;
         if (it < total) {
             const int r = div_ng(it), q4 = it - r * ng;
-            pre[b] = ld4(gXin + (size_t)r * I.ld + 4 * q4);
+            pre[b] = ld4(gXin + (r * I.ld + 4 * q4));
         }
     }
-    const float shift_k = tid < 128 && tid < D ? blk.shift[tid] : 0.f;
+    const float shift_k = tid < 128 && tid < D ? gld(blk.shift + tid) : 0.f;
     const float fx_r = tid < NP ? a.fXin[I.row_off + tid] : INFINITY;
+    // donors of the first pass of the first generation
+    const int nr_first = min(PR, nown);
+    int4 d_first = int4{0, 0, 0, 0};
+    if (tid < nr_first) d_first = a.donors[(size_t)a.donor_gen * a.rows + I.row_off + own0 + tid];
     for (int e = tid; e < nrows_f; e += kEpochThreads) rows[e] = 0.f;
     __syncthreads();
     if (tid < 128) sh[tid] = shift_k;
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     }
     for (int it = tid + kBatch * kEpochThreads; it < total; it += kEpochThreads) {
         const int r = div_ng(it), q4 = it - r * ng;
-        put_row_group(it, ld4(gXin + (size_t)r * I.ld + 4 * q4));
+        put_row_group(it, ld4(gXin + (r * I.ld + 4 * q4)));
     }
     __syncthreads();
     if constexpr (P == 0) {
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
             // 2. the pass's rows: rows of the donors and the target, trial row
             for (int rl = tid; rl < nr; rl += kEpochThreads) {
                 const int r = p0 + rl;
-                const int4 d = dn[r];
+                const int4 d = (g == 0 && p0 == own0 && rl == tid) ? d_first : dn[r];
                 if (in_lds) {
                     doff[rl] = int4{d.x * S, d.y * S, d.z * S, r * S};
                     tinfo[rl] = int2{d.w, trial0 + (r - own0) * S};

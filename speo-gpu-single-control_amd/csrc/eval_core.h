@@ -25,6 +25,16 @@
 namespace speo {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// A load through the global address space.  Pointers read from device tables
+// are generic to the compiler, which then emits flat loads: those also count
+// against the LDS/scalar wait counter, so the waits that follow serialise
+// them with LDS traffic.  T must be a scalar or an ext_vector_type (a class
+// type's copy constructor would take the value back through a generic
+// reference).
+template <typename T> __device__ __forceinline__ T gld(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
@@ -75,7 +85,12 @@ struct CompDev {
     int is_half;             // M16 holds FP16 (scaled by 2^14) instead of BF16
     const float *colw;       // elliptic column weights 10^(6j/(D-1)), [dim]
     // Small-D layouts (dim <= 128), zero padded, or null:
-    const float *B32s;          // [128][128]: B[k][c] = M[c][k], zero padded
+    // B32s: B = M^T (B[k][c] = M[c][k], zero padded to 128 x 128) in MFMA
+    // fragment order: float4 q of lane l (lr = l % 16, lk = l / 16) of column
+    // tile t sits at float offset ((t * 8 + q) * 64 + l) * 4 and holds
+    // B[16 q + 4 e + lk][16 t + lr] for e = 0..3, i.e. the b operands of K
+    // steps ks = 4 q .. 4 q + 3 of v_mfma_f32_16x16x4f32 (k = 4 ks + lk).
+    const float *B32s;
     const uint16_t *M16s[3];    // [128][kp32]: 16-bit parts of M (FP16 scaled by 2^14)
     int kp32;                   // round_up(dim, 32)
 };
@@ -331,6 +346,29 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
 // 16-bit precisions: P * RT * (kp32 + 8) uint16, and the shift: 128 floats.
 // Output: s1[i][q], s2[i][q] for row 16 i + 4 (lane / 16) + q of the tile,
 // summed over this wave's columns and the lane's 16-lane group.
+// Word 3 of a gfx9 buffer resource (raw buffer, 32-bit data format).
+constexpr int kBufferWord3 = 0x00020000;
+
+// The FP32 b fragments of column tile t for K steps ks < KS (zero beyond, and
+// zero when the tile is past the dimension), from the fragment-ordered B32s.
+// The loads go through a buffer resource that covers only the tile's K blocks
+// in use: the hardware returns zeros for offsets past its size, so all eight
+// loads are issued unconditionally (no branches, no register copies that
+// would wait for a load right after issuing it).
+__device__ __forceinline__ void load_b_frag(const float *B32s, int t, bool valid, int KS, int lane,
+                                            float (&b)[32]) {
+    const int nbytes = valid ? min(8, (KS + 3) >> 2) * 1024 : 0;
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(B32s) + (size_t)t * 2048, (short)0, nbytes, kBufferWord3);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const floatx4 f =
+            __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, lane * 16 + q * 1024, 0, 0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[4 * q + e] = f[e];
+    }
+}
+
 template <int RT> constexpr int small_scratch_bytes(int parts) {
     return 128 * 4 + parts * RT * (kSmallD + 8) * 2;
 }
@@ -356,17 +394,9 @@ __device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, 
         const int kp4 = (dim + 3) & ~3, KS = kp4 >> 2;
         // B fragments into registers first (their latency overlaps the shift staging)
         float b0[32], b1[32];
-#pragma unroll
-        for (int ks = 0; ks < 32; ++ks) {
-            b0[ks] = 0.f;
-            b1[ks] = 0.f;
-            if (ks < KS) {
-                const float *bp = c.B32s + (size_t)(ks * 4 + lk) * kSmallD + lr;
-                if (v0) b0[ks] = bp[t0 * 16];
-                if (v1) b1[ks] = bp[t1 * 16];
-            }
-        }
-        for (int k = tid; k < kSmallD; k += 256) sh[k] = k < dim ? c.shift[k] : 0.f;
+        load_b_frag(c.B32s, t0, v0, KS, lane, b0);
+        load_b_frag(c.B32s, t1, v1, KS, lane, b1);
+        for (int k = tid; k < kSmallD; k += 256) sh[k] = k < dim ? gld(c.shift + k) : 0.f;
         __syncthreads();
         const float rate = c.rate;
 #pragma unroll
@@ -388,7 +418,7 @@ __device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, 
         typedef typename std::conditional<H, halfx8, bf16x8>::type vec8;
         const int kp32 = c.kp32, KS = kp32 >> 5, LDA = kSmallD + 8;
         uint16_t *As = reinterpret_cast<uint16_t *>(sh + kSmallD);  // [P][RT][LDA]
-        for (int k = tid; k < kSmallD; k += 256) sh[k] = k < dim ? c.shift[k] : 0.f;
+        for (int k = tid; k < kSmallD; k += 256) sh[k] = k < dim ? gld(c.shift + k) : 0.f;
         __syncthreads();
         const float scale = c.rate * (H ? kF16ScaleA : 1.0f);
         for (int e = tid; e < RT * kp32; e += 256) {
@@ -407,8 +437,8 @@ __device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, 
             vec8 b[2][P];
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                b[0][p] = v0 ? *reinterpret_cast<const vec8 *>(c.M16s[p] + (size_t)(t0 * 16 + lr) * kp32 + kb) : vec8{};
-                b[1][p] = v1 ? *reinterpret_cast<const vec8 *>(c.M16s[p] + (size_t)(t1 * 16 + lr) * kp32 + kb) : vec8{};
+                b[0][p] = v0 ? gld(reinterpret_cast<const vec8 *>(c.M16s[p] + (size_t)(t0 * 16 + lr) * kp32 + kb)) : vec8{};
+                b[1][p] = v1 ? gld(reinterpret_cast<const vec8 *>(c.M16s[p] + (size_t)(t1 * 16 + lr) * kp32 + kb)) : vec8{};
             }
 #pragma unroll
             for (int i = 0; i < NRT; ++i) {

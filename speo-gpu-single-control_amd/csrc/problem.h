@@ -23,7 +23,7 @@ struct CompHost {
     uint16_t *Mh[3] = {nullptr, nullptr, nullptr};   // FP16 parts of 2^14 M
     double *M64 = nullptr;      // [dim][dim]
     // dim <= 128 only: zero-padded layouts of the register-resident-B path
-    float *B32s = nullptr;                                   // [128][128], zero padded
+    float *B32s = nullptr;                                   // 128 x 128, fragment order
     uint16_t *Mbfs[3] = {nullptr, nullptr, nullptr};         // [128][round_up(dim,32)]
     uint16_t *Mhs[3] = {nullptr, nullptr, nullptr};
 };

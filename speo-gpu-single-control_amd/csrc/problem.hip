@@ -151,9 +151,15 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
         }
         if (dim <= kSmallD) {
             const size_t kp32 = (size_t)round_up(dim, 32);
-            std::vector<float> Bs((size_t)kSmallD * kSmallD, 0.f);  // all 128 rows: fragment loads need no bounds
-            for (size_t k = 0; k < D; ++k)
-                for (size_t cc = 0; cc < D; ++cc) Bs[k * kSmallD + cc] = (float)M[cc * D + k];
+            // B = M^T zero padded to 128 x 128, in MFMA fragment order (CompDev::B32s)
+            std::vector<float> Bs((size_t)kSmallD * kSmallD, 0.f);
+            for (size_t t = 0; t < 8; ++t)
+                for (size_t q = 0; q < 8; ++q)
+                    for (size_t l = 0; l < 64; ++l)
+                        for (size_t e = 0; e < 4; ++e) {
+                            const size_t k = 16 * q + 4 * e + l / 16, cc = 16 * t + l % 16;
+                            if (k < D && cc < D) Bs[((t * 8 + q) * 64 + l) * 4 + e] = (float)M[cc * D + k];
+                        }
             SPEO_TRY(upload(&h.B32s, Bs));
             for (int p = 0; p < 3; ++p) {
                 std::vector<uint16_t> sb((size_t)kSmallD * kp32, 0), sh((size_t)kSmallD * kp32, 0);
