@@ -35,7 +35,8 @@ constexpr int kEpochLdsMax = 160 * 1024;
 constexpr int kEpochMaxParts = 8;
 
 struct EpochPlan {
-    int S;          // row stride (floats) of rows in LDS
+    int S;          // row stride (floats) of population rows in LDS
+    int ST;         // row stride (floats) of trial rows in LDS
     int in_lds;     // 1: LDS plan; 0: HBM plan
     int pass_rows;  // rows per generate/evaluate pass (multiple of 16)
     int parts;      // workgroups sharing the island (LDS plan only)
@@ -49,6 +50,8 @@ struct EpochBlock {
     EpochPlan pl;
     int share;                 // this workgroup's share of the island's rows
     int objective, kp32;
+    uint32_t inv_ng;           // ceil(2^32 / ng), ng = ceil(dim / 4) (0 when ng == 1)
+    uint32_t crt;              // cr_threshold(CR)
     float rate, lambda, bias;
     const float *shift, *B32s, *colw;
     const uint16_t *M16s[3];

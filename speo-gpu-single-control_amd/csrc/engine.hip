@@ -613,6 +613,9 @@ int Engine::epoch_setup() {
                                                    "with dim <= 128");
         if (isl[(size_t)i].np > 8192)
             return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: islands of at most 8192 rows");
+        // the epoch kernel reads an island's rows as one block of np * 4 ceil(dim / 4) floats
+        if (isl[(size_t)i].ld != round_up(P.dim, 4))
+            return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: unexpected population row stride");
     }
     int dev = 0, cus = 0;
     SPEO_TRY_HIP(hipGetDevice(&dev));
@@ -648,6 +651,9 @@ int Engine::epoch_setup() {
         e.share = b / I;
         e.objective = c.objective;
         e.kp32 = c.kp32;
+        const uint64_t ng = (uint64_t)(d.dim + 3) / 4;
+        e.inv_ng = ng > 1 ? (uint32_t)((0x100000000ull + ng - 1) / ng) : 0u;
+        e.crt = cr_threshold(d.CR);
         e.rate = c.rate;
         e.lambda = (float)P.comp[0].lambda;
         e.bias = (float)P.comp[0].bias;

@@ -46,17 +46,23 @@ int lds_row_stride(int dim) {
     return S;
 }
 
+// Row stride of trial rows (the evaluation's A operand): the FP32 evaluation
+// reads K steps in blocks of four, up to column 16 ceil(dim / 16) - 1, so the
+// rows carry zeroed padding to a multiple of 16 columns (+4 for the banks).
+int lds_trial_stride(int dim) { return round_up(round_up(dim, 4), 16) + 4; }
+
 }  // namespace
 
 EpochPlan epoch_plan(int np, int dim, int parts) {
     EpochPlan p{};
     p.S = lds_row_stride(dim);
+    p.ST = lds_trial_stride(dim);
     p.parts = parts;
     const int NPp = round_up(np, 16);
     const int own = parts > 1 ? round_up((np + parts - 1) / parts, 16) : NPp;
     p.own_rows = own;
     const int fixed = NPp * 8 + round_up(NPp, 16) + kFixedLds;  // fX, fU, flags, shift, pad
-    const int in_lds = fixed + own * kPassRowBytes + (NPp + own) * p.S * 4;
+    const int in_lds = fixed + own * kPassRowBytes + (NPp * p.S + own * p.ST) * 4;
     if (in_lds <= kEpochLdsMax) {
         p.in_lds = 1;
         p.pass_rows = own;
@@ -67,11 +73,11 @@ EpochPlan epoch_plan(int np, int dim, int parts) {
         p.lds_bytes = kEpochLdsMax + 1;
         return p;
     }
-    int pr = (kEpochLdsMax - fixed) / (kPassRowBytes + 4 * p.S);
+    int pr = (kEpochLdsMax - fixed) / (kPassRowBytes + 4 * p.ST);
     pr = std::min(pr / 16 * 16, NPp);
     p.in_lds = 0;
     p.pass_rows = pr;
-    p.lds_bytes = pr >= 16 ? fixed + pr * (kPassRowBytes + 4 * p.S) : kEpochLdsMax + 1;
+    p.lds_bytes = pr >= 16 ? fixed + pr * (kPassRowBytes + 4 * p.ST) : kEpochLdsMax + 1;
     return p;
 }
 
@@ -140,7 +146,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     const EpochPlan &pl = blk.pl;
     const int share = blk.share;
     const float lambda = blk.lambda, bias = blk.bias;
-    const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, PR = pl.pass_rows;
+    const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, ST = pl.ST, PR = pl.pass_rows;
     const bool in_lds = pl.in_lds != 0;
     // the workgroup's share of the rows: [own0, own0 + nown)
     const int OWN = in_lds ? pl.own_rows : NPp;
@@ -148,12 +154,20 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     const int nown = in_lds ? min(NP, own0 + OWN) - own0 : NP;
     const int ng = (D + 3) >> 2;                        // float4 groups per row
     // it / ng as umulhi(it, ceil(2^32 / ng)): exact for it < 2^32 / ng (it < 2^18 here)
-    const uint32_t inv_ng = ng > 1 ? (uint32_t)((0x100000000ull + ng - 1) / ng) : 0u;
+    const uint32_t inv_ng = blk.inv_ng;
     auto div_ng = [&](int it) -> int { return ng > 1 ? (int)__umulhi((uint32_t)it, inv_ng) : it; };
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int wq = wave & 3, grp = wave >> 2, lr = lane & 15, lk = lane >> 4;
-    const uint32_t crt = cr_threshold(I.CR);
+    const uint32_t crt = blk.crt;
+    const uint32_t gen0 = *a.gen_base + a.gen_offset;
+    // The launch arguments and descriptor fields the prologue needs, and the
+    // generation counter, as one batch of scalar loads with one wait (the
+    // compiler would otherwise issue each at its first use and wait for it
+    // there: one memory latency per argument).
+    asm volatile("" ::"s"(a.Xin), "s"(a.fXin), "s"(a.donors), "s"(a.rows), "s"(a.donor_gen), "s"(gen0),
+                 "s"(blk.B32s), "s"(blk.shift), "s"(blk.colw), "s"(I.elem_off), "s"(I.row_off), "s"(NP),
+                 "s"(D), "s"(pl.S));
 
     // LDS: fX[NPp] fU[NPp] flags[NPp] shift[128] doff[PR] tinfo[PR] aoff[PR]
     //      part[8][PR] rows (+128 zero floats)
@@ -166,18 +180,17 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     int2 *tinfo = reinterpret_cast<int2 *>(doff + PR);     // j_rand, trial row
     int *aoff = reinterpret_cast<int *>(tinfo + PR);       // LDS row of the trial to evaluate
     float *part = reinterpret_cast<float *>(aoff + PR);
-    // LDS plan: current rows [NPp][S], then the share's trial rows [OWN][S];
-    // HBM plan: the pass's trials [PR][S]
+    // LDS plan: current rows [NPp][S], then the share's trial rows [OWN][ST];
+    // HBM plan: the pass's trials [PR][ST]
     float *rows = part + 8 * PR;
     const int trial0 = NPp * S;
-    const int nrows_f = (in_lds ? (NPp + OWN) : PR) * S + 128;
 
     // Row addresses: in LDS a float offset into `rows`; in HBM 2 * (r * ld) + slot.
     // HBM plan (one share, Xin == Xout): slot 0 is the population itself
     float *gX = a.Xout + I.elem_off, *gA = a.Xalt + I.elem_off;
     const float *gXin = a.Xin + I.elem_off;
     auto row_code = [&](int slot, int r) -> int {
-        return in_lds ? (slot ? trial0 + (r - own0) * S : r * S) : 2 * r * I.ld + slot;
+        return in_lds ? (slot ? trial0 + (r - own0) * ST : r * S) : 2 * r * I.ld + slot;
     };
     auto grow = [&](int code) -> float * { return ((code & 1) ? gA : gX) + (code >> 1); };
 
@@ -219,16 +232,26 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     }
     const float w0 = (ell && c0) ? gld(blk.colw + t0 * 16 + lr) : 1.0f;
     const float w1 = (ell && c1) ? gld(blk.colw + t1 * 16 + lr) : 1.0f;
+    // The island's rows are one contiguous block of NP * 4 ng floats (the row
+    // stride is round_up(D, 4) when no island is wider than 128).  They are
+    // read through a buffer resource over that block: offsets past it return
+    // zeros, so the batch needs no bounds branches and no per-load index
+    // arithmetic; with the LDS row stride equal to D (D % 4 == 0 and D not a
+    // multiple of 32) the stores are linear too.
     constexpr int kBatch = 16;
     const int total = in_lds ? NP * ng : 0;
-    float4 pre[kBatch];
+    const bool linear = (D & 3) == 0 && S == D;
+    floatx4 pre[kBatch];
+    {
+        // (a load of batch b is skipped when the whole batch lies past the
+        // block: a wave-uniform branch around it, no value merged after it)
+        const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(gXin), (short)0, total * 16, kBufferWord3);
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b) {
-        const int it = tid + b * kEpochThreads;
-        if (it < total) {
-            const int r = div_ng(it), q4 = it - r * ng;
-            pre[b] = ld4(gXin + (r * I.ld + 4 * q4));
-        }
+        for (int b = 0; b < kBatch; ++b)
+            if (b * kEpochThreads < total)
+                pre[b] = __builtin_bit_cast(
+                    floatx4, __builtin_amdgcn_raw_buffer_load_b128(xr, (tid + b * kEpochThreads) * 16, 0, 0));
     }
     const float shift_k = tid < 128 && tid < D ? gld(blk.shift + tid) : 0.f;
     const float fx_r = tid < NP ? a.fXin[I.row_off + tid] : INFINITY;
@@ -236,15 +259,27 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     const int nr_first = min(PR, nown);
     int4 d_first = int4{0, 0, 0, 0};
     if (tid < nr_first) d_first = a.donors[(size_t)a.donor_gen * a.rows + I.row_off + own0 + tid];
-    for (int e = tid; e < nrows_f; e += kEpochThreads) rows[e] = 0.f;
-    __syncthreads();
     if (tid < 128) sh[tid] = shift_k;
     for (int r = tid; r < NPp; r += kEpochThreads) {
         fXs[r] = r == tid ? fx_r : (r < NP ? a.fXin[I.row_off + r] : INFINITY);
         fUs[r] = INFINITY;
         flags[r] = 0;
     }
+    // Trial rows: columns [4 ng, ST) are zeroed once (trials write the first
+    // 4 ng).  Rows land in LDS with their padding columns zeroed; nothing
+    // else in LDS is read before it is written.
+    {
+        const int pad = ST - 4 * ng, nt = in_lds ? OWN : PR, t0f = in_lds ? trial0 : 0;
+        for (int e = tid; e < nt * pad; e += kEpochThreads) {
+            const int r = e / pad;
+            rows[t0f + r * ST + 4 * ng + (e - r * pad)] = 0.f;
+        }
+    }
     auto put_row_group = [&](int it, float4 x) {
+        if (linear) {
+            st4(rows + 4 * it, x);
+            return;
+        }
         const int r = div_ng(it), q4 = it - r * ng, j = 4 * q4;
         if (j + 1 >= D) x.y = 0.f;
         if (j + 2 >= D) x.z = 0.f;
@@ -254,19 +289,15 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
         const int it = tid + b * kEpochThreads;
-        if (it < total) put_row_group(it, pre[b]);
+        if (it < total) put_row_group(it, float4{pre[b][0], pre[b][1], pre[b][2], pre[b][3]});
     }
-    for (int it = tid + kBatch * kEpochThreads; it < total; it += kEpochThreads) {
-        const int r = div_ng(it), q4 = it - r * ng;
-        put_row_group(it, ld4(gXin + (r * I.ld + 4 * q4)));
-    }
+    for (int it = tid + kBatch * kEpochThreads; it < total; it += kEpochThreads)
+        put_row_group(it, ld4(gXin + 4 * it));
     __syncthreads();
     if constexpr (P == 0) {
 #pragma unroll
         for (int ks = 0; ks < 32; ++ks) shv[ks] = sh[ks * 4 + lk];
     }
-    const uint32_t gen0 = *a.gen_base + a.gen_offset;
-    __syncthreads();
 
     for (int g = 0; g < a.L; ++g) {
         const uint32_t gen = gen0 + (uint32_t)g;
@@ -280,14 +311,14 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                 const int4 d = (g == 0 && p0 == own0 && rl == tid) ? d_first : dn[r];
                 if (in_lds) {
                     doff[rl] = int4{d.x * S, d.y * S, d.z * S, r * S};
-                    tinfo[rl] = int2{d.w, trial0 + (r - own0) * S};
-                    aoff[rl] = trial0 + (r - own0) * S;
+                    tinfo[rl] = int2{d.w, trial0 + (r - own0) * ST};
+                    aoff[rl] = trial0 + (r - own0) * ST;
                 } else {
                     const int cur = flags[r] & 1;
                     doff[rl] = int4{row_code(flags[d.x] & 1, d.x), row_code(flags[d.y] & 1, d.y),
                                     row_code(flags[d.z] & 1, d.z), row_code(cur, r)};
                     tinfo[rl] = int2{d.w, row_code(cur ^ 1, r)};
-                    aoff[rl] = rl * S;
+                    aoff[rl] = rl * ST;
                 }
             }
             __syncthreads();
@@ -345,7 +376,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
             } else {
                 trials([&](int code) -> const float * { return grow(code); },
                        [&](int rl, int code, int q4, float4 v) {
-                           st4(rows + (size_t)rl * S + 4 * q4, v);
+                           st4(rows + (size_t)rl * ST + 4 * q4, v);
                            st4(grow(code) + 4 * q4, v);
                        });
             }
@@ -357,8 +388,8 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                 floatx4 acc0 = floatx4{0.f, 0.f, 0.f, 0.f}, acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
                 if constexpr (P == 0) {
                     // k steps in blocks of four (A read at fixed offsets; the
-                    // steps of the last block past KS have B = 0 in every lane
-                    // and add exact zeros).
+                    // steps of the last block past KS read the trial rows'
+                    // zeroed padding, B = 0 there, and add exact zeros).
                     const float rate = blk.rate;
                     const float *ap = arow + lk;
                     float av[32];
@@ -486,7 +517,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
             __syncthreads();
             for (int it = tid; it < nown * ng; it += kEpochThreads) {
                 const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
-                if (flags[r]) st4(rows + r * S + 4 * q4, ld4(rows + trial0 + rl * S + 4 * q4));
+                if (flags[r]) st4(rows + r * S + 4 * q4, ld4(rows + trial0 + rl * ST + 4 * q4));
             }
             __syncthreads();
         } else {
@@ -509,7 +540,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         float4 cur, tri;
         if (in_lds) {
             cur = ld4(rows + r * S + 4 * q4);
-            tri = ld4(rows + trial0 + rl * S + 4 * q4);
+            tri = ld4(rows + trial0 + rl * ST + 4 * q4);
         } else {
             const int fl = flags[r];
             cur = ld4(grow(row_code(fl & 1, r)) + 4 * q4);

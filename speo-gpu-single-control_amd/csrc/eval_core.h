@@ -349,23 +349,25 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
 // Word 3 of a gfx9 buffer resource (raw buffer, 32-bit data format).
 constexpr int kBufferWord3 = 0x00020000;
 
-// The FP32 b fragments of column tile t for K steps ks < KS (zero beyond, and
-// zero when the tile is past the dimension), from the fragment-ordered B32s.
-// The loads go through a buffer resource that covers only the tile's K blocks
-// in use: the hardware returns zeros for offsets past its size, so all eight
-// loads are issued unconditionally (no branches, no register copies that
-// would wait for a load right after issuing it).
+// The FP32 b fragments of column tile t for K steps ks < KS, rounded up to a
+// block of four, from the fragment-ordered B32s (zero past the dimension).
+// Blocks past that, and every block of a tile past the dimension, are left
+// undefined: callers use b[ks] only for ks < round_up(KS, 4) of valid tiles.
+// The 16-byte loads go through a buffer resource over the tile, behind
+// wave-uniform branches, and nothing merges a loaded value with another one
+// (which would make the wave wait for that load right after issuing it).
 __device__ __forceinline__ void load_b_frag(const float *B32s, int t, bool valid, int KS, int lane,
                                             float (&b)[32]) {
-    const int nbytes = valid ? min(8, (KS + 3) >> 2) * 1024 : 0;
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float *>(B32s) + (size_t)t * 2048, (short)0, nbytes, kBufferWord3);
+        const_cast<float *>(B32s) + (size_t)t * 2048, (short)0, 8192, kBufferWord3);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const floatx4 f =
-            __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, lane * 16 + q * 1024, 0, 0));
+        if (valid && q * 4 < KS) {
+            const floatx4 f =
+                __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, lane * 16 + q * 1024, 0, 0));
 #pragma unroll
-        for (int e = 0; e < 4; ++e) b[4 * q + e] = f[e];
+            for (int e = 0; e < 4; ++e) b[4 * q + e] = f[e];
+        }
     }
 }
 
