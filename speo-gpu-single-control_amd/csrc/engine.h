@@ -78,7 +78,8 @@ struct EpochArgs {
 // workgroups; lds_bytes > kEpochLdsMax means it does not fit.
 EpochPlan epoch_plan(int np, int dim, int parts);
 
-int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, hipStream_t s);
+// split: islands split over workgroups (LDS plan, a.L == 1).
+int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, bool split, hipStream_t s);
 
 // Donors and j_rand of generations *gen_base + gen_offset + [0, L) of every
 // row into out[g * rows + row]; row_island maps a row to its island.

@@ -711,7 +711,7 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
             a.donor_gen = 0;
             a.L = Ls;
             a.write_trials = 1;
-            SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, s));
+            SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, epoch_parts > 1, s));
             continue;
         }
         for (int g = 0; g < Ls; ++g) {
@@ -723,7 +723,7 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
             a.donor_gen = g;
             a.L = 1;
             a.write_trials = g0 + g == L - 1;
-            SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, s));
+            SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, epoch_parts > 1, s));
             swap_epoch_buffers();
         }
     }

@@ -136,7 +136,9 @@ __device__ __forceinline__ void tile_terms(const floatx4 &acc0, const floatx4 &a
     }
 }
 
-template <int PREC>
+// SPLIT: islands split over several workgroups, one generation per launch,
+// rows in LDS (the general kernel's loops and the HBM plan compile away).
+template <int PREC, bool SPLIT>
 __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     extern __shared__ __align__(16) char lds[];
     constexpr int P = Prec<PREC>::parts;
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     const int share = blk.share;
     const float lambda = blk.lambda, bias = blk.bias;
     const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, ST = pl.ST, PR = pl.pass_rows;
-    const bool in_lds = pl.in_lds != 0;
+    const bool in_lds = SPLIT || pl.in_lds != 0;
     // the workgroup's share of the rows: [own0, own0 + nown)
     const int OWN = in_lds ? pl.own_rows : NPp;
     const int own0 = in_lds ? min(NP, share * OWN) : 0;
@@ -299,11 +301,14 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         for (int ks = 0; ks < 32; ++ks) shv[ks] = sh[ks * 4 + lk];
     }
 
-    for (int g = 0; g < a.L; ++g) {
+    const int L = SPLIT ? 1 : a.L;
+    const int npass = SPLIT ? 1 : (nown + PR - 1) / PR;
+    for (int g = 0; g < L; ++g) {
         const uint32_t gen = gen0 + (uint32_t)g;
         // 1. donors and j_rand of this generation, drawn ahead of the launch
         const int4 *dn = a.donors + (size_t)(a.donor_gen + g) * a.rows + I.row_off;
-        for (int p0 = own0; p0 < own0 + nown; p0 += PR) {
+        for (int pass = 0; pass < npass; ++pass) {
+            const int p0 = own0 + pass * PR;
             const int nr = min(PR, own0 + nown - p0);
             // 2. the pass's rows: rows of the donors and the target, trial row
             for (int rl = tid; rl < nr; rl += kEpochThreads) {
@@ -508,18 +513,21 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         }
         // 5. selection, strict "<"
         if (in_lds) {
-            // the share's accepted trials are copied over their rows
+            // the share's accepted trials are copied over their rows (with
+            // one generation per launch the write-back picks them instead)
             for (int r = own0 + tid; r < own0 + nown; r += kEpochThreads) {
                 const bool acc = fUs[r] < fXs[r];
                 if (acc) fXs[r] = fUs[r];
                 flags[r] = acc ? 1 : 0;
             }
             __syncthreads();
-            for (int it = tid; it < nown * ng; it += kEpochThreads) {
-                const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
-                if (flags[r]) st4(rows + r * S + 4 * q4, ld4(rows + trial0 + rl * ST + 4 * q4));
+            if (!SPLIT) {
+                for (int it = tid; it < nown * ng; it += kEpochThreads) {
+                    const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
+                    if (flags[r]) st4(rows + r * S + 4 * q4, ld4(rows + trial0 + rl * ST + 4 * q4));
+                }
+                __syncthreads();
             }
-            __syncthreads();
         } else {
             // an accepted trial's slot becomes the row's current one
             for (int r = tid; r < NP; r += kEpochThreads) {
@@ -539,8 +547,8 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
         float4 cur, tri;
         if (in_lds) {
-            cur = ld4(rows + r * S + 4 * q4);
             tri = ld4(rows + trial0 + rl * ST + 4 * q4);
+            cur = (SPLIT && flags[r]) ? tri : ld4(rows + r * S + 4 * q4);
         } else {
             const int fl = flags[r];
             cur = ld4(grow(row_code(fl & 1, r)) + 4 * q4);
@@ -555,29 +563,35 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     }
 }
 
-template <int PREC>
+template <int PREC, bool SPLIT>
 static int launch_epoch_t(const EpochArgs &a, int nblocks, int lds, hipStream_t s) {
     static std::once_flag once;
     static hipError_t attr = hipSuccess;
     std::call_once(once, [] {
-        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_epoch<PREC>),
+        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_epoch<PREC, SPLIT>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, kEpochLdsMax);
     });
     if (attr != hipSuccess) return check_hip(attr, "hipFuncSetAttribute(k_island_epoch)");
-    hipLaunchKernelGGL((k_island_epoch<PREC>), dim3(nblocks), dim3(kEpochThreads), lds, s, a);
+    hipLaunchKernelGGL((k_island_epoch<PREC, SPLIT>), dim3(nblocks), dim3(kEpochThreads), lds, s, a);
     return check_launch("k_island_epoch");
 }
 
-int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, hipStream_t s) {
+template <int PREC>
+static int launch_epoch_p(const EpochArgs &a, int nblocks, int lds, bool split, hipStream_t s) {
+    return split ? launch_epoch_t<PREC, true>(a, nblocks, lds, s) : launch_epoch_t<PREC, false>(a, nblocks, lds, s);
+}
+
+int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, bool split, hipStream_t s) {
     if (a.L < 1) return set_error(SPEO_ERR_ARG, "epoch launch needs at least one generation");
+    if (split && a.L != 1) return set_error(SPEO_ERR_ARG, "split epoch launches run one generation");
     if (lds_bytes > kEpochLdsMax) return set_error(SPEO_ERR_ARG, "epoch launch: LDS plan too large");
     switch (precision) {
-    case SPEO_PREC_FP32: return launch_epoch_t<SPEO_PREC_FP32>(a, nblocks, lds_bytes, s);
-    case SPEO_PREC_BF16: return launch_epoch_t<SPEO_PREC_BF16>(a, nblocks, lds_bytes, s);
-    case SPEO_PREC_FP16: return launch_epoch_t<SPEO_PREC_FP16>(a, nblocks, lds_bytes, s);
-    case SPEO_PREC_BF16X3: return launch_epoch_t<SPEO_PREC_BF16X3>(a, nblocks, lds_bytes, s);
-    case SPEO_PREC_FP16X3: return launch_epoch_t<SPEO_PREC_FP16X3>(a, nblocks, lds_bytes, s);
-    case SPEO_PREC_BF16X6: return launch_epoch_t<SPEO_PREC_BF16X6>(a, nblocks, lds_bytes, s);
+    case SPEO_PREC_FP32: return launch_epoch_p<SPEO_PREC_FP32>(a, nblocks, lds_bytes, split, s);
+    case SPEO_PREC_BF16: return launch_epoch_p<SPEO_PREC_BF16>(a, nblocks, lds_bytes, split, s);
+    case SPEO_PREC_FP16: return launch_epoch_p<SPEO_PREC_FP16>(a, nblocks, lds_bytes, split, s);
+    case SPEO_PREC_BF16X3: return launch_epoch_p<SPEO_PREC_BF16X3>(a, nblocks, lds_bytes, split, s);
+    case SPEO_PREC_FP16X3: return launch_epoch_p<SPEO_PREC_FP16X3>(a, nblocks, lds_bytes, split, s);
+    case SPEO_PREC_BF16X6: return launch_epoch_p<SPEO_PREC_BF16X6>(a, nblocks, lds_bytes, split, s);
     default: return set_error(SPEO_ERR_ARG, "unknown precision");
     }
 }
