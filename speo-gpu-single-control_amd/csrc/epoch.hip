@@ -328,51 +328,60 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
             }
             __syncthreads();
             // 3. trials (the LDS plan and the HBM plan as separate loops, so that
-            // each compiles to plain LDS or global accesses)
-            // Two items per lane and iteration: their Philox chains are
-            // independent, which doubles the instruction-level parallelism of
-            // the dominant integer work.
+            // each compiles to plain LDS or global accesses).  Two items per
+            // lane and iteration, the operands of both fetched before either
+            // is made.
+            struct Operands {
+                int rl, q4;
+                int2 ti;  // j_rand, trial row
+                float4 e0, e1, e2, xo;
+            };
             auto trials = [&](auto row_of, auto store) {
                 const int n_items = nr * ng;
-                auto one = [&](int it, const U4 &w) {
-                    const int rl = div_ng(it), q4 = it - rl * ng;
-                    const int4 o = doff[rl];
-                    const int2 ti = tinfo[rl];
-                    const float4 e0 = ld4(row_of(o.x) + 4 * q4);
-                    const float4 e1 = ld4(row_of(o.y) + 4 * q4);
-                    const float4 e2 = ld4(row_of(o.z) + 4 * q4);
-                    const float4 xo = ld4(row_of(o.w) + 4 * q4);
-                    // the four coordinates branch-free; the reflective bound
-                    // only when a lane holds a coordinate out of bounds
+                auto fetch = [&](int it) {
+                    Operands o;
+                    o.rl = div_ng(it);
+                    o.q4 = it - o.rl * ng;
+                    const int4 d = doff[o.rl];
+                    o.ti = tinfo[o.rl];
+                    o.e0 = ld4(row_of(d.x) + 4 * o.q4);
+                    o.e1 = ld4(row_of(d.y) + 4 * o.q4);
+                    o.e2 = ld4(row_of(d.z) + 4 * o.q4);
+                    o.xo = ld4(row_of(d.w) + 4 * o.q4);
+                    return o;
+                };
+                // the four coordinates branch-free; the reflective bound only
+                // when a lane holds a coordinate out of bounds
+                auto make = [&](const Operands &o, const U4 &w) {
                     float v[4];
                     bool out = false;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const int j = 4 * q4 + q;
-                        const bool take = (j == ti.x) || cross_take(word(w, q), crt);
-                        const float mt = fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q));
-                        v[q] = j < D ? (take ? mt : comp4(xo, q)) : 0.f;
-                        out |= (j < D) && ((v[q] < a.lo) || (v[q] > a.hi));
+                        const int j = 4 * o.q4 + q;
+                        // (bitwise, not short-circuit, operators: selects, no branches)
+                        const bool take = (j == o.ti.x) | cross_take(word(w, q), crt);
+                        const float mt = fmaf(I.F, comp4(o.e1, q) - comp4(o.e2, q), comp4(o.e0, q));
+                        v[q] = j < D ? (take ? mt : comp4(o.xo, q)) : 0.f;
+                        out |= (j < D) & ((v[q] < a.lo) | (v[q] > a.hi));
                     }
                     if (out) {
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
-                            if (4 * q4 + q < D) v[q] = check_bound(v[q], a.lo, a.hi);
+                            if (4 * o.q4 + q < D) v[q] = check_bound(v[q], a.lo, a.hi);
                     }
-                    store(rl, ti.y, q4, float4{v[0], v[1], v[2], v[3]});
-                };
-                auto draw = [&](int it) {
-                    const int rl = div_ng(it), q4 = it - rl * ng;
-                    return philox_block(a.seed, (uint32_t)q4, (uint32_t)(I.row_off + p0 + rl), gen,
-                                        kPurposeCross);
+                    return float4{v[0], v[1], v[2], v[3]};
                 };
                 for (int it = tid; it < n_items; it += 2 * kEpochThreads) {
                     const int it2 = it + kEpochThreads;
                     const bool two = it2 < n_items;
-                    const U4 w1 = draw(it);
-                    const U4 w2 = draw(two ? it2 : it);
-                    one(it, w1);
-                    if (two) one(it2, w2);
+                    const Operands A = fetch(it), B = fetch(two ? it2 : it);
+                    const U4 wa = philox_block(a.seed, (uint32_t)A.q4, (uint32_t)(I.row_off + p0 + A.rl), gen,
+                                               kPurposeCross);
+                    const U4 wb = philox_block(a.seed, (uint32_t)B.q4, (uint32_t)(I.row_off + p0 + B.rl), gen,
+                                               kPurposeCross);
+                    const float4 va = make(A, wa), vb = make(B, wb);
+                    store(A.rl, A.ti.y, A.q4, va);
+                    if (two) store(B.rl, B.ti.y, B.q4, vb);
                 }
             };
             if (in_lds) {

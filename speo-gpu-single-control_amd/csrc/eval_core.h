@@ -336,16 +336,6 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
     }
 }
 
-// ------------------------------------------------------- small-D path
-//
-// dim <= 128.  The tile's RT rows are in LDS (rows[r * kTS + k], FP32,
-// unshifted; rows >= nrows and columns >= dim hold zeros).  Wave w owns the
-// 16-column tiles w and w + 4 for all RT rows; its B fragments come straight
-// from the padded global layouts into registers (no LDS staging, no barrier
-// inside the K loop).  `scratch` holds the shifted rows' 16-bit parts for the
-// 16-bit precisions: P * RT * (kp32 + 8) uint16, and the shift: 128 floats.
-// Output: s1[i][q], s2[i][q] for row 16 i + 4 (lane / 16) + q of the tile,
-// summed over this wave's columns and the lane's 16-lane group.
 // Word 3 of a gfx9 buffer resource (raw buffer, 32-bit data format).
 constexpr int kBufferWord3 = 0x00020000;
 
@@ -371,6 +361,16 @@ __device__ __forceinline__ void load_b_frag(const float *B32s, int t, bool valid
     }
 }
 
+// ------------------------------------------------------- small-D path
+//
+// dim <= 128.  The tile's RT rows are in LDS (rows[r * kTS + k], FP32,
+// unshifted; rows >= nrows and columns >= dim hold zeros).  Wave w owns the
+// 16-column tiles w and w + 4 for all RT rows; its B fragments come straight
+// from the padded global layouts into registers (no LDS staging, no barrier
+// inside the K loop).  `scratch` holds the shifted rows' 16-bit parts for the
+// 16-bit precisions: P * RT * (kp32 + 8) uint16, and the shift: 128 floats.
+// Output: s1[i][q], s2[i][q] for row 16 i + 4 (lane / 16) + q of the tile,
+// summed over this wave's columns and the lane's 16-lane group.
 template <int RT> constexpr int small_scratch_bytes(int parts) {
     return 128 * 4 + parts * RT * (kSmallD + 8) * 2;
 }
