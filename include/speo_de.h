@@ -331,9 +331,10 @@ SPEO_EXPORT int speo_engine_max_dim(speo_engine e);
 /* Runs `generations` generations eagerly with HIP events recorded on the
  * engine's stream around every launch and returns the average duration (ms)
  * of each kernel role per generation: [0] generate, [1] evaluate, [2] select
- * (plain mode), [3] fused generation (fused mode) and [4] the epoch launch
- * divided by its generations (epoch mode); unused roles are 0.  avg_ms_out
- * holds 5 doubles.  Migration follows the same schedule as speo_engine_run.
+ * (plain mode), [3] fused generation (fused mode), [4] the epoch kernel and
+ * [5] the donor draws ahead of it (epoch mode; launch durations divided by
+ * the generations they cover); unused roles are 0.  avg_ms_out holds 6
+ * doubles.  Migration follows the same schedule as speo_engine_run.
  * Synchronous. */
 SPEO_EXPORT int speo_engine_profile(speo_engine e, int generations, double *avg_ms_out);
 

@@ -393,6 +393,16 @@ struct Engine {
     int run(int generations);
     int profile(int generations, double *avg_ms);
     int profile_epoch(int generations, double *avg_ms);
+    // when set, launch_epoch_chunk records an event pair around every launch:
+    // (role 4 = epoch kernel, role 5 = donor draws, events, generations covered)
+    struct ProfEvent {
+        int role;
+        hipEvent_t a, b;
+        int gens;
+    };
+    std::vector<ProfEvent> *prof = nullptr;
+    int prof_begin(int role, int gens, hipStream_t s);
+    int prof_end(hipStream_t s);
     int next_chunk(int left, bool *mig) const;
     int autotune();
     int best(int *row_out, float *f32_out, double *f64_out);
@@ -700,8 +710,10 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
     a.rows = total_rows;
     for (int g0 = 0; g0 < L; g0 += donor_gens) {
         const int Ls = std::min(donor_gens, L - g0);
+        SPEO_TRY(prof_begin(5, Ls, s));
         SPEO_TRY(epoch_donors(isl_dev, row_island_dev, total_rows, Ls, cfg.seed, gen_base,
                               (uint32_t)g0, donor_buf, s));
+        SPEO_TRY(prof_end(s));
         if (epoch_parts == 1) {
             a.Xin = X;
             a.Xout = X;
@@ -711,9 +723,12 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
             a.donor_gen = 0;
             a.L = Ls;
             a.write_trials = 1;
+            SPEO_TRY(prof_begin(4, Ls, s));
             SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, epoch_parts > 1, s));
+            SPEO_TRY(prof_end(s));
             continue;
         }
+        SPEO_TRY(prof_begin(4, Ls, s));  // one event pair around the sub-chunk's launches
         for (int g = 0; g < Ls; ++g) {
             a.Xin = X;
             a.Xout = Xnext;
@@ -726,6 +741,7 @@ int Engine::launch_epoch_chunk(int L, hipStream_t s) {
             SPEO_TRY(launch_epoch(cfg.precision, a, epoch_blocks, epoch_lds, epoch_parts > 1, s));
             swap_epoch_buffers();
         }
+        SPEO_TRY(prof_end(s));
     }
     return SPEO_OK;
 }
@@ -1003,7 +1019,7 @@ int Engine::run(int generations) {
 
 int Engine::profile(int generations, double *avg_ms) {
     if (cfg.mode == SPEO_MODE_EPOCH) return profile_epoch(generations, avg_ms);
-    const int roles = 5;
+    const int roles = 6;
     std::vector<hipEvent_t> ev;
     std::vector<int> role;
     auto record = [&](int r) -> int {
@@ -1042,8 +1058,8 @@ int Engine::profile(int generations, double *avg_ms) {
         if (mig) ++migrations;
     }
     SPEO_TRY_HIP(hipStreamSynchronize(stream));
-    double sum[roles] = {0, 0, 0, 0, 0};
-    int cnt[roles] = {0, 0, 0, 0, 0};
+    double sum[roles] = {0, 0, 0, 0, 0, 0};
+    int cnt[roles] = {0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < role.size(); ++i) {
         float ms = 0.f;
         SPEO_TRY_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
@@ -1057,37 +1073,53 @@ int Engine::profile(int generations, double *avg_ms) {
 
 // Epoch mode: epochs on the run schedule, HIP events around every epoch
 // launch; role 4 is the launch duration per generation.
+int Engine::prof_begin(int role, int gens_covered, hipStream_t s) {
+    if (!prof) return SPEO_OK;
+    ProfEvent p{role, nullptr, nullptr, gens_covered};
+    SPEO_TRY_HIP(hipEventCreate(&p.a));
+    SPEO_TRY_HIP(hipEventCreate(&p.b));
+    prof->push_back(p);
+    SPEO_TRY_HIP(hipEventRecord(p.a, s));
+    return SPEO_OK;
+}
+
+int Engine::prof_end(hipStream_t s) {
+    if (!prof) return SPEO_OK;
+    SPEO_TRY_HIP(hipEventRecord(prof->back().b, s));
+    return SPEO_OK;
+}
+
 int Engine::profile_epoch(int generations, double *avg_ms) {
-    for (int r = 0; r < 5; ++r) avg_ms[r] = 0.0;
-    hipEvent_t e0, e1;
-    SPEO_TRY_HIP(hipEventCreate(&e0));
-    SPEO_TRY_HIP(hipEventCreate(&e1));
-    double total_ms = 0.0;
+    for (int r = 0; r < 6; ++r) avg_ms[r] = 0.0;
+    std::vector<ProfEvent> events;
+    prof = &events;
     int64_t total_gens = 0;
     int left = generations, rc = SPEO_OK;
     while (left > 0 && rc == SPEO_OK) {
         bool mig = false;
         const int L = next_chunk(left, &mig);
-        float ms = 0.f;
-        if ((rc = check_hip(hipEventRecord(e0, stream), "hipEventRecord")) != SPEO_OK) break;
         if ((rc = launch_epoch_chunk(L, stream)) != SPEO_OK) break;
-        if ((rc = check_hip(hipEventRecord(e1, stream), "hipEventRecord")) != SPEO_OK) break;
         if (mig && (rc = migrate(stream)) != SPEO_OK) break;
         hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, stream, gen_base, (uint32_t)L);
         if ((rc = check_launch("k_add_u32")) != SPEO_OK) break;
-        if ((rc = check_hip(hipEventSynchronize(e1), "hipEventSynchronize")) != SPEO_OK) break;
-        if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != SPEO_OK) break;
-        total_ms += ms;
         total_gens += L;
         gens += L;
         fes += (int64_t)L * total_rows;
         if (mig) ++migrations;
         left -= L;
     }
+    prof = nullptr;
     if (rc == SPEO_OK) rc = check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (total_gens > 0) avg_ms[4] = total_ms / (double)total_gens;
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    for (const ProfEvent &p : events) {
+        float ms = 0.f;
+        if (rc == SPEO_OK) rc = check_hip(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        sum[p.role] += ms;
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    if (total_gens > 0)
+        for (int r = 4; r < 6; ++r) avg_ms[r] = sum[r] / (double)total_gens;
     return rc;
 }
 

@@ -239,10 +239,11 @@ class Engine:
 
     def profile(self, generations: int):
         """Average kernel ms per generation and role: generate, evaluate, select
-        (plain mode), fused (fused mode), epoch (epoch mode: launch / generations)."""
-        out = np.zeros(5, np.float64)
+        (plain mode), fused (fused mode), epoch and donors (epoch mode: the epoch
+        kernel and the donor draws ahead of it, per generation)."""
+        out = np.zeros(6, np.float64)
         check(lib().speo_engine_profile(self.handle, generations, _np_ptr(out)))
-        return dict(zip(("generate", "evaluate", "select", "fused", "epoch"), out.tolist()))
+        return dict(zip(("generate", "evaluate", "select", "fused", "epoch", "donors"), out.tolist()))
 
     def surrogate_stats(self):
         n = len(self.problems)

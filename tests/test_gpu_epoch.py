@@ -112,7 +112,7 @@ def test_epoch_profile_reports_epoch_time(cuda, speo, oracle):
     e = speo.Engine(probs, [64, 64], [0, 0], mode=speo.MODE_EPOCH, interval=10, migration_size=2,
                     seed=SEED)
     kms = e.profile(25)
-    assert kms["epoch"] > 0 and kms["fused"] == 0
+    assert kms["epoch"] > 0 and kms["donors"] > 0 and kms["fused"] == 0
     assert e.stats()["generations"] == 25 and e.stats()["migrations"] == 3
 
 
