@@ -37,7 +37,8 @@ __device__ __forceinline__ float comp4(const float4 &v, int q) {
 }
 
 constexpr int kFixedLds = 128 * 4 + 128 * 4;  // shift vector, zero pad after the rows
-constexpr int kPassRowBytes = 16 + 8 + 4 + 32;  // donor offsets, j_rand + trial row, A row, partials
+// per pass row: donor offsets, j_rand + trial row, A row, per-tile partials
+constexpr int kPassRowBytes = 16 + 8 + 4 + 2 * 8 * 4;
 
 
 int lds_row_stride(int dim) {
@@ -110,30 +111,35 @@ int epoch_donors(const IslandDesc *isl, const int *row_island, int rows, int L, 
     return check_launch("k_epoch_donors");
 }
 
-// Per-coordinate terms of one 16-row tile's two column tiles, summed over the
-// 16 lanes of each row (gemm_small's epilogue with the objective fixed).
+// Per-coordinate terms of one 16-row tile's two column tiles, each tile
+// summed over its 16 lanes per row (gemm_small's epilogue with the objective
+// fixed): s1[t][q], s2[t][q].
 template <int OBJ>
 __device__ __forceinline__ void tile_terms(const floatx4 &acc0, const floatx4 &acc1, bool c0, bool c1,
-                                           float w0, float w1, float unscale, float (&s1)[4],
-                                           float (&s2)[4]) {
+                                           float w0, float w1, float unscale, float (&s1)[2][4],
+                                           float (&s2)[2][4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        s1[q] = 0.f;
-        s2[q] = 0.f;
-    }
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s1[t][q] = 0.f;
+            s2[t][q] = 0.f;
+        }
     if (c0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc0[q] * unscale, w0, s1[q], s2[q]);
+        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc0[q] * unscale, w0, s1[0][q], s2[0][q]);
     }
     if (c1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc1[q] * unscale, w1, s1[q], s2[q]);
+        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc1[q] * unscale, w1, s1[1][q], s2[1][q]);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        s1[q] = sum16(s1[q]);
-        if (OBJ == SPEO_OBJ_ACKLEY) s2[q] = sum16(s2[q]);
-    }
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s1[t][q] = sum16(s1[t][q]);
+            if (OBJ == SPEO_OBJ_ACKLEY) s2[t][q] = sum16(s2[t][q]);
+        }
 }
 
 // SPLIT: islands split over several workgroups, one generation per launch,
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                  "s"(D), "s"(pl.S));
 
     // LDS: fX[NPp] fU[NPp] flags[NPp] shift[128] doff[PR] tinfo[PR] aoff[PR]
-    //      part[8][PR] rows (+128 zero floats)
+    //      part[16][PR] rows (+128 floats)
     float *fXs = reinterpret_cast<float *>(lds);
     float *fUs = fXs + NPp;
     uint8_t *flags = reinterpret_cast<uint8_t *>(fUs + NPp);  // HBM plan: bit 0 current slot, bit 1
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     float *part = reinterpret_cast<float *>(aoff + PR);
     // LDS plan: current rows [NPp][S], then the share's trial rows [OWN][ST];
     // HBM plan: the pass's trials [PR][ST]
-    float *rows = part + 8 * PR;
+    float *rows = part + 16 * PR;
     const int trial0 = NPp * S;
 
     // Row addresses: in LDS a float offset into `rows`; in HBM 2 * (r * ld) + slot.
@@ -486,7 +492,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                     }
                 }
                 const float unscale = H ? kF16Unscale : 1.0f;
-                float s1[4], s2[4];
+                float s1[2][4], s2[2][4];
                 switch (blk.objective) {
                 case SPEO_OBJ_ELLIPTIC:
                     tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, acc1, c0, c1, w0, w1, unscale, s1, s2);
@@ -500,21 +506,23 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                 }
                 if (lr == 0) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int rl = tile * 16 + lk * 4 + q;
-                        part[wq * PR + rl] = s1[q];
-                        part[(4 + wq) * PR + rl] = s2[q];
-                    }
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int rl = tile * 16 + lk * 4 + q, ct = wq + 4 * t;
+                            part[(2 * ct) * PR + rl] = s1[t][q];
+                            part[(2 * ct + 1) * PR + rl] = s2[t][q];
+                        }
                 }
             }
             __syncthreads();
-            // the four column groups' partial sums, in column-group order
+            // the column tiles' partial sums, tiles 0..7 in order
             for (int rl = tid; rl < nr; rl += kEpochThreads) {
                 float s1 = 0.f, s2 = 0.f;
 #pragma unroll
-                for (int w = 0; w < 4; ++w) {
-                    s1 += part[w * PR + rl];
-                    s2 += part[(4 + w) * PR + rl];
+                for (int t = 0; t < 8; ++t) {
+                    s1 += part[(2 * t) * PR + rl];
+                    s2 += part[(2 * t + 1) * PR + rl];
                 }
                 fUs[p0 + rl] = fmaf(lambda, finish_objective(blk.objective, s1, s2, D), bias);
             }
