@@ -52,6 +52,14 @@ int lds_row_stride(int dim) {
 // rows carry zeroed padding to a multiple of 16 columns (+4 for the banks).
 int lds_trial_stride(int dim) { return round_up(round_up(dim, 4), 16) + 4; }
 
+// LDS of k_island_split: fX, fU, flags of the share [own], shift [128],
+// donor offsets [own] int4, j_rand [own], partials [16][own], rows [NPp][S],
+// trial rows and their shifted, scaled copies [own][ST] each.
+int split_lds_bytes(int np, int dim, int own) {
+    const int NPp = round_up(np, 16), S = lds_row_stride(dim), ST = lds_trial_stride(dim);
+    return own * (4 + 4 + 4 + 16 + 4 + 16 * 4) + 128 * 4 + (NPp * S + 2 * own * ST) * 4;
+}
+
 }  // namespace
 
 EpochPlan epoch_plan(int np, int dim, int parts) {
@@ -63,7 +71,8 @@ EpochPlan epoch_plan(int np, int dim, int parts) {
     const int own = parts > 1 ? round_up((np + parts - 1) / parts, 16) : NPp;
     p.own_rows = own;
     const int fixed = NPp * 8 + round_up(NPp, 16) + kFixedLds;  // fX, fU, flags, shift, pad
-    const int in_lds = fixed + own * kPassRowBytes + (NPp * p.S + own * p.ST) * 4;
+    int in_lds = fixed + own * kPassRowBytes + (NPp * p.S + own * p.ST) * 4;
+    if (parts > 1) in_lds = std::max(in_lds, split_lds_bytes(np, dim, own));
     if (in_lds <= kEpochLdsMax) {
         p.in_lds = 1;
         p.pass_rows = own;
@@ -580,6 +589,264 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     }
 }
 
+
+// One generation of islands split over several workgroups, FP32 (the split
+// launches of the rotation in single precision).  1024 threads: 16 waves,
+// two per SIMD more than the general kernel, to hide the LDS and memory
+// latency of the short, single-generation launch.
+//   prologue: the island's rows (linear buffer loads), the share's fitness,
+//     donors and the shift; waves 0..7 load the rotation fragments of their
+//     column tile (t = wave) for the evaluation;
+//   trials: one item (row, four coordinates) per lane; the trial row and its
+//     shifted, scaled copy (the evaluation's A operand, (u - o) * rate, as
+//     the general kernel forms it) go to LDS;
+//   evaluation: waves 0..7, column tile t = wave, every 16-row tile of the
+//     share, two tiles per K loop; per-tile sums as everywhere else;
+//   combine (tiles 0..7 in order), selection, write-back of the selected row.
+constexpr int kSplitThreads = 1024;
+
+template <int OBJ>
+__device__ __forceinline__ void col_tile_terms(const floatx4 &acc, bool c, float w, float (&s1)[4],
+                                               float (&s2)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        s1[q] = 0.f;
+        s2[q] = 0.f;
+    }
+    if (c) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc[q], w, s1[q], s2[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        s1[q] = sum16(s1[q]);
+        if (OBJ == SPEO_OBJ_ACKLEY) s2[q] = sum16(s2[q]);
+    }
+}
+
+__global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    const EpochBlock blk = a.blocks[blockIdx.x];
+    const IslandDesc &I = blk.I;
+    const EpochPlan &pl = blk.pl;
+    const float lambda = blk.lambda, bias = blk.bias;
+    const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, ST = pl.ST;
+    const int OWN = pl.own_rows;
+    const int own0 = min(NP, blk.share * OWN);
+    const int nown = min(NP, own0 + OWN) - own0;
+    const int ng = (D + 3) >> 2;
+    const uint32_t inv_ng = blk.inv_ng;
+    auto div_ng = [&](int it) -> int { return ng > 1 ? (int)__umulhi((uint32_t)it, inv_ng) : it; };
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lr = lane & 15, lk = lane >> 4;
+    const uint32_t crt = blk.crt;
+    const uint32_t gen = *a.gen_base + a.gen_offset;
+    asm volatile("" ::"s"(a.Xin), "s"(a.fXin), "s"(a.donors), "s"(a.rows), "s"(a.donor_gen), "s"(gen),
+                 "s"(blk.B32s), "s"(blk.shift), "s"(blk.colw), "s"(I.elem_off), "s"(I.row_off), "s"(NP),
+                 "s"(D), "s"(pl.S));
+
+    float *fXs = reinterpret_cast<float *>(lds);   // [OWN] the share's rows only
+    float *fUs = fXs + OWN;
+    int *acc_f = reinterpret_cast<int *>(fUs + OWN);
+    float *sh = reinterpret_cast<float *>(acc_f + OWN);
+    int4 *doff = reinterpret_cast<int4 *>(sh + 128);
+    int *jr = reinterpret_cast<int *>(doff + OWN);
+    float *part = reinterpret_cast<float *>(jr + OWN);  // [16][OWN]
+    float *rows = part + 16 * OWN;                       // [NPp][S]
+    float *trial = rows + NPp * S;                        // [OWN][ST]
+    float *arow = trial + OWN * ST;                       // [OWN][ST]
+
+    // ---- prologue: every global load first
+    const float *gXin = a.Xin + I.elem_off;
+    constexpr int kBatch = 8;
+    const int total = NP * ng;
+    floatx4 pre[kBatch];
+    {
+        const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(gXin), (short)0, total * 16, kBufferWord3);
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b)
+            if (b * kSplitThreads < total)
+                pre[b] = __builtin_bit_cast(
+                    floatx4, __builtin_amdgcn_raw_buffer_load_b128(xr, (tid + b * kSplitThreads) * 16, 0, 0));
+    }
+    const float shift_k = tid < 128 && tid < D ? gld(blk.shift + tid) : 0.f;
+    float fx_r;
+    int4 d_r;
+    if (tid < nown) {
+        fx_r = a.fXin[I.row_off + own0 + tid];
+        d_r = a.donors[(size_t)a.donor_gen * a.rows + I.row_off + own0 + tid];
+    }
+    // rotation fragments and column weights of the evaluating waves
+    const int t = wave & 7;
+    const bool evaluates = wave < 8;
+    const bool vt = t * 16 < D, ct = t * 16 + lr < D;
+    const int KS = ng;
+    float b[32];
+    float wcol = 1.0f;
+    if (evaluates) {
+        load_b_frag(blk.B32s, t, vt, KS, lane, b);
+        if (blk.objective == SPEO_OBJ_ELLIPTIC && ct) wcol = gld(blk.colw + t * 16 + lr);
+    }
+
+    // ---- stores
+    if (tid < 128) sh[tid] = shift_k;
+    if (tid < nown) {
+        fXs[tid] = fx_r;
+        const int r = own0 + tid;
+        doff[tid] = int4{d_r.x * S, d_r.y * S, d_r.z * S, r * S};
+        jr[tid] = d_r.w;
+    }
+    const bool linear = (D & 3) == 0 && S == D;
+    auto put_row_group = [&](int it, float4 x) {
+        if (linear) {
+            st4(rows + 4 * it, x);
+            return;
+        }
+        const int r = div_ng(it), q4 = it - r * ng, j = 4 * q4;
+        if (j + 1 >= D) x.y = 0.f;
+        if (j + 2 >= D) x.z = 0.f;
+        if (j + 3 >= D) x.w = 0.f;
+        st4(rows + r * S + 4 * q4, x);
+    };
+#pragma unroll
+    for (int bb = 0; bb < kBatch; ++bb) {
+        const int it = tid + bb * kSplitThreads;
+        if (it < total) put_row_group(it, float4{pre[bb][0], pre[bb][1], pre[bb][2], pre[bb][3]});
+    }
+    for (int it = tid + kBatch * kSplitThreads; it < total; it += kSplitThreads)
+        put_row_group(it, ld4(gXin + 4 * it));
+    {
+        // the shifted copies' columns [4 ng, ST) are read by the evaluation's
+        // blocks of four K steps and stay zero
+        const int pad = ST - 4 * ng;
+        for (int e = tid; e < OWN * pad; e += kSplitThreads) {
+            const int r = e / pad;
+            arow[r * ST + 4 * ng + (e - r * pad)] = 0.f;
+        }
+    }
+    __syncthreads();
+
+    // ---- trials: one item per lane
+    const float rate = blk.rate;
+    for (int it = tid; it < nown * ng; it += kSplitThreads) {
+        const int rl = div_ng(it), q4 = it - rl * ng;
+        const int4 o = doff[rl];
+        const int jrand = jr[rl];
+        const float4 e0 = ld4(rows + o.x + 4 * q4), e1 = ld4(rows + o.y + 4 * q4);
+        const float4 e2 = ld4(rows + o.z + 4 * q4), xo = ld4(rows + o.w + 4 * q4);
+        const float4 sv = ld4(sh + 4 * q4);
+        const U4 w = philox_block(a.seed, (uint32_t)q4, (uint32_t)(I.row_off + own0 + rl), gen, kPurposeCross);
+        float v[4];
+        bool out = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = 4 * q4 + q;
+            const bool take = (j == jrand) | cross_take(word(w, q), crt);
+            const float mt = fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q));
+            v[q] = j < D ? (take ? mt : comp4(xo, q)) : 0.f;
+            out |= (j < D) & ((v[q] < a.lo) | (v[q] > a.hi));
+        }
+        if (out) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (4 * q4 + q < D) v[q] = check_bound(v[q], a.lo, a.hi);
+        }
+        st4(trial + rl * ST + 4 * q4, float4{v[0], v[1], v[2], v[3]});
+        st4(arow + rl * ST + 4 * q4,
+            float4{(v[0] - sv.x) * rate, (v[1] - sv.y) * rate, (v[2] - sv.z) * rate, (v[3] - sv.w) * rate});
+    }
+    __syncthreads();
+
+    // ---- evaluation: waves 0..7, column tile t, the 16-row tiles two at a time
+    if (evaluates) {
+        const int ntile = (nown + 15) >> 4;
+        for (int tile = 0; tile < ntile; tile += 2) {
+            const bool two = tile + 1 < ntile;
+            const float *ap0 = arow + min(tile * 16 + lr, nown - 1) * ST + lk;
+            const float *ap1 = arow + min(tile * 16 + 16 + lr, nown - 1) * ST + lk;
+            floatx4 acc0 = floatx4{0.f, 0.f, 0.f, 0.f}, acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+            if (vt) {
+#pragma unroll
+                for (int kb = 0; kb < 32; kb += 4) {
+                    if (kb >= KS) break;
+                    float a0[4], a1[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        a0[u] = ap0[(kb + u) * 4];
+                        a1[u] = ap1[(kb + u) * 4];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b[kb + u], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b[kb + u], acc1, 0, 0, 0);
+                    }
+                }
+            }
+            float s1[2][4], s2[2][4];
+            switch (blk.objective) {
+            case SPEO_OBJ_ELLIPTIC:
+                col_tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, ct, wcol, s1[0], s2[0]);
+                col_tile_terms<SPEO_OBJ_ELLIPTIC>(acc1, ct, wcol, s1[1], s2[1]);
+                break;
+            case SPEO_OBJ_RASTRIGIN:
+                col_tile_terms<SPEO_OBJ_RASTRIGIN>(acc0, ct, wcol, s1[0], s2[0]);
+                col_tile_terms<SPEO_OBJ_RASTRIGIN>(acc1, ct, wcol, s1[1], s2[1]);
+                break;
+            default:
+                col_tile_terms<SPEO_OBJ_ACKLEY>(acc0, ct, wcol, s1[0], s2[0]);
+                col_tile_terms<SPEO_OBJ_ACKLEY>(acc1, ct, wcol, s1[1], s2[1]);
+                break;
+            }
+            if (lr == 0) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    if (i == 1 && !two) break;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int rl = (tile + i) * 16 + lk * 4 + q;
+                        part[(2 * t) * OWN + rl] = s1[i][q];
+                        part[(2 * t + 1) * OWN + rl] = s2[i][q];
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- combine (tiles 0..7 in order) and selection, strict "<"
+    if (tid < nown) {
+        float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+        for (int tt = 0; tt < 8; ++tt) {
+            s1 += part[(2 * tt) * OWN + tid];
+            s2 += part[(2 * tt + 1) * OWN + tid];
+        }
+        const float fu = fmaf(lambda, finish_objective(blk.objective, s1, s2, D), bias);
+        const float fx = fXs[tid];
+        const bool acc = fu < fx;
+        fUs[tid] = fu;
+        fXs[tid] = acc ? fu : fx;
+        acc_f[tid] = acc ? 1 : 0;
+    }
+    __syncthreads();
+
+    // ---- write-back: the selected rows, and at the end of a chunk the trials
+    float *gX = a.Xout + I.elem_off, *gU = a.U + I.elem_off;
+    for (int it = tid; it < nown * ng; it += kSplitThreads) {
+        const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
+        const float4 tri = ld4(trial + rl * ST + 4 * q4);
+        const float4 cur = acc_f[rl] ? tri : ld4(rows + r * S + 4 * q4);
+        st4(gX + r * I.ld + 4 * q4, cur);
+        if (a.write_trials) st4(gU + r * I.ld + 4 * q4, tri);
+    }
+    if (tid < nown) {
+        a.fXout[I.row_off + own0 + tid] = fXs[tid];
+        if (a.write_trials) a.fU[I.row_off + own0 + tid] = fUs[tid];
+    }
+}
+
 template <int PREC, bool SPLIT>
 static int launch_epoch_t(const EpochArgs &a, int nblocks, int lds, hipStream_t s) {
     static std::once_flag once;
@@ -593,8 +860,21 @@ static int launch_epoch_t(const EpochArgs &a, int nblocks, int lds, hipStream_t 
     return check_launch("k_island_epoch");
 }
 
+static int launch_split_fp32(const EpochArgs &a, int nblocks, int lds, hipStream_t s) {
+    static std::once_flag once;
+    static hipError_t attr = hipSuccess;
+    std::call_once(once, [] {
+        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_split),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kEpochLdsMax);
+    });
+    if (attr != hipSuccess) return check_hip(attr, "hipFuncSetAttribute(k_island_split)");
+    hipLaunchKernelGGL(k_island_split, dim3(nblocks), dim3(kSplitThreads), lds, s, a);
+    return check_launch("k_island_split");
+}
+
 template <int PREC>
 static int launch_epoch_p(const EpochArgs &a, int nblocks, int lds, bool split, hipStream_t s) {
+    if (split && PREC == SPEO_PREC_FP32) return launch_split_fp32(a, nblocks, lds, s);
     return split ? launch_epoch_t<PREC, true>(a, nblocks, lds, s) : launch_epoch_t<PREC, false>(a, nblocks, lds, s);
 }
 
