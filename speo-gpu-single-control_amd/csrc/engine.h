@@ -75,8 +75,11 @@ struct EpochArgs {
 };
 
 // Plan of one island (dim <= 128, single component) split over `parts`
-// workgroups; lds_bytes > kEpochLdsMax means it does not fit.
-EpochPlan epoch_plan(int np, int dim, int parts);
+// workgroups, for a precision with `parts16` 16-bit operand parts (0: FP32);
+// lds_bytes > kEpochLdsMax means it does not fit.
+EpochPlan epoch_plan(int np, int dim, int parts, int parts16);
+// 16-bit operand parts of a precision (0 for FP32).
+int precision_parts(int precision);
 
 // split: islands split over workgroups (LDS plan, a.L == 1).
 int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, bool split, hipStream_t s);

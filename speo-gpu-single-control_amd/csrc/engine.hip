@@ -600,7 +600,7 @@ int Engine::epoch_plan_all(int parts, std::vector<EpochPlan> *plans, int *lds) {
     *lds = 0;
     for (int i = 0; i < I; ++i) {
         const IslandDesc &d = isl[(size_t)i];
-        const EpochPlan p = epoch_plan(d.np, d.dim, parts);
+        const EpochPlan p = epoch_plan(d.np, d.dim, parts, precision_parts(cfg.precision));
         if (p.lds_bytes > kEpochLdsMax) return SPEO_ERR_UNSUPPORTED;
         if (parts > 1 && d.np < 16 * parts) return SPEO_ERR_UNSUPPORTED;
         (*plans)[(size_t)i] = p;

@@ -55,14 +55,31 @@ int lds_trial_stride(int dim) { return round_up(round_up(dim, 4), 16) + 4; }
 // LDS of k_island_split: fX, fU, flags of the share [own], shift [128],
 // donor offsets [own] int4, j_rand [own], partials [16][own], rows [NPp][S],
 // trial rows and their shifted, scaled copies [own][ST] each.
-int split_lds_bytes(int np, int dim, int own) {
+// The A operand region holds the FP32 copies (parts16 == 0) or the 16-bit
+// parts of every element [parts16][own][kp32 + 8].
+int split_a_bytes(int dim, int own, int parts16) {
+    return parts16 == 0 ? own * lds_trial_stride(dim) * 4 : parts16 * own * (round_up(dim, 32) + 8) * 2;
+}
+int split_lds_bytes(int np, int dim, int own, int parts16) {
     const int NPp = round_up(np, 16), S = lds_row_stride(dim), ST = lds_trial_stride(dim);
-    return own * (4 + 4 + 4 + 16 + 4 + 16 * 4) + 128 * 4 + (NPp * S + 2 * own * ST) * 4;
+    return own * (4 + 4 + 4 + 16 + 4 + 16 * 4) + 128 * 4 + (NPp * S + own * ST) * 4 +
+           split_a_bytes(dim, own, parts16);
 }
 
 }  // namespace
 
-EpochPlan epoch_plan(int np, int dim, int parts) {
+int precision_parts(int precision) {
+    switch (precision) {
+    case SPEO_PREC_BF16: return Prec<SPEO_PREC_BF16>::parts;
+    case SPEO_PREC_FP16: return Prec<SPEO_PREC_FP16>::parts;
+    case SPEO_PREC_BF16X3: return Prec<SPEO_PREC_BF16X3>::parts;
+    case SPEO_PREC_FP16X3: return Prec<SPEO_PREC_FP16X3>::parts;
+    case SPEO_PREC_BF16X6: return Prec<SPEO_PREC_BF16X6>::parts;
+    default: return 0;
+    }
+}
+
+EpochPlan epoch_plan(int np, int dim, int parts, int parts16) {
     EpochPlan p{};
     p.S = lds_row_stride(dim);
     p.ST = lds_trial_stride(dim);
@@ -72,7 +89,7 @@ EpochPlan epoch_plan(int np, int dim, int parts) {
     p.own_rows = own;
     const int fixed = NPp * 8 + round_up(NPp, 16) + kFixedLds;  // fX, fU, flags, shift, pad
     int in_lds = fixed + own * kPassRowBytes + (NPp * p.S + own * p.ST) * 4;
-    if (parts > 1) in_lds = std::max(in_lds, split_lds_bytes(np, dim, own));
+    if (parts > 1) in_lds = split_lds_bytes(np, dim, own, parts16);  // the split kernel's layout
     if (in_lds <= kEpochLdsMax) {
         p.in_lds = 1;
         p.pass_rows = own;
@@ -590,16 +607,16 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
 }
 
 
-// One generation of islands split over several workgroups, FP32 (the split
-// launches of the rotation in single precision).  1024 threads: 16 waves,
-// two per SIMD more than the general kernel, to hide the LDS and memory
-// latency of the short, single-generation launch.
+// One generation of islands split over several workgroups.  1024 threads:
+// 16 waves, two per SIMD more than the general kernel, to hide the LDS and
+// memory latency of the short, single-generation launch.
 //   prologue: the island's rows (linear buffer loads), the share's fitness,
 //     donors and the shift; waves 0..7 load the rotation fragments of their
 //     column tile (t = wave) for the evaluation;
-//   trials: one item (row, four coordinates) per lane; the trial row and its
-//     shifted, scaled copy (the evaluation's A operand, (u - o) * rate, as
-//     the general kernel forms it) go to LDS;
+//   trials: one item (row, four coordinates) per lane; the trial row and the
+//     evaluation's A operand go to LDS: the shifted, scaled copy (u - o) *
+//     rate in FP32, or its 16-bit parts, formed as the general kernel and
+//     gemm_small form them;
 //   evaluation: waves 0..7, column tile t = wave, every 16-row tile of the
 //     share, two tiles per K loop; per-tile sums as everywhere else;
 //   combine (tiles 0..7 in order), selection, write-back of the selected row.
@@ -624,8 +641,12 @@ __device__ __forceinline__ void col_tile_terms(const floatx4 &acc, bool c, float
     }
 }
 
+template <int PREC>
 __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     extern __shared__ __align__(16) char lds[];
+    constexpr int P = Prec<PREC>::parts;
+    constexpr bool H = Prec<PREC>::half;
+    typedef typename std::conditional<H, halfx8, bf16x8>::type vec8;
     const EpochBlock blk = a.blocks[blockIdx.x];
     const IslandDesc &I = blk.I;
     const EpochPlan &pl = blk.pl;
@@ -655,7 +676,9 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     float *part = reinterpret_cast<float *>(jr + OWN);  // [16][OWN]
     float *rows = part + 16 * OWN;                       // [NPp][S]
     float *trial = rows + NPp * S;                        // [OWN][ST]
-    float *arow = trial + OWN * ST;                       // [OWN][ST]
+    float *arow = trial + OWN * ST;                       // FP32: [OWN][ST]
+    uint16_t *a16 = reinterpret_cast<uint16_t *>(arow);   // 16-bit: [P][OWN][LDA]
+    const int kp32 = blk.kp32, LDA = kp32 + 8;
 
     // ---- prologue: every global load first
     const float *gXin = a.Xin + I.elem_off;
@@ -682,11 +705,24 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     const int t = wave & 7;
     const bool evaluates = wave < 8;
     const bool vt = t * 16 < D, ct = t * 16 + lr < D;
-    const int KS = ng;
-    float b[32];
+    const int KS = P == 0 ? ng : kp32 >> 5;
+    constexpr int NB32 = P == 0 ? 32 : 1;
+    constexpr int NB16 = P == 0 ? 1 : P;
+    float b[NB32];
+    vec8 bb[4][NB16];
     float wcol = 1.0f;
     if (evaluates) {
-        load_b_frag(blk.B32s, t, vt, KS, lane, b);
+        if constexpr (P == 0) {
+            load_b_frag(blk.B32s, t, vt, KS, lane, b);
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    if (vt && ks < KS)
+                        bb[ks][p] = gld(reinterpret_cast<const vec8 *>(
+                            blk.M16s[p] + (size_t)(t * 16 + lr) * kp32 + ks * 32 + lk * 8));
+        }
         if (blk.objective == SPEO_OBJ_ELLIPTIC && ct) wcol = gld(blk.colw + t * 16 + lr);
     }
 
@@ -717,13 +753,20 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     }
     for (int it = tid + kBatch * kSplitThreads; it < total; it += kSplitThreads)
         put_row_group(it, ld4(gXin + 4 * it));
-    {
+    if constexpr (P == 0) {
         // the shifted copies' columns [4 ng, ST) are read by the evaluation's
         // blocks of four K steps and stay zero
         const int pad = ST - 4 * ng;
         for (int e = tid; e < OWN * pad; e += kSplitThreads) {
             const int r = e / pad;
             arow[r * ST + 4 * ng + (e - r * pad)] = 0.f;
+        }
+    } else {
+        // the parts' columns [4 ng, kp32) are zero
+        const int pad = kp32 - 4 * ng;
+        for (int e = tid; e < P * OWN * pad; e += kSplitThreads) {
+            const int r = e / pad;  // part * OWN + row
+            a16[r * LDA + 4 * ng + (e - r * pad)] = 0;
         }
     }
     __syncthreads();
@@ -754,8 +797,27 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
                 if (4 * q4 + q < D) v[q] = check_bound(v[q], a.lo, a.hi);
         }
         st4(trial + rl * ST + 4 * q4, float4{v[0], v[1], v[2], v[3]});
-        st4(arow + rl * ST + 4 * q4,
-            float4{(v[0] - sv.x) * rate, (v[1] - sv.y) * rate, (v[2] - sv.z) * rate, (v[3] - sv.w) * rate});
+        if constexpr (P == 0) {
+            st4(arow + rl * ST + 4 * q4,
+                float4{(v[0] - sv.x) * rate, (v[1] - sv.y) * rate, (v[2] - sv.z) * rate, (v[3] - sv.w) * rate});
+        } else {
+            const float scale = rate * (H ? kF16ScaleA : 1.0f);
+            typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+            u16x4 pb[P];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float rem = 4 * q4 + q < D ? (v[q] - comp4(sv, q)) * scale : 0.f;
+#pragma unroll
+                for (int pp = 0; pp < P; ++pp) {
+                    const uint16_t bits = to_bits(rem, H);
+                    pb[pp][q] = bits;
+                    rem -= from_bits(bits, H);
+                }
+            }
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp)
+                *reinterpret_cast<u16x4 *>(a16 + (pp * OWN + rl) * LDA + 4 * q4) = pb[pp];
+        }
     }
     __syncthreads();
 
@@ -764,10 +826,11 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
         const int ntile = (nown + 15) >> 4;
         for (int tile = 0; tile < ntile; tile += 2) {
             const bool two = tile + 1 < ntile;
-            const float *ap0 = arow + min(tile * 16 + lr, nown - 1) * ST + lk;
-            const float *ap1 = arow + min(tile * 16 + 16 + lr, nown - 1) * ST + lk;
+            const int r0 = min(tile * 16 + lr, nown - 1), r1 = min(tile * 16 + 16 + lr, nown - 1);
+            const float *ap0 = arow + r0 * ST + lk;
+            const float *ap1 = arow + r1 * ST + lk;
             floatx4 acc0 = floatx4{0.f, 0.f, 0.f, 0.f}, acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
-            if (vt) {
+            if (P == 0 && vt) {
 #pragma unroll
                 for (int kb = 0; kb < 32; kb += 4) {
                     if (kb >= KS) break;
@@ -782,6 +845,43 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
                         acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b[kb + u], acc0, 0, 0, 0);
                         acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b[kb + u], acc1, 0, 0, 0);
                     }
+                }
+            }
+            if constexpr (P > 0) {
+                if (vt) {
+                    // the parts' cross products, in gemm_small's order
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        if (ks >= KS) break;
+                        vec8 a0[P], a1[P];
+#pragma unroll
+                        for (int pp = 0; pp < P; ++pp) {
+                            a0[pp] = *reinterpret_cast<const vec8 *>(a16 + (pp * OWN + r0) * LDA + ks * 32 + lk * 8);
+                            a1[pp] = *reinterpret_cast<const vec8 *>(a16 + (pp * OWN + r1) * LDA + ks * 32 + lk * 8);
+                        }
+#pragma unroll
+                        for (int sum = P - 1; sum >= 0; --sum) {
+#pragma unroll
+                            for (int pa = 0; pa <= sum; ++pa) {
+                                const int pbi = sum - pa;
+                                if (pa < P && pbi < P) {
+                                    if constexpr (H) {
+                                        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0[pa], bb[ks][pbi], acc0, 0, 0, 0);
+                                        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1[pa], bb[ks][pbi], acc1, 0, 0, 0);
+                                    } else {
+                                        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[pa], bb[ks][pbi], acc0, 0, 0, 0);
+                                        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[pa], bb[ks][pbi], acc1, 0, 0, 0);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                const float unscale = H ? kF16Unscale : 1.0f;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    acc0[q] *= unscale;
+                    acc1[q] *= unscale;
                 }
             }
             float s1[2][4], s2[2][4];
@@ -860,22 +960,22 @@ static int launch_epoch_t(const EpochArgs &a, int nblocks, int lds, hipStream_t 
     return check_launch("k_island_epoch");
 }
 
-static int launch_split_fp32(const EpochArgs &a, int nblocks, int lds, hipStream_t s) {
+template <int PREC>
+static int launch_split_t(const EpochArgs &a, int nblocks, int lds, hipStream_t s) {
     static std::once_flag once;
     static hipError_t attr = hipSuccess;
     std::call_once(once, [] {
-        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_split),
+        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_split<PREC>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, kEpochLdsMax);
     });
     if (attr != hipSuccess) return check_hip(attr, "hipFuncSetAttribute(k_island_split)");
-    hipLaunchKernelGGL(k_island_split, dim3(nblocks), dim3(kSplitThreads), lds, s, a);
+    hipLaunchKernelGGL((k_island_split<PREC>), dim3(nblocks), dim3(kSplitThreads), lds, s, a);
     return check_launch("k_island_split");
 }
 
 template <int PREC>
 static int launch_epoch_p(const EpochArgs &a, int nblocks, int lds, bool split, hipStream_t s) {
-    if (split && PREC == SPEO_PREC_FP32) return launch_split_fp32(a, nblocks, lds, s);
-    return split ? launch_epoch_t<PREC, true>(a, nblocks, lds, s) : launch_epoch_t<PREC, false>(a, nblocks, lds, s);
+    return split ? launch_split_t<PREC>(a, nblocks, lds, s) : launch_epoch_t<PREC, false>(a, nblocks, lds, s);
 }
 
 int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, bool split, hipStream_t s) {
