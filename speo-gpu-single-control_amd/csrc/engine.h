@@ -15,16 +15,15 @@ struct FusedProblem {
 
 // ------------------------------------------------------------ epoch kernel
 //
-// A workgroup runs a whole epoch (the generations between two migrations) of
-// one island, or of its share of the rows when an island is split over
-// `parts` workgroups.
-//   LDS plan: the island's current rows and the trial rows of the workgroup's
-//     share live in LDS; an accepted trial is copied over its row.  With
-//     parts > 1 a launch runs one generation: every workgroup loads all rows
-//     of its island from the input population, generates, evaluates and
-//     selects its share and writes the share's rows to the output population
-//     (the next launch's input), so the kernel boundary is the only
-//     synchronisation between the shares.
+// k_island_epoch: a workgroup runs a whole epoch (the generations between
+// two migrations) of one island.
+//   LDS plan: the island's current rows and its trial rows live in LDS; an
+//     accepted trial is copied over its row.
+// k_island_split: an island split over `parts` workgroups, one generation
+//   per launch: every workgroup loads all rows of its island from the input
+//   population, generates, evaluates and selects its share and writes the
+//   share's rows to the output population (the next launch's input), so the
+//   kernel boundary is the only synchronisation between the shares.
 //   HBM plan (one workgroup per island, islands too large for LDS): every row
 //     has two slots in HBM (slot 0 = X, slot 1 = a scratch population), an
 //     accepted trial flips the row's slot bit, and the trials of a pass of

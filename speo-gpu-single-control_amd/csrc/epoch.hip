@@ -168,9 +168,9 @@ __device__ __forceinline__ void tile_terms(const floatx4 &acc0, const floatx4 &a
         }
 }
 
-// SPLIT: islands split over several workgroups, one generation per launch,
-// rows in LDS (the general kernel's loops and the HBM plan compile away).
-template <int PREC, bool SPLIT>
+// One workgroup per island, all generations of an epoch (islands split over
+// several workgroups run k_island_split instead, one generation per launch).
+template <int PREC>
 __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     extern __shared__ __align__(16) char lds[];
     constexpr int P = Prec<PREC>::parts;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     const int share = blk.share;
     const float lambda = blk.lambda, bias = blk.bias;
     const int NP = I.np, NPp = (NP + 15) & ~15, D = I.dim, S = pl.S, ST = pl.ST, PR = pl.pass_rows;
-    const bool in_lds = SPLIT || pl.in_lds != 0;
+    const bool in_lds = pl.in_lds != 0;
     // the workgroup's share of the rows: [own0, own0 + nown)
     const int OWN = in_lds ? pl.own_rows : NPp;
     const int own0 = in_lds ? min(NP, share * OWN) : 0;
@@ -333,8 +333,8 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         for (int ks = 0; ks < 32; ++ks) shv[ks] = sh[ks * 4 + lk];
     }
 
-    const int L = SPLIT ? 1 : a.L;
-    const int npass = SPLIT ? 1 : (nown + PR - 1) / PR;
+    const int L = a.L;
+    const int npass = (nown + PR - 1) / PR;
     for (int g = 0; g < L; ++g) {
         const uint32_t gen = gen0 + (uint32_t)g;
         // 1. donors and j_rand of this generation, drawn ahead of the launch
@@ -556,21 +556,18 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         }
         // 5. selection, strict "<"
         if (in_lds) {
-            // the share's accepted trials are copied over their rows (with
-            // one generation per launch the write-back picks them instead)
+            // the share's accepted trials are copied over their rows
             for (int r = own0 + tid; r < own0 + nown; r += kEpochThreads) {
                 const bool acc = fUs[r] < fXs[r];
                 if (acc) fXs[r] = fUs[r];
                 flags[r] = acc ? 1 : 0;
             }
             __syncthreads();
-            if (!SPLIT) {
-                for (int it = tid; it < nown * ng; it += kEpochThreads) {
-                    const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
-                    if (flags[r]) st4(rows + r * S + 4 * q4, ld4(rows + trial0 + rl * ST + 4 * q4));
-                }
-                __syncthreads();
+            for (int it = tid; it < nown * ng; it += kEpochThreads) {
+                const int rl = div_ng(it), q4 = it - rl * ng, r = own0 + rl;
+                if (flags[r]) st4(rows + r * S + 4 * q4, ld4(rows + trial0 + rl * ST + 4 * q4));
             }
+            __syncthreads();
         } else {
             // an accepted trial's slot becomes the row's current one
             for (int r = tid; r < NP; r += kEpochThreads) {
@@ -591,7 +588,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
         float4 cur, tri;
         if (in_lds) {
             tri = ld4(rows + trial0 + rl * ST + 4 * q4);
-            cur = (SPLIT && flags[r]) ? tri : ld4(rows + r * S + 4 * q4);
+            cur = ld4(rows + r * S + 4 * q4);
         } else {
             const int fl = flags[r];
             cur = ld4(grow(row_code(fl & 1, r)) + 4 * q4);
@@ -947,16 +944,16 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     }
 }
 
-template <int PREC, bool SPLIT>
+template <int PREC>
 static int launch_epoch_t(const EpochArgs &a, int nblocks, int lds, hipStream_t s) {
     static std::once_flag once;
     static hipError_t attr = hipSuccess;
     std::call_once(once, [] {
-        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_epoch<PREC, SPLIT>),
+        attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_island_epoch<PREC>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, kEpochLdsMax);
     });
     if (attr != hipSuccess) return check_hip(attr, "hipFuncSetAttribute(k_island_epoch)");
-    hipLaunchKernelGGL((k_island_epoch<PREC, SPLIT>), dim3(nblocks), dim3(kEpochThreads), lds, s, a);
+    hipLaunchKernelGGL((k_island_epoch<PREC>), dim3(nblocks), dim3(kEpochThreads), lds, s, a);
     return check_launch("k_island_epoch");
 }
 
@@ -975,7 +972,7 @@ static int launch_split_t(const EpochArgs &a, int nblocks, int lds, hipStream_t 
 
 template <int PREC>
 static int launch_epoch_p(const EpochArgs &a, int nblocks, int lds, bool split, hipStream_t s) {
-    return split ? launch_split_t<PREC>(a, nblocks, lds, s) : launch_epoch_t<PREC, false>(a, nblocks, lds, s);
+    return split ? launch_split_t<PREC>(a, nblocks, lds, s) : launch_epoch_t<PREC>(a, nblocks, lds, s);
 }
 
 int launch_epoch(int precision, const EpochArgs &a, int nblocks, int lds_bytes, bool split, hipStream_t s) {
