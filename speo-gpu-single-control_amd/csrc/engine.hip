@@ -69,7 +69,7 @@ constexpr int kTLD = kSmallD;  // widest dimension whose rows stay in LDS
 template <int RT, int PREC>
 constexpr int generation_lds_bytes(bool trial_in_lds) {
     return trial_in_lds ? RT * kTS * 4 + kMaxComp * RT * 4 +
-                              small_scratch_bytes<RT>(Prec<PREC>::parts) + (kSmallRedFloats(RT) + 2 * RT) * 4
+                              small_scratch_bytes<RT>(Prec<PREC>::parts) + 10 * RT * 4
                         : kMaxComp * RT * 4 + gemm_lds_bytes<RT, PREC>();
 }
 
@@ -198,8 +198,8 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
             const CompDev comp = P->comp[c];
             if (lds_trial) {
                 float *red = reinterpret_cast<float *>(gemm_lds + small_scratch_bytes<RT>(Prec<PREC>::parts));
-                float *o1 = red + kSmallRedFloats(RT), *o2 = o1 + RT;
-                float s1[RT / 16][2][4], s2[RT / 16][2][4];
+                float *o1 = red + 8 * RT, *o2 = o1 + RT;
+                float s1[RT / 16][4], s2[RT / 16][4];
                 gemm_small<RT, PREC>(comp, trial, dim, gemm_lds, s1, s2);
                 reduce_rows_small<RT>(s1, s2, red, o1, o2);
                 if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);

@@ -37,8 +37,8 @@ __device__ __forceinline__ float comp4(const float4 &v, int q) {
 }
 
 constexpr int kFixedLds = 128 * 4 + 128 * 4;  // shift vector, zero pad after the rows
-// per pass row: donor offsets, j_rand + trial row, A row, per-tile partials
-constexpr int kPassRowBytes = 16 + 8 + 4 + 2 * 8 * 4;
+// per pass row: donor offsets, j_rand + trial row, A row, partials
+constexpr int kPassRowBytes = 16 + 8 + 4 + 32;
 
 
 int lds_row_stride(int dim) {
@@ -55,6 +55,10 @@ int lds_trial_stride(int dim) { return round_up(round_up(dim, 4), 16) + 4; }
 // LDS of k_island_split: fX, fU, flags of the share [own], shift [128],
 // donor offsets [own] int4, j_rand [own], partials [16][own], rows [NPp][S],
 // trial rows and their shifted, scaled copies [own][ST] each.
+// Accumulators of column tiles 4..7 handed to the waves of tiles 0..3: four
+// waves x two 16-row tiles x 64 lanes x 4 floats.
+constexpr int kSplitXchBytes = 4 * 2 * 64 * 16;
+
 // The A operand region holds the FP32 copies (parts16 == 0) or the 16-bit
 // parts of every element [parts16][own][kp32 + 8].
 int split_a_bytes(int dim, int own, int parts16) {
@@ -62,7 +66,7 @@ int split_a_bytes(int dim, int own, int parts16) {
 }
 int split_lds_bytes(int np, int dim, int own, int parts16) {
     const int NPp = round_up(np, 16), S = lds_row_stride(dim), ST = lds_trial_stride(dim);
-    return own * (4 + 4 + 4 + 16 + 4 + 16 * 4) + 128 * 4 + (NPp * S + own * ST) * 4 +
+    return own * (4 + 4 + 4 + 16 + 4 + 8 * 4) + 128 * 4 + kSplitXchBytes + (NPp * S + own * ST) * 4 +
            split_a_bytes(dim, own, parts16);
 }
 
@@ -137,35 +141,31 @@ int epoch_donors(const IslandDesc *isl, const int *row_island, int rows, int L, 
     return check_launch("k_epoch_donors");
 }
 
-// Per-coordinate terms of one 16-row tile's two column tiles, each tile
-// summed over its 16 lanes per row (gemm_small's epilogue with the objective
-// fixed): s1[t][q], s2[t][q].
+// Per-coordinate terms of one 16-row tile's two column tiles, summed over the
+// 16 lanes of each row (gemm_small's epilogue with the objective fixed): the
+// lane's terms of column tile t0 then t0 + 4, then the 16 lanes.
 template <int OBJ>
 __device__ __forceinline__ void tile_terms(const floatx4 &acc0, const floatx4 &acc1, bool c0, bool c1,
-                                           float w0, float w1, float unscale, float (&s1)[2][4],
-                                           float (&s2)[2][4]) {
+                                           float w0, float w1, float unscale, float (&s1)[4],
+                                           float (&s2)[4]) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            s1[t][q] = 0.f;
-            s2[t][q] = 0.f;
-        }
+    for (int q = 0; q < 4; ++q) {
+        s1[q] = 0.f;
+        s2[q] = 0.f;
+    }
     if (c0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc0[q] * unscale, w0, s1[0][q], s2[0][q]);
+        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc0[q] * unscale, w0, s1[q], s2[q]);
     }
     if (c1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc1[q] * unscale, w1, s1[1][q], s2[1][q]);
+        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc1[q] * unscale, w1, s1[q], s2[q]);
     }
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            s1[t][q] = sum16(s1[t][q]);
-            if (OBJ == SPEO_OBJ_ACKLEY) s2[t][q] = sum16(s2[t][q]);
-        }
+    for (int q = 0; q < 4; ++q) {
+        s1[q] = sum16(s1[q]);
+        if (OBJ == SPEO_OBJ_ACKLEY) s2[q] = sum16(s2[q]);
+    }
 }
 
 // One workgroup per island, all generations of an epoch (islands split over
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                  "s"(D), "s"(pl.S));
 
     // LDS: fX[NPp] fU[NPp] flags[NPp] shift[128] doff[PR] tinfo[PR] aoff[PR]
-    //      part[16][PR] rows (+128 floats)
+    //      part[8][PR] rows (+128 floats)
     float *fXs = reinterpret_cast<float *>(lds);
     float *fUs = fXs + NPp;
     uint8_t *flags = reinterpret_cast<uint8_t *>(fUs + NPp);  // HBM plan: bit 0 current slot, bit 1
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
     float *part = reinterpret_cast<float *>(aoff + PR);
     // LDS plan: current rows [NPp][S], then the share's trial rows [OWN][ST];
     // HBM plan: the pass's trials [PR][ST]
-    float *rows = part + 16 * PR;
+    float *rows = part + 8 * PR;
     const int trial0 = NPp * S;
 
     // Row addresses: in LDS a float offset into `rows`; in HBM 2 * (r * ld) + slot.
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                     }
                 }
                 const float unscale = H ? kF16Unscale : 1.0f;
-                float s1[2][4], s2[2][4];
+                float s1[4], s2[4];
                 switch (blk.objective) {
                 case SPEO_OBJ_ELLIPTIC:
                     tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, acc1, c0, c1, w0, w1, unscale, s1, s2);
@@ -532,23 +532,21 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                 }
                 if (lr == 0) {
 #pragma unroll
-                    for (int t = 0; t < 2; ++t)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const int rl = tile * 16 + lk * 4 + q, ct = wq + 4 * t;
-                            part[(2 * ct) * PR + rl] = s1[t][q];
-                            part[(2 * ct + 1) * PR + rl] = s2[t][q];
-                        }
+                    for (int q = 0; q < 4; ++q) {
+                        const int rl = tile * 16 + lk * 4 + q;
+                        part[wq * PR + rl] = s1[q];
+                        part[(4 + wq) * PR + rl] = s2[q];
+                    }
                 }
             }
             __syncthreads();
-            // the column tiles' partial sums, tiles 0..7 in order
+            // the four column groups' partial sums, in column-group order
             for (int rl = tid; rl < nr; rl += kEpochThreads) {
                 float s1 = 0.f, s2 = 0.f;
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    s1 += part[(2 * t) * PR + rl];
-                    s2 += part[(2 * t + 1) * PR + rl];
+                for (int w = 0; w < 4; ++w) {
+                    s1 += part[w * PR + rl];
+                    s2 += part[(4 + w) * PR + rl];
                 }
                 fUs[p0 + rl] = fmaf(lambda, finish_objective(blk.objective, s1, s2, D), bias);
             }
@@ -614,29 +612,14 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
 //     evaluation's A operand go to LDS: the shifted, scaled copy (u - o) *
 //     rate in FP32, or its 16-bit parts, formed as the general kernel and
 //     gemm_small form them;
-//   evaluation: waves 0..7, column tile t = wave, every 16-row tile of the
-//     share, two tiles per K loop; per-tile sums as everywhere else;
-//   combine (tiles 0..7 in order), selection, write-back of the selected row.
+//   evaluation: waves 0..7, column tile t = wave, the 16-row tiles of the
+//     share two at a time; waves 4..7 hand their accumulators to waves 0..3
+//     through LDS, which form the terms of tiles t and t + 4 in the order of
+//     every other kernel (the lane's terms of t, then of t + 4, then the 16
+//     lanes);
+//   combine (column groups 0..3 in order), selection, write-back of the
+//   selected row.
 constexpr int kSplitThreads = 1024;
-
-template <int OBJ>
-__device__ __forceinline__ void col_tile_terms(const floatx4 &acc, bool c, float w, float (&s1)[4],
-                                               float (&s2)[4]) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        s1[q] = 0.f;
-        s2[q] = 0.f;
-    }
-    if (c) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) add_terms(OBJ, acc[q], w, s1[q], s2[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        s1[q] = sum16(s1[q]);
-        if (OBJ == SPEO_OBJ_ACKLEY) s2[q] = sum16(s2[q]);
-    }
-}
 
 template <int PREC>
 __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
@@ -670,8 +653,9 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     float *sh = reinterpret_cast<float *>(acc_f + OWN);
     int4 *doff = reinterpret_cast<int4 *>(sh + 128);
     int *jr = reinterpret_cast<int *>(doff + OWN);
-    float *part = reinterpret_cast<float *>(jr + OWN);  // [16][OWN]
-    float *rows = part + 16 * OWN;                       // [NPp][S]
+    float *part = reinterpret_cast<float *>(jr + OWN);  // [8][OWN]
+    floatx4 *xch = reinterpret_cast<floatx4 *>(part + 8 * OWN);  // [4][2][64]
+    float *rows = part + 8 * OWN + kSplitXchBytes / 4;   // [NPp][S]
     float *trial = rows + NPp * S;                        // [OWN][ST]
     float *arow = trial + OWN * ST;                       // FP32: [OWN][ST]
     uint16_t *a16 = reinterpret_cast<uint16_t *>(arow);   // 16-bit: [P][OWN][LDA]
@@ -707,7 +691,9 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     constexpr int NB16 = P == 0 ? 1 : P;
     float b[NB32];
     vec8 bb[4][NB16];
-    float wcol = 1.0f;
+    float wcol = 1.0f, wcol4 = 1.0f;  // column weights of tiles t and (waves 0..3) t + 4
+    const bool ct4 = (t + 4) * 16 + lr < D;
+    if (wave < 4 && blk.objective == SPEO_OBJ_ELLIPTIC && ct4) wcol4 = gld(blk.colw + (t + 4) * 16 + lr);
     if (evaluates) {
         if constexpr (P == 0) {
             load_b_frag(blk.B32s, t, vt, KS, lane, b);
@@ -819,15 +805,15 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     __syncthreads();
 
     // ---- evaluation: waves 0..7, column tile t, the 16-row tiles two at a time
-    if (evaluates) {
-        const int ntile = (nown + 15) >> 4;
-        for (int tile = 0; tile < ntile; tile += 2) {
-            const bool two = tile + 1 < ntile;
+    const int ntile = (nown + 15) >> 4;
+    for (int tile = 0; tile < ntile; tile += 2) {  // every wave: the barriers are block-wide
+        const bool two = tile + 1 < ntile;
+        floatx4 acc0 = floatx4{0.f, 0.f, 0.f, 0.f}, acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (evaluates) {
             const int r0 = min(tile * 16 + lr, nown - 1), r1 = min(tile * 16 + 16 + lr, nown - 1);
-            const float *ap0 = arow + r0 * ST + lk;
-            const float *ap1 = arow + r1 * ST + lk;
-            floatx4 acc0 = floatx4{0.f, 0.f, 0.f, 0.f}, acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
             if (P == 0 && vt) {
+                const float *ap0 = arow + r0 * ST + lk;
+                const float *ap1 = arow + r1 * ST + lk;
 #pragma unroll
                 for (int kb = 0; kb < 32; kb += 4) {
                     if (kb >= KS) break;
@@ -874,26 +860,29 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
                         }
                     }
                 }
-                const float unscale = H ? kF16Unscale : 1.0f;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    acc0[q] *= unscale;
-                    acc1[q] *= unscale;
-                }
             }
+            if (wave >= 4) {
+                xch[((wave - 4) * 2 + 0) * 64 + lane] = acc0;
+                xch[((wave - 4) * 2 + 1) * 64 + lane] = acc1;
+            }
+        }
+        __syncthreads();
+        if (wave < 4) {
+            const floatx4 acc0b = xch[(wave * 2 + 0) * 64 + lane], acc1b = xch[(wave * 2 + 1) * 64 + lane];
+            const float unscale = H ? kF16Unscale : 1.0f;
             float s1[2][4], s2[2][4];
             switch (blk.objective) {
             case SPEO_OBJ_ELLIPTIC:
-                col_tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, ct, wcol, s1[0], s2[0]);
-                col_tile_terms<SPEO_OBJ_ELLIPTIC>(acc1, ct, wcol, s1[1], s2[1]);
+                tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, acc0b, ct, ct4, wcol, wcol4, unscale, s1[0], s2[0]);
+                tile_terms<SPEO_OBJ_ELLIPTIC>(acc1, acc1b, ct, ct4, wcol, wcol4, unscale, s1[1], s2[1]);
                 break;
             case SPEO_OBJ_RASTRIGIN:
-                col_tile_terms<SPEO_OBJ_RASTRIGIN>(acc0, ct, wcol, s1[0], s2[0]);
-                col_tile_terms<SPEO_OBJ_RASTRIGIN>(acc1, ct, wcol, s1[1], s2[1]);
+                tile_terms<SPEO_OBJ_RASTRIGIN>(acc0, acc0b, ct, ct4, wcol, wcol4, unscale, s1[0], s2[0]);
+                tile_terms<SPEO_OBJ_RASTRIGIN>(acc1, acc1b, ct, ct4, wcol, wcol4, unscale, s1[1], s2[1]);
                 break;
             default:
-                col_tile_terms<SPEO_OBJ_ACKLEY>(acc0, ct, wcol, s1[0], s2[0]);
-                col_tile_terms<SPEO_OBJ_ACKLEY>(acc1, ct, wcol, s1[1], s2[1]);
+                tile_terms<SPEO_OBJ_ACKLEY>(acc0, acc0b, ct, ct4, wcol, wcol4, unscale, s1[0], s2[0]);
+                tile_terms<SPEO_OBJ_ACKLEY>(acc1, acc1b, ct, ct4, wcol, wcol4, unscale, s1[1], s2[1]);
                 break;
             }
             if (lr == 0) {
@@ -903,22 +892,22 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int rl = (tile + i) * 16 + lk * 4 + q;
-                        part[(2 * t) * OWN + rl] = s1[i][q];
-                        part[(2 * t + 1) * OWN + rl] = s2[i][q];
+                        part[wave * OWN + rl] = s1[i][q];
+                        part[(4 + wave) * OWN + rl] = s2[i][q];
                     }
                 }
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
 
-    // ---- combine (tiles 0..7 in order) and selection, strict "<"
+    // ---- combine (column groups 0..3 in order) and selection, strict "<"
     if (tid < nown) {
         float s1 = 0.f, s2 = 0.f;
 #pragma unroll
-        for (int tt = 0; tt < 8; ++tt) {
-            s1 += part[(2 * tt) * OWN + tid];
-            s2 += part[(2 * tt + 1) * OWN + tid];
+        for (int w = 0; w < 4; ++w) {
+            s1 += part[w * OWN + tid];
+            s2 += part[(4 + w) * OWN + tid];
         }
         const float fu = fmaf(lambda, finish_objective(blk.objective, s1, s2, D), bias);
         const float fx = fXs[tid];

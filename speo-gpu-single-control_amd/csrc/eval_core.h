@@ -369,19 +369,16 @@ __device__ __forceinline__ void load_b_frag(const float *B32s, int t, bool valid
 // from the padded global layouts into registers (no LDS staging, no barrier
 // inside the K loop).  `scratch` holds the shifted rows' 16-bit parts for the
 // 16-bit precisions: P * RT * (kp32 + 8) uint16, and the shift: 128 floats.
-// Output: s1[i][t][q], s2[i][t][q] for row 16 i + 4 (lane / 16) + q of the
-// tile and this wave's column tile t (0: wave, 1: wave + 4), each summed over
-// the tile's 16 columns (the lane's 16-lane group).  Every kernel sums a row's
-// terms in this order: per 16-column tile over its lanes, then the tiles
-// 0..7 in order.
+// Output: s1[i][q], s2[i][q] for row 16 i + 4 (lane / 16) + q of the tile,
+// summed over this wave's columns and the lane's 16-lane group.
 template <int RT> constexpr int small_scratch_bytes(int parts) {
     return 128 * 4 + parts * RT * (kSmallD + 8) * 2;
 }
 
 template <int RT, int PREC>
 __device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, int dim,
-                                           char *scratch, float (&s1)[RT / 16][2][4],
-                                           float (&s2)[RT / 16][2][4]) {
+                                           char *scratch, float (&s1)[RT / 16][4],
+                                           float (&s2)[RT / 16][4]) {
     constexpr int NRT = RT / 16;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
@@ -475,12 +472,10 @@ __device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, 
 #pragma unroll
     for (int i = 0; i < NRT; ++i)
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                s1[i][t][q] = 0.f;
-                s2[i][t][q] = 0.f;
-            }
+        for (int q = 0; q < 4; ++q) {
+            s1[i][q] = 0.f;
+            s2[i][q] = 0.f;
+        }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int col = (t == 0 ? t0 : t1) * 16 + lr;
@@ -489,27 +484,23 @@ __device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, 
 #pragma unroll
             for (int i = 0; i < NRT; ++i)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    add_terms(c.objective, acc[i][t][q] * unscale, w, s1[i][t][q], s2[i][t][q]);
+                for (int q = 0; q < 4; ++q) add_terms(c.objective, acc[i][t][q] * unscale, w, s1[i][q], s2[i][q]);
         }
     }
 #pragma unroll
     for (int i = 0; i < NRT; ++i)
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                s1[i][t][q] = sum16(s1[i][t][q]);
-                s2[i][t][q] = sum16(s2[i][t][q]);
-            }
+        for (int q = 0; q < 4; ++q) {
+            s1[i][q] = sum16(s1[i][q]);
+            s2[i][q] = sum16(s2[i][q]);
+        }
 }
 
-// Per-row sums of gemm_small's per-tile partials: the 16-column tiles 0..7
-// in order.  red: kSmallRedFloats(RT) floats of LDS; out1/out2: RT floats each.
-constexpr int kSmallRedFloats(int rt) { return 2 * 8 * rt; }
+// Per-row sums of gemm_small's per-wave partials (fixed order over waves).
+// red: 2 * 4 * RT floats of LDS; out1/out2: RT floats each.
 template <int RT>
-__device__ __forceinline__ void reduce_rows_small(const float (&s1)[RT / 16][2][4],
-                                                  const float (&s2)[RT / 16][2][4], float *red,
+__device__ __forceinline__ void reduce_rows_small(const float (&s1)[RT / 16][4],
+                                                  const float (&s2)[RT / 16][4], float *red,
                                                   float *out1, float *out2) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     __syncthreads();
@@ -517,20 +508,18 @@ __device__ __forceinline__ void reduce_rows_small(const float (&s1)[RT / 16][2][
 #pragma unroll
         for (int i = 0; i < RT / 16; ++i)
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = i * 16 + (lane >> 4) * 4 + q, slot = wave + 4 * t;
-                    red[(slot * RT + r) * 2 + 0] = s1[i][t][q];
-                    red[(slot * RT + r) * 2 + 1] = s2[i][t][q];
-                }
+            for (int q = 0; q < 4; ++q) {
+                const int r = i * 16 + (lane >> 4) * 4 + q;
+                red[(wave * RT + r) * 2 + 0] = s1[i][q];
+                red[(wave * RT + r) * 2 + 1] = s2[i][q];
+            }
     }
     __syncthreads();
     if (tid < RT) {
         float a = 0.f, b = 0.f;
-        for (int t = 0; t < 8; ++t) {
-            a += red[(t * RT + tid) * 2 + 0];
-            b += red[(t * RT + tid) * 2 + 1];
+        for (int w = 0; w < 4; ++w) {
+            a += red[(w * RT + tid) * 2 + 0];
+            b += red[(w * RT + tid) * 2 + 1];
         }
         out1[tid] = a;
         out2[tid] = b;

@@ -16,7 +16,7 @@ namespace speo {
 // ------------------------------------------------------------------ kernels
 
 template <int RT, int PREC> constexpr int eval_small_lds_bytes() {
-    return RT * kTS * 4 + small_scratch_bytes<RT>(Prec<PREC>::parts) + (kSmallRedFloats(RT) + 2 * RT) * 4;
+    return RT * kTS * 4 + small_scratch_bytes<RT>(Prec<PREC>::parts) + 10 * RT * 4;
 }
 
 template <int RT, int PREC>
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch p) {
         float *rows = reinterpret_cast<float *>(lds);
         char *scratch = lds + RT * kTS * 4;
         float *red = reinterpret_cast<float *>(scratch + small_scratch_bytes<RT>(Prec<PREC>::parts));
-        o1 = red + kSmallRedFloats(RT);
+        o1 = red + 8 * RT;
         o2 = o1 + RT;
         const int kp4 = (p.dim + 3) & ~3;
         for (int e = threadIdx.x; e < RT * kp4; e += 256) {
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch p) {
             rows[r * kTS + k] = (r < nrows && k < p.dim) ? p.X[(size_t)(row0 + r) * p.ld + k] : 0.f;
         }
         __syncthreads();
-        float s1[RT / 16][2][4], s2[RT / 16][2][4];
+        float s1[RT / 16][4], s2[RT / 16][4];
         gemm_small<RT, PREC>(p.comp, rows, p.dim, scratch, s1, s2);
         reduce_rows_small<RT>(s1, s2, red, o1, o2);
     } else {
