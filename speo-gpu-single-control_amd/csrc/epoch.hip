@@ -708,6 +708,11 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
         }
         if (blk.objective == SPEO_OBJ_ELLIPTIC && ct) wcol = gld(blk.colw + t * 16 + lr);
     }
+    // The crossover draws of the lane's first trial item depend only on the
+    // launch: they are computed while the loads are in flight.
+    const int it0 = tid, rl0 = div_ng(it0), q40 = it0 - rl0 * ng;
+    const U4 w0 = philox_block(a.seed, (uint32_t)q40, (uint32_t)(I.row_off + own0 + rl0), gen, kPurposeCross);
+    asm volatile("" ::"v"(w0.x), "v"(w0.y), "v"(w0.z), "v"(w0.w));  // (computed here, not at its use)
 
     // ---- stores
     if (tid < 128) sh[tid] = shift_k;
@@ -757,13 +762,15 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     // ---- trials: one item per lane
     const float rate = blk.rate;
     for (int it = tid; it < nown * ng; it += kSplitThreads) {
-        const int rl = div_ng(it), q4 = it - rl * ng;
+        const bool first = it == it0;
+        const int rl = first ? rl0 : div_ng(it), q4 = first ? q40 : it - rl * ng;
         const int4 o = doff[rl];
         const int jrand = jr[rl];
         const float4 e0 = ld4(rows + o.x + 4 * q4), e1 = ld4(rows + o.y + 4 * q4);
         const float4 e2 = ld4(rows + o.z + 4 * q4), xo = ld4(rows + o.w + 4 * q4);
         const float4 sv = ld4(sh + 4 * q4);
-        const U4 w = philox_block(a.seed, (uint32_t)q4, (uint32_t)(I.row_off + own0 + rl), gen, kPurposeCross);
+        const U4 w = first ? w0
+                           : philox_block(a.seed, (uint32_t)q4, (uint32_t)(I.row_off + own0 + rl), gen, kPurposeCross);
         float v[4];
         bool out = false;
 #pragma unroll
