@@ -392,7 +392,7 @@ def main():
     kms = eng.profile(args.profile_gens)
     dominant = {"epoch": "epoch", "fused": "fused", "plain": "evaluate"}[args.mode]
     kernel_ms = kms[dominant]
-    kernel_name = {"epoch": "k_island_epoch (per generation)", "fused": "k_generation (fused)",
+    kernel_name = {"epoch": "k_island_split / k_island_epoch (per generation)", "fused": "k_generation (fused)",
                    "plain": "k_generation (evaluate)"}[args.mode]
     flops = 2.0 * D * D * rows
     achieved = flops / (kernel_ms * 1e-3) / 1e12
