@@ -685,15 +685,20 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     // rotation fragments and column weights of the evaluating waves
     const int t = wave & 7;
     const bool evaluates = wave < 8;
-    const bool vt = t * 16 < D, ct = t * 16 + lr < D;
+    const bool vt = t * 16 < D;
     const int KS = P == 0 ? ng : kp32 >> 5;
     constexpr int NB32 = P == 0 ? 32 : 1;
     constexpr int NB16 = P == 0 ? 1 : P;
     float b[NB32];
     vec8 bb[4][NB16];
-    float wcol = 1.0f, wcol4 = 1.0f;  // column weights of tiles t and (waves 0..3) t + 4
-    const bool ct4 = (t + 4) * 16 + lr < D;
-    if (wave < 4 && blk.objective == SPEO_OBJ_ELLIPTIC && ct4) wcol4 = gld(blk.colw + (t + 4) * 16 + lr);
+    // the pair of column tiles (tl, tl + 4) whose terms this wave forms
+    const int tl = t & 3;
+    const bool cl = tl * 16 + lr < D, ch = (tl + 4) * 16 + lr < D;
+    float wl = 1.0f, wh = 1.0f;
+    if (evaluates && blk.objective == SPEO_OBJ_ELLIPTIC) {
+        if (cl) wl = gld(blk.colw + tl * 16 + lr);
+        if (ch) wh = gld(blk.colw + (tl + 4) * 16 + lr);
+    }
     if (evaluates) {
         if constexpr (P == 0) {
             load_b_frag(blk.B32s, t, vt, KS, lane, b);
@@ -706,7 +711,6 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
                         bb[ks][p] = gld(reinterpret_cast<const vec8 *>(
                             blk.M16s[p] + (size_t)(t * 16 + lr) * kp32 + ks * 32 + lk * 8));
         }
-        if (blk.objective == SPEO_OBJ_ELLIPTIC && ct) wcol = gld(blk.colw + t * 16 + lr);
     }
     // The crossover draws of the lane's first trial item depend only on the
     // launch: they are computed while the loads are in flight.
@@ -868,40 +872,35 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
                     }
                 }
             }
-            if (wave >= 4) {
-                xch[((wave - 4) * 2 + 0) * 64 + lane] = acc0;
-                xch[((wave - 4) * 2 + 1) * 64 + lane] = acc1;
-            }
+            // waves 4..7 hand over the first row tile's accumulator, waves
+            // 0..3 the second's: each wave then forms one row tile's terms
+            if (wave >= 4) xch[((wave - 4) * 2 + 0) * 64 + lane] = acc0;
+            else xch[(wave * 2 + 1) * 64 + lane] = acc1;
         }
         __syncthreads();
-        if (wave < 4) {
-            const floatx4 acc0b = xch[(wave * 2 + 0) * 64 + lane], acc1b = xch[(wave * 2 + 1) * 64 + lane];
+        if (evaluates && (wave < 4 || two)) {
+            const int i = wave >> 2;  // row tile tile + i
+            const floatx4 lo = wave < 4 ? acc0 : xch[((wave - 4) * 2 + 1) * 64 + lane];
+            const floatx4 hi = wave < 4 ? xch[(wave * 2 + 0) * 64 + lane] : acc1;
             const float unscale = H ? kF16Unscale : 1.0f;
-            float s1[2][4], s2[2][4];
+            float s1[4], s2[4];
             switch (blk.objective) {
             case SPEO_OBJ_ELLIPTIC:
-                tile_terms<SPEO_OBJ_ELLIPTIC>(acc0, acc0b, ct, ct4, wcol, wcol4, unscale, s1[0], s2[0]);
-                tile_terms<SPEO_OBJ_ELLIPTIC>(acc1, acc1b, ct, ct4, wcol, wcol4, unscale, s1[1], s2[1]);
+                tile_terms<SPEO_OBJ_ELLIPTIC>(lo, hi, cl, ch, wl, wh, unscale, s1, s2);
                 break;
             case SPEO_OBJ_RASTRIGIN:
-                tile_terms<SPEO_OBJ_RASTRIGIN>(acc0, acc0b, ct, ct4, wcol, wcol4, unscale, s1[0], s2[0]);
-                tile_terms<SPEO_OBJ_RASTRIGIN>(acc1, acc1b, ct, ct4, wcol, wcol4, unscale, s1[1], s2[1]);
+                tile_terms<SPEO_OBJ_RASTRIGIN>(lo, hi, cl, ch, wl, wh, unscale, s1, s2);
                 break;
             default:
-                tile_terms<SPEO_OBJ_ACKLEY>(acc0, acc0b, ct, ct4, wcol, wcol4, unscale, s1[0], s2[0]);
-                tile_terms<SPEO_OBJ_ACKLEY>(acc1, acc1b, ct, ct4, wcol, wcol4, unscale, s1[1], s2[1]);
+                tile_terms<SPEO_OBJ_ACKLEY>(lo, hi, cl, ch, wl, wh, unscale, s1, s2);
                 break;
             }
             if (lr == 0) {
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    if (i == 1 && !two) break;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int rl = (tile + i) * 16 + lk * 4 + q;
-                        part[wave * OWN + rl] = s1[i][q];
-                        part[(4 + wave) * OWN + rl] = s2[i][q];
-                    }
+                for (int q = 0; q < 4; ++q) {
+                    const int rl = (tile + i) * 16 + lk * 4 + q;
+                    part[tl * OWN + rl] = s1[q];
+                    part[(4 + tl) * OWN + rl] = s2[q];
                 }
             }
         }
