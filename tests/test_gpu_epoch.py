@@ -141,6 +141,24 @@ def test_epoch_split_islands_equal_plain(cuda, speo, oracle, parts):
     _assert_same(plain, split)
 
 
+@pytest.mark.parametrize("prec", [0, 3])
+def test_epoch_split_edge_dims_equal_plain(cuda, speo, oracle, prec):
+    # split islands at the edges of the row layouts: D = 128 (every K block and
+    # column tile in use, LDS stride 132), D = 3 (one float4 group per row),
+    # D = 96 (stride padded to 100), D = 10 (ragged last group)
+    probs = _problems(speo, oracle, [(128, 1), (3, 2), (96, 3), (10, 2)], base=2900)
+    sizes, pidx = [96, 64, 80, 72, 64, 96], [0, 1, 2, 3, 0, 2]
+    kw = dict(F=0.5, CR=0.9, seed=SEED, rows_per_block=32, interval=5, migration_size=2, precision=prec)
+    plain = speo.Engine(probs, sizes, pidx, mode=speo.MODE_PLAIN, **kw)
+    split = speo.Engine(probs, sizes, pidx, mode=speo.MODE_EPOCH, epoch_parts=2, **kw)
+    assert split.stats()["epoch_parts"] == 2
+    for chunk in (1, 6, 4):
+        print(f"precision {prec} chunk {chunk}", flush=True)
+        plain.run(chunk)
+        split.run(chunk)
+    _assert_same(plain, split)
+
+
 def test_epoch_automatic_split(cuda, speo, oracle):
     probs = _problems(speo, oracle, [(100, 1)], base=2800)
     e = speo.Engine(probs, [128] * 8, [0] * 8, mode=speo.MODE_EPOCH, interval=10, migration_size=2,
