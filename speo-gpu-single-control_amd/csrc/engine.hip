@@ -116,18 +116,27 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
         const uint32_t gen = (a.gen_base ? *a.gen_base : 0u) + a.gen_offset;
         const int ngroups = (I.dim + 3) >> 2;
         const uint32_t crt = cr_threshold(I.CR);
-        for (int pass = 0; pass < RT / 16; ++pass) {
-            const int lr = pass * 16 + (tid >> 4), l = tid & 15;
-            if (lr >= t.nrows) {
-                if (lds_trial)
-                    for (int g = l; g < ngroups; g += 16) st4(trial + lr * kTS + 4 * g, float4{0.f, 0.f, 0.f, 0.f});
-                continue;
-            }
-            const int li = t.row0 + lr;
-            const uint32_t gr = (uint32_t)(I.row_off + li);
-            DonorStream ds(a.seed, gr, gen);
-            int r[kMaxR], jr;
-            draw_donors(ds, nullptr, 0, I.np, I.dim, r, jr);
+        // Two passes (rows lr and lr + 16 of the tile, 16 lanes per row) at a
+        // time: the parent-selection loads of both rows, then their row
+        // loads, are in flight together.
+        struct RowCtx {
+            bool valid, own_acc;
+            int lr, li, jr;
+            uint32_t gr;
+            float fx_own, fu_own;
+            const float *p0, *p1, *p2, *own;
+        };
+        const int l = tid & 15;
+        auto prep = [&](int lr) {
+            RowCtx c{};
+            c.lr = lr;
+            c.valid = lr < t.nrows;
+            if (!c.valid) return c;
+            c.li = t.row0 + lr;
+            c.gr = (uint32_t)(I.row_off + c.li);
+            DonorStream ds(a.seed, c.gr, gen);
+            int r[kMaxR];
+            draw_donors(ds, nullptr, 0, I.np, I.dim, r, c.jr);
             // Row `k` of the island as the parent population of this generation.
             auto parent = [&](int k) -> const float * {
                 const int g = I.row_off + k;
@@ -135,41 +144,70 @@ __global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
                     return a.U_prev + I.elem_off + (size_t)k * I.ld;
                 return a.X + I.elem_off + (size_t)k * I.ld;
             };
-            const float *p0 = parent(r[0]), *p1 = parent(r[1]), *p2 = parent(r[2]);
-            bool own_acc = false;
-            float fx_own = 0.f, fu_own = 0.f;
+            c.p0 = parent(r[0]);
+            c.p1 = parent(r[1]);
+            c.p2 = parent(r[2]);
+            c.own_acc = false;
+            c.fx_own = c.fu_own = 0.f;
             if (MODE == kModeFused) {
-                fx_own = a.fX_prev[gr];
-                fu_own = a.fU_prev[gr];
-                own_acc = fu_own < fx_own;
+                c.fx_own = a.fX_prev[c.gr];
+                c.fu_own = a.fU_prev[c.gr];
+                c.own_acc = c.fu_own < c.fx_own;
             }
-            const float *own = own_acc ? a.U_prev + I.elem_off + (size_t)li * I.ld
-                                       : a.X + I.elem_off + (size_t)li * I.ld;
-            float *urow = a.U_new + I.elem_off + (size_t)li * I.ld;
-            float *xrow = a.X + I.elem_off + (size_t)li * I.ld;
-            for (int g = l; g < ngroups; g += 16) {
-                const U4 w = philox_block(a.seed, (uint32_t)g, gr, gen, kPurposeCross);
-                const float4 e0 = ld4(p0 + 4 * g), e1 = ld4(p1 + 4 * g), e2 = ld4(p2 + 4 * g);
-                const float4 xo = ld4(own + 4 * g);
-                float v[4];
+            c.own = c.own_acc ? a.U_prev + I.elem_off + (size_t)c.li * I.ld
+                              : a.X + I.elem_off + (size_t)c.li * I.ld;
+            return c;
+        };
+        // trial coordinates 4 g .. 4 g + 3 of a prepared row
+        auto make = [&](const RowCtx &c, int g, const float4 &e0, const float4 &e1, const float4 &e2,
+                        const float4 &xo) {
+            const U4 w = philox_block(a.seed, (uint32_t)g, c.gr, gen, kPurposeCross);
+            float v[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int j = 4 * g + q;
-                    if (j < I.dim) {
-                        const bool take = (j == jr) || cross_take(word(w, q), crt);
-                        const float m = take ? fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q))
-                                             : comp4(xo, q);
-                        v[q] = check_bound(m, a.lo, a.hi);
-                    } else {
-                        v[q] = 0.f;
-                    }
+            for (int q = 0; q < 4; ++q) {
+                const int j = 4 * g + q;
+                if (j < I.dim) {
+                    const bool take = (j == c.jr) || cross_take(word(w, q), crt);
+                    const float m = take ? fmaf(I.F, comp4(e1, q) - comp4(e2, q), comp4(e0, q))
+                                         : comp4(xo, q);
+                    v[q] = check_bound(m, a.lo, a.hi);
+                } else {
+                    v[q] = 0.f;
                 }
-                const float4 out{v[0], v[1], v[2], v[3]};
-                st4(urow + 4 * g, out);
-                if (lds_trial) st4(trial + lr * kTS + 4 * g, out);
-                if (MODE == kModeFused && own_acc) st4(xrow + 4 * g, xo);
             }
-            if (MODE == kModeFused && l == 0) a.fX_new[gr] = own_acc ? fu_own : fx_own;
+            const float4 out{v[0], v[1], v[2], v[3]};
+            st4(a.U_new + I.elem_off + (size_t)c.li * I.ld + 4 * g, out);
+            if (lds_trial) st4(trial + c.lr * kTS + 4 * g, out);
+            if (MODE == kModeFused && c.own_acc) st4(a.X + I.elem_off + (size_t)c.li * I.ld + 4 * g, xo);
+        };
+        for (int pass = 0; pass < RT / 16; pass += 2) {
+            const int lrA = pass * 16 + (tid >> 4), lrB = lrA + 16;
+            const bool hasB = pass + 1 < RT / 16;
+            const RowCtx A = prep(lrA);
+            const RowCtx B = hasB ? prep(lrB) : RowCtx{};
+            for (int g = l; g < ngroups; g += 16) {
+                float4 a0, a1, a2, ao, b0, b1, b2, bo;
+                if (A.valid) {
+                    a0 = ld4(A.p0 + 4 * g);
+                    a1 = ld4(A.p1 + 4 * g);
+                    a2 = ld4(A.p2 + 4 * g);
+                    ao = ld4(A.own + 4 * g);
+                }
+                if (B.valid) {
+                    b0 = ld4(B.p0 + 4 * g);
+                    b1 = ld4(B.p1 + 4 * g);
+                    b2 = ld4(B.p2 + 4 * g);
+                    bo = ld4(B.own + 4 * g);
+                }
+                if (A.valid) make(A, g, a0, a1, a2, ao);
+                else if (lds_trial) st4(trial + lrA * kTS + 4 * g, float4{0.f, 0.f, 0.f, 0.f});
+                if (B.valid) make(B, g, b0, b1, b2, bo);
+                else if (hasB && lds_trial) st4(trial + lrB * kTS + 4 * g, float4{0.f, 0.f, 0.f, 0.f});
+            }
+            if (MODE == kModeFused && l == 0) {
+                if (A.valid) a.fX_new[A.gr] = A.own_acc ? A.fu_own : A.fx_own;
+                if (B.valid) a.fX_new[B.gr] = B.own_acc ? B.fu_own : B.fx_own;
+            }
         }
     }
 
