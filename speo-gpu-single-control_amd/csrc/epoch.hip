@@ -699,19 +699,6 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
         if (cl) wl = gld(blk.colw + tl * 16 + lr);
         if (ch) wh = gld(blk.colw + (tl + 4) * 16 + lr);
     }
-    if (evaluates) {
-        if constexpr (P == 0) {
-            load_b_frag(blk.B32s, t, vt, KS, lane, b);
-        } else {
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-                for (int p = 0; p < P; ++p)
-                    if (vt && ks < KS)
-                        bb[ks][p] = gld(reinterpret_cast<const vec8 *>(
-                            blk.M16s[p] + (size_t)(t * 16 + lr) * kp32 + ks * 32 + lk * 8));
-        }
-    }
     // The crossover draws of the lane's first trial item depend only on the
     // launch: they are computed while the loads are in flight.
     const int it0 = tid, rl0 = div_ng(it0), q40 = it0 - rl0 * ng;
@@ -762,6 +749,22 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
         }
     }
     __syncthreads();
+
+    // the rotation fragments, needed from the evaluation on: issued now, they
+    // load while the trials are made rather than ahead of the population
+    if (evaluates) {
+        if constexpr (P == 0) {
+            load_b_frag(blk.B32s, t, vt, KS, lane, b);
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    if (vt && ks < KS)
+                        bb[ks][p] = gld(reinterpret_cast<const vec8 *>(
+                            blk.M16s[p] + (size_t)(t * 16 + lr) * kp32 + ks * 32 + lk * 8));
+        }
+    }
 
     // ---- trials: one item per lane
     const float rate = blk.rate;
