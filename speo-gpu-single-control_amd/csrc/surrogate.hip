@@ -30,12 +30,20 @@ struct GemmArgs {
     int kc;             // K per split (blockIdx.z); C of split z at C + z * c_split
     long c_split;
     int vec_a, vec_b;   // 16-byte loads along the unit-stride index are aligned
+    int bytes_a, bytes_b;  // extent of A and B in bytes (the buffer range of their loads)
 };
 
 // Whether 16-byte loads along the unit-stride index of a matrix are aligned.
 static int vec_ok(const float *p, long s_row, long s_k) {
     const long other = s_k == 1 ? s_row : s_k;
     return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && (other % 4 == 0) && (s_row == 1 || s_k == 1);
+}
+
+// Bytes spanned by the elements (r, k), r < rows, k < K, of a strided matrix
+// (0 when it does not fit the 32-bit buffer range).
+static int extent_bytes(long rows, long K, long s_row, long s_k) {
+    const long e = ((rows - 1) * s_row + (K - 1) * s_k + 1) * 4;
+    return e > 0 && e < 0x7fffffffL ? (int)e : 0;
 }
 
 __device__ __forceinline__ float sigmoidf_(float a) { return 1.0f / (1.0f + expf(-a)); }
@@ -45,13 +53,17 @@ __device__ __forceinline__ float dsiluf_(float a) {
     return s * (1.0f + a * (1.0f - s));
 }
 
-// 64 x 64 tile, 4 waves as 2 x 2, each wave 32 x 32 = 2 x 2 MFMA 16x16 tiles,
-// K staged 128 at a time (one or two stages for the MLP's K <= 256, so the
-// global-load latency is paid once or twice per tile); split z of the K range [z kc, (z + 1) kc) writes
-// its own C.  Operand tiles are staged with the unit-stride index fastest,
-// so the global reads coalesce for either layout.
+// 32W x 32W tile, 4 waves as 2 x 2, each wave 16W x 16W = W x W MFMA 16x16
+// tiles (W = 2: 64 x 64; W = 1: 32 x 32 for the small GEMMs, four times the
+// workgroups and a quarter of each one's staging and epilogue), K staged
+// 128 at a time, so the MLP's K <= 256 costs one or two global-load round
+// trips per tile; split z of the K range
+// [z kc, (z + 1) kc) writes its own C.  Operand tiles are staged with the
+// unit-stride index fastest, so the global reads coalesce for either layout.
+// Every output element accumulates its K blocks in the same order for any W.
+template <int W>
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
-    constexpr int TM = 64, TN = 64, TK = 128, LDK = TK + 8;
+    constexpr int TM = 32 * W, TN = 32 * W, TK = 128, LDK = TK + 8;
     __shared__ __align__(16) uint16_t As[TM * LDK];
     __shared__ __align__(16) uint16_t Bs[TN * LDK];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -60,76 +72,119 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
     const int kbeg = blockIdx.z * g.kc, kend = min(g.K, kbeg + g.kc);
     float *C = g.C + (long)blockIdx.z * g.c_split;
     const bool a_kfast = g.a_k == 1, b_kfast = g.b_k == 1;
-    floatx4 acc[2][2];
+    floatx4 acc[W][W];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < W; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < W; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
     // One operand tile: `rows` x TK (K fast) or TK x `rows` (rows fast) from a
-    // global matrix whose unit-stride index is the fast one; 16-byte loads
-    // when the tile is aligned (vec), element loads at the matrix edges.
-    auto stage = [&](const float *src, long s_row, long s_k, bool kfast, bool vec, int row0,
-                     int nrows_total, int k0, float scale, uint16_t *dst) {
-        constexpr int NV = TM * TK / 4;  // float4 groups per tile
+    // global matrix whose unit-stride index is the fast one.  Every load of
+    // the tile is issued before any is used: a 16-byte load per group of four
+    // that lies inside the tile (when the matrix allows them), 4-byte loads
+    // for the groups that straddle its edge (rare: a second pass) or for the
+    // whole tile otherwise; elements outside the tile's rows or the split's
+    // K range are 0.  Buffer loads, their range the matrix extent.
+    auto stage = [&](const float *src, long s_row, long s_k, bool kfast, bool vec, int bytes, int row0,
+                     int nrows_total, int k0, float scale, uint16_t *dst) __attribute__((always_inline)) {
+        constexpr int NI = TM * TK / 4 / 256;  // float4 groups per thread
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(src), (short)0, bytes, kBufferWord3);
+        const long s_q = kfast ? s_k : s_row;  // stride between the group's four elements
+        auto elem_ok = [&](int gr, int gk, int q) {
+            return kfast ? (gr < nrows_total && gk + q < kend) : (gr + q < nrows_total && gk < kend);
+        };
+        auto elem_off = [&](int gr, int gk, int q) -> uint32_t {
+            return elem_ok(gr, gk, q) ? (uint32_t)(((long)gr * s_row + (long)gk * s_k + q * s_q) * 4) : 0u;
+        };
+        auto ld1 = [&](uint32_t off) {
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+        };
+        // the group of thread slot i: tile row r, tile column k
+        auto slot = [&](int i, int &r, int &k) {
+            const int e4 = tid + i * 256;
+            r = kfast ? e4 / (TK / 4) : (e4 % (TM / 4)) * 4;
+            k = kfast ? (e4 % (TK / 4)) * 4 : e4 / (TM / 4);
+        };
+        // raw loads first (no use of a loaded value until all are issued)
+        float v[NI][4];
+        if (vec) {
 #pragma unroll
-        for (int e4 = tid; e4 < NV; e4 += 256) {
-            int r, k;
-            if (kfast) {
-                r = e4 / (TK / 4);
-                k = (e4 % (TK / 4)) * 4;
-            } else {
-                k = e4 / (TM / 4);
-                r = (e4 % (TM / 4)) * 4;
+            for (int i = 0; i < NI; ++i) {
+                int r, k;
+                slot(i, r, k);
+                const int gr = row0 + r, gk = k0 + k;
+                const bool full = elem_ok(gr, gk, 3) && elem_ok(gr, gk, 0);
+                const floatx4 x = __builtin_bit_cast(
+                    floatx4, __builtin_amdgcn_raw_buffer_load_b128(rs, full ? elem_off(gr, gk, 0) : 0u, 0, 0));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[i][q] = x[q];
             }
-            float v[4];
-            const int gr = row0 + r, gk = k0 + k;
-            const bool full = kfast ? (gr < nrows_total && gk + 3 < kend)
-                                    : (gk < kend && gr + 3 < nrows_total);
-            if (vec && full) {
-                const float4 x = *reinterpret_cast<const float4 *>(src + (long)gr * s_row + (long)gk * s_k);
-                v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-            } else {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int rr = kfast ? gr : gr + q, kk = kfast ? gk + q : gk;
-                    v[q] = (rr < nrows_total && kk < kend) ? src[(long)rr * s_row + (long)kk * s_k] : 0.f;
+            for (int i = 0; i < NI; ++i) {
+                int r, k;
+                slot(i, r, k);
+                const int gr = row0 + r, gk = k0 + k;
+                if (elem_ok(gr, gk, 0) && !elem_ok(gr, gk, 3)) {  // straddles the tile's edge
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v[i][q] = ld1(elem_off(gr, gk, q));
                 }
             }
+        } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int rr = kfast ? r : r + q, kk = kfast ? k + q : k;
-                dst[rr * LDK + kk] = to_bits(v[q] * scale, false);
+            for (int i = 0; i < NI; ++i) {
+                int r, k;
+                slot(i, r, k);
+                const int gr = row0 + r, gk = k0 + k;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[i][q] = ld1(elem_off(gr, gk, q));
+            }
+        }
+        // then the elements outside the tile set to 0, BF16, LDS
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            int r, k;
+            slot(i, r, k);
+            const int gr = row0 + r, gk = k0 + k;
+            uint16_t h[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) h[q] = to_bits((elem_ok(gr, gk, q) ? v[i][q] : 0.f) * scale, false);
+            if (kfast) {
+                typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+                *reinterpret_cast<u16x4 *>(&dst[r * LDK + k]) = u16x4{h[0], h[1], h[2], h[3]};
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) dst[(r + q) * LDK + k] = h[q];
             }
         }
     };
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
         __syncthreads();
-        stage(g.A, g.a_m, g.a_k, a_kfast, g.vec_a != 0, m0, g.M, k0, g.a_scale, As);
-        stage(g.B, g.b_n, g.b_k, b_kfast, g.vec_b != 0, n0, g.N, k0, g.b_scale, Bs);
+        stage(g.A, g.a_m, g.a_k, a_kfast, g.vec_a != 0, g.bytes_a, m0, g.M, k0, g.a_scale, As);
+        stage(g.B, g.b_n, g.b_k, b_kfast, g.vec_b != 0, g.bytes_b, n0, g.N, k0, g.b_scale, Bs);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < TK; ks += 32) {
-            bf16x8 a[2], b[2];
+            bf16x8 a[W], b[W];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * 32 + i * 16 + lr) * LDK + ks + lk * 8]);
-                b[i] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * 32 + i * 16 + lr) * LDK + ks + lk * 8]);
+            for (int i = 0; i < W; ++i) {
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm * 16 * W + i * 16 + lr) * LDK + ks + lk * 8]);
+                b[i] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn * 16 * W + i * 16 + lr) * LDK + ks + lk * 8]);
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < W; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < W; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < W; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < W; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int m = m0 + wm * 32 + i * 16 + lk * 4 + q;
-                const int n = n0 + wn * 32 + j * 16 + lr;
+                const int m = m0 + wm * 16 * W + i * 16 + lk * 4 + q;
+                const int n = n0 + wn * 16 * W + j * 16 + lr;
                 if (m >= g.M || n >= g.N) continue;
                 float v = acc[i][j][q];
                 if (g.epi == kEpiBiasSilu) {
@@ -152,6 +207,28 @@ __global__ void k_splitk_reduce(const float *part, int splits, int M, int N, flo
     C[(i / N) * c_m + i % N] = v;
 }
 
+// The operands' buffer ranges (their extents must fit 31 bits).
+static int set_extents(GemmArgs &g) {
+    g.bytes_a = extent_bytes(g.M, g.K, g.a_m, g.a_k);
+    g.bytes_b = extent_bytes(g.N, g.K, g.b_n, g.b_k);
+    if (g.bytes_a == 0 || g.bytes_b == 0) return set_error(SPEO_ERR_ARG, "mlp gemm: operand larger than 2 GiB");
+    return SPEO_OK;
+}
+
+// 64 x 64 tiles when they give the device at least one workgroup per compute
+// unit (256), else 32 x 32.
+static int launch_gemm(const GemmArgs &g, int splits, hipStream_t s) {
+    const long tiles64 = (long)((g.N + 63) / 64) * ((g.M + 63) / 64) * splits;
+    if (tiles64 >= 256) {
+        dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, splits);
+        hipLaunchKernelGGL(k_gemm_bf16<2>, grid, dim3(256), 0, s, g);
+    } else {
+        dim3 grid((g.N + 31) / 32, (g.M + 31) / 32, splits);
+        hipLaunchKernelGGL(k_gemm_bf16<1>, grid, dim3(256), 0, s, g);
+    }
+    return check_launch("k_gemm_bf16");
+}
+
 static int gemm(const GemmArgs &g0, hipStream_t s) {
     if (g0.M <= 0 || g0.N <= 0) return SPEO_OK;
     GemmArgs g = g0;
@@ -159,9 +236,8 @@ static int gemm(const GemmArgs &g0, hipStream_t s) {
     g.c_split = 0;
     g.vec_a = vec_ok(g.A, g.a_m, g.a_k);
     g.vec_b = vec_ok(g.B, g.b_n, g.b_k);
-    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
-    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
-    return check_launch("k_gemm_bf16");
+    SPEO_TRY(set_extents(g));
+    return launch_gemm(g, 1, s);
 }
 
 // K split into chunks of 128 (plain store epilogue only), partial products
@@ -173,14 +249,13 @@ static int gemm_splitk(const GemmArgs &g0, float *part, hipStream_t s) {
     GemmArgs g = g0;
     g.vec_a = vec_ok(g.A, g.a_m, g.a_k);
     g.vec_b = vec_ok(g.B, g.b_n, g.b_k);
+    SPEO_TRY(set_extents(g));
     g.kc = (g.K + splits - 1) / splits;
     g.kc = (g.kc + 31) / 32 * 32;
     g.C = part;
     g.c_m = g.N;
     g.c_split = (long)g.M * g.N;
-    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, splits);
-    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
-    SPEO_TRY(check_launch("k_gemm_bf16"));
+    SPEO_TRY(launch_gemm(g, splits, s));
     const long total = (long)g.M * g.N;
     hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, part,
                        splits, g.M, g.N, g0.C, g0.c_m);
@@ -225,8 +300,19 @@ __global__ __launch_bounds__(1024) void k_colsum(const float *Mx, const float *w
     const int c = blockIdx.x * 64 + (threadIdx.x & 63), grp = threadIdx.x >> 6;
     float acc = 0.f;
     if (c < cols) {
-#pragma unroll 8
-        for (int b = grp; b < n; b += 16) acc = fmaf(Mx[(long)b * cols + c], w ? w[b] : 1.0f, acc);
+        // rows grp, grp + 16, ... in order; 32 of them loaded before they are added
+        int b = grp;
+        for (; b + 16 * 31 < n; b += 16 * 32) {
+            float x[32], y[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                x[u] = Mx[(long)(b + 16 * u) * cols + c];
+                y[u] = w ? w[b + 16 * u] : 1.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u) acc = fmaf(x[u], y[u], acc);
+        }
+        for (; b < n; b += 16) acc = fmaf(Mx[(long)b * cols + c], w ? w[b] : 1.0f, acc);
     }
     red[grp][threadIdx.x & 63] = acc;
     __syncthreads();
