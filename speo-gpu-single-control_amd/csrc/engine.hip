@@ -79,8 +79,14 @@ __device__ __forceinline__ float comp4(const float4 &v, int q) {
     return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
 }
 
+// Row tile 32: four waves per SIMD (at most 128 registers).  The fused FP32
+// kernel would otherwise take 158 and run three; the few values it then
+// spills cost less than the lost occupancy (7 % faster on an uneven batch of
+// 256 islands, D 10..100).  The other row tiles lose by it (row tile 16: 12 %
+// slower on 32 islands x 256 at D = 100) and keep their allocation.
 template <int RT, int PREC, int MODE>
-__global__ __launch_bounds__(256) void k_generation(GenerationArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 4 : 1))) void k_generation(
+    GenerationArgs a) {
     extern __shared__ __align__(16) char lds[];
     const TileDesc t = a.tiles[blockIdx.x];
     const IslandDesc I = a.isl[t.island];
