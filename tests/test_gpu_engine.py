@@ -171,3 +171,42 @@ def test_convergence_on_small_problem(cuda, speo, oracle):
     e.run(600)
     _, _, f_end = e.best()
     assert f_end.min() < 1e-4 * f_start.min()
+
+
+@pytest.mark.parametrize("case", ["no_islands", "small_island", "problem_index", "bounds", "mode",
+                                  "precision", "rows_per_block", "migration_size", "epoch_parts",
+                                  "surrogate_dims", "run_negative"])
+def test_engine_rejects_bad_configs(cuda, speo, oracle, case):
+    # every rejected configuration returns SPEO_ERR_ARG with a message naming
+    # the problem, before any device work
+    probs = _problems(speo, oracle, DIMS)
+    sizes, pidx, kw = [16, 24], [0, 1], {}
+    match = "speo"
+    if case == "no_islands":
+        sizes, pidx, match = [], [], "islands and problems are required"
+    elif case == "small_island":
+        sizes, match = [7, 24], "island size must be >= 8"
+    elif case == "problem_index":
+        pidx, match = [0, 2], "problem index in range"
+    elif case == "bounds":
+        kw, match = dict(lo=5.0, hi=5.0), "bad migration or bounds"
+    elif case == "mode":
+        kw, match = dict(mode=7), "unknown mode"
+    elif case == "precision":
+        kw, match = dict(precision=9), "unknown precision"
+    elif case == "rows_per_block":
+        kw, match = dict(rows_per_block=48), "rows_per_block must be"
+    elif case == "migration_size":
+        kw, match = dict(migration_size=9, interval=5), ">= 2 \\* migration_size"
+    elif case == "epoch_parts":
+        kw, match = dict(mode=speo.MODE_EPOCH, epoch_parts=3), "epoch_parts must be"
+    elif case == "surrogate_dims":
+        kw, match = dict(surrogate_candidates=4), "share one dimension"
+    if case == "run_negative":
+        e = _engine(speo, probs, sizes, pidx)
+        with pytest.raises(speo.SpeoError, match="bad arguments"):
+            e.run(-1)
+        e.run(0)
+        return
+    with pytest.raises(speo.SpeoError, match=match):
+        _engine(speo, probs, sizes, pidx, **kw)
