@@ -319,8 +319,9 @@ SPEO_EXPORT int speo_engine_best(speo_engine e, int *row_out, float *f32_out, do
 SPEO_EXPORT int speo_engine_read(speo_engine e, float *X_out, float *f_out);
 /* Host copies of the most recent trial vectors and their fitness, same
  * layout as speo_engine_read; call before anything that applies a pending
- * selection (speo_engine_best / _read / _write), which clears the trial
- * fitness.  Synchronous. */
+ * selection (speo_engine_best / _read / _write, or a migration after the last
+ * generation run), which in SPEO_MODE_FUSED clears the trial fitness.
+ * Synchronous. */
 SPEO_EXPORT int speo_engine_read_trials(speo_engine e, float *U_out, float *fU_out);
 /* Overwrites the population and fitness from host arrays (same layout). */
 SPEO_EXPORT int speo_engine_write(speo_engine e, const float *X_in, const float *f_in);
