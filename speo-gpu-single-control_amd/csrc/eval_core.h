@@ -195,12 +195,123 @@ struct RowSource {
     int nrows;
 };
 
+// Vector staging of one K chunk of gemm_chunk, for rows in global memory that
+// are 16-byte aligned with a row stride of a multiple of 4 floats: 16-byte
+// loads, every one of the chunk issued before the first is used (scalar
+// loads only for a group of four that straddles the end of K).  The values
+// are those of the scalar loops: A = (x - o) * rate (* sa) for r < nrows,
+// k < kc, else 0; B from the operand's zero-padded rows (B32 rows padded to a
+// multiple of 4 floats, M16 rows to a multiple of 8).
+template <int RT>
+__device__ __forceinline__ void stage_chunk_f32(const CompDev &c, const RowSource &src, int k0, int kc, int n0,
+                                                int ncols, float *As, float *Bs, int tid) {
+    constexpr int LDA = kKC + 2, LDB = kNC + 16;
+    constexpr int GA = RT * kKC / 4 / 256, GB = kKC * kNC / 4 / 256;
+    floatx4 xa[GA], oa[GA], xb[GB];
+#pragma unroll
+    for (int i = 0; i < GA; ++i) {
+        const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
+        const bool full = r < src.nrows && k + 3 < kc;
+        xa[i] = gld(reinterpret_cast<const floatx4 *>(full ? src.rows + (size_t)r * src.ld + k0 + k : src.rows));
+        oa[i] = gld(reinterpret_cast<const floatx4 *>(full ? c.shift + k0 + k : c.shift));
+    }
+#pragma unroll
+    for (int i = 0; i < GB; ++i) {
+        const int e = tid + i * 256, k = e / (kNC / 4), n = (e % (kNC / 4)) * 4;
+        const bool ok = k < kc && n < ncols;
+        xb[i] = gld(reinterpret_cast<const floatx4 *>(c.B32 + (ok ? (size_t)(k0 + k) * c.ldb32 + n0 + n : 0)));
+    }
+#pragma unroll
+    for (int i = 0; i < GA; ++i) {
+        const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
+        const bool full = r < src.nrows && k + 3 < kc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float v;
+            if (full)
+                v = (xa[i][q] - oa[i][q]) * c.rate;
+            else
+                v = (r < src.nrows && k + q < kc)
+                        ? (src.rows[(size_t)r * src.ld + k0 + k + q] - c.shift[k0 + k + q]) * c.rate
+                        : 0.f;
+            As[r * LDA + k + q] = v;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < GB; ++i) {
+        const int e = tid + i * 256, k = e / (kNC / 4), n = (e % (kNC / 4)) * 4;
+        const bool ok = k < kc && n < ncols;
+        *reinterpret_cast<floatx4 *>(Bs + k * LDB + n) = ok ? xb[i] : floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
+template <int RT, int P, bool H>
+__device__ __forceinline__ void stage_chunk_16(const CompDev &c, const RowSource &src, int k0, int kc, int n0,
+                                               int ncols, uint16_t *As, uint16_t *Bs, int tid) {
+    constexpr int LDA = kKC + 8, LDB = kKC + 8;
+    constexpr int GA = RT * kKC / 4 / 256, GB = kNC * kKC / 8 / 256;
+    typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const float sa = H ? kF16ScaleA : 1.0f;
+    floatx4 xa[GA], oa[GA];
+    u32x4 xb[P][GB];
+#pragma unroll
+    for (int i = 0; i < GA; ++i) {
+        const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
+        const bool full = r < src.nrows && k + 3 < kc;
+        xa[i] = gld(reinterpret_cast<const floatx4 *>(full ? src.rows + (size_t)r * src.ld + k0 + k : src.rows));
+        oa[i] = gld(reinterpret_cast<const floatx4 *>(full ? c.shift + k0 + k : c.shift));
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int i = 0; i < GB; ++i) {
+            const int e = tid + i * 256, n = e / (kKC / 8), k = (e % (kKC / 8)) * 8;
+            const bool ok = k < kc && n < ncols;
+            xb[p][i] = gld(reinterpret_cast<const u32x4 *>(c.M16[p] + (ok ? (size_t)(n0 + n) * c.ldm16 + k0 + k : 0)));
+        }
+#pragma unroll
+    for (int i = 0; i < GA; ++i) {
+        const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
+        const bool full = r < src.nrows && k + 3 < kc;
+        u16x4 h[P];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float v;
+            if (full)
+                v = (xa[i][q] - oa[i][q]) * c.rate * sa;
+            else
+                v = (r < src.nrows && k + q < kc)
+                        ? (src.rows[(size_t)r * src.ld + k0 + k + q] - c.shift[k0 + k + q]) * c.rate * sa
+                        : 0.f;
+            float rem = v;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const uint16_t b = to_bits(rem, H);
+                h[p][q] = b;
+                rem -= from_bits(b, H);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) *reinterpret_cast<u16x4 *>(As + (p * RT + r) * LDA + k) = h[p];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int i = 0; i < GB; ++i) {
+            const int e = tid + i * 256, n = e / (kKC / 8), k = (e % (kKC / 8)) * 8;
+            const bool ok = k < kc && n < ncols;
+            *reinterpret_cast<u32x4 *>(Bs + (p * kNC + n) * LDB + k) = ok ? xb[p][i] : u32x4{0u, 0u, 0u, 0u};
+        }
+}
+
 // ------------------------------------------------------------------ GEMM
 //
 // Computes the Z columns [n0, n0 + kNC) of the tile's RT rows for component
 // `c` and adds their terms to s1[i], s2[i] (row 4*(lane/16)+i of this wave's
 // row tile; only lanes with lane%16 == 0 hold the full sums after the final
 // sum16, done by the caller).  All 256 threads of the block must call it.
+// src.rows is in global memory.
 // `lds` must hold gemm_lds_bytes<RT, PREC>() bytes.
 template <int RT, int PREC>
 __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int dim, int n0,
@@ -217,6 +328,8 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
     for (int t = 0; t < TPW; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     const int ncols = min(kNC, dim - n0);
+    // rows aligned for the vector staging (uniform over the block)
+    const bool vec = (src.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(src.rows) & 15) == 0;
 
     if constexpr (Prec<PREC>::parts == 0) {
         float *As = reinterpret_cast<float *>(lds);  // [RT][kKC+2]
@@ -225,18 +338,22 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
         for (int k0 = 0; k0 < dim; k0 += kKC) {
             const int kc = min(kKC, dim - k0);
             __syncthreads();
-            for (int e = tid; e < RT * kKC; e += 256) {
-                const int r = e / kKC, k = e % kKC;
-                float v = 0.f;
-                if (r < src.nrows && k < kc)
-                    v = (src.rows[(size_t)r * src.ld + k0 + k] - c.shift[k0 + k]) * c.rate;
-                As[r * LDA + k] = v;
-            }
-            for (int e = tid; e < kKC * kNC; e += 256) {
-                const int k = e / kNC, n = e % kNC;
-                float v = 0.f;
-                if (k < kc && n < ncols) v = c.B32[(size_t)(k0 + k) * c.ldb32 + n0 + n];
-                Bs[k * LDB + n] = v;
+            if (vec) {
+                stage_chunk_f32<RT>(c, src, k0, kc, n0, ncols, As, Bs, tid);
+            } else {
+                for (int e = tid; e < RT * kKC; e += 256) {
+                    const int r = e / kKC, k = e % kKC;
+                    float v = 0.f;
+                    if (r < src.nrows && k < kc)
+                        v = (src.rows[(size_t)r * src.ld + k0 + k] - c.shift[k0 + k]) * c.rate;
+                    As[r * LDA + k] = v;
+                }
+                for (int e = tid; e < kKC * kNC; e += 256) {
+                    const int k = e / kNC, n = e % kNC;
+                    float v = 0.f;
+                    if (k < kc && n < ncols) v = c.B32[(size_t)(k0 + k) * c.ldb32 + n0 + n];
+                    Bs[k * LDB + n] = v;
+                }
             }
             __syncthreads();
             const int ksteps = (kc + 3) >> 2;
@@ -262,26 +379,30 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
         for (int k0 = 0; k0 < dim; k0 += kKC) {
             const int kc = min(kKC, dim - k0);
             __syncthreads();
-            for (int e = tid; e < RT * kKC; e += 256) {
-                const int r = e / kKC, k = e % kKC;
-                float v = 0.f;
-                if (r < src.nrows && k < kc)
-                    v = (src.rows[(size_t)r * src.ld + k0 + k] - c.shift[k0 + k]) * c.rate * sa;
-                float rem = v;
+            if (vec) {
+                stage_chunk_16<RT, P, H>(c, src, k0, kc, n0, ncols, As, Bs, tid);
+            } else {
+                for (int e = tid; e < RT * kKC; e += 256) {
+                    const int r = e / kKC, k = e % kKC;
+                    float v = 0.f;
+                    if (r < src.nrows && k < kc)
+                        v = (src.rows[(size_t)r * src.ld + k0 + k] - c.shift[k0 + k]) * c.rate * sa;
+                    float rem = v;
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    const uint16_t b = to_bits(rem, H);
-                    As[(p * RT + r) * LDA + k] = b;
-                    rem -= from_bits(b, H);
+                    for (int p = 0; p < P; ++p) {
+                        const uint16_t b = to_bits(rem, H);
+                        As[(p * RT + r) * LDA + k] = b;
+                        rem -= from_bits(b, H);
+                    }
                 }
-            }
-            for (int e = tid; e < kNC * kKC; e += 256) {
-                const int n = e / kKC, k = e % kKC;
-                const bool ok = (k < kc && n < ncols);
+                for (int e = tid; e < kNC * kKC; e += 256) {
+                    const int n = e / kKC, k = e % kKC;
+                    const bool ok = (k < kc && n < ncols);
 #pragma unroll
-                for (int p = 0; p < P; ++p)
-                    Bs[(p * kNC + n) * LDB + k] =
-                        ok ? c.M16[p][(size_t)(n0 + n) * c.ldm16 + k0 + k] : (uint16_t)0;
+                    for (int p = 0; p < P; ++p)
+                        Bs[(p * kNC + n) * LDB + k] =
+                            ok ? c.M16[p][(size_t)(n0 + n) * c.ldm16 + k0 + k] : (uint16_t)0;
+                }
             }
             __syncthreads();
             const int ksteps = (kc + 31) >> 5;
