@@ -33,9 +33,34 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch p) {
         o1 = red + 8 * RT;
         o2 = o1 + RT;
         const int kp4 = (p.dim + 3) & ~3;
-        for (int e = threadIdx.x; e < RT * kp4; e += 256) {
-            const int r = e / kp4, k = e % kp4;
-            rows[r * kTS + k] = (r < nrows && k < p.dim) ? p.X[(size_t)(row0 + r) * p.ld + k] : 0.f;
+        if ((p.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(p.X) & 15) == 0) {
+            // 16-byte loads, all of the tile issued before the first is
+            // stored; columns >= dim set to 0 (the row's padding may hold
+            // anything)
+            constexpr int G = RT * (kSmallD / 4) / 256;  // groups per thread at most
+            const int ng = kp4 >> 2;
+            floatx4 x[G];
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                const int e = threadIdx.x + i * 256, r = e / ng, g = e - r * ng;
+                const bool ok = e < RT * ng && r < nrows;
+                x[i] = gld(reinterpret_cast<const floatx4 *>(p.X + (ok ? (size_t)(row0 + r) * p.ld + 4 * g : 0)));
+            }
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                const int e = threadIdx.x + i * 256, r = e / ng, g = e - r * ng;
+                if (e >= RT * ng) break;
+                floatx4 v = x[i];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (r >= nrows || 4 * g + q >= p.dim) v[q] = 0.f;
+                *reinterpret_cast<floatx4 *>(rows + r * kTS + 4 * g) = v;
+            }
+        } else {
+            for (int e = threadIdx.x; e < RT * kp4; e += 256) {
+                const int r = e / kp4, k = e % kp4;
+                rows[r * kTS + k] = (r < nrows && k < p.dim) ? p.X[(size_t)(row0 + r) * p.ld + k] : 0.f;
+            }
         }
         __syncthreads();
         float s1[RT / 16][4], s2[RT / 16][4];
