@@ -533,11 +533,32 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *X, int ld, int n
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
     const int row0 = blockIdx.x * TM;
-    for (int e = tid; e < TM * KP; e += 256) {
-        const int r = e / KP, k = e % KP;
-        float v = 0.f;
-        if (row0 + r < n && k < D) v = X[(size_t)(row0 + r) * ld + k] * kXScale;
-        Xs[r * LX + k] = to_bits(v, false);
+    if ((ld & 3) == 0 && ld >= KP && (reinterpret_cast<uintptr_t>(X) & 15) == 0) {
+        // 16-byte loads, all of the tile issued before the first is converted
+        constexpr int G = TM * KP / 4 / 256;
+        floatx4 x[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            const int e = tid + i * 256, r = e / (KP / 4), k = (e % (KP / 4)) * 4;
+            x[i] = gld(reinterpret_cast<const floatx4 *>(X + (row0 + r < n ? (size_t)(row0 + r) * ld + k : 0)));
+        }
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            const int e = tid + i * 256, r = e / (KP / 4), k = (e % (KP / 4)) * 4;
+            typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+            u16x4 h;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                h[q] = to_bits(row0 + r < n && k + q < D ? x[i][q] * kXScale : 0.f, false);
+            *reinterpret_cast<u16x4 *>(&Xs[r * LX + k]) = h;
+        }
+    } else {
+        for (int e = tid; e < TM * KP; e += 256) {
+            const int r = e / KP, k = e % KP;
+            float v = 0.f;
+            if (row0 + r < n && k < D) v = X[(size_t)(row0 + r) * ld + k] * kXScale;
+            Xs[r * LX + k] = to_bits(v, false);
+        }
     }
     __syncthreads();
     const int c0 = wave * 64;
