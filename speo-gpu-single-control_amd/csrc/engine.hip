@@ -251,7 +251,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 
                 float *red = reinterpret_cast<float *>(gemm_lds);
                 float *o1 = red + 8 * RT, *o2 = o1 + RT;
                 float s1[4] = {0.f, 0.f, 0.f, 0.f}, s2[4] = {0.f, 0.f, 0.f, 0.f};
-                for (int n0 = 0; n0 < dim; n0 += kNC) gemm_chunk<RT, PREC>(comp, src, dim, n0, gemm_lds, s1, s2);
+                for (int n0 = 0; n0 < dim; n0 += kNC)
+                    gemm_chunk<RT, PREC, false>(comp, src, dim, n0, gemm_lds, s1, s2);
                 reduce_rows<RT>(s1, s2, red, o1, o2);
                 if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);
             }

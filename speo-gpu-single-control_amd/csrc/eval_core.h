@@ -313,7 +313,11 @@ __device__ __forceinline__ void stage_chunk_16(const CompDev &c, const RowSource
 // sum16, done by the caller).  All 256 threads of the block must call it.
 // src.rows is in global memory.
 // `lds` must hold gemm_lds_bytes<RT, PREC>() bytes.
-template <int RT, int PREC>
+// VS: vector staging (stage_chunk_*) where the rows allow it.  Its registers
+// count in every kernel that contains the call, so a kernel whose other paths
+// need them (the generation kernel's small-D phases) keeps the element-wise
+// form.
+template <int RT, int PREC, bool VS = true>
 __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int dim, int n0,
                                            char *lds, float (&s1)[4], float (&s2)[4]) {
     constexpr int NSPLIT = 4 / (RT / 16);  // waves sharing one 16-row tile
@@ -329,7 +333,7 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
 
     const int ncols = min(kNC, dim - n0);
     // rows aligned for the vector staging (uniform over the block)
-    const bool vec = (src.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(src.rows) & 15) == 0;
+    const bool vec = VS && (src.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(src.rows) & 15) == 0;
 
     if constexpr (Prec<PREC>::parts == 0) {
         float *As = reinterpret_cast<float *>(lds);  // [RT][kKC+2]
