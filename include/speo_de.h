@@ -383,6 +383,49 @@ SPEO_EXPORT int speo_mlp_adam_step(speo_mlp m, float lr);
 SPEO_EXPORT int speo_spearman(const float *a, const float *b, int n, float *rho_out,
                               speo_stream_t stream);
 
+/* ---- run driver and result record ------------------------------------- *
+ * The evaluation-budget loop and the result row of the reference's island
+ * driver (IslandEA::Execute / PrintResult, src/island_EA.cc:78-105, 139-160;
+ * header line :24), for runs whose output must line up row for row with the
+ * reference's SPEO_GPU.csv. */
+
+/* Generations the budget loop runs: while current < max_FEs, one generation
+ * of per_generation evaluations (island_EA.cc:86-104; the initial population
+ * counts, :81).  0 when current >= max_FEs; negative arguments give 0. */
+SPEO_EXPORT int64_t speo_generations_for_fes(int64_t max_FEs, int64_t current, int64_t per_generation);
+/* Runs the engine until its evaluation count (speo_engine_stats.evaluations,
+ * all islands) reaches max_FEs, migrating on the engine's interval;
+ * *generations_out (may be NULL) receives the generations run.  The
+ * reference's budget is per GPU island, max_base_FEs * dim / island_num; an
+ * engine holding every island is given max_base_FEs * dim.  Asynchronous, as
+ * speo_engine_run. */
+SPEO_EXPORT int speo_engine_run_fes(speo_engine e, int64_t max_FEs, int *generations_out);
+/* The reference's per-run seed (src/main.cc:138). */
+SPEO_EXPORT uint64_t speo_run_seed(int node_ID, int function_ID, int run_ID);
+/* "CR/F=<CR>/<F>_<strategy>" as DE_CUDA::GetParameters (DE_CUDA.cc:18-56);
+ * writes at most buflen bytes including the terminator and returns the
+ * length, or SPEO_ERR_ARG for an unknown strategy or a short buffer. */
+SPEO_EXPORT int speo_de_parameters(double CR, double F, int strategy_ID, char *buf, int buflen);
+
+/* One result row: the 21 columns of the reference's record, in its order. */
+typedef struct speo_run_record {
+    int function_ID, run_ID, dim;
+    double best_fitness, time_period, communication_percentage;
+    int64_t total_FEs;
+    int node_num, GPU_num, island_num, total_pop_size, subisland_size, interval, migration_size;
+    double migration_rate;
+    int buffer_capacity;
+    const char *migration_topology, *buffer_manage, *regroup_option, *configure_EA,
+        *EA_parameters;
+} speo_run_record;
+
+/* The header line of the record file (no newline). */
+SPEO_EXPORT const char *speo_result_header(void);
+/* The row as the reference's stream output writes it (numbers with six
+ * significant digits, %g), without the newline; returns the length, or
+ * SPEO_ERR_ARG for a null argument, a null string field or a short buffer. */
+SPEO_EXPORT int speo_result_record(const speo_run_record *r, char *buf, int buflen);
+
 #ifdef __cplusplus
 }
 #endif

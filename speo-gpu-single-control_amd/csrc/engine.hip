@@ -1359,6 +1359,17 @@ int speo_engine_run(speo_engine e, int generations) {
     return e->e.run(generations);
 }
 
+// IslandEA::Execute's budget loop (src/island_EA.cc:86-104) over the engine's
+// evaluation count; speo_engine_run splits the generations at the migration
+// points as the reference migrates at generation % interval == 0.
+int speo_engine_run_fes(speo_engine e, int64_t max_FEs, int *generations_out) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_run_fes: null engine");
+    const int64_t g = speo_generations_for_fes(max_FEs, e->e.fes, e->e.total_rows);
+    if (g > 0x7fffffff) return set_error(SPEO_ERR_ARG, "speo_engine_run_fes: budget beyond 2^31 generations");
+    if (generations_out) *generations_out = (int)g;
+    return e->e.run((int)g);
+}
+
 int speo_engine_sync(speo_engine e) {
     if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_sync: null engine");
     return check_hip(hipStreamSynchronize(e->e.stream), "speo_engine_sync");

@@ -31,6 +31,7 @@ __all__ = [
     "OBJ_ELLIPTIC", "OBJ_RASTRIGIN", "OBJ_ACKLEY", "PREC_FP32", "PREC_BF16", "PREC_FP16",
     "PREC_BF16X3", "PREC_FP16X3", "PREC_BF16X6", "PREC_NAMES", "MODE_PLAIN", "MODE_FUSED",
     "MODE_EPOCH",    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
+    "generations_for_fes", "run_seed", "de_parameters", "result_header", "result_record",
 ]
 
 
@@ -148,6 +149,44 @@ def segment_argmin(f, off, best_out, stream=None):
                                     _stream(stream)))
 
 
+def generations_for_fes(max_fes: int, current: int, per_generation: int) -> int:
+    return int(lib().speo_generations_for_fes(int(max_fes), int(current), int(per_generation)))
+
+
+def run_seed(node_id: int, function_id: int, run_id: int) -> int:
+    """The reference driver's per-run seed (src/main.cc:138)."""
+    return int(lib().speo_run_seed(node_id, function_id, run_id))
+
+
+def de_parameters(cr: float, f: float, strategy: int = 4) -> str:
+    """DE_CUDA::GetParameters' string, e.g. "CR/F=0.9/0.5_rand/1/bin"."""
+    buf = ctypes.create_string_buffer(128)
+    n = lib().speo_de_parameters(float(cr), float(f), int(strategy), buf, len(buf))
+    check(n if n < 0 else 0)
+    return buf.value.decode()
+
+
+def result_header() -> str:
+    """The header line of the reference's SPEO_GPU.csv (island_EA.cc:24)."""
+    return lib().speo_result_header().decode()
+
+
+def result_record(**fields) -> str:
+    """One result row (island_EA.cc:139-160) from the 21 named columns of
+    speo_run_record; string columns are str."""
+    from ._lib import RunRecord
+    rec = RunRecord()
+    for name, _ in RunRecord._fields_:
+        if name not in fields:
+            raise TypeError(f"result_record: missing column {name}")
+        v = fields[name]
+        setattr(rec, name, v.encode() if isinstance(v, str) else v)
+    buf = ctypes.create_string_buffer(1024)
+    n = lib().speo_result_record(byref(rec), buf, len(buf))
+    check(n if n < 0 else 0)
+    return buf.value.decode()
+
+
 def de_info_bytes(cr, f, strategy=4, pop_size=-1) -> bytes:
     """Packed DEInfo array (EA_CUDA.h:33-39) for upload to the device."""
     cr = np.atleast_1d(np.asarray(cr, np.float32))
@@ -208,6 +247,13 @@ class Engine:
 
     def run(self, generations: int):
         check(lib().speo_engine_run(self.handle, generations))
+
+    def run_fes(self, max_fes: int) -> int:
+        """Generations until the evaluation count reaches max_fes (the budget
+        loop of IslandEA::Execute, src/island_EA.cc:86-104); returns them."""
+        g = c_int(0)
+        check(lib().speo_engine_run_fes(self.handle, int(max_fes), byref(g)))
+        return g.value
 
     def sync(self):
         check(lib().speo_engine_sync(self.handle))

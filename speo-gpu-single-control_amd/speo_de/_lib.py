@@ -68,6 +68,18 @@ class EngineConfig(Structure):
                 ("surrogate_rho_min", c_float), ("epoch_parts", c_int)]
 
 
+class RunRecord(Structure):
+    """speo_run_record: the 21 columns of the reference's result row."""
+    _fields_ = [("function_ID", c_int), ("run_ID", c_int), ("dim", c_int),
+                ("best_fitness", c_double), ("time_period", c_double),
+                ("communication_percentage", c_double), ("total_FEs", c_int64),
+                ("node_num", c_int), ("GPU_num", c_int), ("island_num", c_int),
+                ("total_pop_size", c_int), ("subisland_size", c_int), ("interval", c_int),
+                ("migration_size", c_int), ("migration_rate", c_double), ("buffer_capacity", c_int),
+                ("migration_topology", c_char_p), ("buffer_manage", c_char_p),
+                ("regroup_option", c_char_p), ("configure_EA", c_char_p), ("EA_parameters", c_char_p)]
+
+
 class EngineStats(Structure):
     _fields_ = [("generations", c_int64), ("evaluations", c_int64), ("migrations", c_int64),
                 ("graph_launches", c_int64), ("rows_per_block", c_int),
@@ -129,6 +141,12 @@ _SIGS = {
                                    c_void_p, c_void_p]),
     "speo_mlp_adam_step": (c_int, [c_void_p, c_float]),
     "speo_spearman": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "speo_generations_for_fes": (c_int64, [c_int64, c_int64, c_int64]),
+    "speo_engine_run_fes": (c_int, [c_void_p, c_int64, POINTER(c_int)]),
+    "speo_run_seed": (c_uint64, [c_int, c_int, c_int]),
+    "speo_de_parameters": (c_int, [c_double, c_double, c_int, c_char_p, c_int]),
+    "speo_result_header": (c_char_p, []),
+    "speo_result_record": (c_int, [POINTER(RunRecord), c_char_p, c_int]),
 }
 
 _lib = None

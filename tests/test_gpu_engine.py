@@ -210,3 +210,28 @@ def test_engine_rejects_bad_configs(cuda, speo, oracle, case):
         return
     with pytest.raises(speo.SpeoError, match=match):
         _engine(speo, probs, sizes, pidx, **kw)
+
+
+def test_run_fes_budget_loop(cuda, speo, oracle):
+    # IslandEA::Execute (src/island_EA.cc:86-104): generations while the
+    # evaluation count (initial population included) is below the budget,
+    # migration at generation % interval == 0; then the record row
+    probs = _problems(speo, oracle, DIMS)
+    sizes, pidx = [16, 24], [0, 1]
+    e = _engine(speo, probs, sizes, pidx, interval=5, migration_size=2, use_graph=True)
+    g = e.run_fes(1000)
+    st = e.stats()
+    assert g == 24 == st["generations"]           # 40 rows: 40 + 24 * 40 = 1000
+    assert st["evaluations"] == 1000 and st["migrations"] == 5   # generations 0, 5, ..., 20
+    assert e.run_fes(1000) == 0 and e.stats()["generations"] == 24
+    assert e.run_fes(1001) == 1 and e.stats()["evaluations"] == 1040
+    _, _, f64 = e.best()
+    row = speo.result_record(function_ID=9, run_ID=1, dim=10, best_fitness=float(f64.min()),
+                             time_period=0.0, communication_percentage=0.0, total_FEs=1001,
+                             node_num=1, GPU_num=1, island_num=2, total_pop_size=40,
+                             subisland_size=-1, interval=5, migration_size=2, migration_rate=0.5,
+                             buffer_capacity=32, migration_topology="ring", buffer_manage="none",
+                             regroup_option="static_and_ordered", configure_EA="constant",
+                             EA_parameters=speo.de_parameters(0.9, 0.5, 4))
+    cols = row.split(",")
+    assert len(cols) == 21 and float(cols[3]) == pytest.approx(float(f64.min()), rel=1e-5)
