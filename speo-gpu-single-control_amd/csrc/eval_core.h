@@ -196,54 +196,64 @@ struct RowSource {
 };
 
 // Vector staging of one K chunk of gemm_chunk, for rows in global memory that
-// are 16-byte aligned with a row stride of a multiple of 4 floats: 16-byte
-// loads, every one of the chunk issued before the first is used (scalar
-// loads only for a group of four that straddles the end of K).  The values
-// are those of the scalar loops: A = (x - o) * rate (* sa) for r < nrows,
-// k < kc, else 0; B from the operand's zero-padded rows (B32 rows padded to a
-// multiple of 4 floats, M16 rows to a multiple of 8).
+// are 16-byte aligned with a row stride of a multiple of 4 floats: every
+// 16-byte load of the chunk is issued before the LDS operands are formed
+// (scalar loads only for a group of four that straddles the end of K).
+// FP32 (ChunkF32) splits this in two: load() fills registers, store() writes
+// LDS, and gemm_chunk stores chunk k and then loads chunk k + 1, whose loads
+// are in flight during chunk k's MFMA work.  The 16-bit form
+// (stage_chunk_16) loads and stores each chunk in turn: holding its P-fold B
+// operand across the MFMA loop measured slower than the overlap.
+// The values are those of the scalar loops: A = (x - o) * rate (* sa) for
+// r < nrows, k < kc, else 0; B from the operand's zero-padded rows (B32 rows
+// padded to a multiple of 4 floats, M16 rows to a multiple of 8).
 template <int RT>
-__device__ __forceinline__ void stage_chunk_f32(const CompDev &c, const RowSource &src, int k0, int kc, int n0,
-                                                int ncols, float *As, float *Bs, int tid) {
-    constexpr int LDA = kKC + 2, LDB = kNC + 16;
-    constexpr int GA = RT * kKC / 4 / 256, GB = kKC * kNC / 4 / 256;
+struct ChunkF32 {
+    static constexpr int LDA = kKC + 2, LDB = kNC + 16;
+    static constexpr int GA = RT * kKC / 4 / 256, GB = kKC * kNC / 4 / 256;
     floatx4 xa[GA], oa[GA], xb[GB];
+    __device__ __forceinline__ void load(const CompDev &c, const RowSource &src, int k0, int kc, int n0,
+                                         int ncols, int tid) {
 #pragma unroll
-    for (int i = 0; i < GA; ++i) {
-        const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
-        const bool full = r < src.nrows && k + 3 < kc;
-        xa[i] = gld(reinterpret_cast<const floatx4 *>(full ? src.rows + (size_t)r * src.ld + k0 + k : src.rows));
-        oa[i] = gld(reinterpret_cast<const floatx4 *>(full ? c.shift + k0 + k : c.shift));
-    }
+        for (int i = 0; i < GA; ++i) {
+            const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
+            const bool full = r < src.nrows && k + 3 < kc;
+            xa[i] = gld(reinterpret_cast<const floatx4 *>(full ? src.rows + (size_t)r * src.ld + k0 + k : src.rows));
+            oa[i] = gld(reinterpret_cast<const floatx4 *>(full ? c.shift + k0 + k : c.shift));
+        }
 #pragma unroll
-    for (int i = 0; i < GB; ++i) {
-        const int e = tid + i * 256, k = e / (kNC / 4), n = (e % (kNC / 4)) * 4;
-        const bool ok = k < kc && n < ncols;
-        xb[i] = gld(reinterpret_cast<const floatx4 *>(c.B32 + (ok ? (size_t)(k0 + k) * c.ldb32 + n0 + n : 0)));
-    }
-#pragma unroll
-    for (int i = 0; i < GA; ++i) {
-        const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
-        const bool full = r < src.nrows && k + 3 < kc;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float v;
-            if (full)
-                v = (xa[i][q] - oa[i][q]) * c.rate;
-            else
-                v = (r < src.nrows && k + q < kc)
-                        ? (src.rows[(size_t)r * src.ld + k0 + k + q] - c.shift[k0 + k + q]) * c.rate
-                        : 0.f;
-            As[r * LDA + k + q] = v;
+        for (int i = 0; i < GB; ++i) {
+            const int e = tid + i * 256, k = e / (kNC / 4), n = (e % (kNC / 4)) * 4;
+            const bool ok = k < kc && n < ncols;
+            xb[i] = gld(reinterpret_cast<const floatx4 *>(c.B32 + (ok ? (size_t)(k0 + k) * c.ldb32 + n0 + n : 0)));
         }
     }
+    __device__ __forceinline__ void store(const CompDev &c, const RowSource &src, int k0, int kc, int ncols,
+                                          float *As, float *Bs, int tid) const {
 #pragma unroll
-    for (int i = 0; i < GB; ++i) {
-        const int e = tid + i * 256, k = e / (kNC / 4), n = (e % (kNC / 4)) * 4;
-        const bool ok = k < kc && n < ncols;
-        *reinterpret_cast<floatx4 *>(Bs + k * LDB + n) = ok ? xb[i] : floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < GA; ++i) {
+            const int e = tid + i * 256, r = e / (kKC / 4), k = (e % (kKC / 4)) * 4;
+            const bool full = r < src.nrows && k + 3 < kc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float v;
+                if (full)
+                    v = (xa[i][q] - oa[i][q]) * c.rate;
+                else
+                    v = (r < src.nrows && k + q < kc)
+                            ? (src.rows[(size_t)r * src.ld + k0 + k + q] - c.shift[k0 + k + q]) * c.rate
+                            : 0.f;
+                As[r * LDA + k + q] = v;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < GB; ++i) {
+            const int e = tid + i * 256, k = e / (kNC / 4), n = (e % (kNC / 4)) * 4;
+            const bool ok = k < kc && n < ncols;
+            *reinterpret_cast<floatx4 *>(Bs + k * LDB + n) = ok ? xb[i] : floatx4{0.f, 0.f, 0.f, 0.f};
+        }
     }
-}
+};
 
 template <int RT, int P, bool H>
 __device__ __forceinline__ void stage_chunk_16(const CompDev &c, const RowSource &src, int k0, int kc, int n0,
@@ -313,7 +323,7 @@ __device__ __forceinline__ void stage_chunk_16(const CompDev &c, const RowSource
 // sum16, done by the caller).  All 256 threads of the block must call it.
 // src.rows is in global memory.
 // `lds` must hold gemm_lds_bytes<RT, PREC>() bytes.
-// VS: vector staging (stage_chunk_*) where the rows allow it.  Its registers
+// VS: vector staging (ChunkF32 / stage_chunk_16) where the rows allow it.  Its registers
 // count in every kernel that contains the call, so a kernel whose other paths
 // need them (the generation kernel's small-D phases) keeps the element-wise
 // form.
@@ -339,11 +349,14 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
         float *As = reinterpret_cast<float *>(lds);  // [RT][kKC+2]
         float *Bs = As + RT * (kKC + 2);             // [kKC][kNC+16]
         constexpr int LDA = kKC + 2, LDB = kNC + 16;
+        ChunkF32<RT> stg;
+        if (VS && vec) stg.load(c, src, 0, min(kKC, dim), n0, ncols, tid);
         for (int k0 = 0; k0 < dim; k0 += kKC) {
             const int kc = min(kKC, dim - k0);
             __syncthreads();
-            if (vec) {
-                stage_chunk_f32<RT>(c, src, k0, kc, n0, ncols, As, Bs, tid);
+            if (VS && vec) {
+                stg.store(c, src, k0, kc, ncols, As, Bs, tid);
+                if (k0 + kKC < dim) stg.load(c, src, k0 + kKC, min(kKC, dim - k0 - kKC), n0, ncols, tid);
             } else {
                 for (int e = tid; e < RT * kKC; e += 256) {
                     const int r = e / kKC, k = e % kKC;
@@ -383,7 +396,7 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
         for (int k0 = 0; k0 < dim; k0 += kKC) {
             const int kc = min(kKC, dim - k0);
             __syncthreads();
-            if (vec) {
+            if (VS && vec) {
                 stage_chunk_16<RT, P, H>(c, src, k0, kc, n0, ncols, As, Bs, tid);
             } else {
                 for (int e = tid; e < RT * kKC; e += 256) {
