@@ -20,6 +20,8 @@
  *       elliptic                            CUDA_EA_library/CEC2014.cc:481-497
  *       Ackley                              CUDA_EA_library/CEC2014.cc:590-612
  *       Rastrigin                           CUDA_EA_library/CEC2014.cc:668-683
+ *       bent cigar                          CUDA_EA_library/CEC2014.cc:499-516
+ *       discus                              CUDA_EA_library/CEC2014.cc:518-530
  *       composition weights (cf_cal)        CUDA_EA_library/CEC2014.cc:1481-1519
  *   - Philox4x32-10 (Salmon et al., "Parallel random numbers: as easy as
  *     1, 2, 3", SC'11), replacing cuRAND XORWOW, which is closed and is not
@@ -243,7 +245,7 @@ ORACLE_EXPORT void oracle_init_population(float *X, int ld, int NP, int D, float
 
 /* ------------------------------------------------------------- objectives */
 
-enum { OBJ_ELLIPTIC = 1, OBJ_RASTRIGIN = 2, OBJ_ACKLEY = 3 };
+enum { OBJ_ELLIPTIC = 1, OBJ_RASTRIGIN = 2, OBJ_ACKLEY = 3, OBJ_BENT_CIGAR = 4, OBJ_DISCUS = 5 };
 
 static const double ORACLE_PI = 3.1415926535897932384626433832795029;
 static const double ORACLE_E = 2.7182818284590452353602874713526625;
@@ -285,9 +287,25 @@ ORACLE_EXPORT double oracle_ackley(const double *z, int D) {
     return ORACLE_E - 20.0 * exp(s1) + 20.0 - exp(s2);
 }
 
+/* CEC2014.cc:507-515: z_0^2, then 10^6 z_i^2 for i >= 1, in index order. */
+ORACLE_EXPORT double oracle_bent_cigar(const double *z, int D) {
+    double f = z[0] * z[0];
+    for (int i = 1; i < D; ++i) f += pow(10.0, 6.0) * z[i] * z[i];
+    return f;
+}
+
+/* CEC2014.cc:521-529: 10^6 z_0^2, then z_i^2 for i >= 1. */
+ORACLE_EXPORT double oracle_discus(const double *z, int D) {
+    double f = pow(10.0, 6.0) * z[0] * z[0];
+    for (int i = 1; i < D; ++i) f += z[i] * z[i];
+    return f;
+}
+
 ORACLE_EXPORT double oracle_basic(int objective, const double *z, int D) {
     switch (objective) {
     case OBJ_ELLIPTIC: return oracle_elliptic(z, D);
+    case OBJ_BENT_CIGAR: return oracle_bent_cigar(z, D);
+    case OBJ_DISCUS: return oracle_discus(z, D);
     case OBJ_RASTRIGIN: return oracle_rastrigin(z, D);
     case OBJ_ACKLEY: return oracle_ackley(z, D);
     default: return NAN;

@@ -33,6 +33,10 @@ __device__ inline double eval_row_f64(const CompParams *comps, int ncomp, int di
             if (lane == 0) {
                 if (cp.objective == SPEO_OBJ_ELLIPTIC) {
                     s1 += pow(10.0, 6.0 * i / (dim - 1)) * z * z;
+                } else if (cp.objective == SPEO_OBJ_BENT_CIGAR) {
+                    s1 += (i == 0 ? 1.0 : 1.0e6) * z * z;
+                } else if (cp.objective == SPEO_OBJ_DISCUS) {
+                    s1 += (i == 0 ? 1.0e6 : 1.0) * z * z;
                 } else if (cp.objective == SPEO_OBJ_RASTRIGIN) {
                     s1 += (z * z - 10.0 * cos(2.0 * PI * z) + 10.0);
                 } else {

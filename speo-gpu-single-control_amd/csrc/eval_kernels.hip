@@ -250,7 +250,7 @@ int default_rows_per_block(int n, int dim) {
 
 namespace {
 struct DimTables {
-    float *colw = nullptr;
+    float *colw[3] = {nullptr, nullptr, nullptr};  // elliptic, bent cigar, discus weights
     float *identity = nullptr;
 };
 std::mutex g_tables_mu;
@@ -265,13 +265,14 @@ int dim_tables(int dim, DimTables &out) {
     }
     DimTables t;
     std::vector<float> w((size_t)dim), I((size_t)dim * dim, 0.f);
-    for (int j = 0; j < dim; ++j) {
-        w[(size_t)j] = (float)pow(10.0, 6.0 * j / (dim > 1 ? dim - 1 : 1));
-        I[(size_t)j * dim + j] = 1.f;
+    for (int j = 0; j < dim; ++j) I[(size_t)j * dim + j] = 1.f;
+    const int kinds[3] = {SPEO_OBJ_ELLIPTIC, SPEO_OBJ_BENT_CIGAR, SPEO_OBJ_DISCUS};
+    for (int k = 0; k < 3; ++k) {
+        column_weights(kinds[k], dim, w.data());
+        SPEO_TRY_HIP(hipMalloc(&t.colw[k], sizeof(float) * dim));
+        SPEO_TRY_HIP(hipMemcpy(t.colw[k], w.data(), sizeof(float) * dim, hipMemcpyHostToDevice));
     }
-    SPEO_TRY_HIP(hipMalloc(&t.colw, sizeof(float) * dim));
     SPEO_TRY_HIP(hipMalloc(&t.identity, sizeof(float) * dim * dim));
-    SPEO_TRY_HIP(hipMemcpy(t.colw, w.data(), sizeof(float) * dim, hipMemcpyHostToDevice));
     SPEO_TRY_HIP(hipMemcpy(t.identity, I.data(), sizeof(float) * dim * dim, hipMemcpyHostToDevice));
     g_tables[dim] = t;
     out = t;
@@ -293,8 +294,13 @@ extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real 
     }
     CompDev c{};
     bool rotate = true;
+    int weights = 0;  // DimTables::colw entry of the weighted-sphere objectives
     switch (function_ID) {
     case 1: c.objective = SPEO_OBJ_ELLIPTIC; c.rate = 1.0f; break;       // F1, kernel_CEC2014_CUDA.cu:1105
+    // F2 bent cigar and F3 discus (:1111, :1117; global_f2 / global_f3 at
+    // :287-352): the weighted sphere with their own column weights.
+    case 2: c.objective = SPEO_OBJ_ELLIPTIC; c.rate = 1.0f; weights = 1; break;
+    case 3: c.objective = SPEO_OBJ_ELLIPTIC; c.rate = 1.0f; weights = 2; break;
     case 5: c.objective = SPEO_OBJ_ACKLEY; c.rate = 1.0f; break;         // F5, :1130
     case 8: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; rotate = false; break;  // F8, :1150
     case 9: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; break;   // F9, :1156
@@ -315,7 +321,7 @@ extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real 
     c.shift = d_shift_data;
     c.B32 = rotate ? d_rotated_data : t.identity;
     c.ldb32 = problem_dim;  // reference layout [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255)
-    c.colw = t.colw;
+    c.colw = t.colw[weights];
     if (problem_dim > kNC) {
         defer_error(SPEO_ERR_UNSUPPORTED, "API_evaluateFitness: dim > 128 needs a workspace; "
                                           "use speo_evaluate");

@@ -12,6 +12,12 @@ namespace speo {
 
 constexpr int kMaxComp = 8;
 
+// Column weights w_j of the weighted-sphere objectives, sum_j w_j z_j^2:
+// elliptic 10^(6 j / (D - 1)), bent cigar (1, 10^6, 10^6, ...), discus
+// (10^6, 1, 1, ...).  The FP32 kernels evaluate all three through one path.
+bool weighted_sphere(int objective);
+void column_weights(int objective, int dim, float *w);
+
 struct CompHost {
     int objective = SPEO_OBJ_RASTRIGIN;
     float rate = 1.0f;
@@ -26,6 +32,7 @@ struct CompHost {
     float *B32s = nullptr;                                   // 128 x 128, fragment order
     uint16_t *Mbfs[3] = {nullptr, nullptr, nullptr};         // [128][round_up(dim,32)]
     uint16_t *Mhs[3] = {nullptr, nullptr, nullptr};
+    float *colw = nullptr;      // [dim] bent cigar / discus weights (elliptic: ProblemHost::colw)
 };
 
 // Per-component parameters of the FP64 path and of the composition combine,
@@ -42,7 +49,7 @@ struct CompParams {
 struct ProblemHost {
     int dim = 0, ncomp = 0;
     int ldb32 = 0, ldm16 = 0;
-    float *colw = nullptr;          // [dim]
+    float *colw = nullptr;          // [dim] elliptic weights
     CompParams *params_dev = nullptr;  // [ncomp]
     CompHost comp[kMaxComp];
     CompDev view(int c, int precision) const;

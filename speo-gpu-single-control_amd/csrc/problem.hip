@@ -40,11 +40,29 @@ static double f16_val(uint16_t b) {
     return (double)(float)h;
 }
 
+bool weighted_sphere(int objective) {
+    return objective == SPEO_OBJ_ELLIPTIC || objective == SPEO_OBJ_BENT_CIGAR ||
+           objective == SPEO_OBJ_DISCUS;
+}
+
+// CEC2014.cc:488-496 (elliptic), 507-515 (bent cigar), 521-529 (discus).
+void column_weights(int objective, int dim, float *w) {
+    for (int j = 0; j < dim; ++j) {
+        if (objective == SPEO_OBJ_BENT_CIGAR)
+            w[j] = j == 0 ? 1.0f : 1.0e6f;
+        else if (objective == SPEO_OBJ_DISCUS)
+            w[j] = j == 0 ? 1.0e6f : 1.0f;
+        else
+            w[j] = (float)std::pow(10.0, 6.0 * (double)j / (dim > 1 ? dim - 1 : 1));
+    }
+}
+
 CompDev ProblemHost::view(int c, int precision) const {
     const CompHost &h = comp[c];
     const bool half = precision == SPEO_PREC_FP16 || precision == SPEO_PREC_FP16X3;
     CompDev d{};
-    d.objective = h.objective;
+    // The kernels' elliptic path is the weighted sphere with the table in colw.
+    d.objective = weighted_sphere(h.objective) ? SPEO_OBJ_ELLIPTIC : h.objective;
     d.rate = h.rate;
     d.shift = h.shift32;
     d.B32 = h.B32;
@@ -52,7 +70,7 @@ CompDev ProblemHost::view(int c, int precision) const {
     for (int p = 0; p < 3; ++p) d.M16[p] = half ? h.Mh[p] : h.Mbf[p];
     d.ldm16 = ldm16;
     d.is_half = half ? 1 : 0;
-    d.colw = colw;
+    d.colw = h.colw ? h.colw : colw;
     d.B32s = h.B32s;
     for (int p = 0; p < 3; ++p) d.M16s[p] = half ? h.Mhs[p] : h.Mbfs[p];
     d.kp32 = round_up(dim, 32);
@@ -80,6 +98,7 @@ static void free_problem(ProblemHost &P) {
             (void)hipFree(h.Mbfs[p]);
             (void)hipFree(h.Mhs[p]);
         }
+        (void)hipFree(h.colw);
         h = CompHost{};
     }
     (void)hipFree(P.colw);
@@ -95,7 +114,7 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
         return set_error(SPEO_ERR_ARG, "speo_problem_create: need dim >= 2, 1 <= ncomp <= 8, "
                                        "objectives and shift");
     for (int c = 0; c < ncomp; ++c)
-        if (objectives[c] < SPEO_OBJ_ELLIPTIC || objectives[c] > SPEO_OBJ_ACKLEY)
+        if (objectives[c] < SPEO_OBJ_ELLIPTIC || objectives[c] > SPEO_OBJ_DISCUS)
             return set_error(SPEO_ERR_ARG, "speo_problem_create: unknown objective");
     P.dim = dim;
     P.ncomp = ncomp;
@@ -103,12 +122,16 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
     P.ldm16 = round_up(dim, 8);
     const size_t D = (size_t)dim;
     std::vector<float> w(D);
-    for (size_t j = 0; j < D; ++j) w[j] = (float)std::pow(10.0, 6.0 * (double)j / (dim - 1));
+    column_weights(SPEO_OBJ_ELLIPTIC, dim, w.data());
     SPEO_TRY(upload(&P.colw, w));
     std::vector<CompParams> params((size_t)ncomp);
     for (int c = 0; c < ncomp; ++c) {
         CompHost &h = P.comp[c];
         h.objective = objectives[c];
+        if (h.objective == SPEO_OBJ_BENT_CIGAR || h.objective == SPEO_OBJ_DISCUS) {
+            column_weights(h.objective, dim, w.data());
+            SPEO_TRY(upload(&h.colw, w));
+        }
         h.rate = rate ? (float)rate[c] : 1.0f;
         h.sigma = sigma ? sigma[c] : 10.0;
         h.bias = bias ? bias[c] : 0.0;

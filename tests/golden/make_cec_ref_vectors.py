@@ -3,8 +3,9 @@ code (CUDA_EA_library/CEC2014.cc, built from its source into oracle/_ref by
 oracle/Makefile).  Run in the build container, where the reference tree is
 mounted; the fixture is committed and the tests only read it.
 
-Cases (func_id -> objective; CEC2014.cc:154, 170, 186):
+Cases (func_id -> objective; CEC2014.cc:154-171, 186):
   F1 rotated elliptic, rate 1;  F5 rotated Ackley, rate 1;
+  F2 rotated bent cigar, rate 1;  F3 rotated discus, rate 1;
   F9 rotated Rastrigin, rate 5.12/100 -- evaluated at x' = o + (x - o)/0.0512
   so that the reference computes Rastrigin(M (x - o)) at rate 1.
 Two data sources:
@@ -27,7 +28,10 @@ sys.path.insert(0, str(REPO / "oracle"))
 import oracle as O  # noqa: E402
 
 REF_BIN = Path("/root/reference/SPEO-GPU-single-control/bin")
-FUNCS = {1: O.OBJ_ELLIPTIC, 5: O.OBJ_ACKLEY, 9: O.OBJ_RASTRIGIN}
+# F2 and F3 come last so that the random points of the earlier cases keep
+# their values.
+FUNCS = {1: O.OBJ_ELLIPTIC, 5: O.OBJ_ACKLEY, 9: O.OBJ_RASTRIGIN, 2: O.OBJ_BENT_CIGAR,
+         3: O.OBJ_DISCUS}
 NPTS = 12
 
 

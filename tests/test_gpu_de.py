@@ -221,7 +221,8 @@ def test_dropin_generation_cycle(cuda, speo, oracle):
     np.testing.assert_array_equal(fpop.cpu().numpy(), f2)
 
 
-@pytest.mark.parametrize("fid,obj,rate,rotate", [(1, 1, 1.0, True), (5, 3, 1.0, True),
+@pytest.mark.parametrize("fid,obj,rate,rotate", [(1, 1, 1.0, True), (2, 4, 1.0, True),
+                                                 (3, 5, 1.0, True), (5, 3, 1.0, True),
                                                  (8, 2, 0.0512, False), (9, 2, 0.0512, True)])
 def test_dropin_evaluate_functions(cuda, speo, oracle, fid, obj, rate, rotate):
     import torch

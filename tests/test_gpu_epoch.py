@@ -54,7 +54,7 @@ def test_epoch_equals_plain_small(cuda, speo, oracle):
         _assert_same(a, b)
 
 
-@pytest.mark.parametrize("obj", [1, 2, 3])
+@pytest.mark.parametrize("obj", [1, 2, 3, 4, 5])
 def test_epoch_equals_plain_d100(cuda, speo, oracle, obj):
     # NP=128 at D=100 keeps both slots in LDS; NP=200 and 333 do not (HBM slots,
     # trials staged in passes of rows); NP=17 leaves a ragged last row tile.

@@ -2,16 +2,24 @@
 FP64 oracle (CEC2014.cc forms, pinned to the reference's compiled CEC2014.cc).
 
 Tolerances: FP32 (and the FP32-recovering splits BF16x6 / FP16x3): 1e-5
-relative on random points of [-100, 100]^D.  BF16x3 (two-part BF16 split):
+relative on random points of [-100, 100]^D.  For the weighted-sphere
+objectives (elliptic, bent cigar, discus) a row whose 1e6-weighted
+coordinates cancel in the rotation is ill-conditioned for any FP32 GEMM and
+is held to 4 u kappa instead (tolerance.py: kappa is the row's condition
+number, u the precision's unit roundoff).  BF16x3 (two-part BF16 split):
 1e-4.  Single BF16 / FP16: reported, bounded loosely (2e-2 / 5e-3).
 FP64 device path: 1e-12 relative.
 """
 import numpy as np
 import pytest
 
+from tolerance import assert_rows, composition_rtol, row_rtol
+
 pytestmark = pytest.mark.gpu
 
 TOL = {0: 1e-5, 1: 2e-2, 2: 5e-3, 3: 1e-4, 4: 1e-5, 5: 1e-5}
+# unit roundoff of each precision's rotation (operand split residual)
+U_PREC = {0: 2.0 ** -24, 1: 2.0 ** -8, 2: 2.0 ** -11, 3: 2.0 ** -16, 4: 2.0 ** -22, 5: 2.0 ** -24}
 
 
 def _t(a, dev):
@@ -26,7 +34,7 @@ def _points(rng, n, D, ld):
 
 
 @pytest.mark.parametrize("D", [2, 10, 30, 50, 100, 128, 129, 200, 1000])
-@pytest.mark.parametrize("obj", [1, 2, 3])
+@pytest.mark.parametrize("obj", [1, 2, 3, 4, 5])
 def test_fp32_against_fp64_oracle(cuda, speo, oracle, D, obj):
     import torch
     rng = np.random.default_rng(D * 10 + obj)
@@ -40,7 +48,7 @@ def test_fp32_against_fp64_oracle(cuda, speo, oracle, D, obj):
     speo.evaluate(prob, _t(X, cuda), f)
     torch.cuda.synchronize()
     ref = oracle.evaluate(obj, X[:, :D], o, M)
-    np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-5)
+    assert_rows(f.cpu().numpy(), ref, row_rtol(obj, X[:, :D], o, M))
 
 
 def test_fp32_near_optimum(cuda, speo, oracle):
@@ -50,7 +58,7 @@ def test_fp32_near_optimum(cuda, speo, oracle):
     import torch
     D = 100
     rng = np.random.default_rng(1)
-    for obj in (1, 2, 3):
+    for obj in (1, 2, 3, 4, 5):
         o, M = oracle.gen_problem(17, D)
         prob = speo.Problem([obj], o[None, :], M[None, :, :])
         X = (o[None, :] + rng.normal(0, 1e-2, (64, D))).astype(np.float32)
@@ -58,7 +66,7 @@ def test_fp32_near_optimum(cuda, speo, oracle):
         speo.evaluate(prob, _t(X, cuda), f)
         torch.cuda.synchronize()
         ref = oracle.evaluate(obj, X, o, M)
-        np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=2e-5)
+        assert_rows(f.cpu().numpy(), ref, row_rtol(obj, X, o, M, base=2e-5))
 
 
 @pytest.mark.parametrize("D", [100, 1000])
@@ -68,7 +76,7 @@ def test_precision_modes(cuda, speo, oracle, D):
     n = 256 if D == 100 else 64
     X = _points(rng, n, D, D)
     report = {}
-    for obj in (1, 2, 3):
+    for obj in (1, 2, 3, 4, 5):
         o, M = oracle.gen_problem(1000 + obj, D)
         prob = speo.Problem([obj], o[None, :], M[None, :, :])
         ref = oracle.evaluate(obj, X[:, :D], o, M)
@@ -76,9 +84,13 @@ def test_precision_modes(cuda, speo, oracle, D):
             f = torch.zeros(n, dtype=torch.float32, device=cuda)
             speo.evaluate(prob, _t(X, cuda), f, precision=prec)
             torch.cuda.synchronize()
-            rel = float(np.max(np.abs(f.cpu().numpy() - ref) / np.abs(ref)))
+            got = f.cpu().numpy()
+            rel = float(np.max(np.abs(got - ref) / np.abs(ref)))
             report[(obj, speo.PREC_NAMES[prec])] = rel
-            assert rel <= TOL[prec], (obj, speo.PREC_NAMES[prec], rel)
+            if obj in (4, 5):
+                assert_rows(got, ref, row_rtol(obj, X[:, :D], o, M, TOL[prec], U_PREC[prec]))
+            else:
+                assert rel <= TOL[prec], (obj, speo.PREC_NAMES[prec], rel)
     print("\nmax relative error vs FP64, D=%d:" % D)
     for k, v in sorted(report.items()):
         print("  obj=%d %-7s %.3e" % (k[0], k[1], v))
@@ -90,7 +102,7 @@ def test_fp64_device_path(cuda, speo, oracle, D):
     rng = np.random.default_rng(3)
     n = 32
     X = _points(rng, n, D, D)
-    for obj in (1, 2, 3):
+    for obj in (1, 2, 3, 4, 5):
         o, M = oracle.gen_problem(5 + obj, D)
         prob = speo.Problem([obj], o[None, :], M[None, :, :])
         f = torch.zeros(n, dtype=torch.float64, device=cuda)
@@ -100,11 +112,11 @@ def test_fp64_device_path(cuda, speo, oracle, D):
                                    rtol=1e-12)
 
 
+@pytest.mark.parametrize("objs", [[1, 2, 3], [4, 1, 5]], ids=["ell-ras-ack", "bent-ell-dis"])
 @pytest.mark.parametrize("D", [10, 100, 1000])
-def test_composition(cuda, speo, oracle, D):
+def test_composition(cuda, speo, oracle, D, objs):
     import torch
     rng = np.random.default_rng(D)
-    objs = [1, 2, 3]
     os_ = np.stack([oracle.gen_shift(s, D) for s in (21, 22, 23)])
     Ms = np.stack([oracle.gen_rotation(s, D) for s in (21, 22, 23)])
     sigma, bias = np.array([10.0, 20.0, 30.0]), np.array([0.0, 100.0, 200.0])
@@ -114,11 +126,14 @@ def test_composition(cuda, speo, oracle, D):
     X[0, :D] = os_[1]                         # d = 0 for component 2 -> f = bias_2
     X[1, :D] = os_[0] + 1e-3                  # close to component 1
     ref = oracle.composition(X[:, :D], objs, os_, Ms, sigma, bias)
+    rtol = composition_rtol(objs, X[:, :D], os_, Ms)
     for prec in (0, 5):
         f = torch.zeros(n, dtype=torch.float32, device=cuda)
         speo.evaluate(prob, _t(X, cuda), f, precision=prec)
         torch.cuda.synchronize()
-        np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-5, atol=1e-4)
+        got = f.cpu().numpy()
+        assert np.all(np.abs(got - ref) <= rtol * np.abs(ref) + 1e-4), (
+            np.max(np.abs(got - ref) / np.abs(ref)))
     f64 = torch.zeros(n, dtype=torch.float64, device=cuda)
     speo.evaluate_f64(prob, _t(X, cuda), f64)
     torch.cuda.synchronize()

@@ -1,4 +1,4 @@
-"""Pins the oracle's FP64 objectives (CEC2014.cc:481-497, 590-612, 668-683,
+"""Pins the oracle's FP64 objectives (CEC2014.cc:481-530, 590-612, 668-683,
 1373-1393, 1481-1519) against the reference's own CEC2014.cc (golden vectors
 produced by tests/golden/make_cec_ref_vectors.py from oracle/_ref), closed
 forms, and the properties a transposed operand or a wrong index would break.
@@ -21,7 +21,8 @@ def _cases():
 def test_against_reference_cec2014(oracle, case):
     d = np.load(GOLDEN)
     fid = int(case.split("_F")[1].split("_")[0])
-    obj = {1: oracle.OBJ_ELLIPTIC, 5: oracle.OBJ_ACKLEY, 9: oracle.OBJ_RASTRIGIN}[fid]
+    obj = {1: oracle.OBJ_ELLIPTIC, 2: oracle.OBJ_BENT_CIGAR, 3: oracle.OBJ_DISCUS,
+           5: oracle.OBJ_ACKLEY, 9: oracle.OBJ_RASTRIGIN}[fid]
     x = d[case + "_x"]
     if case.startswith("haar"):
         o, M = oracle.gen_problem(int(d[case + "_seed"]), x.shape[1])
@@ -37,13 +38,14 @@ def test_live_reference_if_available(oracle, tmp_path):
         pytest.skip("oracle/_ref not built (reference tree absent)")
     o, M = oracle.gen_problem(555, 20)
     x = np.random.default_rng(0).uniform(-100, 100, (5, 20))
-    for fid, obj in ((1, oracle.OBJ_ELLIPTIC), (5, oracle.OBJ_ACKLEY)):
+    for fid, obj in ((1, oracle.OBJ_ELLIPTIC), (2, oracle.OBJ_BENT_CIGAR), (3, oracle.OBJ_DISCUS),
+                     (5, oracle.OBJ_ACKLEY)):
         oracle.write_cec_files(tmp_path, fid, o, M)
         fr = oracle.ref_eval(tmp_path, fid, x)
         np.testing.assert_allclose(oracle.evaluate(obj, x, o, M), fr, rtol=1e-12)
 
 
-@pytest.mark.parametrize("obj", [1, 2, 3])
+@pytest.mark.parametrize("obj", [1, 2, 3, 4, 5])
 def test_zero_at_shift(oracle, obj):
     o, M = oracle.gen_problem(3, 30)
     f = oracle.evaluate(obj, o[None, :], o, M)[0]
@@ -61,6 +63,24 @@ def test_elliptic_unit_vectors(oracle):
         x = M[k]                      # M x = e_k for orthogonal M
         f = oracle.evaluate(oracle.OBJ_ELLIPTIC, x[None, :], o, M)[0]
         assert f == pytest.approx(10.0 ** (6.0 * k / (D - 1)), rel=1e-12)
+
+
+@pytest.mark.parametrize("obj,w0,wk", [(4, 1.0, 1.0e6), (5, 1.0e6, 1.0)],
+                         ids=["bent_cigar", "discus"])
+def test_weighted_sphere_unit_vectors(oracle, obj, w0, wk):
+    # z = t e_0 gives w0 t^2 and z = t e_k (k >= 1) gives wk t^2: bent cigar
+    # weights the first coordinate 1 and the rest 1e6, discus the reverse
+    # (CEC2014.cc:507-515, 521-529), which a swapped weight or an off-by-one
+    # first index fails.
+    D = 6
+    rng = np.random.default_rng(2)
+    M = np.linalg.qr(rng.normal(size=(D, D)))[0]
+    o = rng.uniform(-5, 5, D)
+    for k in range(D):
+        t = 0.75 + k
+        x = o + t * M[k]                       # M (x - o) = t e_k
+        f = oracle.evaluate(obj, x[None, :], o, M)[0]
+        assert f == pytest.approx((w0 if k == 0 else wk) * t * t, rel=1e-12)
 
 
 def test_rastrigin_integer_lattice(oracle):
