@@ -134,7 +134,8 @@ SPEO_EXPORT int API_DE_SelectSurvival(GPU_Population d_population, GPU_Populatio
 /* Replaces API_evaluateFitness (CUDA_EA_library/CEC2014_CUDA.h:35-37,
  * kernel_CEC2014_CUDA.cu:1672-1797) for function_ID 1 (rotated elliptic),
  * 2 (rotated bent cigar), 3 (rotated discus), 5 (rotated Ackley), 8 (shifted
- * Rastrigin, rate 0.0512) and 9 (rotated Rastrigin, rate 0.0512), in the
+ * Rastrigin, rate 0.0512), 9 (rotated Rastrigin, rate 0.0512), 13 (rotated
+ * HappyCat, rate 0.05) and 14 (rotated HGBat, rate 0.05), in the
  * reference's layouts: d_original
  * [pop_size][next_pow2(dim)], d_shift [next_pow2(dim)], d_rotated_data =
  * M transposed and row-padded, [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255).
@@ -164,13 +165,19 @@ SPEO_EXPORT int speo_device_check(void); /* SPEO_OK if a gfx950 device is curren
  *   ACKLEY      -20 e^(-0.2 sqrt(|z|^2 / D)) - e^(mean cos(2 pi z_j)) + 20 + e
  *                                                       CEC2014.cc:590-612
  *   BENT_CIGAR  z_0^2 + 10^6 sum_{j >= 1} z_j^2         CEC2014.cc:499-516
- *   DISCUS      10^6 z_0^2 + sum_{j >= 1} z_j^2         CEC2014.cc:518-530 */
+ *   DISCUS      10^6 z_0^2 + sum_{j >= 1} z_j^2         CEC2014.cc:518-530
+ *   HAPPYCAT    |r2 - D|^(1/4) + (r2 / 2 + s) / D + 1/2   CEC2014.cc:907-929
+ *   HGBAT       |r2^2 - s^2|^(1/2) + (r2 / 2 + s) / D + 1/2   CEC2014.cc:931-953
+ *               with u_j = z_j - 1, r2 = sum_j u_j^2, s = sum_j u_j.
+ * SPEO_MODE_EPOCH takes objectives 1-5. */
 enum {
     SPEO_OBJ_ELLIPTIC = 1,
     SPEO_OBJ_RASTRIGIN = 2,
     SPEO_OBJ_ACKLEY = 3,
     SPEO_OBJ_BENT_CIGAR = 4,
-    SPEO_OBJ_DISCUS = 5
+    SPEO_OBJ_DISCUS = 5,
+    SPEO_OBJ_HAPPYCAT = 6,
+    SPEO_OBJ_HGBAT = 7
 };
 
 /* Precision of the rotation GEMM (accumulation is always FP32). */

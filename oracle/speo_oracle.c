@@ -22,6 +22,8 @@
  *       Rastrigin                           CUDA_EA_library/CEC2014.cc:668-683
  *       bent cigar                          CUDA_EA_library/CEC2014.cc:499-516
  *       discus                              CUDA_EA_library/CEC2014.cc:518-530
+ *       HappyCat                            CUDA_EA_library/CEC2014.cc:907-929
+ *       HGBat                               CUDA_EA_library/CEC2014.cc:931-953
  *       composition weights (cf_cal)        CUDA_EA_library/CEC2014.cc:1481-1519
  *   - Philox4x32-10 (Salmon et al., "Parallel random numbers: as easy as
  *     1, 2, 3", SC'11), replacing cuRAND XORWOW, which is closed and is not
@@ -245,7 +247,15 @@ ORACLE_EXPORT void oracle_init_population(float *X, int ld, int NP, int D, float
 
 /* ------------------------------------------------------------- objectives */
 
-enum { OBJ_ELLIPTIC = 1, OBJ_RASTRIGIN = 2, OBJ_ACKLEY = 3, OBJ_BENT_CIGAR = 4, OBJ_DISCUS = 5 };
+enum {
+    OBJ_ELLIPTIC = 1,
+    OBJ_RASTRIGIN = 2,
+    OBJ_ACKLEY = 3,
+    OBJ_BENT_CIGAR = 4,
+    OBJ_DISCUS = 5,
+    OBJ_HAPPYCAT = 6,
+    OBJ_HGBAT = 7
+};
 
 static const double ORACLE_PI = 3.1415926535897932384626433832795029;
 static const double ORACLE_E = 2.7182818284590452353602874713526625;
@@ -301,11 +311,37 @@ ORACLE_EXPORT double oracle_discus(const double *z, int D) {
     return f;
 }
 
+/* CEC2014.cc:919-928: u_i = z_i - 1, r2 = sum u_i^2, s = sum u_i,
+ * |r2 - D|^(2 alpha) + (r2 / 2 + s) / D + 1/2 with alpha = 1/8. */
+ORACLE_EXPORT double oracle_happycat(const double *z, int D) {
+    double r2 = 0.0, s = 0.0;
+    for (int i = 0; i < D; ++i) {
+        const double u = z[i] - 1.0;
+        r2 += u * u;
+        s += u;
+    }
+    return pow(fabs(r2 - D), 2.0 * (1.0 / 8.0)) + (0.5 * r2 + s) / D + 0.5;
+}
+
+/* CEC2014.cc:943-952: the same sums, |r2^2 - s^2|^(2 alpha) + ... with
+ * alpha = 1/4. */
+ORACLE_EXPORT double oracle_hgbat(const double *z, int D) {
+    double r2 = 0.0, s = 0.0;
+    for (int i = 0; i < D; ++i) {
+        const double u = z[i] - 1.0;
+        r2 += u * u;
+        s += u;
+    }
+    return pow(fabs(pow(r2, 2.0) - pow(s, 2.0)), 2.0 * (1.0 / 4.0)) + (0.5 * r2 + s) / D + 0.5;
+}
+
 ORACLE_EXPORT double oracle_basic(int objective, const double *z, int D) {
     switch (objective) {
     case OBJ_ELLIPTIC: return oracle_elliptic(z, D);
     case OBJ_BENT_CIGAR: return oracle_bent_cigar(z, D);
     case OBJ_DISCUS: return oracle_discus(z, D);
+    case OBJ_HAPPYCAT: return oracle_happycat(z, D);
+    case OBJ_HGBAT: return oracle_hgbat(z, D);
     case OBJ_RASTRIGIN: return oracle_rastrigin(z, D);
     case OBJ_ACKLEY: return oracle_ackley(z, D);
     default: return NAN;

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 
                 float s1[RT / 16][4], s2[RT / 16][4];
                 gemm_small<RT, PREC>(comp, trial, dim, gemm_lds, s1, s2);
                 reduce_rows_small<RT>(s1, s2, red, o1, o2);
-                if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);
+                if (tid < t.nrows) gval[c * RT + tid] = finish_objective_ext(comp.objective, o1[tid], o2[tid], dim);
             } else {
                 float *red = reinterpret_cast<float *>(gemm_lds);
                 float *o1 = red + 8 * RT, *o2 = o1 + RT;
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 
                 for (int n0 = 0; n0 < dim; n0 += kNC)
                     gemm_chunk<RT, PREC, false>(comp, src, dim, n0, gemm_lds, s1, s2);
                 reduce_rows<RT>(s1, s2, red, o1, o2);
-                if (tid < t.nrows) gval[c * RT + tid] = finish_objective(comp.objective, o1[tid], o2[tid], dim);
+                if (tid < t.nrows) gval[c * RT + tid] = finish_objective_ext(comp.objective, o1[tid], o2[tid], dim);
             }
         }
         if (tid < t.nrows) {
@@ -666,6 +666,8 @@ int Engine::epoch_setup() {
         if (P.ncomp != 1 || P.dim > kSmallD)
             return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode needs single-component objectives "
                                                    "with dim <= 128");
+        if (P.view(0, cfg.precision).objective > SPEO_OBJ_ACKLEY)
+            return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: objectives 1-5 only");
         if (isl[(size_t)i].np > 8192)
             return set_error(SPEO_ERR_UNSUPPORTED, "epoch mode: islands of at most 8192 rows");
         // the epoch kernel reads an island's rows as one block of np * 4 ceil(dim / 4) floats

@@ -161,10 +161,15 @@ __host__ __device__ __forceinline__ float sin2pi(float z) {
 //   elliptic:  s1 += w_j z^2
 //   Rastrigin: s1 += z^2 + 20 sin^2(pi z)   (== z^2 - 10 cos(2 pi z) + 10)
 //   Ackley:    s1 += z^2, s2 += sin^2(pi z) (cos(2 pi z) - 1 == -2 sin^2(pi z))
+//   HappyCat, HGBat: s1 += (z - 1)^2, s2 += z - 1
 __device__ __forceinline__ void add_terms(int objective, float z, float w, float &s1,
                                           float &s2) {
     if (objective == SPEO_OBJ_ELLIPTIC) {
         s1 = fmaf(w * z, z, s1);
+    } else if (objective == SPEO_OBJ_HAPPYCAT || objective == SPEO_OBJ_HGBAT) {
+        const float u = z - 1.0f;
+        s1 = fmaf(u, u, s1);
+        s2 += u;
     } else {
         const float s2z = sin2pi(z);
         if (objective == SPEO_OBJ_RASTRIGIN) {
@@ -185,6 +190,19 @@ __device__ __forceinline__ float finish_objective(int objective, float s1, float
         return -20.0f * expm1f(-0.2f * r) - 2.71828182845904523536f * expm1f(-2.0f * s2 / (float)dim);
     }
     return s1;
+}
+
+// finish_objective plus HappyCat and HGBat (CEC2014.cc:907-953; the GPU forms
+// global_f11 / global_f12, kernel_CEC2014_CUDA.cu:702-808), which the
+// evaluation and fused-generation kernels take; r2^2 - s^2 is formed as
+// (r2 - s)(r2 + s).
+__device__ __forceinline__ float finish_objective_ext(int objective, float s1, float s2, int dim) {
+    const float D = (float)dim;
+    if (objective == SPEO_OBJ_HAPPYCAT)
+        return sqrtf(sqrtf(fabsf(s1 - D))) + (0.5f * s1 + s2) / D + 0.5f;
+    if (objective == SPEO_OBJ_HGBAT)
+        return sqrtf(fabsf((s1 - s2) * (s1 + s2))) + (0.5f * s1 + s2) / D + 0.5f;
+    return finish_objective(objective, s1, s2, dim);
 }
 
 // Rows of the A tile: row r of the tile is `rows + r * ld` (global or LDS),

@@ -1,7 +1,8 @@
 // eval_f64.h -- FP64 objective of one row by one 256-thread block
 // (re-verification path), following the reference's FP64 CPU forms:
-// shift/rotate CEC2014.cc:1373-1393, elliptic 481-497, Ackley 590-612,
-// Rastrigin 668-683, composition weights cf_cal 1481-1519.
+// shift/rotate CEC2014.cc:1373-1393, elliptic 481-497, bent cigar 499-516,
+// discus 518-530, Ackley 590-612, Rastrigin 668-683, HappyCat 907-929, HGBat
+// 931-953, composition weights cf_cal 1481-1519.
 #pragma once
 
 #include "problem.h"
@@ -37,6 +38,9 @@ __device__ inline double eval_row_f64(const CompParams *comps, int ncomp, int di
                     s1 += (i == 0 ? 1.0 : 1.0e6) * z * z;
                 } else if (cp.objective == SPEO_OBJ_DISCUS) {
                     s1 += (i == 0 ? 1.0e6 : 1.0) * z * z;
+                } else if (cp.objective == SPEO_OBJ_HAPPYCAT || cp.objective == SPEO_OBJ_HGBAT) {
+                    s1 += (z - 1.0) * (z - 1.0);
+                    s2 += z - 1.0;
                 } else if (cp.objective == SPEO_OBJ_RASTRIGIN) {
                     s1 += (z * z - 10.0 * cos(2.0 * PI * z) + 10.0);
                 } else {
@@ -67,6 +71,10 @@ __device__ inline double eval_row_f64(const CompParams *comps, int ncomp, int di
         if (cp.objective == SPEO_OBJ_ACKLEY) {
             const double r = -0.2 * sqrt(S1 / (double)dim);
             g = E - 20.0 * exp(r) + 20.0 - exp(S2 / (double)dim);
+        } else if (cp.objective == SPEO_OBJ_HAPPYCAT) {
+            g = pow(fabs(S1 - dim), 0.25) + (0.5 * S1 + S2) / dim + 0.5;
+        } else if (cp.objective == SPEO_OBJ_HGBAT) {
+            g = pow(fabs(S1 * S1 - S2 * S2), 0.5) + (0.5 * S1 + S2) / dim + 0.5;
         } else {
             g = S1;
         }

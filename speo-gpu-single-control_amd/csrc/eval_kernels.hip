@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch p) {
     if (t < RT && row0 + t < p.n) {
         const int row = row0 + t;
         if (p.nchunks == 1) {
-            const float g = finish_objective(p.comp.objective, o1[t], o2[t], p.dim);
+            const float g = finish_objective_ext(p.comp.objective, o1[t], o2[t], p.dim);
             p.f_out[row] = p.raw ? g : fmaf(p.lambda, g, p.bias);
         } else {
             p.part[((size_t)chunk * 2 + 0) * p.n + row] = o1[t];
@@ -97,7 +97,7 @@ __global__ void k_eval_finalize(const float *part, int nchunks, int n, int dim, 
         a += part[((size_t)c * 2 + 0) * n + row];
         b += part[((size_t)c * 2 + 1) * n + row];
     }
-    const float g = finish_objective(objective, a, b, dim);
+    const float g = finish_objective_ext(objective, a, b, dim);
     f_out[row] = raw ? g : fmaf(lambda, g, bias);
 }
 
@@ -304,6 +304,8 @@ extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real 
     case 5: c.objective = SPEO_OBJ_ACKLEY; c.rate = 1.0f; break;         // F5, :1130
     case 8: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; rotate = false; break;  // F8, :1150
     case 9: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; break;   // F9, :1156
+    case 13: c.objective = SPEO_OBJ_HAPPYCAT; c.rate = 0.05f; break;     // F13, :1180
+    case 14: c.objective = SPEO_OBJ_HGBAT; c.rate = 0.05f; break;        // F14, :1186
     default:
         defer_error(SPEO_ERR_UNSUPPORTED, "API_evaluateFitness: function_ID " +
                                               std::to_string(function_ID) + " is not built");

@@ -6,6 +6,8 @@ mounted; the fixture is committed and the tests only read it.
 Cases (func_id -> objective; CEC2014.cc:154-171, 186):
   F1 rotated elliptic, rate 1;  F5 rotated Ackley, rate 1;
   F2 rotated bent cigar, rate 1;  F3 rotated discus, rate 1;
+  F13 rotated HappyCat and F14 rotated HGBat, rate 5/100 -- evaluated at a
+  rate-adjusted input like F9;
   F9 rotated Rastrigin, rate 5.12/100 -- evaluated at x' = o + (x - o)/0.0512
   so that the reference computes Rastrigin(M (x - o)) at rate 1.
 Two data sources:
@@ -28,10 +30,10 @@ sys.path.insert(0, str(REPO / "oracle"))
 import oracle as O  # noqa: E402
 
 REF_BIN = Path("/root/reference/SPEO-GPU-single-control/bin")
-# F2 and F3 come last so that the random points of the earlier cases keep
-# their values.
+# Functions added later come last so that the random points of the earlier
+# cases keep their values.
 FUNCS = {1: O.OBJ_ELLIPTIC, 5: O.OBJ_ACKLEY, 9: O.OBJ_RASTRIGIN, 2: O.OBJ_BENT_CIGAR,
-         3: O.OBJ_DISCUS}
+         3: O.OBJ_DISCUS, 13: O.OBJ_HAPPYCAT, 14: O.OBJ_HGBAT}
 NPTS = 12
 
 

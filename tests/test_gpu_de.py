@@ -223,7 +223,8 @@ def test_dropin_generation_cycle(cuda, speo, oracle):
 
 @pytest.mark.parametrize("fid,obj,rate,rotate", [(1, 1, 1.0, True), (2, 4, 1.0, True),
                                                  (3, 5, 1.0, True), (5, 3, 1.0, True),
-                                                 (8, 2, 0.0512, False), (9, 2, 0.0512, True)])
+                                                 (8, 2, 0.0512, False), (9, 2, 0.0512, True),
+                                                 (13, 6, 0.05, True), (14, 7, 0.05, True)])
 def test_dropin_evaluate_functions(cuda, speo, oracle, fid, obj, rate, rotate):
     import torch
     L = speo.lib()

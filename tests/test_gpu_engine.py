@@ -137,6 +137,35 @@ def test_weighted_sphere_objectives_all_modes(cuda, speo, oracle):
     np.testing.assert_array_equal(a.read()[0], b.read()[0])
 
 
+def test_happycat_hgbat_plain_and_fused(cuda, speo, oracle):
+    # HappyCat (6) and HGBat (7): first-generation trials against the FP64
+    # oracle, plain and fused bit-identical over migrating runs (D = 40 and
+    # the large-D path at D = 160); epoch mode declines them.
+    dims_objs = [(40, 6), (40, 7), (160, 6), (160, 7)]
+    probs = _problems(speo, oracle, dims_objs)
+    sizes, pidx = [32, 40, 48, 24], [0, 1, 2, 3]
+    first = _engine(speo, probs, sizes, pidx, mode=speo.MODE_PLAIN)
+    first.run(1)
+    U, fU = first.read_trials()
+    for i, p in enumerate(pidx):
+        D, obj = dims_objs[p]
+        rows = slice(first.row_off[i], first.row_off[i + 1])
+        ref = oracle.evaluate(obj, U[rows][:, :D], probs[p].shifts[0], probs[p].rotations[0])
+        np.testing.assert_allclose(fU[rows], ref, rtol=1e-5)
+    kw = dict(interval=3, migration_size=2)
+    a = _engine(speo, probs, sizes, pidx, mode=speo.MODE_PLAIN, **kw)
+    b = _engine(speo, probs, sizes, pidx, mode=speo.MODE_FUSED, **kw)
+    for chunk in (1, 4, 9):
+        a.run(chunk)
+        b.run(chunk)
+    Xa, fa = a.read()
+    Xb, fb = b.read()
+    np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(Xa, Xb)
+    with pytest.raises(speo.SpeoError, match="objectives 1-5"):
+        _engine(speo, probs[:1], [32], [0], mode=speo.MODE_EPOCH)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_migration_matches_oracle(cuda, speo, oracle, mode):
     probs = _problems(speo, oracle, DIMS)

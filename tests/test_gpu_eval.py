@@ -34,7 +34,7 @@ def _points(rng, n, D, ld):
 
 
 @pytest.mark.parametrize("D", [2, 10, 30, 50, 100, 128, 129, 200, 1000])
-@pytest.mark.parametrize("obj", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("obj", [1, 2, 3, 4, 5, 6, 7])
 def test_fp32_against_fp64_oracle(cuda, speo, oracle, D, obj):
     import torch
     rng = np.random.default_rng(D * 10 + obj)
@@ -76,7 +76,7 @@ def test_precision_modes(cuda, speo, oracle, D):
     n = 256 if D == 100 else 64
     X = _points(rng, n, D, D)
     report = {}
-    for obj in (1, 2, 3, 4, 5):
+    for obj in (1, 2, 3, 4, 5, 6, 7):
         o, M = oracle.gen_problem(1000 + obj, D)
         prob = speo.Problem([obj], o[None, :], M[None, :, :])
         ref = oracle.evaluate(obj, X[:, :D], o, M)
@@ -102,7 +102,7 @@ def test_fp64_device_path(cuda, speo, oracle, D):
     rng = np.random.default_rng(3)
     n = 32
     X = _points(rng, n, D, D)
-    for obj in (1, 2, 3, 4, 5):
+    for obj in (1, 2, 3, 4, 5, 6, 7):
         o, M = oracle.gen_problem(5 + obj, D)
         prob = speo.Problem([obj], o[None, :], M[None, :, :])
         f = torch.zeros(n, dtype=torch.float64, device=cuda)

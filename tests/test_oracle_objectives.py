@@ -1,5 +1,5 @@
 """Pins the oracle's FP64 objectives (CEC2014.cc:481-530, 590-612, 668-683,
-1373-1393, 1481-1519) against the reference's own CEC2014.cc (golden vectors
+907-953, 1373-1393, 1481-1519) against the reference's own CEC2014.cc (golden vectors
 produced by tests/golden/make_cec_ref_vectors.py from oracle/_ref), closed
 forms, and the properties a transposed operand or a wrong index would break.
 """
@@ -22,7 +22,8 @@ def test_against_reference_cec2014(oracle, case):
     d = np.load(GOLDEN)
     fid = int(case.split("_F")[1].split("_")[0])
     obj = {1: oracle.OBJ_ELLIPTIC, 2: oracle.OBJ_BENT_CIGAR, 3: oracle.OBJ_DISCUS,
-           5: oracle.OBJ_ACKLEY, 9: oracle.OBJ_RASTRIGIN}[fid]
+           5: oracle.OBJ_ACKLEY, 9: oracle.OBJ_RASTRIGIN, 13: oracle.OBJ_HAPPYCAT,
+           14: oracle.OBJ_HGBAT}[fid]
     x = d[case + "_x"]
     if case.startswith("haar"):
         o, M = oracle.gen_problem(int(d[case + "_seed"]), x.shape[1])
@@ -45,7 +46,7 @@ def test_live_reference_if_available(oracle, tmp_path):
         np.testing.assert_allclose(oracle.evaluate(obj, x, o, M), fr, rtol=1e-12)
 
 
-@pytest.mark.parametrize("obj", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("obj", [1, 2, 3, 4, 5, 6, 7])
 def test_zero_at_shift(oracle, obj):
     o, M = oracle.gen_problem(3, 30)
     f = oracle.evaluate(obj, o[None, :], o, M)[0]
@@ -81,6 +82,22 @@ def test_weighted_sphere_unit_vectors(oracle, obj, w0, wk):
         x = o + t * M[k]                       # M (x - o) = t e_k
         f = oracle.evaluate(obj, x[None, :], o, M)[0]
         assert f == pytest.approx((w0 if k == 0 else wk) * t * t, rel=1e-12)
+
+
+@pytest.mark.parametrize("t", [0.0, 0.5, -3.0, 40.0])
+def test_happycat_hgbat_closed_forms(oracle, t):
+    # z = 1 + t e_0: u = t e_0, r2 = t^2, s = t, so HappyCat is
+    # |t^2 - D|^(1/4) + (t^2 / 2 + t) / D + 1/2 and HGBat |t^4 - t^2|^(1/2) +
+    # the same tail (CEC2014.cc:919-928, 943-952): pins the shift by one, the
+    # two exponents and which sum D divides.
+    D = 5
+    z = np.ones(D)
+    z[0] += t
+    tail = (0.5 * t * t + t) / D + 0.5
+    f6 = oracle.evaluate(oracle.OBJ_HAPPYCAT, z[None, :], np.zeros(D), np.eye(D))[0]
+    f7 = oracle.evaluate(oracle.OBJ_HGBAT, z[None, :], np.zeros(D), np.eye(D))[0]
+    assert f6 == pytest.approx(abs(t * t - D) ** 0.25 + tail, rel=1e-13, abs=1e-13)
+    assert f7 == pytest.approx(abs(t ** 4 - t * t) ** 0.5 + tail, rel=1e-13, abs=1e-13)
 
 
 def test_rastrigin_integer_lattice(oracle):
