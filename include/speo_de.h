@@ -134,9 +134,9 @@ SPEO_EXPORT int API_DE_SelectSurvival(GPU_Population d_population, GPU_Populatio
 /* Replaces API_evaluateFitness (CUDA_EA_library/CEC2014_CUDA.h:35-37,
  * kernel_CEC2014_CUDA.cu:1672-1797) for function_ID 1 (rotated elliptic),
  * 2 (rotated bent cigar), 3 (rotated discus), 5 (rotated Ackley), 8 (shifted
- * Rastrigin, rate 0.0512), 9 (rotated Rastrigin, rate 0.0512), 13 (rotated
- * HappyCat, rate 0.05) and 14 (rotated HGBat, rate 0.05), in the
- * reference's layouts: d_original
+ * Rastrigin, rate 0.0512), 9 (rotated Rastrigin, rate 0.0512), 10 (shifted
+ * Schwefel, rate 10), 11 (rotated Schwefel, rate 10), 13 (rotated HappyCat,
+ * rate 0.05) and 14 (rotated HGBat, rate 0.05), in the reference's layouts: d_original
  * [pop_size][next_pow2(dim)], d_shift [next_pow2(dim)], d_rotated_data =
  * M transposed and row-padded, [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255).
  * The shift-rotate GEMM and the objective run as one fused MFMA kernel, so
@@ -169,6 +169,9 @@ SPEO_EXPORT int speo_device_check(void); /* SPEO_OK if a gfx950 device is curren
  *   HAPPYCAT    |r2 - D|^(1/4) + (r2 / 2 + s) / D + 1/2   CEC2014.cc:907-929
  *   HGBAT       |r2^2 - s^2|^(1/2) + (r2 / 2 + s) / D + 1/2   CEC2014.cc:931-953
  *               with u_j = z_j - 1, r2 = sum_j u_j^2, s = sum_j u_j.
+ *   SCHWEFEL    418.9828872724338 D - sum_j g(z_j + 420.9687462275036), the
+ *               modified Schwefel function folded beyond |v| = 500
+ *                                                       CEC2014.cc:705-738
  * SPEO_MODE_EPOCH takes objectives 1-5. */
 enum {
     SPEO_OBJ_ELLIPTIC = 1,
@@ -177,7 +180,8 @@ enum {
     SPEO_OBJ_BENT_CIGAR = 4,
     SPEO_OBJ_DISCUS = 5,
     SPEO_OBJ_HAPPYCAT = 6,
-    SPEO_OBJ_HGBAT = 7
+    SPEO_OBJ_HGBAT = 7,
+    SPEO_OBJ_SCHWEFEL = 8
 };
 
 /* Precision of the rotation GEMM (accumulation is always FP32). */

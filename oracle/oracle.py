@@ -29,7 +29,7 @@ LIB_PATH = HERE / "libspeo_oracle.so"
 REF_DIR = HERE / "_ref"
 
 OBJ_ELLIPTIC, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_BENT_CIGAR, OBJ_DISCUS = 1, 2, 3, 4, 5
-OBJ_HAPPYCAT, OBJ_HGBAT = 6, 7
+OBJ_HAPPYCAT, OBJ_HGBAT, OBJ_SCHWEFEL = 6, 7, 8
 PURPOSE_DONOR, PURPOSE_CROSS, PURPOSE_INIT = 0, 1, 2
 PURPOSE_SHIFT, PURPOSE_GAUSS, PURPOSE_PROBLEM = 16, 17, 18
 
@@ -295,10 +295,10 @@ def migrate_ring(X, fX, ld, off, dims, dst, k):
 # ------------------------------------------------- the reference's CEC2014.cc
 
 REF_FUNC = {OBJ_ELLIPTIC: 1, OBJ_BENT_CIGAR: 2, OBJ_DISCUS: 3, OBJ_ACKLEY: 5, OBJ_RASTRIGIN: 9,
-            OBJ_HAPPYCAT: 13, OBJ_HGBAT: 14}
-# CEC2014.cc:154-171, 186, 201-208: F1, F2, F3 and F5 use rate 1; F9 scales
-# x - o by 5.12/100, F13 and F14 by 5/100.
-REF_RATE = {1: 1.0, 2: 1.0, 3: 1.0, 5: 1.0, 9: 0.0512, 13: 0.05, 14: 0.05}
+            OBJ_HAPPYCAT: 13, OBJ_HGBAT: 14, OBJ_SCHWEFEL: 11}
+# CEC2014.cc:154-208: F1, F2, F3 and F5 use rate 1; F9 scales x - o by
+# 5.12/100, F11 by 10, F13 and F14 by 5/100.
+REF_RATE = {1: 1.0, 2: 1.0, 3: 1.0, 5: 1.0, 9: 0.0512, 11: 10.0, 13: 0.05, 14: 0.05}
 
 
 def write_cec_files(directory, func_id, o, M):

@@ -22,6 +22,7 @@
  *       Rastrigin                           CUDA_EA_library/CEC2014.cc:668-683
  *       bent cigar                          CUDA_EA_library/CEC2014.cc:499-516
  *       discus                              CUDA_EA_library/CEC2014.cc:518-530
+ *       Schwefel (modified)                 CUDA_EA_library/CEC2014.cc:705-738
  *       HappyCat                            CUDA_EA_library/CEC2014.cc:907-929
  *       HGBat                               CUDA_EA_library/CEC2014.cc:931-953
  *       composition weights (cf_cal)        CUDA_EA_library/CEC2014.cc:1481-1519
@@ -254,7 +255,8 @@ enum {
     OBJ_BENT_CIGAR = 4,
     OBJ_DISCUS = 5,
     OBJ_HAPPYCAT = 6,
-    OBJ_HGBAT = 7
+    OBJ_HGBAT = 7,
+    OBJ_SCHWEFEL = 8
 };
 
 static const double ORACLE_PI = 3.1415926535897932384626433832795029;
@@ -335,6 +337,30 @@ ORACLE_EXPORT double oracle_hgbat(const double *z, int D) {
     return pow(fabs(pow(r2, 2.0) - pow(s, 2.0)), 2.0 * (1.0 / 4.0)) + (0.5 * r2 + s) / D + 0.5;
 }
 
+/* CEC2014.cc:717-737: v = z + 420.9687462275036; inside [-500, 500] the
+ * term -v sin(sqrt|v|); outside, |v| folded back by fmod into
+ * m = 500 - fmod(|v|, 500) with -m sin(sqrt m) for v > 500 and
+ * +m sin(sqrt m) for v < -500, plus ((|v| - 500) / 100)^2 / D; then
+ * 418.9828872724338 D. */
+ORACLE_EXPORT double oracle_schwefel(const double *z, int D) {
+    double f = 0.0;
+    for (int i = 0; i < D; ++i) {
+        const double v = z[i] + 4.209687462275036e+002;
+        if (v > 500) {
+            f -= (500.0 - fmod(v, 500)) * sin(pow(500.0 - fmod(v, 500), 0.5));
+            const double t = (v - 500.0) / 100;
+            f += t * t / D;
+        } else if (v < -500) {
+            f -= (-500.0 + fmod(fabs(v), 500)) * sin(pow(500.0 - fmod(fabs(v), 500), 0.5));
+            const double t = (v + 500.0) / 100;
+            f += t * t / D;
+        } else {
+            f -= v * sin(pow(fabs(v), 0.5));
+        }
+    }
+    return f + 4.189828872724338e+002 * D;
+}
+
 ORACLE_EXPORT double oracle_basic(int objective, const double *z, int D) {
     switch (objective) {
     case OBJ_ELLIPTIC: return oracle_elliptic(z, D);
@@ -342,6 +368,7 @@ ORACLE_EXPORT double oracle_basic(int objective, const double *z, int D) {
     case OBJ_DISCUS: return oracle_discus(z, D);
     case OBJ_HAPPYCAT: return oracle_happycat(z, D);
     case OBJ_HGBAT: return oracle_hgbat(z, D);
+    case OBJ_SCHWEFEL: return oracle_schwefel(z, D);
     case OBJ_RASTRIGIN: return oracle_rastrigin(z, D);
     case OBJ_ACKLEY: return oracle_ackley(z, D);
     default: return NAN;

@@ -157,15 +157,50 @@ __host__ __device__ __forceinline__ float sin2pi(float z) {
     return q * u;
 }
 
+// sin x for 0 <= x <= 23 (the Schwefel terms take the sine of a square root
+// of at most 500): x = k pi + r with k = rint(x / pi) <= 7 and |r| <= pi / 2,
+// reduced with pi = 3.140625 + 9.6765358979e-4 (k * 3.140625 is exact and each
+// step is one fma), then sin r by its Taylor polynomial through r^11, whose
+// remainder is below (pi/2)^13 / 13! = 5.7e-8, negated for odd k.  Small
+// enough to inline into the unrolled epilogues, where the general sinf's
+// large-argument reduction is not.
+__host__ __device__ __forceinline__ float sin_small(float x) {
+    const float k = rintf(x * 0.318309886183790672f);
+    float r = fmaf(-k, 3.140625f, x);
+    r = fmaf(-k, 9.67653589793e-4f, r);
+    const float r2 = r * r;
+    float p = -1.0f / 39916800.0f;
+    p = fmaf(p, r2, 1.0f / 362880.0f);
+    p = fmaf(p, r2, -1.0f / 5040.0f);
+    p = fmaf(p, r2, 1.0f / 120.0f);
+    p = fmaf(p, r2, -1.0f / 6.0f);
+    const float s = fmaf(p * r2, r, r);
+    return (static_cast<int>(k) & 1) ? -s : s;
+}
+
 // Per-coordinate terms.  s1/s2 accumulate, per row:
 //   elliptic:  s1 += w_j z^2
 //   Rastrigin: s1 += z^2 + 20 sin^2(pi z)   (== z^2 - 10 cos(2 pi z) + 10)
 //   Ackley:    s1 += z^2, s2 += sin^2(pi z) (cos(2 pi z) - 1 == -2 sin^2(pi z))
 //   HappyCat, HGBat: s1 += (z - 1)^2, s2 += z - 1
+//   Schwefel: v = z + 420.9687462275036; s1 -= v sin(sqrt|v|) for |v| <= 500,
+//             else with m = 500 - fmod(|v|, 500): s1 -= sign(v) m sin(sqrt m),
+//             s2 += ((|v| - 500) / 100)^2 (divided by D when finished)
 __device__ __forceinline__ void add_terms(int objective, float z, float w, float &s1,
                                           float &s2) {
     if (objective == SPEO_OBJ_ELLIPTIC) {
         s1 = fmaf(w * z, z, s1);
+    } else if (objective == SPEO_OBJ_SCHWEFEL) {
+        const float v = z + 420.9687462275036f;
+        const float av = fabsf(v);
+        if (av <= 500.0f) {
+            s1 -= v * sin_small(sqrtf(av));
+        } else {
+            const float m = 500.0f - fmodf(av, 500.0f);
+            s1 -= copysignf(m, v) * sin_small(sqrtf(m));
+            const float t = (av - 500.0f) * 0.01f;
+            s2 = fmaf(t, t, s2);
+        }
     } else if (objective == SPEO_OBJ_HAPPYCAT || objective == SPEO_OBJ_HGBAT) {
         const float u = z - 1.0f;
         s1 = fmaf(u, u, s1);
@@ -181,6 +216,21 @@ __device__ __forceinline__ void add_terms(int objective, float z, float w, float
     }
 }
 
+// Calls f(std::integral_constant<int, OBJ>{}) for the runtime objective, so
+// that a loop of add_terms over a tile is compiled once per objective with
+// its branches resolved, instead of testing the objective at every element.
+// HappyCat and HGBat share their terms.
+template <typename F>
+__device__ __forceinline__ void with_objective(int objective, F &&f) {
+    switch (objective) {
+    case SPEO_OBJ_ELLIPTIC: f(std::integral_constant<int, SPEO_OBJ_ELLIPTIC>{}); break;
+    case SPEO_OBJ_RASTRIGIN: f(std::integral_constant<int, SPEO_OBJ_RASTRIGIN>{}); break;
+    case SPEO_OBJ_ACKLEY: f(std::integral_constant<int, SPEO_OBJ_ACKLEY>{}); break;
+    case SPEO_OBJ_SCHWEFEL: f(std::integral_constant<int, SPEO_OBJ_SCHWEFEL>{}); break;
+    default: f(std::integral_constant<int, SPEO_OBJ_HAPPYCAT>{}); break;
+    }
+}
+
 // Objective value from the two row sums.  Ackley in the cancellation-free
 // form -20 expm1(-0.2 r) - e expm1(mean(cos) - 1) of
 // e - 20 exp(-0.2 r) + 20 - exp(mean(cos)) (CEC2014.cc:604-611).
@@ -193,7 +243,8 @@ __device__ __forceinline__ float finish_objective(int objective, float s1, float
 }
 
 // finish_objective plus HappyCat and HGBat (CEC2014.cc:907-953; the GPU forms
-// global_f11 / global_f12, kernel_CEC2014_CUDA.cu:702-808), which the
+// global_f11 / global_f12, kernel_CEC2014_CUDA.cu:702-808) and Schwefel
+// (CEC2014.cc:705-738; global_f9, kernel_CEC2014_CUDA.cu:588-648), which the
 // evaluation and fused-generation kernels take; r2^2 - s^2 is formed as
 // (r2 - s)(r2 + s).
 __device__ __forceinline__ float finish_objective_ext(int objective, float s1, float s2, int dim) {
@@ -202,6 +253,7 @@ __device__ __forceinline__ float finish_objective_ext(int objective, float s1, f
         return sqrtf(sqrtf(fabsf(s1 - D))) + (0.5f * s1 + s2) / D + 0.5f;
     if (objective == SPEO_OBJ_HGBAT)
         return sqrtf(fabsf((s1 - s2) * (s1 + s2))) + (0.5f * s1 + s2) / D + 0.5f;
+    if (objective == SPEO_OBJ_SCHWEFEL) return fmaf(418.9828872724338f, D, s1 + s2 / D);
     return finish_objective(objective, s1, s2, dim);
 }
 
@@ -486,8 +538,10 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
         const int col = n0 + (wn + t * NSPLIT) * 16 + lr;
         if (col < dim) {
             const float w = (c.objective == SPEO_OBJ_ELLIPTIC) ? c.colw[col] : 1.0f;
+            with_objective(c.objective, [&](auto obj) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) add_terms(c.objective, acc[t][i] * unscale, w, s1[i], s2[i]);
+                for (int i = 0; i < 4; ++i) add_terms(decltype(obj)::value, acc[t][i] * unscale, w, s1[i], s2[i]);
+            });
         }
     }
 }
@@ -637,10 +691,13 @@ __device__ __forceinline__ void gemm_small(const CompDev &c, const float *rows, 
         const int col = (t == 0 ? t0 : t1) * 16 + lr;
         if (col < dim) {
             const float w = (c.objective == SPEO_OBJ_ELLIPTIC) ? c.colw[col] : 1.0f;
+            with_objective(c.objective, [&](auto obj) {
 #pragma unroll
-            for (int i = 0; i < NRT; ++i)
+                for (int i = 0; i < NRT; ++i)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) add_terms(c.objective, acc[i][t][q] * unscale, w, s1[i][q], s2[i][q]);
+                    for (int q = 0; q < 4; ++q)
+                        add_terms(decltype(obj)::value, acc[i][t][q] * unscale, w, s1[i][q], s2[i][q]);
+            });
         }
     }
 #pragma unroll

@@ -1,7 +1,8 @@
 // eval_f64.h -- FP64 objective of one row by one 256-thread block
 // (re-verification path), following the reference's FP64 CPU forms:
 // shift/rotate CEC2014.cc:1373-1393, elliptic 481-497, bent cigar 499-516,
-// discus 518-530, Ackley 590-612, Rastrigin 668-683, HappyCat 907-929, HGBat
+// discus 518-530, Ackley 590-612, Rastrigin 668-683, Schwefel 705-738,
+// HappyCat 907-929, HGBat
 // 931-953, composition weights cf_cal 1481-1519.
 #pragma once
 
@@ -38,6 +39,19 @@ __device__ inline double eval_row_f64(const CompParams *comps, int ncomp, int di
                     s1 += (i == 0 ? 1.0 : 1.0e6) * z * z;
                 } else if (cp.objective == SPEO_OBJ_DISCUS) {
                     s1 += (i == 0 ? 1.0e6 : 1.0) * z * z;
+                } else if (cp.objective == SPEO_OBJ_SCHWEFEL) {
+                    const double v = z + 4.209687462275036e+002;
+                    if (v > 500.0) {
+                        const double m = 500.0 - fmod(v, 500.0);
+                        s1 -= m * sin(sqrt(m));
+                        s1 += (v - 500.0) / 100.0 * ((v - 500.0) / 100.0) / dim;
+                    } else if (v < -500.0) {
+                        const double m = 500.0 - fmod(fabs(v), 500.0);
+                        s1 += m * sin(sqrt(m));
+                        s1 += (v + 500.0) / 100.0 * ((v + 500.0) / 100.0) / dim;
+                    } else {
+                        s1 -= v * sin(sqrt(fabs(v)));
+                    }
                 } else if (cp.objective == SPEO_OBJ_HAPPYCAT || cp.objective == SPEO_OBJ_HGBAT) {
                     s1 += (z - 1.0) * (z - 1.0);
                     s2 += z - 1.0;
@@ -71,6 +85,8 @@ __device__ inline double eval_row_f64(const CompParams *comps, int ncomp, int di
         if (cp.objective == SPEO_OBJ_ACKLEY) {
             const double r = -0.2 * sqrt(S1 / (double)dim);
             g = E - 20.0 * exp(r) + 20.0 - exp(S2 / (double)dim);
+        } else if (cp.objective == SPEO_OBJ_SCHWEFEL) {
+            g = S1 + 4.189828872724338e+002 * dim;
         } else if (cp.objective == SPEO_OBJ_HAPPYCAT) {
             g = pow(fabs(S1 - dim), 0.25) + (0.5 * S1 + S2) / dim + 0.5;
         } else if (cp.objective == SPEO_OBJ_HGBAT) {

@@ -304,6 +304,8 @@ extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real 
     case 5: c.objective = SPEO_OBJ_ACKLEY; c.rate = 1.0f; break;         // F5, :1130
     case 8: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; rotate = false; break;  // F8, :1150
     case 9: c.objective = SPEO_OBJ_RASTRIGIN; c.rate = 0.0512f; break;   // F9, :1156
+    case 10: c.objective = SPEO_OBJ_SCHWEFEL; c.rate = 10.0f; rotate = false; break;  // F10, :1162
+    case 11: c.objective = SPEO_OBJ_SCHWEFEL; c.rate = 10.0f; break;     // F11, :1168
     case 13: c.objective = SPEO_OBJ_HAPPYCAT; c.rate = 0.05f; break;     // F13, :1180
     case 14: c.objective = SPEO_OBJ_HGBAT; c.rate = 0.05f; break;        // F14, :1186
     default:

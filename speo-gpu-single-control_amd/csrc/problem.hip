@@ -114,7 +114,7 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
         return set_error(SPEO_ERR_ARG, "speo_problem_create: need dim >= 2, 1 <= ncomp <= 8, "
                                        "objectives and shift");
     for (int c = 0; c < ncomp; ++c)
-        if (objectives[c] < SPEO_OBJ_ELLIPTIC || objectives[c] > SPEO_OBJ_HGBAT)
+        if (objectives[c] < SPEO_OBJ_ELLIPTIC || objectives[c] > SPEO_OBJ_SCHWEFEL)
             return set_error(SPEO_ERR_ARG, "speo_problem_create: unknown objective");
     P.dim = dim;
     P.ncomp = ncomp;

@@ -22,7 +22,7 @@ import numpy as np
 
 from ._lib import (DEInfo, EngineConfig, EngineStats, GPU_IslandInfo, GPU_Population,
                    MODE_EPOCH, MODE_FUSED, MODE_PLAIN, OBJ_ACKLEY, OBJ_BENT_CIGAR, OBJ_DISCUS,
-                   OBJ_ELLIPTIC, OBJ_HAPPYCAT, OBJ_HGBAT, OBJ_RASTRIGIN, PREC_BF16,
+                   OBJ_ELLIPTIC, OBJ_HAPPYCAT, OBJ_HGBAT, OBJ_RASTRIGIN, OBJ_SCHWEFEL, PREC_BF16,
                    PREC_BF16X3, PREC_BF16X6, PREC_FP16, PREC_FP16X3, PREC_FP32, PREC_NAMES,
                    ProblemInfo, SPEO_RandState, SpeoError, check, last_error, lib)
 
@@ -30,7 +30,7 @@ __all__ = [
     "Problem", "Engine", "generate_problem_data", "problem_seed", "init_population",
     "de_generate", "de_select", "evaluate", "evaluate_f64", "segment_argmin", "SpeoError",
     "OBJ_ELLIPTIC", "OBJ_RASTRIGIN", "OBJ_ACKLEY", "OBJ_BENT_CIGAR", "OBJ_DISCUS", "OBJ_HAPPYCAT",
-    "OBJ_HGBAT", "PREC_FP32", "PREC_BF16", "PREC_FP16",
+    "OBJ_HGBAT", "OBJ_SCHWEFEL", "PREC_FP32", "PREC_BF16", "PREC_FP16",
     "PREC_BF16X3", "PREC_FP16X3", "PREC_BF16X6", "PREC_NAMES", "MODE_PLAIN", "MODE_FUSED",
     "MODE_EPOCH",    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
     "generations_for_fes", "run_seed", "de_parameters", "result_header", "result_record",

@@ -18,7 +18,7 @@ HEADER = PKG_DIR.parent / "include" / "speo_de.h"
 SPEO_OK, SPEO_ERR_ARG, SPEO_ERR_HIP, SPEO_ERR_UNSUPPORTED, SPEO_ERR_NO_DEVICE, SPEO_ERR_ALLOC = (
     0, -1, -2, -3, -4, -5)
 OBJ_ELLIPTIC, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_BENT_CIGAR, OBJ_DISCUS = 1, 2, 3, 4, 5
-OBJ_HAPPYCAT, OBJ_HGBAT = 6, 7
+OBJ_HAPPYCAT, OBJ_HGBAT, OBJ_SCHWEFEL = 6, 7, 8
 PREC_FP32, PREC_BF16, PREC_FP16, PREC_BF16X3, PREC_FP16X3, PREC_BF16X6 = range(6)
 PREC_NAMES = {PREC_FP32: "fp32", PREC_BF16: "bf16", PREC_FP16: "fp16", PREC_BF16X3: "bf16x3",
               PREC_FP16X3: "fp16x3", PREC_BF16X6: "bf16x6"}

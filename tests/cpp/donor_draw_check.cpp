@@ -5,6 +5,7 @@
 //   B <v as u32 bits> <lo> <hi>
 //   T <CR as u32 bits>          (crossover threshold)
 //   S <z as u32 bits>           (sin^2(pi z) of the objective terms)
+//   N <x as u32 bits>           (sin x, 0 <= x <= 23, of the Schwefel terms)
 // Output per line: "r0 r1 r2 r3 r4 jr draws", or one u32 (bits of a float, or
 // the threshold).
 #include <cstdint>
@@ -51,6 +52,14 @@ int main() {
             float z;
             std::memcpy(&z, &bits, 4);
             const float o = speo::sin2pi(z);
+            std::memcpy(&bits, &o, 4);
+            std::printf("%u\n", bits);
+        } else if (kind == "N") {
+            uint32_t bits;
+            in >> bits;
+            float x;
+            std::memcpy(&x, &bits, 4);
+            const float o = speo::sin_small(x);
             std::memcpy(&bits, &o, 4);
             std::printf("%u\n", bits);
         } else if (kind == "B") {

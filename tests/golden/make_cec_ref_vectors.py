@@ -8,6 +8,8 @@ Cases (func_id -> objective; CEC2014.cc:154-171, 186):
   F2 rotated bent cigar, rate 1;  F3 rotated discus, rate 1;
   F13 rotated HappyCat and F14 rotated HGBat, rate 5/100 -- evaluated at a
   rate-adjusted input like F9;
+  F11 rotated Schwefel, rate 10 -- at the rate-adjusted input x' = o + (x - o)/10,
+  with x in [-1000, 1000] so that |z| reaches the folded range beyond 500;
   F9 rotated Rastrigin, rate 5.12/100 -- evaluated at x' = o + (x - o)/0.0512
   so that the reference computes Rastrigin(M (x - o)) at rate 1.
 Two data sources:
@@ -33,7 +35,9 @@ REF_BIN = Path("/root/reference/SPEO-GPU-single-control/bin")
 # Functions added later come last so that the random points of the earlier
 # cases keep their values.
 FUNCS = {1: O.OBJ_ELLIPTIC, 5: O.OBJ_ACKLEY, 9: O.OBJ_RASTRIGIN, 2: O.OBJ_BENT_CIGAR,
-         3: O.OBJ_DISCUS, 13: O.OBJ_HAPPYCAT, 14: O.OBJ_HGBAT}
+         3: O.OBJ_DISCUS, 13: O.OBJ_HAPPYCAT, 14: O.OBJ_HGBAT, 11: O.OBJ_SCHWEFEL}
+# point range per function (default [-100, 100])
+XRANGE = {11: 1000.0}
 NPTS = 12
 
 
@@ -53,14 +57,16 @@ def main():
         for D in (10, 30):
             o = np.loadtxt(REF_BIN / "input_data" / f"shift_data_{fid}.txt").reshape(-1)[:D]
             M = np.loadtxt(REF_BIN / "input_data" / f"M_{fid}_D{D}.txt").reshape(D, D)
-            x = rng.uniform(-100, 100, (NPTS, D)).astype(np.float32).astype(np.float64)
+            r = XRANGE.get(fid, 100.0)
+            x = rng.uniform(-r, r, (NPTS, D)).astype(np.float32).astype(np.float64)
             f = O.ref_eval(REF_BIN, fid, ref_inputs(fid, x, o))
             key = f"refdata_F{fid}_D{D}"
             out[key + "_x"], out[key + "_o"], out[key + "_M"], out[key + "_f"] = x, o, M, f
             cases.append(key)
         for D, seed in ((10, 101), (100, 102)):
             o, M = O.gen_problem(seed, D)
-            x = rng.uniform(-100, 100, (NPTS, D)).astype(np.float32).astype(np.float64)
+            r = XRANGE.get(fid, 100.0)
+            x = rng.uniform(-r, r, (NPTS, D)).astype(np.float32).astype(np.float64)
             with tempfile.TemporaryDirectory() as td:
                 O.write_cec_files(td, fid, o, M)
                 f = O.ref_eval(td, fid, ref_inputs(fid, x, o))

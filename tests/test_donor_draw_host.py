@@ -7,7 +7,7 @@ which runs into the 4096-draw cap), and for injected sequences; and the same
 bits of the bound for in-range, reflected, far-out, infinite and NaN values.
 Also the integer form of the crossover test (exhaustive over all 2^24 draw
 values for a set of CR) and the accuracy of the objective's sin^2(pi z)
-polynomial against FP64."""
+polynomial and of the Schwefel terms' bounded sine against FP64."""
 import shutil
 import struct
 import subprocess
@@ -112,3 +112,19 @@ def test_sin2pi_accuracy(exe):
     assert np.all(got[tiny] <= 1e-30)
     rel = np.abs(got[~tiny] - ref[~tiny]) / ref[~tiny]
     assert rel.max() < 3e-7, rel.max()
+
+
+def test_sin_small_accuracy(exe):
+    # sin x on [0, 23] (the Schwefel terms' sin(sqrt v), v <= 500): absolute
+    # error against FP64 within a few FP32 ulps of 1, including the
+    # reduction points k pi +- pi/2 and the ends of the range.
+    rng = np.random.default_rng(10)
+    k = np.arange(8)
+    x = np.concatenate([rng.uniform(0, 23, 40000), np.sqrt(rng.uniform(0, 500, 20000)),
+                        k * np.pi, k * np.pi + np.pi / 2, k * np.pi - np.pi / 2,
+                        [0.0, 1e-30, 1e-7, 22.36068, 23.0]]).astype(np.float32)
+    x = x[(x >= 0) & (x <= 23)]
+    got = np.array([_f32(int(v)) for v in _run(exe, [f"N {_bits(float(t))}" for t in x])],
+                   np.float64)
+    err = np.abs(got - np.sin(x.astype(np.float64)))
+    assert err.max() < 4e-7, err.max()

@@ -6,6 +6,8 @@ import ctypes
 import numpy as np
 import pytest
 
+from tolerance import schwefel_rows
+
 pytestmark = pytest.mark.gpu
 
 
@@ -224,6 +226,7 @@ def test_dropin_generation_cycle(cuda, speo, oracle):
 @pytest.mark.parametrize("fid,obj,rate,rotate", [(1, 1, 1.0, True), (2, 4, 1.0, True),
                                                  (3, 5, 1.0, True), (5, 3, 1.0, True),
                                                  (8, 2, 0.0512, False), (9, 2, 0.0512, True),
+                                                 (10, 8, 10.0, False), (11, 8, 10.0, True),
                                                  (13, 6, 0.05, True), (14, 7, 0.05, True)])
 def test_dropin_evaluate_functions(cuda, speo, oracle, fid, obj, rate, rotate):
     import torch
@@ -248,7 +251,11 @@ def test_dropin_evaluate_functions(cuda, speo, oracle, fid, obj, rate, rotate):
     # z = rate * M (x - o): the oracle at x' = o + rate (x - o)
     xs = o[None, :] + rate * (X[:, :D].astype(np.float64) - o[None, :])
     ref = oracle.evaluate(obj, xs, o, Mu)
-    np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-5)
+    keep = np.ones(NP, bool)
+    if obj == 8:   # rows at a jump of Schwefel have no well-defined FP32 value
+        keep = ~schwefel_rows(X[:, :D], o, Mu, rate)[1]
+        assert keep.sum() >= NP - 2
+    np.testing.assert_allclose(f.cpu().numpy()[keep], ref[keep], rtol=1e-5)
 
 
 def test_dropin_unsupported_function_reports(cuda, speo):

@@ -7,7 +7,7 @@ equal to the plain three-launch one, and graph replays equal to eager runs."""
 import numpy as np
 import pytest
 
-from tolerance import assert_rows, row_rtol
+from tolerance import assert_rows, row_rtol, schwefel_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -164,6 +164,38 @@ def test_happycat_hgbat_plain_and_fused(cuda, speo, oracle):
     np.testing.assert_array_equal(Xa, Xb)
     with pytest.raises(speo.SpeoError, match="objectives 1-5"):
         _engine(speo, probs[:1], [32], [0], mode=speo.MODE_EPOCH)
+
+
+def test_schwefel_plain_and_fused(cuda, speo, oracle):
+    # Schwefel (8) at rate 10, D = 50 and the large-D path at D = 200:
+    # first-generation trials against the FP64 oracle (rows at a jump of the
+    # function left out), plain and fused bit-identical over migrating runs.
+    probs, dims = [], [50, 200]
+    for k, D in enumerate(dims):
+        o, M = oracle.gen_problem(1900 + k, D)
+        probs.append(speo.Problem([8], o[None, :], M[None, :, :], rate=[10.0]))
+    sizes, pidx = [40, 64, 48], [0, 1, 0]
+    first = _engine(speo, probs, sizes, pidx, mode=speo.MODE_PLAIN)
+    first.run(1)
+    U, fU = first.read_trials()
+    for i, p in enumerate(pidx):
+        D = dims[p]
+        rows = slice(first.row_off[i], first.row_off[i + 1])
+        o, M = probs[p].shifts[0], probs[p].rotations[0]
+        Ui = U[rows][:, :D].astype(np.float64)
+        ref = oracle.evaluate(8, o[None, :] + 10.0 * (Ui - o[None, :]), o, M)
+        keep = ~schwefel_rows(Ui, o, M, 10.0)[1]
+        np.testing.assert_allclose(fU[rows][keep], ref[keep], rtol=1e-5)
+    kw = dict(interval=3, migration_size=2)
+    a = _engine(speo, probs, sizes, pidx, mode=speo.MODE_PLAIN, **kw)
+    b = _engine(speo, probs, sizes, pidx, mode=speo.MODE_FUSED, **kw)
+    for chunk in (2, 7):
+        a.run(chunk)
+        b.run(chunk)
+    Xa, fa = a.read()
+    Xb, fb = b.read()
+    np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(Xa, Xb)
 
 
 @pytest.mark.parametrize("mode", [0, 1])

@@ -13,7 +13,7 @@ FP64 device path: 1e-12 relative.
 import numpy as np
 import pytest
 
-from tolerance import assert_rows, composition_rtol, row_rtol
+from tolerance import assert_rows, composition_rtol, row_rtol, schwefel_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +49,35 @@ def test_fp32_against_fp64_oracle(cuda, speo, oracle, D, obj):
     torch.cuda.synchronize()
     ref = oracle.evaluate(obj, X[:, :D], o, M)
     assert_rows(f.cpu().numpy(), ref, row_rtol(obj, X[:, :D], o, M))
+
+
+@pytest.mark.parametrize("D", [2, 10, 100, 128, 129, 1000])
+def test_schwefel_rate10(cuda, speo, oracle, D):
+    # Schwefel at the reference's rate 10 (CEC2014.cc:189-194): |v| reaches
+    # well past 500, so most coordinates take the folded branches.  Rows
+    # within rounding distance of a jump of the function are left out and
+    # counted (tolerance.py).
+    import torch
+    rng = np.random.default_rng(700 + D)
+    o, M = oracle.gen_problem(800 + D, D)
+    prob = speo.Problem([8], o[None, :], M[None, :, :], rate=[10.0])
+    n = 64 if D >= 500 else 300
+    ld = (D + 3) // 4 * 4
+    X = _points(rng, n, D, ld)
+    f = torch.zeros(n, dtype=torch.float32, device=cuda)
+    print(f"eval schwefel D={D}", flush=True)
+    speo.evaluate(prob, _t(X, cuda), f)
+    torch.cuda.synchronize()
+    xs = o[None, :] + 10.0 * (X[:, :D].astype(np.float64) - o[None, :])
+    ref = oracle.evaluate(8, xs, o, M)
+    rtol, near = schwefel_rows(X[:, :D], o, M, 10.0)
+    assert near.sum() <= max(1, n // 50), int(near.sum())
+    keep = ~near
+    assert_rows(f.cpu().numpy()[keep], ref[keep], rtol[keep])
+    f64 = torch.zeros(n, dtype=torch.float64, device=cuda)
+    speo.evaluate_f64(prob, _t(X, cuda), f64)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(f64.cpu().numpy(), ref, rtol=1e-12)
 
 
 def test_fp32_near_optimum(cuda, speo, oracle):

@@ -22,8 +22,8 @@ def test_against_reference_cec2014(oracle, case):
     d = np.load(GOLDEN)
     fid = int(case.split("_F")[1].split("_")[0])
     obj = {1: oracle.OBJ_ELLIPTIC, 2: oracle.OBJ_BENT_CIGAR, 3: oracle.OBJ_DISCUS,
-           5: oracle.OBJ_ACKLEY, 9: oracle.OBJ_RASTRIGIN, 13: oracle.OBJ_HAPPYCAT,
-           14: oracle.OBJ_HGBAT}[fid]
+           5: oracle.OBJ_ACKLEY, 9: oracle.OBJ_RASTRIGIN, 11: oracle.OBJ_SCHWEFEL,
+           13: oracle.OBJ_HAPPYCAT, 14: oracle.OBJ_HGBAT}[fid]
     x = d[case + "_x"]
     if case.startswith("haar"):
         o, M = oracle.gen_problem(int(d[case + "_seed"]), x.shape[1])
@@ -98,6 +98,18 @@ def test_happycat_hgbat_closed_forms(oracle, t):
     f7 = oracle.evaluate(oracle.OBJ_HGBAT, z[None, :], np.zeros(D), np.eye(D))[0]
     assert f6 == pytest.approx(abs(t * t - D) ** 0.25 + tail, rel=1e-13, abs=1e-13)
     assert f7 == pytest.approx(abs(t ** 4 - t * t) ** 0.5 + tail, rel=1e-13, abs=1e-13)
+
+
+def test_schwefel_branches(oracle):
+    # v = z + 420.9687462275036 at 300 (inside), 600 (folded down to
+    # m = 400, penalty 1) and -700 (folded to m = 300 with the sign flipped,
+    # penalty 4), D = 3 (CEC2014.cc:717-737).
+    c = 4.209687462275036e+002
+    z = np.array([300.0, 600.0, -700.0]) - c
+    f = oracle.evaluate(oracle.OBJ_SCHWEFEL, z[None, :], np.zeros(3), np.eye(3))[0]
+    want = (-300.0 * math.sin(math.sqrt(300.0)) - 400.0 * math.sin(20.0) + 1.0 / 3
+            + 300.0 * math.sin(math.sqrt(300.0)) + 4.0 / 3 + 4.189828872724338e+002 * 3)
+    assert f == pytest.approx(want, rel=1e-13)
 
 
 def test_rastrigin_integer_lattice(oracle):

@@ -12,6 +12,12 @@ weighted coordinate cancels (z_j small against the terms it sums) has a large
 kappa for any implementation; discus, whose value is 1e6 z_0^2 plus the rest,
 has such rows often.  Rows are held to max(base, 4 u kappa); for the other
 objectives the bound is `base`.
+
+Schwefel (objective 8) sums g(v_j), v_j = z_j + 420.97, and rows are held to
+`base`.  The function jumps (by up to about 400) where |v_j| crosses 1000,
+1500, ...: a row with a coordinate within d_j = 8 u (S_j + |v_j|), a bound on
+the FP32 error of v_j, of such a point has no well-defined FP32 value and is
+left out (schwefel_rows).
 """
 import numpy as np
 
@@ -50,3 +56,14 @@ def assert_rows(f, ref, rtol):
     rel = np.abs(np.asarray(f, np.float64) - ref) / np.abs(ref)
     bad = np.flatnonzero(rel > rtol)
     assert bad.size == 0, [(int(i), float(rel[i]), float(rtol[i])) for i in bad[:5]]
+
+
+def schwefel_rows(X, o, M, rate, base=1e-5, u=U_FP32):
+    """(per-row rtol, rows within rounding distance of a jump) for Schwefel."""
+    a = (np.asarray(X, np.float64) - o[None, :]) * rate
+    v = a @ M.T + 4.209687462275036e+002
+    d = 8.0 * u * (np.abs(a) @ np.abs(M).T + np.abs(v))
+    av = np.abs(v)
+    dist = np.abs(av - 500.0 * np.rint(av / 500.0))
+    near = ((av >= 1000.0 - d) & (dist <= d)).any(1)
+    return np.full(av.shape[0], base), near
