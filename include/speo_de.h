@@ -220,6 +220,23 @@ SPEO_EXPORT int speo_generate_problem_data(uint64_t seed, int dim, double *shift
                                            double *rotation_out);
 SPEO_EXPORT uint64_t speo_problem_seed(uint64_t base_seed, uint32_t index);
 
+/* Reads the reference's problem data files, as CEC2014::Initilize
+ * (CEC2014.cc:353-420) and CEC2014_CUDA::Initilize (CEC2014_CUDA.cc:195-277):
+ * <dir>/M_<func_id>_D<dim>.txt holds ncomp * dim * dim matrix values and
+ * <dir>/shift_data_<func_id>.txt one line per component of which the first
+ * dim values are taken.  Host only.  shift_out [ncomp][dim], rotation_out
+ * [ncomp][dim][dim] (either may be NULL).  SPEO_ERR_ARG with the file name
+ * when a file is missing or holds too few values. */
+SPEO_EXPORT int speo_load_cec_data(const char *dir, int func_id, int dim, int ncomp,
+                                   double *shift_out, double *rotation_out);
+
+/* The problem of CEC2014 function `func_id` at `dim` from those files, with
+ * the reference GPU path's objective, rate and rotation flag
+ * (kernel_CEC2014_CUDA.cu:1099-1190): 1, 2, 3, 5, 8 (no rotation), 9, 10 (no
+ * rotation), 11, 13, 14; others SPEO_ERR_UNSUPPORTED.  Values are the raw
+ * objective (no 100 * func_id), as the reference's API_evaluateFitness. */
+SPEO_EXPORT int speo_problem_create_cec(const char *dir, int func_id, int dim, speo_problem *out);
+
 /* f_out[i] = f(X[i]) for n rows of stride ld (device pointers), FP32
  * evaluation with the rotation GEMM in `precision`.  Rows must have ld >= dim. */
 SPEO_EXPORT int speo_evaluate(speo_problem p, const float *X, int ld, int n, float *f_out,

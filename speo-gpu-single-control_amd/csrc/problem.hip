@@ -8,7 +8,9 @@
 //   hi/lo/lo2 splits of M (16-bit MFMA operands), M64 (FP64 re-verification),
 //   and the elliptic column weights.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "eval.h"
@@ -314,6 +316,70 @@ int speo_problem_create(int dim, int ncomp, const int *objectives, const double 
     }
     *out = p;
     return SPEO_OK;
+}
+
+// CEC2014.cc:353-420: fscanf("%lf") over the matrix file; per component the
+// first dim values of its shift line, the rest of the line skipped.
+int speo_load_cec_data(const char *dir, int func_id, int dim, int ncomp, double *shift_out,
+                       double *rotation_out) {
+    if (!dir || dim < 1 || ncomp < 1 || ncomp > kMaxComp)
+        return set_error(SPEO_ERR_ARG, "speo_load_cec_data: bad arguments");
+    const std::string base(dir);
+    if (rotation_out) {
+        const std::string name = base + "/M_" + std::to_string(func_id) + "_D" + std::to_string(dim) + ".txt";
+        FILE *f = std::fopen(name.c_str(), "r");
+        if (!f) return set_error(SPEO_ERR_ARG, "speo_load_cec_data: cannot open " + name);
+        const size_t n = (size_t)ncomp * dim * dim;
+        size_t got = 0;
+        while (got < n && std::fscanf(f, "%lf", &rotation_out[got]) == 1) ++got;
+        std::fclose(f);
+        if (got < n) return set_error(SPEO_ERR_ARG, "speo_load_cec_data: too few values in " + name);
+    }
+    if (shift_out) {
+        const std::string name = base + "/shift_data_" + std::to_string(func_id) + ".txt";
+        FILE *f = std::fopen(name.c_str(), "r");
+        if (!f) return set_error(SPEO_ERR_ARG, "speo_load_cec_data: cannot open " + name);
+        bool ok = true;
+        for (int c = 0; c < ncomp && ok; ++c) {
+            for (int j = 0; j < dim && ok; ++j) ok = std::fscanf(f, "%lf", &shift_out[(size_t)c * dim + j]) == 1;
+            if (ok && c + 1 < ncomp) {
+                int ch;
+                while ((ch = std::fgetc(f)) != EOF && ch != '\n') {
+                }
+            }
+        }
+        std::fclose(f);
+        if (!ok) return set_error(SPEO_ERR_ARG, "speo_load_cec_data: too few values in " + name);
+    }
+    return SPEO_OK;
+}
+
+int speo_problem_create_cec(const char *dir, int func_id, int dim, speo_problem *out) {
+    if (!out) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: null output");
+    *out = nullptr;
+    int obj = 0;
+    double rate = 1.0;
+    bool rotate = true;
+    switch (func_id) {  // F1-F14 of kernel_CEC2014_CUDA.cu:1099-1190
+    case 1: obj = SPEO_OBJ_ELLIPTIC; break;
+    case 2: obj = SPEO_OBJ_BENT_CIGAR; break;
+    case 3: obj = SPEO_OBJ_DISCUS; break;
+    case 5: obj = SPEO_OBJ_ACKLEY; break;
+    case 8: obj = SPEO_OBJ_RASTRIGIN; rate = 0.0512; rotate = false; break;
+    case 9: obj = SPEO_OBJ_RASTRIGIN; rate = 0.0512; break;
+    case 10: obj = SPEO_OBJ_SCHWEFEL; rate = 10.0; rotate = false; break;
+    case 11: obj = SPEO_OBJ_SCHWEFEL; rate = 10.0; break;
+    case 13: obj = SPEO_OBJ_HAPPYCAT; rate = 0.05; break;
+    case 14: obj = SPEO_OBJ_HGBAT; rate = 0.05; break;
+    default:
+        return set_error(SPEO_ERR_UNSUPPORTED,
+                         "speo_problem_create_cec: function " + std::to_string(func_id) + " is not built");
+    }
+    if (dim < 2) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: dim < 2");
+    std::vector<double> o((size_t)dim), M(rotate ? (size_t)dim * dim : 0);
+    SPEO_TRY(speo_load_cec_data(dir, func_id, dim, 1, o.data(), rotate ? M.data() : nullptr));
+    return speo_problem_create(dim, 1, &obj, o.data(), rotate ? M.data() : nullptr, nullptr, nullptr,
+                               nullptr, &rate, out);
 }
 
 int speo_problem_destroy(speo_problem p) {

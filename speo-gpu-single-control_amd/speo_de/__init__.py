@@ -33,7 +33,7 @@ __all__ = [
     "OBJ_HGBAT", "OBJ_SCHWEFEL", "PREC_FP32", "PREC_BF16", "PREC_FP16",
     "PREC_BF16X3", "PREC_FP16X3", "PREC_BF16X6", "PREC_NAMES", "MODE_PLAIN", "MODE_FUSED",
     "MODE_EPOCH",    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
-    "generations_for_fes", "run_seed", "de_parameters", "result_header", "result_record",
+    "generations_for_fes", "run_seed", "load_cec_data", "de_parameters", "result_header", "result_record",
 ]
 
 
@@ -90,6 +90,18 @@ class Problem:
         self.sigma, self.bias, self.lam, self.rate = arrs
 
     @classmethod
+    def from_cec(cls, directory, func_id: int, dim: int):
+        """CEC2014 function `func_id` from the reference's data files in
+        `directory` (speo_problem_create_cec)."""
+        self = cls.__new__(cls)
+        h = c_void_p()
+        check(lib().speo_problem_create_cec(str(directory).encode(), func_id, dim, byref(h)))
+        self.handle, self.dim, self.ncomp = h, dim, 1
+        self.objectives = self.shifts = self.rotations = None
+        self.sigma = self.bias = self.lam = self.rate = None
+        return self
+
+    @classmethod
     def seeded(cls, objective: int, dim: int, seed: int):
         o, M = generate_problem_data(seed, dim)
         return cls([objective], o[None, :], M[None, :, :])
@@ -104,6 +116,16 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+
+def load_cec_data(directory, func_id: int, dim: int, ncomp: int = 1):
+    """(shift [ncomp][dim], rotation [ncomp][dim][dim]) from the reference's
+    data files (speo_load_cec_data)."""
+    o = np.zeros((ncomp, dim), np.float64)
+    M = np.zeros((ncomp, dim, dim), np.float64)
+    check(lib().speo_load_cec_data(str(directory).encode(), func_id, dim, ncomp,
+                                   o.ctypes.data_as(c_void_p), M.ctypes.data_as(c_void_p)))
+    return o, M
 
 
 def evaluate(problem: Problem, X, f_out, precision=PREC_FP32, ld=None, n=None, stream=None):

@@ -107,6 +107,8 @@ _SIGS = {
     "speo_problem_dim": (c_int, [c_void_p]),
     "speo_generate_problem_data": (c_int, [c_uint64, c_int, c_void_p, c_void_p]),
     "speo_problem_seed": (c_uint64, [c_uint64, c_uint32]),
+    "speo_load_cec_data": (c_int, [c_char_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "speo_problem_create_cec": (c_int, [c_char_p, c_int, c_int, POINTER(c_void_p)]),
     "speo_evaluate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
     "speo_evaluate_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "speo_init_population": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_float, c_uint64,

@@ -80,6 +80,43 @@ def test_schwefel_rate10(cuda, speo, oracle, D):
     np.testing.assert_allclose(f64.cpu().numpy(), ref, rtol=1e-12)
 
 
+def _refdata_cases():
+    from pathlib import Path
+    g = np.load(Path(__file__).resolve().parent / "golden" / "cec2014_ref_vectors.npz")
+    return [str(c) for c in g["cases"] if str(c).startswith("refdata")]
+
+
+@pytest.mark.parametrize("case", _refdata_cases())
+def test_reference_data_files_end_to_end(cuda, speo, tmp_path, case):
+    # The reference's own o and M (held in the golden vectors) written in its
+    # text format, loaded by speo_problem_create_cec with the reference's
+    # objective and rate, evaluated by the GPU kernels at the inputs the
+    # reference's CEC2014.cc was given, against its outputs.
+    import torch
+    from pathlib import Path
+    from test_cec_data import _write
+    g = np.load(Path(__file__).resolve().parent / "golden" / "cec2014_ref_vectors.npz")
+    fid = int(case.split("_F")[1].split("_")[0])
+    x, o, M, ref = g[case + "_x"], g[case + "_o"], g[case + "_M"], g[case + "_f"]
+    D = o.size
+    _write(tmp_path, fid, o[None, :], M[None, :, :])
+    prob = speo.Problem.from_cec(tmp_path, fid, D)
+    rate = {9: 0.0512, 11: 10.0, 13: 0.05, 14: 0.05}.get(fid, 1.0)
+    xs = (o[None, :] + (x - o[None, :]) / rate).astype(np.float32)
+    f = torch.zeros(x.shape[0], dtype=torch.float32, device=cuda)
+    print(f"reference data {case}", flush=True)
+    speo.evaluate(prob, _t(xs, cuda), f)
+    torch.cuda.synchronize()
+    got = f.cpu().numpy()
+    xs64 = xs.astype(np.float64)
+    if fid == 11:
+        keep = ~schwefel_rows(xs64, o, M, rate)[1]
+        np.testing.assert_allclose(got[keep], ref[keep], rtol=1e-5)
+    else:
+        obj = {1: 1, 2: 4, 3: 5}.get(fid, 0)
+        assert_rows(got, ref, row_rtol(obj, (xs64 - o) * rate + o, o, M))
+
+
 def test_fp32_near_optimum(cuda, speo, oracle):
     # |x - o| ~ 1e-2, so f is small.  x and o are FP32 and close, so x - o is
     # exact in FP32 (Sterbenz); what remains is the GEMM and the transcendental
