@@ -233,7 +233,9 @@ SPEO_EXPORT int speo_load_cec_data(const char *dir, int func_id, int dim, int nc
 /* The problem of CEC2014 function `func_id` at `dim` from those files, with
  * the reference GPU path's objective, rate and rotation flag
  * (kernel_CEC2014_CUDA.cu:1099-1190): 1, 2, 3, 5, 8 (no rotation), 9, 10 (no
- * rotation), 11, 13, 14; others SPEO_ERR_UNSUPPORTED.  Values are the raw
+ * rotation), 11, 13, 14, and the compositions 24 and 25 (cf02 / cf03,
+ * CEC2014.cc:1235-1267: three components with the reference's sigma, bias,
+ * lambda and rates); others SPEO_ERR_UNSUPPORTED.  Values are the raw
  * objective (no 100 * func_id), as the reference's API_evaluateFitness. */
 SPEO_EXPORT int speo_problem_create_cec(const char *dir, int func_id, int dim, speo_problem *out);
 

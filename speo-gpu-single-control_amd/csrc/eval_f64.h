@@ -24,7 +24,7 @@ __device__ inline double eval_row_f64(const CompParams *comps, int ncomp, int di
     for (int c = 0; c < ncomp; ++c) {
         const CompParams cp = comps[c];
         __syncthreads();
-        for (int j = tid; j < dim; j += 256) a[j] = ((double)x[j] - cp.shift64[j]) * (double)cp.rate;
+        for (int j = tid; j < dim; j += 256) a[j] = ((double)x[j] - cp.shift64[j]) * cp.rate;
         __syncthreads();
         double s1 = 0.0, s2 = 0.0, dd = 0.0;
         for (int i = wave; i < dim; i += 4) {

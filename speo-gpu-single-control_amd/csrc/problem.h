@@ -20,7 +20,8 @@ void column_weights(int objective, int dim, float *w);
 
 struct CompHost {
     int objective = SPEO_OBJ_RASTRIGIN;
-    float rate = 1.0f;
+    float rate = 1.0f;     // FP32 kernels
+    double rate64 = 1.0;   // FP64 verifier (a rate such as 0.0512 is not a float)
     double sigma = 10.0, bias = 0.0, lambda = 1.0;
     float *shift32 = nullptr;   // [dim]
     double *shift64 = nullptr;  // [dim]
@@ -39,7 +40,7 @@ struct CompHost {
 // stored on the device as an array.
 struct CompParams {
     int objective;
-    float rate;
+    double rate;
     double sigma, bias, lambda;
     const float *shift32;
     const double *shift64;

@@ -7,6 +7,7 @@
 //   shift32/shift64, B32 = M^T (FP32 GEMM operand), BF16 and 2^14-scaled FP16
 //   hi/lo/lo2 splits of M (16-bit MFMA operands), M64 (FP64 re-verification),
 //   and the elliptic column weights.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -135,6 +136,7 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
             SPEO_TRY(upload(&h.colw, w));
         }
         h.rate = rate ? (float)rate[c] : 1.0f;
+        h.rate64 = rate ? rate[c] : 1.0;
         h.sigma = sigma ? sigma[c] : 10.0;
         h.bias = bias ? bias[c] : 0.0;
         h.lambda = lambda ? lambda[c] : 1.0;
@@ -197,7 +199,7 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
                 SPEO_TRY(upload(&h.Mhs[p], sh));
             }
         }
-        params[(size_t)c] = CompParams{h.objective, h.rate, h.sigma, h.bias, h.lambda,
+        params[(size_t)c] = CompParams{h.objective, h.rate64, h.sigma, h.bias, h.lambda,
                                        h.shift32, h.shift64, h.M64};
     }
     SPEO_TRY(upload(&P.params_dev, params));
@@ -357,6 +359,26 @@ int speo_load_cec_data(const char *dir, int func_id, int dim, int ncomp, double 
 int speo_problem_create_cec(const char *dir, int func_id, int dim, speo_problem *out) {
     if (!out) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: null output");
     *out = nullptr;
+    if (func_id == 24 || func_id == 25) {
+        // cf02 / cf03 (CEC2014.cc:1235-1267): three components over the data
+        // files' three shifts and matrices; F24's Schwefel is not rotated.
+        if (dim < 2) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: dim < 2");
+        const size_t D = (size_t)dim;
+        std::vector<double> o(3 * D), M(3 * D * D);
+        SPEO_TRY(speo_load_cec_data(dir, func_id, dim, 3, o.data(), M.data()));
+        const int objs[3] = {SPEO_OBJ_SCHWEFEL, SPEO_OBJ_RASTRIGIN,
+                             func_id == 24 ? SPEO_OBJ_HGBAT : SPEO_OBJ_ELLIPTIC};
+        const double rate[3] = {10.0, 0.0512, func_id == 24 ? 0.05 : 1.0};
+        const double bias[3] = {0.0, 100.0, 200.0};
+        const double sigma24[3] = {20.0, 20.0, 20.0}, sigma25[3] = {10.0, 30.0, 50.0};
+        const double lam24[3] = {1.0, 1.0, 1.0}, lam25[3] = {1000.0 / 4e3, 1000.0 / 1e3, 1000.0 / 1e10};
+        if (func_id == 24) {
+            std::fill(M.begin(), M.begin() + D * D, 0.0);
+            for (size_t i = 0; i < D; ++i) M[i * D + i] = 1.0;
+        }
+        return speo_problem_create(dim, 3, objs, o.data(), M.data(), func_id == 24 ? sigma24 : sigma25,
+                                   bias, func_id == 24 ? lam24 : lam25, rate, out);
+    }
     int obj = 0;
     double rate = 1.0;
     bool rotate = true;

@@ -17,6 +17,8 @@ Two data sources:
   "haar":    this project's seeded problems (o, M regenerated from the seed by
              the oracle's generator; only the seed is stored), written in the
              reference's text format (D = 10, 100).
+Compositions "comp": F24 (cf02) and F25 (cf03) on the reference's data files
+(D = 10, 30), at the points themselves.
 Points x are float32-representable, uniform in [-100, 100].
 """
 from __future__ import annotations
@@ -73,6 +75,19 @@ def main():
             key = f"haar_F{fid}_D{D}"
             out[key + "_x"], out[key + "_f"] = x, f
             out[key + "_seed"] = np.array(seed)
+            cases.append(key)
+    # Compositions F24 (cf02) and F25 (cf03) on the reference's own data files:
+    # three shift lines and three matrices; points taken as they are (each
+    # component applies its own rate).
+    for fid in (24, 25):
+        for D in (10, 30):
+            o = np.loadtxt(REF_BIN / "input_data" / f"shift_data_{fid}.txt")[:3, :D]
+            M = np.loadtxt(REF_BIN / "input_data" / f"M_{fid}_D{D}.txt").reshape(-1)[:3 * D * D]
+            M = M.reshape(3, D, D)
+            x = rng.uniform(-100, 100, (NPTS, D)).astype(np.float32).astype(np.float64)
+            f = O.ref_eval(REF_BIN, fid, x)
+            key = f"comp_F{fid}_D{D}"
+            out[key + "_x"], out[key + "_o"], out[key + "_M"], out[key + "_f"] = x, o, M, f
             cases.append(key)
     out["cases"] = np.array(cases)
     dst = Path(__file__).resolve().parent / "cec2014_ref_vectors.npz"

@@ -14,7 +14,8 @@ GOLDEN = Path(__file__).resolve().parent / "golden" / "cec2014_ref_vectors.npz"
 
 def _cases():
     d = np.load(GOLDEN)
-    return [str(c) for c in d["cases"]]
+    # the compositions ("comp_F24", "comp_F25") are checked end to end on the GPU
+    return [str(c) for c in d["cases"] if not str(c).startswith("comp")]
 
 
 @pytest.mark.parametrize("case", _cases())
