@@ -89,6 +89,9 @@ def lib():
                                       _f64p, ctypes.c_int, _f64p]
         L.oracle_composition.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p,
                                          _f64p, _f64p, _f64p, _f64p, _f64p, ctypes.c_int, _f64p]
+        L.oracle_composition_rated.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                               ctypes.c_int, _f64p]
         L.oracle_migrate_ring.argtypes = [_f32p, _f32p, ctypes.c_int, _i32p, _i32p, _i32p,
                                           ctypes.c_int, ctypes.c_int]
         _lib = L
@@ -267,19 +270,21 @@ def evaluate(objective, x, o, M):
     return out
 
 
-def composition(x, objectives, os_, Ms, sigma, bias, lam=None):
+def composition(x, objectives, os_, Ms, sigma, bias, lam=None, rate=None):
     x2 = np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1, np.asarray(x).shape[-1]))
     nc = len(objectives)
     D = os_.shape[1]
     lam = np.ones(nc) if lam is None else np.asarray(lam, np.float64)
+    rate = np.ones(nc) if rate is None else np.asarray(rate, np.float64)
     out = np.zeros(x2.shape[0], np.float64)
-    lib().oracle_composition(x2, x2.shape[1], x2.shape[0], nc,
-                             np.ascontiguousarray(objectives, np.int32),
-                             np.ascontiguousarray(os_, np.float64),
-                             np.ascontiguousarray(Ms, np.float64),
-                             np.ascontiguousarray(sigma, np.float64),
-                             np.ascontiguousarray(bias, np.float64),
-                             np.ascontiguousarray(lam, np.float64), D, out)
+    lib().oracle_composition_rated(x2, x2.shape[1], x2.shape[0], nc,
+                                   np.ascontiguousarray(objectives, np.int32),
+                                   np.ascontiguousarray(os_, np.float64),
+                                   np.ascontiguousarray(Ms, np.float64),
+                                   np.ascontiguousarray(sigma, np.float64),
+                                   np.ascontiguousarray(bias, np.float64),
+                                   np.ascontiguousarray(lam, np.float64),
+                                   np.ascontiguousarray(rate, np.float64), D, out)
     return out
 
 

@@ -404,10 +404,13 @@ ORACLE_EXPORT void oracle_eval_f32(int objective, const float *x, int ldx, int n
  * (w_i = 1e99 when d_i == 0), all-zero weights fall back to uniform, and
  * f = sum_i w_i / sum_j w_j * fit_i.  os: [ncomp][D], Ms: [ncomp][D][D].
  */
-ORACLE_EXPORT void oracle_composition(const float *x, int ldx, int n, int ncomp,
-                                      const int *objectives, const double *os, const double *Ms,
-                                      const double *sigma, const double *bias,
-                                      const double *lambda, int D, double *f_out) {
+/* Same, with each component's z = M_c ((x - o_c) rate_c), the order of
+ * sr_func (CEC2014.cc:1373-1393); rate NULL = 1. */
+ORACLE_EXPORT void oracle_composition_rated(const float *x, int ldx, int n, int ncomp,
+                                            const int *objectives, const double *os,
+                                            const double *Ms, const double *sigma,
+                                            const double *bias, const double *lambda,
+                                            const double *rate, int D, double *f_out) {
     double *xd = (double *)malloc(sizeof(double) * (size_t)D * 3);
     double *z = xd + D, *tmp = xd + 2 * D;
     double w[16], fit[16];
@@ -416,7 +419,14 @@ ORACLE_EXPORT void oracle_composition(const float *x, int ldx, int n, int ncomp,
         double wmax = 0.0, wsum = 0.0;
         for (int c = 0; c < ncomp; ++c) {
             const double *o = os + (size_t)c * D;
-            oracle_shift_rotate(xd, o, Ms + (size_t)c * D * D, D, z, tmp);
+            const double *M = Ms + (size_t)c * D * D;
+            const double r = rate ? rate[c] : 1.0;
+            for (int j = 0; j < D; ++j) tmp[j] = (xd[j] - o[j]) * r;
+            for (int i = 0; i < D; ++i) {
+                double acc = 0.0;
+                for (int j = 0; j < D; ++j) acc = acc + tmp[j] * M[(size_t)i * D + j];
+                z[i] = acc;
+            }
             fit[c] = lambda[c] * oracle_basic(objectives[c], z, D) + bias[c];
             double d = 0.0;
             for (int j = 0; j < D; ++j) d += pow(xd[j] - o[j], 2.0);
@@ -436,6 +446,14 @@ ORACLE_EXPORT void oracle_composition(const float *x, int ldx, int n, int ncomp,
         f_out[p] = fv;
     }
     free(xd);
+}
+
+ORACLE_EXPORT void oracle_composition(const float *x, int ldx, int n, int ncomp,
+                                      const int *objectives, const double *os, const double *Ms,
+                                      const double *sigma, const double *bias,
+                                      const double *lambda, int D, double *f_out) {
+    oracle_composition_rated(x, ldx, n, ncomp, objectives, os, Ms, sigma, bias, lambda, NULL, D,
+                             f_out);
 }
 
 /* ------------------------------------------------------- island driver */

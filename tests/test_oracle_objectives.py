@@ -169,3 +169,24 @@ def test_composition_special_cases(oracle):
                      for c in range(3)])
     f = oracle.composition(x, objs, os_, Ms, sigma, bias)[0]
     assert f == pytest.approx(float((w / w.sum() * fits).sum()), rel=1e-12)
+
+
+@pytest.mark.parametrize("fid", [24, 25])
+def test_composition_against_reference_cf02_cf03(oracle, fid):
+    # the oracle's composition with per-component rates and lambdas against
+    # the reference's cf02 / cf03 (CEC2014.cc:1235-1267) on its own F24 / F25
+    # data files: F24 = unrotated Schwefel (rate 10), Rastrigin (5.12/100),
+    # HGBat (5/100), sigma 20; F25 = Schwefel, Rastrigin, elliptic with
+    # lambda 1000/4e3, 1000/1e3, 1000/1e10, sigma 10, 30, 50.
+    d = np.load(GOLDEN)
+    for D in (10, 30):
+        case = f"comp_F{fid}_D{D}"
+        x, o, M = d[case + "_x"], d[case + "_o"], d[case + "_M"]
+        if fid == 24:
+            objs, rate, lam, sig = [8, 2, 7], [10.0, 0.0512, 0.05], [1.0, 1.0, 1.0], [20.0] * 3
+            M = M.copy()
+            M[0] = np.eye(D)
+        else:
+            objs, rate, lam, sig = [8, 2, 1], [10.0, 0.0512, 1.0], [0.25, 1.0, 1e-7], [10.0, 30.0, 50.0]
+        f = oracle.composition(x, objs, o, M, sig, [0.0, 100.0, 200.0], lam, rate)
+        np.testing.assert_allclose(f, d[case + "_f"], rtol=1e-12)
