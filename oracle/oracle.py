@@ -30,6 +30,8 @@ REF_DIR = HERE / "_ref"
 
 OBJ_ELLIPTIC, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_BENT_CIGAR, OBJ_DISCUS = 1, 2, 3, 4, 5
 OBJ_HAPPYCAT, OBJ_HGBAT, OBJ_SCHWEFEL = 6, 7, 8
+OBJ_ROSENBROCK, OBJ_WEIERSTRASS, OBJ_GRIEWANK, OBJ_KATSUURA = 9, 10, 11, 12
+OBJ_GRIE_ROSEN, OBJ_ESCAFFER6 = 13, 14
 PURPOSE_DONOR, PURPOSE_CROSS, PURPOSE_INIT = 0, 1, 2
 PURPOSE_SHIFT, PURPOSE_GAUSS, PURPOSE_PROBLEM = 16, 17, 18
 
@@ -92,6 +94,13 @@ def lib():
         L.oracle_composition_rated.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                                ctypes.c_int, _f64p]
+        L.oracle_cec.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 _f64p, _f64p, ctypes.c_void_p, _f64p]
+        L.oracle_cec.restype = ctypes.c_int
+        L.oracle_cec_ncomp.argtypes = [ctypes.c_int]
+        L.oracle_cec_ncomp.restype = ctypes.c_int
+        L.oracle_hybrid_segments.argtypes = [ctypes.c_int, ctypes.c_int, _i32p]
+        L.oracle_hybrid_segments.restype = ctypes.c_int
         L.oracle_migrate_ring.argtypes = [_f32p, _f32p, ctypes.c_int, _i32p, _i32p, _i32p,
                                           ctypes.c_int, ctypes.c_int]
         _lib = L
@@ -288,6 +297,57 @@ def composition(x, objectives, os_, Ms, sigma, bias, lam=None, rate=None):
     return out
 
 
+def cec_ncomp(func_id: int) -> int:
+    return lib().oracle_cec_ncomp(func_id)
+
+
+def is_hybrid(func_id: int) -> bool:
+    return 17 <= func_id <= 22 or func_id in (29, 30)
+
+
+def hybrid_segments(h: int, D: int):
+    out = np.zeros(5, np.int32)
+    n = lib().oracle_hybrid_segments(h, D, out)
+    return out[:n].tolist()
+
+
+def cec(func_id, x, os_, Ms, SS=None):
+    """CEC2014 F<func_id> (1..30) in FP64 at the rows of x, as the reference's
+    CEC2014::EvaluateFitness minus its 100 * func_id.  os_ [ncomp][D],
+    Ms [ncomp][D][D], SS [ncomp][D] 1-based shuffle (hybrids only)."""
+    x2 = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1, np.asarray(x).shape[-1]))
+    D = x2.shape[1]
+    nc = cec_ncomp(func_id)
+    os_ = np.ascontiguousarray(np.asarray(os_, np.float64).reshape(nc, D))
+    Ms = np.ascontiguousarray(np.asarray(Ms, np.float64).reshape(nc, D, D))
+    ss = None
+    if is_hybrid(func_id):
+        ss = np.ascontiguousarray(np.asarray(SS, np.int32).reshape(nc, D))
+    out = np.zeros(x2.shape[0], np.float64)
+    rc = lib().oracle_cec(func_id, x2, D, x2.shape[0], D, os_, Ms, _ptr(ss), out)
+    assert rc == 0
+    return out
+
+
+def gen_shuffle(seed: int, D: int, ncomp: int = 1) -> np.ndarray:
+    """Seeded 1-based permutations of 1..D, one per component (the format of
+    the reference's shuffle_data files)."""
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.permutation(D) + 1 for _ in range(ncomp)]).astype(np.int32)
+
+
+def gen_cec_problem(seed: int, func_id: int, D: int):
+    """Seeded data of F<func_id> at dimension D: shifts, Haar rotations and
+    (hybrids) shuffles, one per component."""
+    nc = cec_ncomp(func_id)
+    os_ = np.zeros((nc, D))
+    Ms = np.zeros((nc, D, D))
+    for c in range(nc):
+        os_[c], Ms[c] = gen_problem(seed * 16 + c, D)
+    SS = gen_shuffle(seed, D, nc) if is_hybrid(func_id) else None
+    return os_, Ms, SS
+
+
 def migrate_ring(X, fX, ld, off, dims, dst, k):
     X = np.array(X, np.float32, copy=True)
     fX = np.array(fX, np.float32, copy=True)
@@ -306,13 +366,20 @@ REF_FUNC = {OBJ_ELLIPTIC: 1, OBJ_BENT_CIGAR: 2, OBJ_DISCUS: 3, OBJ_ACKLEY: 5, OB
 REF_RATE = {1: 1.0, 2: 1.0, 3: 1.0, 5: 1.0, 9: 0.0512, 11: 10.0, 13: 0.05, 14: 0.05}
 
 
-def write_cec_files(directory, func_id, o, M):
-    """Write o and M in the reference's text format (CEC2014.cc:353-420)."""
+def write_cec_files(directory, func_id, o, M, SS=None):
+    """Write o, M (and the shuffle SS) in the reference's text format
+    (CEC2014.cc:353-460): one shift line per component, ncomp * D * D matrix
+    values, ncomp * D 1-based shuffle indices."""
     d = Path(directory) / "input_data"
     d.mkdir(parents=True, exist_ok=True)
-    np.savetxt(d / f"shift_data_{func_id}.txt", np.asarray(o, np.float64)[None, :],
+    o = np.asarray(o, np.float64)
+    D = o.shape[-1]
+    np.savetxt(d / f"shift_data_{func_id}.txt", o.reshape(-1, D), fmt="%.17e")
+    np.savetxt(d / f"M_{func_id}_D{D}.txt", np.asarray(M, np.float64).reshape(-1, D),
                fmt="%.17e")
-    np.savetxt(d / f"M_{func_id}_D{o.size}.txt", np.asarray(M, np.float64), fmt="%.17e")
+    if SS is not None:
+        np.savetxt(d / f"shuffle_data_{func_id}_D{D}.txt",
+                   np.asarray(SS, np.int64).reshape(1, -1), fmt="%d")
 
 
 def ref_binary(opt="O2") -> Path:

@@ -25,6 +25,15 @@
  *       Schwefel (modified)                 CUDA_EA_library/CEC2014.cc:705-738
  *       HappyCat                            CUDA_EA_library/CEC2014.cc:907-929
  *       HGBat                               CUDA_EA_library/CEC2014.cc:931-953
+ *       Rosenbrock                          CUDA_EA_library/CEC2014.cc:552-568
+ *       Weierstrass                         CUDA_EA_library/CEC2014.cc:615-644
+ *       Griewank                            CUDA_EA_library/CEC2014.cc:646-666
+ *       Katsuura                            CUDA_EA_library/CEC2014.cc:739-765
+ *       Griewank-Rosenbrock                 CUDA_EA_library/CEC2014.cc:842-868
+ *       expanded Schaffer F6                CUDA_EA_library/CEC2014.cc:870-905
+ *       hybrid functions hf01-hf06          CUDA_EA_library/CEC2014.cc:954-1207
+ *       compositions cf01-cf08              CUDA_EA_library/CEC2014.cc:1209-1371
+ *       F1-F30 dispatch                     CUDA_EA_library/CEC2014.cc:14-145
  *       composition weights (cf_cal)        CUDA_EA_library/CEC2014.cc:1481-1519
  *   - Philox4x32-10 (Salmon et al., "Parallel random numbers: as easy as
  *     1, 2, 3", SC'11), replacing cuRAND XORWOW, which is closed and is not
@@ -256,7 +265,13 @@ enum {
     OBJ_DISCUS = 5,
     OBJ_HAPPYCAT = 6,
     OBJ_HGBAT = 7,
-    OBJ_SCHWEFEL = 8
+    OBJ_SCHWEFEL = 8,
+    OBJ_ROSENBROCK = 9,
+    OBJ_WEIERSTRASS = 10,
+    OBJ_GRIEWANK = 11,
+    OBJ_KATSUURA = 12,
+    OBJ_GRIE_ROSEN = 13,
+    OBJ_ESCAFFER6 = 14
 };
 
 static const double ORACLE_PI = 3.1415926535897932384626433832795029;
@@ -361,8 +376,104 @@ ORACLE_EXPORT double oracle_schwefel(const double *z, int D) {
     return f + 4.189828872724338e+002 * D;
 }
 
+/* CEC2014.cc:552-568 with the fixed shift of 1 its callers pass (F4 at
+ * :32, hf03 / hf05 at :1066, :1151, cf01 at :1217): y = z + 1, then
+ * sum_{i < D-1} 100 (y_i^2 - y_{i+1})^2 + (y_i - 1)^2. */
+ORACLE_EXPORT double oracle_rosenbrock(const double *z, int D) {
+    double f = 0.0;
+    for (int i = 0; i < D - 1; ++i) {
+        const double y0 = z[i] + 1.0, y1 = z[i + 1] + 1.0;
+        const double t1 = y0 * y0 - y1, t2 = y0 - 1.0;
+        f += 100.0 * t1 * t1 + t2 * t2;
+    }
+    return f;
+}
+
+/* CEC2014.cc:615-644: a = 0.5, b = 3, k_max = 20;
+ * sum_i sum_k a^k cos(2 pi b^k (z_i + 0.5)) - D sum_k a^k cos(2 pi b^k 0.5),
+ * each term formed as pow(a, k) * cos(2.0 * PI * pow(b, k) * (z + 0.5)). */
+ORACLE_EXPORT double oracle_weierstrass(const double *z, int D) {
+    double f = 0.0, c = 0.0;
+    for (int k = 0; k <= 20; ++k) c += pow(0.5, k) * cos(2.0 * ORACLE_PI * pow(3.0, k) * 0.5);
+    for (int i = 0; i < D; ++i) {
+        double s = 0.0;
+        for (int k = 0; k <= 20; ++k)
+            s += pow(0.5, k) * cos(2.0 * ORACLE_PI * pow(3.0, k) * (z[i] + 0.5));
+        f += s;
+    }
+    return f - D * c;
+}
+
+/* CEC2014.cc:646-666: 1 + sum z_i^2 / 4000 - prod cos(z_i / sqrt(i + 1)). */
+ORACLE_EXPORT double oracle_griewank(const double *z, int D) {
+    double s = 0.0, p = 1.0;
+    for (int i = 0; i < D; ++i) {
+        s += z[i] * z[i];
+        p *= cos(z[i] / sqrt(1.0 + i));
+    }
+    return 1.0 + s / 4000.0 - p;
+}
+
+/* CEC2014.cc:739-765: t_i = sum_{j=1..32} |2^j z_i - floor(2^j z_i + 1/2)| / 2^j,
+ * f = 10/D^2 prod_i (1 + (i + 1) t_i)^(10 / D^1.2) - 10/D^2. */
+ORACLE_EXPORT double oracle_katsuura(const double *z, int D) {
+    double f = 1.0;
+    const double e = 10.0 / pow(1.0 * D, 1.2);
+    for (int i = 0; i < D; ++i) {
+        double t = 0.0;
+        for (int j = 1; j <= 32; ++j) {
+            const double s = pow(2.0, j), v = s * z[i];
+            t += fabs(v - floor(v + 0.5)) / s;
+        }
+        f *= pow(1.0 + (i + 1) * t, e);
+    }
+    const double c = 10.0 / D / D;
+    return f * c - c;
+}
+
+/* Griewank of a Rosenbrock term, t -> t^2 / 4000 - cos t + 1. */
+static double grie_of_rosen(double a, double b) {
+    const double t1 = a * a - b, t2 = a - 1.0;
+    const double t = 100.0 * t1 * t1 + t2 * t2;
+    return (t * t) / 4000.0 - cos(t) + 1.0;
+}
+
+/* CEC2014.cc:842-868 with the fixed shift of 1 of its callers (F15 at :76,
+ * hf04 / hf06 at :1109, :1196, cf06 at :1324): y = z + 1, the pairs
+ * (y_i, y_{i+1}) for i < D-1, then the wrap-around pair (y_{D-1}, y_0). */
+ORACLE_EXPORT double oracle_grie_rosen(const double *z, int D) {
+    double f = 0.0;
+    for (int i = 0; i < D - 1; ++i) f += grie_of_rosen(z[i] + 1.0, z[i + 1] + 1.0);
+    f += grie_of_rosen(z[D - 1] + 1.0, z[0] + 1.0);
+    return f;
+}
+
+/* One Schaffer F6 term of r2 = a^2 + b^2. */
+static double scaffer6(double r2) {
+    double t1 = sin(sqrt(r2));
+    t1 = t1 * t1;
+    const double t2 = 1.0 + 0.001 * r2;
+    return 0.5 + (t1 - 0.5) / (t2 * t2);
+}
+
+/* CEC2014.cc:870-905: pairs (z_i, z_{i+1}) for i < D-1 and the wrap-around
+ * pair (z_{D-1}, z_0); a single coordinate gives the term of z_0^2. */
+ORACLE_EXPORT double oracle_escaffer6(const double *z, int D) {
+    if (D == 1) return scaffer6(z[0] * z[0]);
+    double f = 0.0;
+    for (int i = 0; i < D - 1; ++i) f += scaffer6(z[i] * z[i] + z[i + 1] * z[i + 1]);
+    f += scaffer6(z[D - 1] * z[D - 1] + z[0] * z[0]);
+    return f;
+}
+
 ORACLE_EXPORT double oracle_basic(int objective, const double *z, int D) {
     switch (objective) {
+    case OBJ_ROSENBROCK: return oracle_rosenbrock(z, D);
+    case OBJ_WEIERSTRASS: return oracle_weierstrass(z, D);
+    case OBJ_GRIEWANK: return oracle_griewank(z, D);
+    case OBJ_KATSUURA: return oracle_katsuura(z, D);
+    case OBJ_GRIE_ROSEN: return oracle_grie_rosen(z, D);
+    case OBJ_ESCAFFER6: return oracle_escaffer6(z, D);
     case OBJ_ELLIPTIC: return oracle_elliptic(z, D);
     case OBJ_BENT_CIGAR: return oracle_bent_cigar(z, D);
     case OBJ_DISCUS: return oracle_discus(z, D);
@@ -454,6 +565,218 @@ ORACLE_EXPORT void oracle_composition(const float *x, int ldx, int n, int ncomp,
                                       const double *lambda, int D, double *f_out) {
     oracle_composition_rated(x, ldx, n, ncomp, objectives, os, Ms, sigma, bias, lambda, NULL, D,
                              f_out);
+}
+
+/* ------------------------------------------- CEC2014 F1-F30 (CEC2014.cc) */
+
+/* sr_func (CEC2014.cc:1395-1435): y = (x - o) * rate, then z = M y when
+ * rotated (rotatefunc, :1382-1393, z_i = sum_j y_j M[i D + j]). */
+static void sr(const double *x, const double *o, const double *M, double rate, int rotate, int D,
+               double *z, double *tmp) {
+    for (int j = 0; j < D; ++j) tmp[j] = (x[j] - o[j]) * rate;
+    if (!rotate) {
+        for (int j = 0; j < D; ++j) z[j] = tmp[j];
+        return;
+    }
+    for (int i = 0; i < D; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < D; ++j) acc = acc + tmp[j] * M[(size_t)i * D + j];
+        z[i] = acc;
+    }
+}
+
+/* The six hybrid functions hf01-hf06 (CEC2014.cc:954-1207): fractions of
+ * the dimension, each segment's objective and the rate it scales its
+ * segment by (sr_func with s_flag = r_flag = 0 inside each segment). */
+typedef struct {
+    int nseg;
+    double p[5];
+    int obj[5];
+    double rate[5];
+} hybrid_def;
+
+static const hybrid_def HYBRID[6] = {
+    {3, {0.3, 0.3, 0.4}, {OBJ_SCHWEFEL, OBJ_RASTRIGIN, OBJ_ELLIPTIC},
+     {1000.0 / 100.0, 5.12 / 100.0, 1.0}},                                    /* :954-991  */
+    {3, {0.3, 0.3, 0.4}, {OBJ_BENT_CIGAR, OBJ_HGBAT, OBJ_RASTRIGIN},
+     {1.0, 5.0 / 100.0, 5.12 / 100.0}},                                       /* :993-1032 */
+    {4, {0.2, 0.2, 0.3, 0.3}, {OBJ_GRIEWANK, OBJ_WEIERSTRASS, OBJ_ROSENBROCK, OBJ_ESCAFFER6},
+     {600.0 / 100.0, 0.5 / 100.0, 2.048 / 100.0, 1.0}},                       /* :1034-1075 */
+    {4, {0.2, 0.2, 0.3, 0.3}, {OBJ_HGBAT, OBJ_DISCUS, OBJ_GRIE_ROSEN, OBJ_RASTRIGIN},
+     {5.0 / 100.0, 1.0, 5.0 / 100.0, 5.12 / 100.0}},                          /* :1077-1117 */
+    {5, {0.1, 0.2, 0.2, 0.2, 0.3},
+     {OBJ_ESCAFFER6, OBJ_HGBAT, OBJ_ROSENBROCK, OBJ_SCHWEFEL, OBJ_ELLIPTIC},
+     {1.0, 5.0 / 100.0, 2.048 / 100.0, 1000.0 / 100.0, 1.0}},                 /* :1119-1162 */
+    {5, {0.1, 0.2, 0.2, 0.2, 0.2},
+     {OBJ_KATSUURA, OBJ_HAPPYCAT, OBJ_GRIE_ROSEN, OBJ_SCHWEFEL, OBJ_ACKLEY},
+     {5.0 / 100.0, 5.0 / 100.0, 5.0 / 100.0, 1000.0 / 100.0, 1.0}},           /* :1164-1207 */
+};
+
+/* Segment lengths of a hybrid: ceil(p_i D) in FP64 for all but the last,
+ * which takes the rest (CEC2014.cc:961-967). */
+ORACLE_EXPORT int oracle_hybrid_segments(int h, int D, int *len_out) {
+    if (h < 1 || h > 6) return -1;
+    const hybrid_def *H = &HYBRID[h - 1];
+    int used = 0;
+    for (int i = 0; i < H->nseg - 1; ++i) {
+        len_out[i] = (int)ceil(H->p[i] * D);
+        used += len_out[i];
+    }
+    len_out[H->nseg - 1] = D - used;
+    return H->nseg;
+}
+
+/* Hybrid h (1..6) at x: z = M (x - o) (rate 1), y_i = z[S_i - 1] (S is
+ * 1-based, as in the shuffle_data files), then the segments' objectives of
+ * y_seg * rate_seg, summed (CEC2014.cc:970-990). */
+static double hybrid(int h, const double *x, const double *o, const double *M, const int *S,
+                     int D, double *z, double *tmp) {
+    const hybrid_def *H = &HYBRID[h - 1];
+    int len[5];
+    oracle_hybrid_segments(h, D, len);
+    sr(x, o, M, 1.0, 1, D, z, tmp);
+    for (int i = 0; i < D; ++i) tmp[i] = z[S[i] - 1];
+    double f = 0.0;
+    int start = 0;
+    for (int s = 0; s < H->nseg; ++s) {
+        for (int i = 0; i < len[s]; ++i) z[i] = tmp[start + i] * H->rate[s];
+        f += oracle_basic(H->obj[s], z, len[s]);
+        start += len[s];
+    }
+    return f;
+}
+
+/* cf_cal (CEC2014.cc:1481-1519) over fit[] (bias added first). */
+static double cf_combine(const double *x, const double *os, int D, const double *delta,
+                         const double *bias, double *fit, int n) {
+    double w[8], wmax = 0.0, wsum = 0.0;
+    for (int c = 0; c < n; ++c) {
+        fit[c] += bias[c];
+        double d = 0.0;
+        for (int j = 0; j < D; ++j) d += pow(x[j] - os[(size_t)c * D + j], 2.0);
+        w[c] = d != 0.0 ? pow(1.0 / d, 0.5) * exp(-d / 2.0 / D / pow(delta[c], 2.0)) : 1.0e99;
+        if (w[c] > wmax) wmax = w[c];
+    }
+    for (int c = 0; c < n; ++c) wsum += w[c];
+    if (wmax == 0.0) {
+        for (int c = 0; c < n; ++c) w[c] = 1.0;
+        wsum = n;
+    }
+    double f = 0.0;
+    for (int c = 0; c < n; ++c) f += w[c] / wsum * fit[c];
+    return f;
+}
+
+/* Components of the compositions cf01-cf06 (CEC2014.cc:1209-1339): objective,
+ * rate, rotation flag, and the scaling num * g / den applied before cf_cal. */
+typedef struct {
+    int obj;
+    double rate;
+    int rotate;
+    double num, den;
+} comp_def;
+
+typedef struct {
+    int n;
+    comp_def c[5];
+    double delta[5], bias[5];
+} cf_def;
+
+static const cf_def CF[6] = {
+    {5,
+     {{OBJ_ROSENBROCK, 2.048 / 100.0, 1, 10000, 1e+4}, {OBJ_ELLIPTIC, 1, 1, 10000, 1e+10},
+      {OBJ_BENT_CIGAR, 1, 1, 10000, 1e+30}, {OBJ_DISCUS, 1, 1, 10000, 1e+10},
+      {OBJ_ELLIPTIC, 1, 0, 10000, 1e+10}},
+     {10, 20, 30, 40, 50}, {0, 100, 200, 300, 400}},                         /* cf01 :1209 */
+    {3,
+     {{OBJ_SCHWEFEL, 1000.0 / 100.0, 0, 1, 1}, {OBJ_RASTRIGIN, 5.12 / 100.0, 1, 1, 1},
+      {OBJ_HGBAT, 5.0 / 100.0, 1, 1, 1}},
+     {20, 20, 20}, {0, 100, 200}},                                            /* cf02 :1235 */
+    {3,
+     {{OBJ_SCHWEFEL, 1000.0 / 100.0, 1, 1000, 4e+3}, {OBJ_RASTRIGIN, 5.12 / 100.0, 1, 1000, 1e+3},
+      {OBJ_ELLIPTIC, 1, 1, 1000, 1e+10}},
+     {10, 30, 50}, {0, 100, 200}},                                            /* cf03 :1251 */
+    {5,
+     {{OBJ_SCHWEFEL, 1000.0 / 100.0, 1, 1000, 4e+3}, {OBJ_HAPPYCAT, 5.0 / 100.0, 1, 1000, 1e+3},
+      {OBJ_ELLIPTIC, 1, 1, 1000, 1e+10}, {OBJ_WEIERSTRASS, 0.5 / 100.0, 1, 1000, 400},
+      {OBJ_GRIEWANK, 600.0 / 100.0, 1, 1000, 100}},
+     {10, 10, 10, 10, 10}, {0, 100, 200, 300, 400}},                         /* cf04 :1269 */
+    {5,
+     {{OBJ_HGBAT, 5.0 / 100.0, 1, 10000, 1000}, {OBJ_RASTRIGIN, 5.12 / 100.0, 1, 10000, 1e+3},
+      {OBJ_SCHWEFEL, 1000.0 / 100.0, 1, 10000, 4e+3},
+      {OBJ_WEIERSTRASS, 0.5 / 100.0, 1, 10000, 400}, {OBJ_ELLIPTIC, 1, 1, 10000, 1e+10}},
+     {10, 10, 10, 20, 20}, {0, 100, 200, 300, 400}},                         /* cf05 :1293 */
+    {5,
+     {{OBJ_GRIE_ROSEN, 5.0 / 100.0, 1, 10000, 4e+3}, {OBJ_HAPPYCAT, 5.0 / 100.0, 1, 10000, 1e+3},
+      {OBJ_SCHWEFEL, 1000.0 / 100.0, 1, 10000, 4e+3}, {OBJ_ESCAFFER6, 1, 1, 10000, 2e+7},
+      {OBJ_ELLIPTIC, 1, 1, 10000, 1e+10}},
+     {10, 20, 30, 40, 50}, {0, 100, 200, 300, 400}},                         /* cf06 :1317 */
+};
+
+/* CEC2014 function func_id (1..30) as CEC2014::EvaluateFitness
+ * (CEC2014.cc:14-145) without the 100 * func_id it adds and removes:
+ *   F1-F16   the basic functions with the rates and rotation flags of :17-79;
+ *   F17-F22  hf01-hf06 (:80-97);   F23-F28  cf01-cf06 (:98-115);
+ *   F29, F30 cf07 / cf08 (:116-121): hf01-03 / hf04-06 per component, each on
+ *            its own o, M and shuffle, weighted by cf_cal with delta {10, 30, 50}
+ *            and bias {0, 100, 200} (:1341-1371).
+ * os [ncomp][D], Ms [ncomp][D][D], SS [ncomp][D] (1-based; NULL unless
+ * F17-F22, F29, F30).  Returns 0, or -1 for an unknown func_id. */
+ORACLE_EXPORT int oracle_cec(int func_id, const double *x, int ldx, int n, int D,
+                             const double *os, const double *Ms, const int *SS, double *f_out) {
+    static const struct {
+        int obj;
+        double rate;
+        int rotate;
+    } BASIC[16] = {{OBJ_ELLIPTIC, 1, 1},           {OBJ_BENT_CIGAR, 1, 1},
+                   {OBJ_DISCUS, 1, 1},             {OBJ_ROSENBROCK, 2.048 / (double)100, 1},
+                   {OBJ_ACKLEY, 1, 1},             {OBJ_WEIERSTRASS, 0.5 / (double)100, 1},
+                   {OBJ_GRIEWANK, 6, 1},           {OBJ_RASTRIGIN, 5.12 / (double)100, 0},
+                   {OBJ_RASTRIGIN, 5.12 / (double)100, 1}, {OBJ_SCHWEFEL, 10, 0},
+                   {OBJ_SCHWEFEL, 10, 1},          {OBJ_KATSUURA, 5 / (double)100, 1},
+                   {OBJ_HAPPYCAT, 5 / (double)100, 1}, {OBJ_HGBAT, 5 / (double)100, 1},
+                   {OBJ_GRIE_ROSEN, 5 / (double)100, 1}, {OBJ_ESCAFFER6, 1, 1}};
+    if (func_id < 1 || func_id > 30) return -1;
+    double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)D);
+    double *z = buf, *tmp = buf + D;
+    for (int p = 0; p < n; ++p) {
+        const double *x_p = x + (size_t)p * ldx;
+        double f;
+        if (func_id <= 16) {
+            sr(x_p, os, Ms, BASIC[func_id - 1].rate, BASIC[func_id - 1].rotate, D, z, tmp);
+            f = oracle_basic(BASIC[func_id - 1].obj, z, D);
+        } else if (func_id <= 22) {
+            f = hybrid(func_id - 16, x_p, os, Ms, SS, D, z, tmp);
+        } else if (func_id <= 28) {
+            const cf_def *C = &CF[func_id - 23];
+            double fit[5];
+            for (int c = 0; c < C->n; ++c) {
+                sr(x_p, os + (size_t)c * D, Ms + (size_t)c * D * D, C->c[c].rate, C->c[c].rotate,
+                   D, z, tmp);
+                fit[c] = C->c[c].num * oracle_basic(C->c[c].obj, z, D) / C->c[c].den;
+            }
+            f = cf_combine(x_p, os, D, C->delta, C->bias, fit, C->n);
+        } else {
+            static const double delta[3] = {10, 30, 50}, bias[3] = {0, 100, 200};
+            const int h0 = func_id == 29 ? 1 : 4;
+            double fit[3];
+            for (int c = 0; c < 3; ++c)
+                fit[c] = hybrid(h0 + c, x_p, os + (size_t)c * D, Ms + (size_t)c * D * D,
+                                SS + (size_t)c * D, D, z, tmp);
+            f = cf_combine(x_p, os, D, delta, bias, fit, 3);
+        }
+        f_out[p] = f;
+    }
+    free(buf);
+    return 0;
+}
+
+/* Components per function: 5 for F23, F26-F28; 3 for F24, F25, F29, F30;
+ * else 1 (CEC2014.cc:304-330). */
+ORACLE_EXPORT int oracle_cec_ncomp(int func_id) {
+    if (func_id == 23 || (func_id >= 26 && func_id <= 28)) return 5;
+    if (func_id == 24 || func_id == 25 || func_id == 29 || func_id == 30) return 3;
+    return 1;
 }
 
 /* ------------------------------------------------------- island driver */

@@ -412,8 +412,17 @@ __device__ __forceinline__ void gemm_chunk(const CompDev &c, RowSource src, int 
     for (int t = 0; t < TPW; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     const int ncols = min(kNC, dim - n0);
-    // rows aligned for the vector staging (uniform over the block)
-    const bool vec = VS && (src.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(src.rows) & 15) == 0;
+    // rows and B operand aligned for the vector staging (uniform over the
+    // block): FP32 reads B32 rows as float4 (stride a multiple of 4 floats),
+    // the 16-bit forms read M16 rows as groups of 8 (stride a multiple of 8).
+    // The drop-in evaluation hands the caller's B with a row stride of dim,
+    // which at dim = 10, 30, 50 takes the element-wise form.
+    const bool b_vec = Prec<PREC>::parts == 0
+                           ? (c.ldb32 & 3) == 0 && (reinterpret_cast<uintptr_t>(c.B32) & 15) == 0
+                           : (c.ldm16 & 7) == 0 && (reinterpret_cast<uintptr_t>(c.M16[0]) & 15) == 0 &&
+                                 (Prec<PREC>::parts < 2 || (reinterpret_cast<uintptr_t>(c.M16[1]) & 15) == 0) &&
+                                 (Prec<PREC>::parts < 3 || (reinterpret_cast<uintptr_t>(c.M16[2]) & 15) == 0);
+    const bool vec = VS && b_vec && (src.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(src.rows) & 15) == 0;
 
     if constexpr (Prec<PREC>::parts == 0) {
         float *As = reinterpret_cast<float *>(lds);  // [RT][kKC+2]
