@@ -264,28 +264,121 @@ def precision_study(S, n_points=10000, D=1000, reps=5):
     return out
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def host_cores() -> int:
+    """Cores this process may use, capped at 16 (the GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def ref_time_parallel(O, td, fids, x, seconds, procs):
+    """Evals/s of `procs` concurrent processes of the reference's
+    CEC2014::EvaluateFitness (objectives round-robin), summed."""
+    import subprocess
+    xs = np.ascontiguousarray(x, np.float64).reshape(-1).tobytes()
+    running = []
+    for k in range(procs):
+        fid = fids[k % len(fids)]
+        pr = subprocess.Popen([str(O.ref_binary("O2")), "time", str(fid), str(x.size), str(seconds)],
+                              stdin=subprocess.PIPE, stdout=subprocess.PIPE, cwd=td)
+        pr.stdin.write(xs)
+        pr.stdin.close()
+        running.append(pr)
+    total = 0.0
+    for pr in running:
+        evals, el, _ = pr.stdout.read().decode().split()
+        pr.wait()
+        total += int(evals) / float(el)
+    return total
+
+
+def config1_cpu(seconds_cap=30.0):
+    """Config 1 end to end on the CPU: one island of NP=100, D=10, F=0.5,
+    CR=0.9, 1000 generations of DE/rand/1/bin on shifted-rotated Rastrigin
+    (rate 1), through the oracle (its FP32 restatement of the GPU DE kernels
+    and its FP64 objective), one thread."""
+    import oracle as O
+    NP, D, G = 100, 10, 1000
+    o, M = O.gen_problem(O.problem_seed(BASE_SEED + 1, 0), D)
+    X = O.init_population(NP, D, D, -100.0, 100.0, BASE_SEED + 1)
+    t0 = time.perf_counter()
+    fX = O.evaluate(O.OBJ_RASTRIGIN, X, o, M).astype(np.float32)
+    g = 0
+    while g < G and time.perf_counter() - t0 < seconds_cap:
+        U, *_ = O.de_generate(X, D, NP, D, seed=BASE_SEED + 1, gen=g)
+        fU = O.evaluate(O.OBJ_RASTRIGIN, U, o, M).astype(np.float32)
+        X, fX, _ = O.de_select(X, fX, U, fU, D, NP, D)
+        g += 1
+    dt = time.perf_counter() - t0
+    return {"generations": g, "seconds": dt, "best_f": float(fX.min()),
+            "workload": "NP=100, D=10, 1000 generations, shifted-rotated Rastrigin, 1 thread "
+                        "(oracle: FP32 DE step + FP64 objective)"}
+
+
+def config1_gpu(S):
+    """Config 1 on the GPU engine (one island, plain mode: the reference's
+    three launches per generation), 1000 generations timed."""
+    import oracle as O
+    NP, D = 100, 10
+    o, M = O.gen_problem(O.problem_seed(BASE_SEED + 1, 0), D)
+    prob = S.Problem([S.OBJ_RASTRIGIN], o[None, :], M[None, :, :])
+    eng = S.Engine([prob], [NP], [0], F=0.5, CR=0.9, seed=BASE_SEED + 1, mode=S.MODE_PLAIN,
+                   rows_per_block=16)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.run(1000)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    _, f32, f64 = eng.best()
+    eng.close()
+    prob.close()
+    return {"seconds": dt, "best_f64": float(f64[0]), "mode": "plain (3 launches / generation)"}
+
+
 def cpu_baseline(data, seconds, D, NP):
-    """Single-thread CPU trial evals/s: objective evaluation + DE step."""
+    """CPU trial evals/s on this host: the reference's CEC2014::EvaluateFitness
+    (built from its source, oracle/_ref) on one core and on every core (one
+    process per core), combined with the DE step of the oracle's FP32
+    restatement of the GPU DE kernels (kernel_DE_CUDA.cu:172-327; the
+    reference's own CPU DE step, DE_CUDA.cc:121-198, cannot be built here: it
+    includes curand.h through EA_CUDA.h)."""
     import oracle as O
     rng = np.random.default_rng(0)
     x = rng.uniform(-100, 100, D)
     eval_rates, eval_rates_o0 = {}, {}
     kind = "reference" if O.ref_available() else "port"
     fid_of = {O.OBJ_ELLIPTIC: 1, O.OBJ_ACKLEY: 5, O.OBJ_RASTRIGIN: 9}
-    for obj, o, M in data:
-        fid = fid_of[obj]
-        if kind == "reference":
-            with tempfile.TemporaryDirectory() as td:
+    cores = host_cores()
+    all_core = None
+    with tempfile.TemporaryDirectory() as td:
+        for obj, o, M in data:
+            fid = fid_of[obj]
+            if kind == "reference":
                 O.write_cec_files(td, fid, o, M)
                 eval_rates[fid] = O.ref_time(td, fid, x, seconds, "O2")
                 if O.ref_binary("O0").exists():
                     eval_rates_o0[fid] = O.ref_time(td, fid, x, seconds / 3, "O0")
-        else:
-            n, t0 = 0, time.perf_counter()
-            while time.perf_counter() - t0 < seconds:
-                O.evaluate(obj, x[None, :], o, M)
-                n += 1
-            eval_rates[fid] = n / (time.perf_counter() - t0)
+            else:
+                n, t0 = 0, time.perf_counter()
+                while time.perf_counter() - t0 < seconds:
+                    O.evaluate(obj, x[None, :], o, M)
+                    n += 1
+                eval_rates[fid] = n / (time.perf_counter() - t0)
+        if kind == "reference":
+            all_core = ref_time_parallel(O, td, sorted(eval_rates), x, seconds, cores)
     # DE step (generate + select) of one island, FP32, the oracle's C code
     X = O.init_population(NP, D, D, -100.0, 100.0, 1)
     fX = rng.uniform(0, 1, NP).astype(np.float32)
@@ -298,24 +391,31 @@ def cpu_baseline(data, seconds, D, NP):
         gen += 1
     de_rate = n / (time.perf_counter() - t0)
     t_eval = float(np.mean([1.0 / r for r in eval_rates.values()]))
-    value = 1.0 / (t_eval + 1.0 / de_rate)
-    out = {"value": value, "unit": "trial evals/s", "cores": 1, "kind": kind,
-           "sample": (f"{'reference CEC2014::EvaluateFitness (-O2, built from CEC2014.cc)' if kind == 'reference' else 'oracle FP64 evaluation'}"
-                      f" on this workload's elliptic/Rastrigin/Ackley at D={D}, ~{seconds:.0f} s each,"
-                      f" plus the oracle's FP32 DE/rand/1/bin step on one NP={NP} island for ~{seconds:.0f} s;"
-                      f" 1 thread"),
-           "eval_per_s": {f"F{k}": v for k, v in eval_rates.items()},
-           "de_step_trials_per_s": de_rate}
+    one = 1.0 / (t_eval + 1.0 / de_rate)
+    src = ("reference CEC2014::EvaluateFitness (-O2, built from CEC2014.cc)" if kind == "reference"
+           else "oracle FP64 evaluation")
+    if all_core is not None:
+        # islands are independent: every core evaluates and steps its own
+        value = 1.0 / (1.0 / all_core + 1.0 / (de_rate * cores))
+        used = cores
+        sample = (f"{src} on this workload's elliptic/Rastrigin/Ackley at D={D}: {cores} "
+                  f"concurrent processes (objectives round-robin) for ~{seconds:.0f} s, plus the "
+                  f"oracle's FP32 DE/rand/1/bin step (its restatement of kernel_DE_CUDA.cu) on "
+                  f"one NP={NP} island for ~{seconds:.0f} s, scaled by {cores} cores")
+    else:
+        value, used = one, 1
+        sample = (f"{src} at D={D}, ~{seconds:.0f} s each, plus the oracle's FP32 DE step "
+                  f"(its restatement of kernel_DE_CUDA.cu), 1 thread")
+    out = {"value": value, "unit": "trial evals/s", "cores": used, "kind": kind,
+           "sample": sample, "value_1_core": one,
+           "eval_per_s_1_core": {f"F{k}": v for k, v in eval_rates.items()},
+           "eval_per_s_all_cores": all_core,
+           "de_step_trials_per_s_1_core": de_rate,
+           "host_cpu": cpu_model(), "host_nproc": os.cpu_count()}
     if eval_rates_o0:
         t0e = float(np.mean([1.0 / r for r in eval_rates_o0.values()]))
-        out["value_reference_O0"] = 1.0 / (t0e + 1.0 / de_rate)
-    try:
-        import platform
-        out["host_cpu"] = platform.processor() or open("/proc/cpuinfo").read().split(
-            "model name")[1].split("\n")[0].strip(": ")
-    except Exception:
-        pass
-    out["host_nproc"] = os.cpu_count()
+        out["value_1_core_reference_O0"] = 1.0 / (t0e + 1.0 / de_rate)
+    out["config1_end_to_end"] = config1_cpu()
     return out
 
 
@@ -373,6 +473,9 @@ def main():
           f"rows_per_block={eng.stats()['rows_per_block']}", file=sys.stderr, flush=True)
 
     eng.run(args.warmup)
+    # capture and instantiate the graphs of exactly the chunks the timed run
+    # launches, so the timed region holds graph launches only
+    eng.prepare(args.steps)
     eng.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -458,6 +561,7 @@ def main():
         out["time_to_error"] = time_to_error(S, D, 32, 256, args.tte_gens, 10)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(data, args.cpu_seconds, D, args.island_size)
+        out["cpu_baseline"]["config1_gpu"] = config1_gpu(S)
     print(json.dumps(out), flush=True)
 
 

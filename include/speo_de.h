@@ -181,7 +181,32 @@ enum {
     SPEO_OBJ_DISCUS = 5,
     SPEO_OBJ_HAPPYCAT = 6,
     SPEO_OBJ_HGBAT = 7,
-    SPEO_OBJ_SCHWEFEL = 8
+    SPEO_OBJ_SCHWEFEL = 8,
+    /* Evaluated with an FP64 rotation and FP64 terms (any precision setting;
+     * SPEO_MODE_PLAIN only; dim <= 1024):                                     */
+    SPEO_OBJ_ROSENBROCK = 9,   /* y = z + 1: sum 100 (y_i^2 - y_{i+1})^2 + (y_i - 1)^2
+                                                           CEC2014.cc:552-568 */
+    SPEO_OBJ_WEIERSTRASS = 10, /* sum_i sum_k 0.5^k cos(2 pi 3^k (z_i + 1/2)) - D c,
+                                  k = 0..20                CEC2014.cc:615-644 */
+    SPEO_OBJ_GRIEWANK = 11,    /* 1 + |z|^2 / 4000 - prod cos(z_i / sqrt(i + 1))
+                                                           CEC2014.cc:646-666 */
+    SPEO_OBJ_KATSUURA = 12,    /* 10/D^2 prod (1 + (i+1) sum_j |2^j z_i - round(2^j z_i)| 2^-j)^(10/D^1.2)
+                                  - 10/D^2                 CEC2014.cc:739-765 */
+    SPEO_OBJ_GRIE_ROSEN = 13,  /* Griewank of Rosenbrock pairs, y = z + 1, wrap-around
+                                                           CEC2014.cc:842-868 */
+    SPEO_OBJ_ESCAFFER6 = 14,   /* expanded Schaffer F6, wrap-around pairs
+                                                           CEC2014.cc:870-905 */
+    /* Hybrid functions hf01-hf06 (CEC2014.cc:954-1207): z = M (x - o), shuffled
+     * by the component's permutation, cut into segments of ceil(p_i D)
+     * coordinates, each its own objective at its own rate, summed.  They need a
+     * shuffle (speo_problem_create_ex).                                        */
+    SPEO_OBJ_HYBRID_1 = 17, /* Schwefel 0.3, Rastrigin 0.3, elliptic 0.4           */
+    SPEO_OBJ_HYBRID_2 = 18, /* bent cigar 0.3, HGBat 0.3, Rastrigin 0.4            */
+    SPEO_OBJ_HYBRID_3 = 19, /* Griewank 0.2, Weierstrass 0.2, Rosenbrock 0.3, Schaffer F6 0.3 */
+    SPEO_OBJ_HYBRID_4 = 20, /* HGBat 0.2, discus 0.2, Griewank-Rosenbrock 0.3, Rastrigin 0.3 */
+    SPEO_OBJ_HYBRID_5 = 21, /* Schaffer F6 0.1, HGBat 0.2, Rosenbrock 0.2, Schwefel 0.2, elliptic 0.3 */
+    SPEO_OBJ_HYBRID_6 = 22  /* Katsuura 0.1, HappyCat 0.2, Griewank-Rosenbrock 0.2, Schwefel 0.2,
+                               Ackley 0.2                                            */
 };
 
 /* Precision of the rotation GEMM (accumulation is always FP32). */
@@ -210,8 +235,18 @@ SPEO_EXPORT int speo_problem_create(int dim, int ncomp, const int *objectives,
                                     const double *sigma, const double *bias,
                                     const double *lambda, const double *rate,
                                     speo_problem *out);
+/* Same, for components that may be hybrid functions: shuffle [ncomp][dim]
+ * holds each component's 1-based permutation of 1..dim (the reference's
+ * shuffle_data format, CEC2014.cc:440-458), read for hybrid components only;
+ * NULL when there are none. */
+SPEO_EXPORT int speo_problem_create_ex(int dim, int ncomp, const int *objectives,
+                                       const double *shift, const double *rotation,
+                                       const int *shuffle, const double *sigma,
+                                       const double *bias, const double *lambda,
+                                       const double *rate, speo_problem *out);
 SPEO_EXPORT int speo_problem_destroy(speo_problem p);
 SPEO_EXPORT int speo_problem_dim(speo_problem p);
+SPEO_EXPORT int speo_problem_ncomp(speo_problem p); /* components (1 unless a composition) */
 
 /* Seeded problem data: o_j = fl32(-80 + 160 u), M = Haar orthogonal (QR of a
  * Philox/Box-Muller N(0,1) matrix, sign-fixed so diag(R) > 0).  Host-side,
@@ -321,9 +356,10 @@ typedef struct {
     float surrogate_lr;           /* Adam learning rate (1e-3)                    */
     float surrogate_rho_min;      /* minimum held-out Spearman rho (0.5)          */
     /* Epoch mode: workgroups per island (0 = automatic: islands are split while
-     * there are fewer islands than compute units; 1, 2, 4 or 8 = fixed).  Split
-     * islands exchange their rows through HBM every generation, inside one
-     * cooperative launch per epoch (not captured in graphs). */
+     * there are fewer islands than compute units; 1, 2, 4 or 8 = fixed).  A
+     * split island runs one launch per generation: every workgroup reads the
+     * whole island from HBM, evolves its share of the rows and writes it back;
+     * the launch boundary is the only synchronisation (graph-capturable). */
     int epoch_parts;
 } speo_engine_config;
 
@@ -350,6 +386,11 @@ SPEO_EXPORT int speo_engine_destroy(speo_engine e);
  * engine's life) islands migrate when g % interval == 0 (island_EA.cc:93-99).
  * Asynchronous on the engine's stream. */
 SPEO_EXPORT int speo_engine_run(speo_engine e, int generations);
+/* Captures and instantiates every hipGraph that speo_engine_run(e, generations)
+ * would launch next (use_graph engines), without running anything, so that a
+ * timed or latency-critical run that follows launches only.  Enqueues no
+ * work; no effect without graphs or with the surrogate. */
+SPEO_EXPORT int speo_engine_prepare(speo_engine e, int generations);
 SPEO_EXPORT int speo_engine_sync(speo_engine e);
 
 /* Applies any pending selection, then per island: best row, its FP32 fitness
@@ -429,8 +470,8 @@ SPEO_EXPORT int speo_spearman(const float *a, const float *b, int n, float *rho_
 
 /* ---- run driver and result record ------------------------------------- *
  * The evaluation-budget loop and the result row of the reference's island
- * driver (IslandEA::Execute / PrintResult, src/island_EA.cc:78-105, 139-160;
- * header line :24), for runs whose output must line up row for row with the
+ * driver (IslandEA::Execute / PrintResult, src/island_EA.cc:78-105, 146-171;
+ * header line :23), for runs whose output must line up row for row with the
  * reference's SPEO_GPU.csv. */
 
 /* Generations the budget loop runs: while current < max_FEs, one generation

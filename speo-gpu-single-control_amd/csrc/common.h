@@ -131,7 +131,7 @@ inline int next_pow2_dim(int dim) {
     return p < 32 ? 32 : p;
 }
 
-inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+__host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 hipStream_t as_stream(speo_stream_t s);
 

@@ -9,6 +9,7 @@ namespace speo {
 // One problem as the generation kernels see it.
 struct FusedProblem {
     int ncomp, dim;
+    int generic;  // evaluated by the generic kernel (cec_generic.h), skipped by k_generation
     float lambda[kMaxComp], bias[kMaxComp], sigma[kMaxComp];
     CompDev comp[kMaxComp];
 };

@@ -219,6 +219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 
 
     if constexpr (MODE == kModeEval || MODE == kModeFused) {
         const FusedProblem *P = a.probs + I.problem;
+        if (MODE == kModeEval && P->generic) return;  // the generic kernel evaluates this island
         const int ncomp = P->ncomp, dim = I.dim;
         if (MODE == kModeEval && lds_trial) {
             // stage the rows to evaluate into the LDS tile (zero rows past nrows)
@@ -264,8 +265,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 
             } else {
                 // cf_cal weights (CEC2014.cc:1481-1519) in the log domain.
                 const float *x = src.rows + (size_t)tid * src.ld;
+                // (components at distance 0 share the weight equally, as
+                // cf_cal's w = 1e99 each)
                 float lw[kMaxComp], lwmax = -INFINITY;
-                int zero_c = -1;
+                bool zero[kMaxComp];
+                bool any_zero = false;
                 for (int c = 0; c < ncomp; ++c) {
                     const float *o = P->comp[c].shift;
                     float d = 0.f;
@@ -273,14 +277,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 
                         const float v = x[j] - o[j];
                         d = fmaf(v, v, d);
                     }
-                    if (d == 0.f && zero_c < 0) zero_c = c;
+                    zero[c] = d == 0.f;
+                    any_zero = any_zero || zero[c];
                     const float s = P->sigma[c];
                     lw[c] = -0.5f * logf(d) - d / (2.0f * (float)dim * s * s);
                     lwmax = fmaxf(lwmax, lw[c]);
                 }
                 float wsum = 0.f, acc = 0.f;
                 for (int c = 0; c < ncomp; ++c) {
-                    const float w = zero_c >= 0 ? (c == zero_c ? 1.f : 0.f) : expf(lw[c] - lwmax);
+                    const float w = any_zero ? (zero[c] ? 1.f : 0.f) : expf(lw[c] - lwmax);
                     wsum += w;
                     acc = fmaf(w, fmaf(P->lambda[c], gval[c * RT + tid], P->bias[c]), acc);
                 }
@@ -374,6 +379,8 @@ struct Engine {
     std::vector<int> row_off, dst;
     std::vector<speo_problem> problems;
     bool all_small = true;  // every dim <= kTLD
+    bool any_generic = false;  // some problem needs the generic kernel (plain mode only)
+    int eval_generic_islands(const float *src, float *out, hipStream_t s);
 
     IslandDesc *isl_dev = nullptr;
     int *row_off_dev = nullptr, *dst_dev = nullptr, *best_buf = nullptr, *worst_buf = nullptr;
@@ -435,7 +442,10 @@ struct Engine {
     int enqueue_chunk(int L, bool migrate, hipStream_t s);
     int finalize_pending(hipStream_t s);
     int migrate(hipStream_t s);
-    int run(int generations);
+    // dry: capture and instantiate the chunks' graphs, launch nothing (host
+    // state is advanced as if launched; prepare() restores it)
+    int run(int generations, bool dry = false);
+    int prepare(int generations);
     int profile(int generations, double *avg_ms);
     int profile_epoch(int generations, double *avg_ms);
     // when set, launch_epoch_chunk records an event pair around every launch:
@@ -502,6 +512,8 @@ int Engine::upload_tables() {
         FusedProblem f{};
         f.ncomp = P.ncomp;
         f.dim = P.dim;
+        f.generic = P.generic ? 1 : 0;
+        any_generic = any_generic || P.generic;
         for (int c = 0; c < P.ncomp; ++c) {
             f.lambda[c] = (float)P.comp[c].lambda;
             f.bias[c] = (float)P.comp[c].bias;
@@ -596,6 +608,9 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
         own_stream = true;
     }
     SPEO_TRY(upload_tables());
+    if (any_generic && (c.mode != SPEO_MODE_PLAIN || c.surrogate_candidates > 0))
+        return set_error(SPEO_ERR_UNSUPPORTED,
+                         "speo_engine_create: objectives 9-22 run in SPEO_MODE_PLAIN without the surrogate");
     const size_t E = (size_t)total_elems, R = (size_t)total_rows;
     SPEO_TRY_HIP(hipMalloc(&X, sizeof(float) * E));
     for (int b = 0; b < 2; ++b) {
@@ -613,10 +628,9 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
         hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, fU[b], total_rows, INFINITY);
         SPEO_TRY(check_launch("k_fill"));
     }
-    if (c.surrogate_candidates > 0) {
-        cfg.mode = SPEO_MODE_PLAIN;  // the screened trial replaces generate; selection each generation
-        cfg.use_graph = 0;
-    }
+    if (c.surrogate_candidates > 0 && (c.mode != SPEO_MODE_PLAIN || c.use_graph))
+        return set_error(SPEO_ERR_UNSUPPORTED, "speo_engine_create: the surrogate runs in "
+                                               "SPEO_MODE_PLAIN without graphs");
     if (cfg.mode == SPEO_MODE_EPOCH) SPEO_TRY(epoch_setup());
     rt = c.rows_per_block;
     if (rt == 0) SPEO_TRY(autotune());
@@ -631,6 +645,7 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     a.eval_out = fX[0];
     a.trial_in_lds = all_small ? 1 : 0;
     SPEO_TRY(launch_any(rt, cfg.precision, kModeEval, a, ntiles[rt], stream));
+    SPEO_TRY(eval_generic_islands(X, fX[0], stream));
     hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, fX[1], total_rows, INFINITY);
     SPEO_TRY(check_launch("k_fill"));
     fes = total_rows;
@@ -941,6 +956,19 @@ int Engine::launch(int mode, int gen_offset, int rt_, hipStream_t s, int clear) 
     return launch_any(rt_, cfg.precision, mode, a, ntiles[rt_], s);
 }
 
+// Islands whose problem the generic kernel evaluates: one launch per island.
+int Engine::eval_generic_islands(const float *src, float *out, hipStream_t s) {
+    if (!any_generic) return SPEO_OK;
+    for (int i = 0; i < I; ++i) {
+        const IslandDesc &d = isl[(size_t)i];
+        const ProblemHost &P = problems[(size_t)d.problem]->h;
+        if (!P.generic) continue;
+        SPEO_TRY(eval_problem_generic(P, src + d.elem_off, d.ld, d.np, out + d.row_off, nullptr,
+                                      nullptr, 0, s));
+    }
+    return SPEO_OK;
+}
+
 int Engine::finalize_pending(hipStream_t s) {
     if (cfg.mode != SPEO_MODE_FUSED || !pending) return SPEO_OK;
     SPEO_TRY(launch(kModeSelect, 0, rt, s, 1));
@@ -972,6 +1000,7 @@ int Engine::enqueue_chunk(int L, bool do_migrate, hipStream_t s) {
             } else {
                 SPEO_TRY(launch(kModeGen, g, rt, s));
                 SPEO_TRY(launch(kModeEval, g, rt, s));
+                SPEO_TRY(eval_generic_islands(U[0], fU[0], s));
                 SPEO_TRY(launch(kModeSelect, g, rt, s));
             }
         }
@@ -998,7 +1027,23 @@ int Engine::next_chunk(int left, bool *mig) const {
     return left;
 }
 
-int Engine::run(int generations) {
+int Engine::prepare(int generations) {
+    if (C > 0 || !cfg.use_graph || graph_failed) return SPEO_OK;
+    const int64_t g0 = gens, f0 = fes, m0 = migrations, gl0 = graph_launches;
+    const int p0 = p, ep0 = ep_par;
+    const bool pend0 = pending;
+    const int rc = run(generations, true);
+    gens = g0;
+    fes = f0;
+    migrations = m0;
+    graph_launches = gl0;
+    p = p0;
+    pending = pend0;
+    if (ep_par != ep0) swap_epoch_buffers();
+    return rc;
+}
+
+int Engine::run(int generations, bool dry) {
     if (C > 0) {
         for (int g = 0; g < generations; ++g) {
             const bool mig = cfg.interval > 0 && cfg.migration_size > 0 && gens % cfg.interval == 0;
@@ -1045,7 +1090,7 @@ int Engine::run(int generations) {
                 pending = pend_before;
                 if (ep_par != ep_before) swap_epoch_buffers();
             }
-            SPEO_TRY_HIP(hipGraphLaunch(it->second, stream));
+            if (!dry) SPEO_TRY_HIP(hipGraphLaunch(it->second, stream));
             ++graph_launches;
             if (cfg.mode == SPEO_MODE_FUSED) {
                 if (L % 2) p = 1 - p;
@@ -1053,7 +1098,7 @@ int Engine::run(int generations) {
                 if (mig && cfg.migration_size > 0) pending = false;
             }
             if (cfg.mode == SPEO_MODE_EPOCH && epoch_parts > 1 && L % 2) swap_epoch_buffers();
-        } else {
+        } else if (!dry) {
             SPEO_TRY(enqueue_chunk(L, mig, stream));
         }
         gens += L;
@@ -1093,6 +1138,7 @@ int Engine::profile(int generations, double *avg_ms) {
                 SPEO_TRY(record(m));
                 SPEO_TRY_HIP(hipEventRecord(ev[ev.size() - 2], stream));
                 SPEO_TRY(launch(modes[m], 0, rt, stream));
+                if (modes[m] == kModeEval) SPEO_TRY(eval_generic_islands(U[0], fU[0], stream));
                 SPEO_TRY_HIP(hipEventRecord(ev.back(), stream));
             }
         }
@@ -1266,6 +1312,13 @@ int Engine::best(int *row_out, float *f32_out, double *f64_out) {
     hipLaunchKernelGGL(k_eval_f64_rows, dim3(I), dim3(256), lds, stream, params_tab_dev,
                        ncomp_dev, dims_dev, isl_dev, best_row_dev, X, best_f64_dev);
     SPEO_TRY(check_launch("k_eval_f64_rows"));
+    for (int i = 0; i < I && any_generic; ++i) {  // generic problems: the FP64 generic kernel
+        const IslandDesc &d = isl[(size_t)i];
+        const ProblemHost &P = problems[(size_t)d.problem]->h;
+        if (P.generic)
+            SPEO_TRY(eval_problem_generic(P, X + d.elem_off, d.ld, 1, nullptr, best_f64_dev + i,
+                                          best_row_dev + i, d.row_off, stream));
+    }
     std::vector<int> rows((size_t)I);
     std::vector<double> f64((size_t)I);
     std::vector<float> fall((size_t)total_rows);
@@ -1370,6 +1423,11 @@ int speo_engine_run_fes(speo_engine e, int64_t max_FEs, int *generations_out) {
     if (g > 0x7fffffff) return set_error(SPEO_ERR_ARG, "speo_engine_run_fes: budget beyond 2^31 generations");
     if (generations_out) *generations_out = (int)g;
     return e->e.run((int)g);
+}
+
+int speo_engine_prepare(speo_engine e, int generations) {
+    if (!e || generations < 0) return set_error(SPEO_ERR_ARG, "speo_engine_prepare: bad arguments");
+    return e->e.prepare(generations);
 }
 
 int speo_engine_sync(speo_engine e) {

@@ -35,4 +35,10 @@ int eval_problem_f64(const ProblemHost &P, const float *X, int ld, int n, double
 
 int default_rows_per_block(int n, int dim);
 
+// Problems with objectives outside the fused FP32 epilogues (ProblemHost::
+// generic): the FP64 generic kernel, FP32 and/or FP64 outputs; rows (device,
+// optional) select rows[i] - row_base of X for launch row i.
+int eval_problem_generic(const ProblemHost &P, const float *X, int ld, int n, float *f32,
+                         double *f64, const int *rows, int row_base, hipStream_t s);
+
 }  // namespace speo

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <unordered_map>
 
+#include "cec_generic.h"
 #include "eval.h"
 #include "eval_f64.h"
 
@@ -104,7 +105,8 @@ __global__ void k_eval_finalize(const float *part, int nchunks, int n, int dim, 
 // Composition (cf_cal, CEC2014.cc:1481-1519) in FP32: one wave per row.
 // Weights in the log domain, lw_c = -log(d_c)/2 - d_c / (2 D sigma_c^2),
 // normalised by their maximum, so no weight underflows before the ratio is
-// formed; a component with d_c == 0 takes all the weight (cf_cal's INF).
+// formed; components with d_c == 0 share all the weight equally (cf_cal gives
+// each of them w = 1e99, beside which every other weight vanishes).
 __global__ void k_comp_combine(const CompParams *comps, int ncomp, int dim, const float *X,
                                int ld, int n, const float *g, float *f_out) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -112,7 +114,8 @@ __global__ void k_comp_combine(const CompParams *comps, int ncomp, int dim, cons
     if (wave >= n) return;
     const float *x = X + (size_t)wave * ld;
     float lw[kMaxComp];
-    int zero_c = -1;
+    bool zero[kMaxComp];
+    bool any_zero = false;
     float lwmax = -INFINITY;
     for (int c = 0; c < ncomp; ++c) {
         float d = 0.f;
@@ -121,7 +124,8 @@ __global__ void k_comp_combine(const CompParams *comps, int ncomp, int dim, cons
             d = fmaf(v, v, d);
         }
         for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-        if (d == 0.f && zero_c < 0) zero_c = c;
+        zero[c] = d == 0.f;
+        any_zero = any_zero || zero[c];
         const float s = (float)comps[c].sigma;
         lw[c] = -0.5f * logf(d) - d / (2.0f * (float)dim * s * s);
         lwmax = fmaxf(lwmax, lw[c]);
@@ -129,7 +133,7 @@ __global__ void k_comp_combine(const CompParams *comps, int ncomp, int dim, cons
     if (lane == 0) {
         float wsum = 0.f, acc = 0.f;
         for (int c = 0; c < ncomp; ++c) {
-            const float w = zero_c >= 0 ? (c == zero_c ? 1.f : 0.f) : expf(lw[c] - lwmax);
+            const float w = any_zero ? (zero[c] ? 1.f : 0.f) : expf(lw[c] - lwmax);
             const float fit = fmaf((float)comps[c].lambda, g[(size_t)c * n + wave], (float)comps[c].bias);
             wsum += w;
             acc = fmaf(w, fit, acc);
@@ -198,6 +202,20 @@ int eval_component(const CompDev &comp, const float *X, int ld, int n, int dim, 
     return check_launch("k_eval_finalize");
 }
 
+int eval_problem_generic(const ProblemHost &P, const float *X, int ld, int n, float *f32,
+                         double *f64, const int *rows, int row_base, hipStream_t s) {
+    GenLaunch g{};
+    g.P = *P.gen;
+    g.X = X;
+    g.ld = ld;
+    g.n = n;
+    g.rows = rows;
+    g.row_base = row_base;
+    g.f32 = f32;
+    g.f64 = f64;
+    return eval_generic(g, s);
+}
+
 size_t eval_scratch_floats(int ncomp, int dim, int n) {
     const int nchunks = (dim + kNC - 1) / kNC;
     size_t part = nchunks > 1 ? (size_t)nchunks * 2 * n : 0;
@@ -208,6 +226,7 @@ size_t eval_scratch_floats(int ncomp, int dim, int n) {
 int eval_problem(const ProblemHost &P, const float *X, int ld, int n, float *f_out,
                  float *scratch, int precision, int rows_per_block, hipStream_t s) {
     if (n <= 0) return SPEO_OK;
+    if (P.generic) return eval_problem_generic(P, X, ld, n, f_out, nullptr, nullptr, 0, s);
     const int nchunks = (P.dim + kNC - 1) / kNC;
     float *part = nchunks > 1 ? scratch : nullptr;
     float *gbuf = scratch + (nchunks > 1 ? (size_t)nchunks * 2 * n : 0);
@@ -229,6 +248,7 @@ int eval_problem(const ProblemHost &P, const float *X, int ld, int n, float *f_o
 int eval_problem_f64(const ProblemHost &P, const float *X, int ld, int n, double *f_out,
                      hipStream_t s) {
     if (n <= 0) return SPEO_OK;
+    if (P.generic) return eval_problem_generic(P, X, ld, n, nullptr, f_out, nullptr, 0, s);
     const size_t lds = sizeof(double) * ((size_t)P.dim + 16);
     if (lds > 48 * 1024)
         SPEO_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eval_f64),
@@ -280,16 +300,68 @@ int dim_tables(int dim, DimTables &out) {
 }
 }  // namespace
 
+// The reference's CEC2014_CUDA (CEC2014_CUDA.cc:68-128, 160-289) hands every
+// call its device buffers: d_shift_data [ncomp][next_pow2(dim)],
+// d_rotated_data [ncomp][next_pow2(dim)][dim] (B = M^T, row-padded),
+// d_shuffle_data [ncomp][next_pow2(dim)] (1-based), d_sigma [ncomp] and the
+// scratch rows d_rotated_elements [pop_size][next_pow2(dim)].
+static int dropin_generic(real *f, real *X, real *shift, real *B, int *shuffle, real *sigma,
+                          int n, int dim, real bias, int fid) {
+    int nc = 0;
+    CecComp cc[kMaxComp];
+    if (!cec_function(fid, &nc, cc)) return set_error(SPEO_ERR_UNSUPPORTED, "unknown function");
+    const int Dp = next_pow2_dim(dim);
+    GenLaunch g{};
+    g.P.dim = dim;
+    g.P.ncomp = nc;
+    g.P.sigma32 = nc > 1 ? sigma : nullptr;
+    for (int c = 0; c < nc; ++c) {
+        GenComp &G = g.P.comp[c];
+        if (cc[c].objective >= SPEO_OBJ_HYBRID_1) {
+            if (!shuffle) return set_error(SPEO_ERR_ARG, "missing shuffle data");
+            hybrid_segments(cc[c].objective, dim, G);
+            G.perm = shuffle + (size_t)c * Dp;
+        } else {
+            G.nseg = 1;
+            G.seg[0] = GenSeg{cc[c].objective, dim, 1.0};
+        }
+        if (cc[c].rotate && !B) return set_error(SPEO_ERR_ARG, "missing rotation matrix");
+        G.rate = cc[c].rate;
+        G.rotate = cc[c].rotate;
+        G.o32 = shift + (size_t)c * Dp;
+        G.B32 = B ? B + (size_t)c * Dp * dim : nullptr;
+        G.ldb = dim;
+        G.lambda = cc[c].lambda;
+        G.bias = cc[c].bias;
+        G.sigma = cc[c].sigma;
+    }
+    g.X = X;
+    g.ld = Dp;
+    g.n = n;
+    g.f32 = f;
+    // F1-F16 ignore the call's bias; the hybrid and composition chains add it
+    // in CombineFitness (kernel_CEC2014_CUDA.cu:205, 1227, 1447).
+    g.out_bias = fid >= 17 ? bias : 0.0;
+    return eval_generic(g, nullptr);
+}
+
 }  // namespace speo
 
 using namespace speo;
 
-extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real *d_original_elements,
-                                    real *d_shift_data, real *d_rotated_data, int *, real *, real *,
-                                    real *, int pop_size, int problem_dim, real, int function_ID) {
+extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *d_rotated_elements,
+                                    real *d_original_elements, real *d_shift_data,
+                                    real *d_rotated_data, int *d_shuffle_data, real *,
+                                    real *, real *d_sigma, int pop_size, int problem_dim,
+                                    real bias, int function_ID) {
     if (pop_size <= 0 || problem_dim < 2 || !d_fitness_value || !d_original_elements ||
         !d_shift_data) {
         defer_error(SPEO_ERR_ARG, "API_evaluateFitness: invalid arguments");
+        return;
+    }
+    if (function_ID < 1 || function_ID > 30) {
+        defer_error(SPEO_ERR_UNSUPPORTED, "API_evaluateFitness: function_ID " +
+                                              std::to_string(function_ID) + " is not in 1..30");
         return;
     }
     CompDev c{};
@@ -308,10 +380,15 @@ extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real 
     case 11: c.objective = SPEO_OBJ_SCHWEFEL; c.rate = 10.0f; break;     // F11, :1168
     case 13: c.objective = SPEO_OBJ_HAPPYCAT; c.rate = 0.05f; break;     // F13, :1180
     case 14: c.objective = SPEO_OBJ_HGBAT; c.rate = 0.05f; break;        // F14, :1186
-    default:
-        defer_error(SPEO_ERR_UNSUPPORTED, "API_evaluateFitness: function_ID " +
-                                              std::to_string(function_ID) + " is not built");
+    default: {
+        // F4, F6, F7, F12, F15, F16, the hybrids F17-F22 and the compositions
+        // F23-F30 (:1123-1671): the FP64 generic kernel.
+        const int rc = dropin_generic(d_fitness_value, d_original_elements, d_shift_data,
+                                      d_rotated_data, d_shuffle_data, d_sigma, pop_size,
+                                      problem_dim, bias, function_ID);
+        if (rc != SPEO_OK) defer_error(rc, std::string("API_evaluateFitness: ") + speo_last_error());
         return;
+    }
     }
     if (rotate && !d_rotated_data) {
         defer_error(SPEO_ERR_ARG, "API_evaluateFitness: missing rotation matrix");
@@ -326,14 +403,19 @@ extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *, real 
     c.B32 = rotate ? d_rotated_data : t.identity;
     c.ldb32 = problem_dim;  // reference layout [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255)
     c.colw = t.colw[weights];
-    if (problem_dim > kNC) {
-        defer_error(SPEO_ERR_UNSUPPORTED, "API_evaluateFitness: dim > 128 needs a workspace; "
-                                          "use speo_evaluate");
-        return;
-    }
     const int ld = next_pow2_dim(problem_dim);
+    // Above 128 columns the per-chunk row sums go to the caller's rotated-row
+    // buffer (pop_size * next_pow2(dim) floats, at least 2 * chunks * pop_size).
+    float *part = nullptr;
+    if (problem_dim > kNC) {
+        if (!d_rotated_elements) {
+            defer_error(SPEO_ERR_ARG, "API_evaluateFitness: dim > 128 needs d_rotated_elements");
+            return;
+        }
+        part = d_rotated_elements;
+    }
     const int rc = eval_component(c, d_original_elements, ld, pop_size, problem_dim,
-                                  d_fitness_value, nullptr, 1.f, 0.f, 0, SPEO_PREC_FP32,
+                                  d_fitness_value, part, 1.f, 0.f, 0, SPEO_PREC_FP32,
                                   default_rows_per_block(pop_size, problem_dim), nullptr);
     if (rc != SPEO_OK) defer_error(rc, speo_last_error());
 }

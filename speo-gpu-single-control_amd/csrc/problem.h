@@ -47,6 +47,8 @@ struct CompParams {
     const double *M64;
 };
 
+struct GenProblem;
+
 struct ProblemHost {
     int dim = 0, ncomp = 0;
     int ldb32 = 0, ldm16 = 0;
@@ -54,6 +56,12 @@ struct ProblemHost {
     CompParams *params_dev = nullptr;  // [ncomp]
     CompHost comp[kMaxComp];
     CompDev view(int c, int precision) const;
+    // Objectives outside the fused FP32 epilogues (Rosenbrock, Weierstrass,
+    // Griewank, Katsuura, Griewank-Rosenbrock, Schaffer F6, hybrids): the
+    // whole problem is evaluated by the FP64 generic kernel (cec_generic.h).
+    bool generic = false;
+    int *perm[kMaxComp] = {};       // hybrids: 1-based shuffle [dim] on the device
+    GenProblem *gen = nullptr;      // host description with device pointers
 };
 
 }  // namespace speo

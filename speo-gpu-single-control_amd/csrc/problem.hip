@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "cec_generic.h"
 #include "eval.h"
 
 namespace speo {
@@ -104,21 +105,58 @@ static void free_problem(ProblemHost &P) {
         (void)hipFree(h.colw);
         h = CompHost{};
     }
+    for (int c = 0; c < kMaxComp; ++c) {
+        (void)hipFree(P.perm[c]);
+        P.perm[c] = nullptr;
+    }
+    delete P.gen;
+    P.gen = nullptr;
+    P.generic = false;
     (void)hipFree(P.colw);
     (void)hipFree(P.params_dev);
     P.colw = nullptr;
     P.params_dev = nullptr;
 }
 
+static bool known_objective(int o) {
+    return (o >= SPEO_OBJ_ELLIPTIC && o <= SPEO_OBJ_ESCAFFER6) ||
+           (o >= SPEO_OBJ_HYBRID_1 && o <= SPEO_OBJ_HYBRID_6);
+}
+
+static bool is_identity(const double *M, int dim) {
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j)
+            if (M[(size_t)i * dim + j] != (i == j ? 1.0 : 0.0)) return false;
+    return true;
+}
+
 static int create_problem(int dim, int ncomp, const int *objectives, const double *shift,
-                          const double *rotation, const double *sigma, const double *bias,
-                          const double *lambda, const double *rate, ProblemHost &P) {
+                          const double *rotation, const int *shuffle, const double *sigma,
+                          const double *bias, const double *lambda, const double *rate,
+                          ProblemHost &P) {
     if (dim < 2 || ncomp < 1 || ncomp > kMaxComp || !objectives || !shift)
         return set_error(SPEO_ERR_ARG, "speo_problem_create: need dim >= 2, 1 <= ncomp <= 8, "
                                        "objectives and shift");
-    for (int c = 0; c < ncomp; ++c)
-        if (objectives[c] < SPEO_OBJ_ELLIPTIC || objectives[c] > SPEO_OBJ_SCHWEFEL)
+    bool generic = false;
+    for (int c = 0; c < ncomp; ++c) {
+        if (!known_objective(objectives[c]))
             return set_error(SPEO_ERR_ARG, "speo_problem_create: unknown objective");
+        generic = generic || generic_objective(objectives[c]);
+        if (objectives[c] >= SPEO_OBJ_HYBRID_1) {
+            if (!shuffle)
+                return set_error(SPEO_ERR_ARG, "speo_problem_create: a hybrid objective needs a shuffle");
+            std::vector<char> seen((size_t)dim, 0);
+            for (int j = 0; j < dim; ++j) {
+                const int v = shuffle[(size_t)c * dim + j];
+                if (v < 1 || v > dim || seen[(size_t)v - 1])
+                    return set_error(SPEO_ERR_ARG, "speo_problem_create: the shuffle of a hybrid "
+                                                   "component must be a permutation of 1..dim");
+                seen[(size_t)v - 1] = 1;
+            }
+        }
+    }
+    if (generic && dim > kGenMaxDim)
+        return set_error(SPEO_ERR_UNSUPPORTED, "speo_problem_create: these objectives take dim <= 1024");
     P.dim = dim;
     P.ncomp = ncomp;
     P.ldb32 = round_up(dim, 4);
@@ -203,6 +241,33 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
                                        h.shift32, h.shift64, h.M64};
     }
     SPEO_TRY(upload(&P.params_dev, params));
+    if (generic) {
+        P.generic = true;
+        P.gen = new (std::nothrow) GenProblem{};
+        if (!P.gen) return set_error(SPEO_ERR_ALLOC, "speo_problem_create: out of host memory");
+        P.gen->dim = dim;
+        P.gen->ncomp = ncomp;
+        for (int c = 0; c < ncomp; ++c) {
+            const CompHost &h = P.comp[c];
+            GenComp &g = P.gen->comp[c];
+            if (objectives[c] >= SPEO_OBJ_HYBRID_1) {
+                std::vector<int> perm(shuffle + (size_t)c * dim, shuffle + (size_t)(c + 1) * dim);
+                SPEO_TRY(upload(&P.perm[c], perm));
+                hybrid_segments(objectives[c], dim, g);
+                g.perm = P.perm[c];
+            } else {
+                g.nseg = 1;
+                g.seg[0] = GenSeg{objectives[c], dim, 1.0};
+            }
+            g.rate = h.rate64;
+            g.rotate = rotation && !is_identity(rotation + (size_t)c * D * D, dim) ? 1 : 0;
+            g.o64 = h.shift64;
+            g.M64 = h.M64;
+            g.lambda = h.lambda;
+            g.bias = h.bias;
+            g.sigma = h.sigma;
+        }
+    }
     return SPEO_OK;
 }
 
@@ -305,12 +370,20 @@ int speo_generate_problem_data(uint64_t seed, int dim, double *shift_out, double
 int speo_problem_create(int dim, int ncomp, const int *objectives, const double *shift,
                         const double *rotation, const double *sigma, const double *bias,
                         const double *lambda, const double *rate, speo_problem *out) {
+    return speo_problem_create_ex(dim, ncomp, objectives, shift, rotation, nullptr, sigma, bias,
+                                  lambda, rate, out);
+}
+
+int speo_problem_create_ex(int dim, int ncomp, const int *objectives, const double *shift,
+                           const double *rotation, const int *shuffle, const double *sigma,
+                           const double *bias, const double *lambda, const double *rate,
+                           speo_problem *out) {
     if (!out) return set_error(SPEO_ERR_ARG, "speo_problem_create: null output");
     *out = nullptr;
     auto *p = new (std::nothrow) speo_problem_s;
     if (!p) return set_error(SPEO_ERR_ALLOC, "speo_problem_create: out of host memory");
-    const int rc = create_problem(dim, ncomp, objectives, shift, rotation, sigma, bias, lambda,
-                                  rate, p->h);
+    const int rc = create_problem(dim, ncomp, objectives, shift, rotation, shuffle, sigma, bias,
+                                  lambda, rate, p->h);
     if (rc != SPEO_OK) {
         free_problem(p->h);
         delete p;
@@ -356,52 +429,55 @@ int speo_load_cec_data(const char *dir, int func_id, int dim, int ncomp, double 
     return SPEO_OK;
 }
 
+// The shuffle of a hybrid function (CEC2014.cc:440-458, CEC2014_CUDA.cc:
+// 275-279): <dir>/shuffle_data_<func_id>_D<dim>.txt, ncomp * dim 1-based indices.
+static int load_cec_shuffle(const char *dir, int func_id, int dim, int ncomp, int *out) {
+    const std::string name = std::string(dir) + "/shuffle_data_" + std::to_string(func_id) + "_D" +
+                             std::to_string(dim) + ".txt";
+    FILE *f = std::fopen(name.c_str(), "r");
+    if (!f) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: cannot open " + name);
+    const size_t n = (size_t)ncomp * dim;
+    size_t got = 0;
+    while (got < n && std::fscanf(f, "%d", &out[got]) == 1) ++got;
+    std::fclose(f);
+    if (got < n) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: too few values in " + name);
+    return SPEO_OK;
+}
+
 int speo_problem_create_cec(const char *dir, int func_id, int dim, speo_problem *out) {
     if (!out) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: null output");
     *out = nullptr;
-    if (func_id == 24 || func_id == 25) {
-        // cf02 / cf03 (CEC2014.cc:1235-1267): three components over the data
-        // files' three shifts and matrices; F24's Schwefel is not rotated.
-        if (dim < 2) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: dim < 2");
-        const size_t D = (size_t)dim;
-        std::vector<double> o(3 * D), M(3 * D * D);
-        SPEO_TRY(speo_load_cec_data(dir, func_id, dim, 3, o.data(), M.data()));
-        const int objs[3] = {SPEO_OBJ_SCHWEFEL, SPEO_OBJ_RASTRIGIN,
-                             func_id == 24 ? SPEO_OBJ_HGBAT : SPEO_OBJ_ELLIPTIC};
-        const double rate[3] = {10.0, 0.0512, func_id == 24 ? 0.05 : 1.0};
-        const double bias[3] = {0.0, 100.0, 200.0};
-        const double sigma24[3] = {20.0, 20.0, 20.0}, sigma25[3] = {10.0, 30.0, 50.0};
-        const double lam24[3] = {1.0, 1.0, 1.0}, lam25[3] = {1000.0 / 4e3, 1000.0 / 1e3, 1000.0 / 1e10};
-        if (func_id == 24) {
-            std::fill(M.begin(), M.begin() + D * D, 0.0);
-            for (size_t i = 0; i < D; ++i) M[i * D + i] = 1.0;
-        }
-        return speo_problem_create(dim, 3, objs, o.data(), M.data(), func_id == 24 ? sigma24 : sigma25,
-                                   bias, func_id == 24 ? lam24 : lam25, rate, out);
-    }
-    int obj = 0;
-    double rate = 1.0;
-    bool rotate = true;
-    switch (func_id) {  // F1-F14 of kernel_CEC2014_CUDA.cu:1099-1190
-    case 1: obj = SPEO_OBJ_ELLIPTIC; break;
-    case 2: obj = SPEO_OBJ_BENT_CIGAR; break;
-    case 3: obj = SPEO_OBJ_DISCUS; break;
-    case 5: obj = SPEO_OBJ_ACKLEY; break;
-    case 8: obj = SPEO_OBJ_RASTRIGIN; rate = 0.0512; rotate = false; break;
-    case 9: obj = SPEO_OBJ_RASTRIGIN; rate = 0.0512; break;
-    case 10: obj = SPEO_OBJ_SCHWEFEL; rate = 10.0; rotate = false; break;
-    case 11: obj = SPEO_OBJ_SCHWEFEL; rate = 10.0; break;
-    case 13: obj = SPEO_OBJ_HAPPYCAT; rate = 0.05; break;
-    case 14: obj = SPEO_OBJ_HGBAT; rate = 0.05; break;
-    default:
-        return set_error(SPEO_ERR_UNSUPPORTED,
-                         "speo_problem_create_cec: function " + std::to_string(func_id) + " is not built");
-    }
+    int nc = 0;
+    CecComp cc[kMaxComp];
+    if (!cec_function(func_id, &nc, cc))
+        return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: function " + std::to_string(func_id) +
+                                           " is not in 1..30");
     if (dim < 2) return set_error(SPEO_ERR_ARG, "speo_problem_create_cec: dim < 2");
-    std::vector<double> o((size_t)dim), M(rotate ? (size_t)dim * dim : 0);
-    SPEO_TRY(speo_load_cec_data(dir, func_id, dim, 1, o.data(), rotate ? M.data() : nullptr));
-    return speo_problem_create(dim, 1, &obj, o.data(), rotate ? M.data() : nullptr, nullptr, nullptr,
-                               nullptr, &rate, out);
+    const size_t D = (size_t)dim;
+    std::vector<double> o(nc * D), M(nc * D * D);
+    bool any_rot = false, any_hybrid = false;
+    for (int c = 0; c < nc; ++c) {
+        any_rot = any_rot || cc[c].rotate;
+        any_hybrid = any_hybrid || cc[c].objective >= SPEO_OBJ_HYBRID_1;
+    }
+    SPEO_TRY(speo_load_cec_data(dir, func_id, dim, nc, o.data(), any_rot ? M.data() : nullptr));
+    std::vector<int> S(any_hybrid ? nc * D : 0);
+    if (any_hybrid) SPEO_TRY(load_cec_shuffle(dir, func_id, dim, nc, S.data()));
+    int objs[kMaxComp];
+    double rate[kMaxComp], sigma[kMaxComp], bias[kMaxComp], lam[kMaxComp];
+    for (int c = 0; c < nc; ++c) {
+        objs[c] = cc[c].objective;
+        rate[c] = cc[c].rate;
+        sigma[c] = cc[c].sigma;
+        bias[c] = cc[c].bias;
+        lam[c] = cc[c].lambda;
+        if (!cc[c].rotate) {  // sr_func with r_flag = 0: no rotation for this component
+            std::fill(M.begin() + c * D * D, M.begin() + (c + 1) * D * D, 0.0);
+            for (size_t i = 0; i < D; ++i) M[c * D * D + i * D + i] = 1.0;
+        }
+    }
+    return speo_problem_create_ex(dim, nc, objs, o.data(), any_rot ? M.data() : nullptr,
+                                  any_hybrid ? S.data() : nullptr, sigma, bias, lam, rate, out);
 }
 
 int speo_problem_destroy(speo_problem p) {
@@ -414,6 +490,7 @@ int speo_problem_destroy(speo_problem p) {
 }
 
 int speo_problem_dim(speo_problem p) { return p ? p->h.dim : 0; }
+int speo_problem_ncomp(speo_problem p) { return p ? p->h.ncomp : 0; }
 
 int speo_evaluate(speo_problem p, const float *X, int ld, int n, float *f_out, int precision,
                   speo_stream_t stream) {

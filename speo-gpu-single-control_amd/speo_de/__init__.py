@@ -22,7 +22,10 @@ import numpy as np
 
 from ._lib import (DEInfo, EngineConfig, EngineStats, GPU_IslandInfo, GPU_Population,
                    MODE_EPOCH, MODE_FUSED, MODE_PLAIN, OBJ_ACKLEY, OBJ_BENT_CIGAR, OBJ_DISCUS,
-                   OBJ_ELLIPTIC, OBJ_HAPPYCAT, OBJ_HGBAT, OBJ_RASTRIGIN, OBJ_SCHWEFEL, PREC_BF16,
+                   OBJ_ELLIPTIC, OBJ_ESCAFFER6, OBJ_GRIE_ROSEN, OBJ_GRIEWANK, OBJ_HAPPYCAT,
+                   OBJ_HGBAT, OBJ_HYBRID_1, OBJ_HYBRID_2, OBJ_HYBRID_3, OBJ_HYBRID_4,
+                   OBJ_HYBRID_5, OBJ_HYBRID_6, OBJ_KATSUURA, OBJ_RASTRIGIN, OBJ_ROSENBROCK,
+                   OBJ_SCHWEFEL, OBJ_WEIERSTRASS, PREC_BF16,
                    PREC_BF16X3, PREC_BF16X6, PREC_FP16, PREC_FP16X3, PREC_FP32, PREC_NAMES,
                    ProblemInfo, SPEO_RandState, SpeoError, check, last_error, lib)
 
@@ -30,7 +33,9 @@ __all__ = [
     "Problem", "Engine", "generate_problem_data", "problem_seed", "init_population",
     "de_generate", "de_select", "evaluate", "evaluate_f64", "segment_argmin", "SpeoError",
     "OBJ_ELLIPTIC", "OBJ_RASTRIGIN", "OBJ_ACKLEY", "OBJ_BENT_CIGAR", "OBJ_DISCUS", "OBJ_HAPPYCAT",
-    "OBJ_HGBAT", "OBJ_SCHWEFEL", "PREC_FP32", "PREC_BF16", "PREC_FP16",
+    "OBJ_HGBAT", "OBJ_SCHWEFEL", "OBJ_ROSENBROCK", "OBJ_WEIERSTRASS", "OBJ_GRIEWANK",
+    "OBJ_KATSUURA", "OBJ_GRIE_ROSEN", "OBJ_ESCAFFER6", "OBJ_HYBRID_1", "OBJ_HYBRID_2",
+    "OBJ_HYBRID_3", "OBJ_HYBRID_4", "OBJ_HYBRID_5", "OBJ_HYBRID_6", "PREC_FP32", "PREC_BF16", "PREC_FP16",
     "PREC_BF16X3", "PREC_FP16X3", "PREC_BF16X6", "PREC_NAMES", "MODE_PLAIN", "MODE_FUSED",
     "MODE_EPOCH",    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
     "generations_for_fes", "run_seed", "load_cec_data", "de_parameters", "result_header", "result_record",
@@ -70,7 +75,7 @@ class Problem:
     """A shifted-rotated problem (single component or cf_cal composition)."""
 
     def __init__(self, objectives, shifts, rotations=None, sigma=None, bias=None, lam=None,
-                 rate=None):
+                 rate=None, shuffle=None):
         objectives = np.atleast_1d(np.asarray(objectives, np.int32))
         nc = objectives.size
         shifts = np.ascontiguousarray(np.asarray(shifts, np.float64).reshape(nc, -1))
@@ -82,10 +87,15 @@ class Problem:
                                                                                 self.dim))
         arrs = [None if v is None else np.ascontiguousarray(np.asarray(v, np.float64).reshape(nc))
                 for v in (sigma, bias, lam, rate)]
+        shuf = None
+        if shuffle is not None:
+            shuf = np.ascontiguousarray(np.asarray(shuffle, np.int32).reshape(nc, self.dim))
         h = c_void_p()
-        check(lib().speo_problem_create(self.dim, nc, _np_ptr(objectives), _np_ptr(shifts),
-                                        _np_ptr(rot), *[_np_ptr(a) for a in arrs], byref(h)))
+        check(lib().speo_problem_create_ex(self.dim, nc, _np_ptr(objectives), _np_ptr(shifts),
+                                           _np_ptr(rot), _np_ptr(shuf),
+                                           *[_np_ptr(a) for a in arrs], byref(h)))
         self.handle = h
+        self.shuffle = shuf
         self.objectives, self.shifts, self.rotations = objectives, shifts, rot
         self.sigma, self.bias, self.lam, self.rate = arrs
 
@@ -96,8 +106,9 @@ class Problem:
         self = cls.__new__(cls)
         h = c_void_p()
         check(lib().speo_problem_create_cec(str(directory).encode(), func_id, dim, byref(h)))
-        self.handle, self.dim, self.ncomp = h, dim, 1
-        self.objectives = self.shifts = self.rotations = None
+        self.handle, self.dim = h, dim
+        self.ncomp = int(lib().speo_problem_ncomp(h))
+        self.objectives = self.shifts = self.rotations = self.shuffle = None
         self.sigma = self.bias = self.lam = self.rate = None
         return self
 
@@ -191,12 +202,12 @@ def de_parameters(cr: float, f: float, strategy: int = 4) -> str:
 
 
 def result_header() -> str:
-    """The header line of the reference's SPEO_GPU.csv (island_EA.cc:24)."""
+    """The header line of the reference's SPEO_GPU.csv (island_EA.cc:23)."""
     return lib().speo_result_header().decode()
 
 
 def result_record(**fields) -> str:
-    """One result row (island_EA.cc:139-160) from the 21 named columns of
+    """One result row (island_EA.cc:146-171) from the 21 named columns of
     speo_run_record; string columns are str."""
     from ._lib import RunRecord
     rec = RunRecord()
@@ -271,6 +282,10 @@ class Engine:
 
     def run(self, generations: int):
         check(lib().speo_engine_run(self.handle, generations))
+
+    def prepare(self, generations: int):
+        """Capture the graphs the next run(generations) launches (no work)."""
+        check(lib().speo_engine_prepare(self.handle, generations))
 
     def run_fes(self, max_fes: int) -> int:
         """Generations until the evaluation count reaches max_fes (the budget

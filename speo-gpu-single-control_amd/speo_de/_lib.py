@@ -19,6 +19,9 @@ SPEO_OK, SPEO_ERR_ARG, SPEO_ERR_HIP, SPEO_ERR_UNSUPPORTED, SPEO_ERR_NO_DEVICE, S
     0, -1, -2, -3, -4, -5)
 OBJ_ELLIPTIC, OBJ_RASTRIGIN, OBJ_ACKLEY, OBJ_BENT_CIGAR, OBJ_DISCUS = 1, 2, 3, 4, 5
 OBJ_HAPPYCAT, OBJ_HGBAT, OBJ_SCHWEFEL = 6, 7, 8
+OBJ_ROSENBROCK, OBJ_WEIERSTRASS, OBJ_GRIEWANK, OBJ_KATSUURA = 9, 10, 11, 12
+OBJ_GRIE_ROSEN, OBJ_ESCAFFER6 = 13, 14
+OBJ_HYBRID_1, OBJ_HYBRID_2, OBJ_HYBRID_3, OBJ_HYBRID_4, OBJ_HYBRID_5, OBJ_HYBRID_6 = range(17, 23)
 PREC_FP32, PREC_BF16, PREC_FP16, PREC_BF16X3, PREC_FP16X3, PREC_BF16X6 = range(6)
 PREC_NAMES = {PREC_FP32: "fp32", PREC_BF16: "bf16", PREC_FP16: "fp16", PREC_BF16X3: "bf16x3",
               PREC_FP16X3: "fp16x3", PREC_BF16X6: "bf16x6"}
@@ -103,8 +106,12 @@ _SIGS = {
     "speo_device_check": (c_int, []),
     "speo_problem_create": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
+    "speo_problem_create_ex": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p,
+                                       POINTER(c_void_p)]),
     "speo_problem_destroy": (c_int, [c_void_p]),
     "speo_problem_dim": (c_int, [c_void_p]),
+    "speo_problem_ncomp": (c_int, [c_void_p]),
     "speo_generate_problem_data": (c_int, [c_uint64, c_int, c_void_p, c_void_p]),
     "speo_problem_seed": (c_uint64, [c_uint64, c_uint32]),
     "speo_load_cec_data": (c_int, [c_char_p, c_int, c_int, c_int, c_void_p, c_void_p]),
@@ -122,6 +129,7 @@ _SIGS = {
     "speo_engine_create": (c_int, [POINTER(EngineConfig), c_void_p, POINTER(c_void_p)]),
     "speo_engine_destroy": (c_int, [c_void_p]),
     "speo_engine_run": (c_int, [c_void_p, c_int]),
+    "speo_engine_prepare": (c_int, [c_void_p, c_int]),
     "speo_engine_sync": (c_int, [c_void_p]),
     "speo_engine_best": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "speo_engine_read": (c_int, [c_void_p, c_void_p, c_void_p]),

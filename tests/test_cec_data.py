@@ -66,10 +66,30 @@ def test_missing_and_short_files(speo, tmp_path):
         speo.load_cec_data(tmp_path, 9, 4)
 
 
-def test_unsupported_function_id(speo, tmp_path):
-    with pytest.raises(speo.SpeoError, match="function 23 is not built") as e:
-        speo.Problem.from_cec(tmp_path, 23, 10)
-    assert e.value.code == -3
+def test_unknown_function_id(speo, tmp_path):
+    with pytest.raises(speo.SpeoError, match="function 31 is not in 1..30") as e:
+        speo.Problem.from_cec(tmp_path, 31, 10)
+    assert e.value.code == -1
+
+
+def test_missing_shuffle_file_is_reported(speo, oracle, tmp_path):
+    # F17 needs shuffle_data_17_D10.txt besides its shift and matrix files
+    o, M, S = oracle.gen_cec_problem(5, 17, 10)
+    oracle.write_cec_files(tmp_path, 17, o, M)
+    with pytest.raises(speo.SpeoError, match="shuffle_data_17_D10.txt") as e:
+        speo.Problem.from_cec(tmp_path / "input_data", 17, 10)
+    assert e.value.code == -1
+
+
+def test_hybrid_shuffle_must_be_a_permutation(speo, oracle):
+    o, M, S = oracle.gen_cec_problem(5, 17, 10)
+    S = S.copy()
+    S[0, 0] = S[0, 1]
+    with pytest.raises(speo.SpeoError, match="permutation") as e:
+        speo.Problem([speo.OBJ_HYBRID_1], o, M, shuffle=S)
+    assert e.value.code == -1
+    with pytest.raises(speo.SpeoError, match="needs a shuffle"):
+        speo.Problem([speo.OBJ_HYBRID_1], o, M)
 
 
 def test_reference_files_if_present(speo):

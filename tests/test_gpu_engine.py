@@ -244,6 +244,34 @@ def test_graph_replay_equals_eager(cuda, speo, oracle, mode):
     assert st["migrations"] == eager.stats()["migrations"] == 6   # after 0, 7, ..., 35
 
 
+@pytest.mark.parametrize("mode", ["fused", "epoch"])
+def test_prepare_then_run_equals_eager(cuda, speo, oracle, mode):
+    # speo_engine_prepare captures the graphs of the next run without running
+    # anything: the generation count and the populations are unchanged, and
+    # the run that follows gives the eager result bit for bit.
+    m = {"fused": speo.MODE_FUSED, "epoch": speo.MODE_EPOCH}[mode]
+    probs = _problems(speo, oracle, DIMS)
+    kw = dict(mode=m, interval=7, migration_size=2)
+    eager = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], use_graph=False, **kw)
+    graph = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], use_graph=True, **kw)
+    eager.run(5)
+    graph.run(5)
+    X0, f0 = graph.read()
+    graph.prepare(20)
+    st = graph.stats()
+    assert st["generations"] == 5
+    X1, f1 = graph.read()
+    np.testing.assert_array_equal(X0, X1)
+    np.testing.assert_array_equal(f0, f1)
+    eager.run(20)
+    graph.run(20)
+    Xe, fe = eager.read()
+    Xg, fg = graph.read()
+    np.testing.assert_array_equal(Xe, Xg)
+    np.testing.assert_array_equal(fe, fg)
+    assert graph.stats()["generations"] == 25
+
+
 def test_best_reverified_in_fp64(cuda, speo, oracle):
     probs = _problems(speo, oracle, DIMS)
     e = _engine(speo, probs, LAYOUT["sizes"], LAYOUT["pidx"], mode=speo.MODE_FUSED)
@@ -293,7 +321,7 @@ def test_convergence_on_small_problem(cuda, speo, oracle):
 
 @pytest.mark.parametrize("case", ["no_islands", "small_island", "problem_index", "bounds", "mode",
                                   "precision", "rows_per_block", "migration_size", "epoch_parts",
-                                  "surrogate_dims", "run_negative"])
+                                  "surrogate_dims", "surrogate_mode", "run_negative"])
 def test_engine_rejects_bad_configs(cuda, speo, oracle, case):
     # every rejected configuration returns SPEO_ERR_ARG with a message naming
     # the problem, before any device work
@@ -319,7 +347,9 @@ def test_engine_rejects_bad_configs(cuda, speo, oracle, case):
     elif case == "epoch_parts":
         kw, match = dict(mode=speo.MODE_EPOCH, epoch_parts=3), "epoch_parts must be"
     elif case == "surrogate_dims":
-        kw, match = dict(surrogate_candidates=4), "share one dimension"
+        kw, match = dict(surrogate_candidates=4, mode=speo.MODE_PLAIN), "share one dimension"
+    elif case == "surrogate_mode":
+        kw, match = dict(surrogate_candidates=4, mode=speo.MODE_FUSED), "SPEO_MODE_PLAIN without graphs"
     if case == "run_negative":
         e = _engine(speo, probs, sizes, pidx)
         with pytest.raises(speo.SpeoError, match="bad arguments"):

@@ -230,18 +230,54 @@ def test_composition(cuda, speo, oracle, D, objs):
     X[0, :D] = os_[1]                         # d = 0 for component 2 -> f = bias_2
     X[1, :D] = os_[0] + 1e-3                  # close to component 1
     ref = oracle.composition(X[:, :D], objs, os_, Ms, sigma, bias)
-    rtol = composition_rtol(objs, X[:, :D], os_, Ms)
-    for prec in (0, 5):
+    # Per precision: the bound on a well-conditioned row (measured maxima at
+    # D = 1000 over 10k points: BF16 7.6e-4, FP16 9.6e-5, BF16x3 3.3e-6) and
+    # the unit roundoff of the rotation's operands for ill-conditioned
+    # weighted-sphere rows (BF16 2^-8, FP16 2^-11 with M scaled into range,
+    # the two- and three-part BF16 splits 2^-16 and 2^-24).
+    bounds = {0: (1e-5, 2.0 ** -24), 1: (3e-3, 2.0 ** -8), 2: (4e-4, 2.0 ** -11),
+              3: (2e-5, 2.0 ** -16), 5: (1e-5, 2.0 ** -24)}
+    for prec, (base, u) in bounds.items():
+        if D != 1000 and prec in (1, 2, 3):
+            continue
+        rtol = composition_rtol(objs, X[:, :D], os_, Ms, base=base, u=u)
         f = torch.zeros(n, dtype=torch.float32, device=cuda)
         speo.evaluate(prob, _t(X, cuda), f, precision=prec)
         torch.cuda.synchronize()
         got = f.cpu().numpy()
         assert np.all(np.abs(got - ref) <= rtol * np.abs(ref) + 1e-4), (
-            np.max(np.abs(got - ref) / np.abs(ref)))
+            prec, np.max(np.abs(got - ref) / np.abs(ref)))
     f64 = torch.zeros(n, dtype=torch.float64, device=cuda)
     speo.evaluate_f64(prob, _t(X, cuda), f64)
     torch.cuda.synchronize()
     np.testing.assert_allclose(f64.cpu().numpy(), ref, rtol=1e-12, atol=1e-9)
+
+
+def test_composition_components_sharing_a_shift(cuda, speo, oracle):
+    # Two components with the same shift: at that point both have d = 0 and
+    # cf_cal gives each w = 1e99 (CEC2014.cc:1495-1498), so they share the
+    # weight equally and the third gets none.
+    import torch
+    D = 20
+    os_ = np.stack([oracle.gen_shift(s, D) for s in (31, 32, 33)])
+    os_[2] = os_[1]
+    Ms = np.stack([oracle.gen_rotation(s, D) for s in (31, 32, 33)])
+    objs = [1, 2, 3]
+    sigma, bias = np.array([10.0, 20.0, 30.0]), np.array([0.0, 100.0, 200.0])
+    prob = speo.Problem(objs, os_, Ms, sigma=sigma, bias=bias)
+    X = np.zeros((3, D), np.float32)
+    X[0] = os_[1]
+    X[1] = os_[0]
+    X[2] = os_[1] + 0.5
+    ref = oracle.composition(X, objs, os_, Ms, sigma, bias)
+    assert ref[0] == pytest.approx(150.0)          # (fit_2 + fit_3) / 2 = (100 + 200) / 2
+    f = torch.zeros(3, dtype=torch.float32, device=cuda)
+    speo.evaluate(prob, _t(X, cuda), f)
+    f64 = torch.zeros(3, dtype=torch.float64, device=cuda)
+    speo.evaluate_f64(prob, _t(X, cuda), f64)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-5)
+    np.testing.assert_allclose(f64.cpu().numpy(), ref, rtol=1e-12)
 
 
 def test_strided_rows_and_ragged_counts(cuda, speo, oracle):
