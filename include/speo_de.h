@@ -361,7 +361,33 @@ typedef struct {
      * whole island from HBM, evolves its share of the rows and writes it back;
      * the launch boundary is the only synchronisation (graph-capturable). */
     int epoch_parts;
+    /* The reference island's sub-islands and DE configuration (IslandInfo,
+     * config.h:62-84; DE_CUDA::Initilize / ConfigureEA, DE_CUDA.cc:57-119;
+     * Migrate::RegroupIslands at every migration point, migrate.cc:59,
+     * 147-186).  Anything but the defaults needs SPEO_MODE_PLAIN without
+     * graphs; the drawn sizes, permutations and configurations follow
+     * speo_island_regroup. */
+    int subisland_size;  /* rows per sub-island (>= 4, divides every island); 0 = whole island */
+    int de_strategy;     /* 0 = rand/1/bin; SPEO_DE_STRATEGY(id) for the reference's id 0..8  */
+    int configure_EA;    /* SPEO_CONFIGURE_*                                                  */
+    int regroup_option;  /* SPEO_REGROUP_*; the dynamic options need islands of 4 * 2^k rows  */
 } speo_engine_config;
+
+/* Sub-island options of the reference's island (IslandInfo::regroup_option and
+ * configure_EA, config.h:62-84, src/main.cc:62-63, 104-117). */
+enum {
+    SPEO_REGROUP_STATIC_ORDERED = 0,  /* "static_and_ordered" (the reference's default)  */
+    SPEO_REGROUP_STATIC_RANDOM = 1,   /* "static_and_random": new permutation           */
+    SPEO_REGROUP_DYNAMIC_RANDOM = 2,  /* "dynamic_and_random": new size and permutation */
+    SPEO_REGROUP_DYNAMIC_ORDERED = 3  /* "dynamic_and_ordered": new size               */
+};
+enum {
+    SPEO_CONFIGURE_CONSTANT = 0, /* "constant": every sub-island CR, F, strategy of the config */
+    SPEO_CONFIGURE_DYNAMIC = 1   /* "dynamic": CR in {0.1, 0.9}, strategy in {4, 6} drawn per
+                                    sub-island (DE_CUDA.cc:92-103)                           */
+};
+/* speo_engine_config.de_strategy from the reference's strategy ID 0..8 */
+#define SPEO_DE_STRATEGY(reference_id) ((reference_id) + 1)
 
 typedef struct speo_engine_s *speo_engine;
 
@@ -430,6 +456,27 @@ SPEO_EXPORT int speo_engine_profile(speo_engine e, int generations, double *avg_
  * Synchronous. */
 SPEO_EXPORT int speo_engine_surrogate_stats(speo_engine e, float *rho_out, int *active_out,
                                             float *loss_out);
+
+/* Island `island`'s current sub-island partition: *subisland_num_out, the
+ * permutation perm_out [island_size] (row perm[i] plays member i; members
+ * [s S, (s+1) S) form sub-island s) and info_out [subisland_num] (each may be
+ * NULL).  Synchronous. */
+SPEO_EXPORT int speo_engine_subislands(speo_engine e, int island, int *subisland_num_out,
+                                       int *perm_out, DEInfo *info_out);
+
+/* One regroup event of the reference's island, host side: event 0 is the
+ * initial configuration (DE_CUDA::Initilize -> ConfigureEA), event k > 0 the
+ * k-th migration point (Migrate::RegroupIslands, migrate.cc:147-186, then
+ * ConfigureEA for the dynamic options).  Draws: word k of Philox4x32-10 with
+ * key = seed and counter {k / 4, island, event, 19}; RandIntUnif(a, b) =
+ * a + w % (b - a + 1), no draw when a == b (src/random.cc:14-19); the
+ * permutation is Random::Permutate (src/random.cc:29-53).  perm [np] and
+ * info_out [np / 4] are in/out: the current partition and configuration on
+ * entry, the new ones on return; base holds the constant CR, F, strategy. */
+SPEO_EXPORT int speo_island_regroup(uint64_t seed, int island, int event, int regroup_option,
+                                    int configure_EA, int np, int subisland_num, int *perm,
+                                    const DEInfo *base, DEInfo *info_out,
+                                    int *subisland_num_out);
 
 /* Device pointers of the engine's buffers (for profiling and tests). */
 SPEO_EXPORT int speo_engine_buffers(speo_engine e, float **X, float **fX, int64_t *elem_off,

@@ -348,6 +348,82 @@ def gen_cec_problem(seed: int, func_id: int, D: int):
     return os_, Ms, SS
 
 
+# ------------------------------------------------- sub-island regroups
+
+PURPOSE_REGROUP = 19
+REGROUP_STATIC_ORDERED, REGROUP_STATIC_RANDOM, REGROUP_DYNAMIC_RANDOM, REGROUP_DYNAMIC_ORDERED = range(4)
+CONFIGURE_CONSTANT, CONFIGURE_DYNAMIC = 0, 1
+
+
+class HostDraws:
+    """Host-side draws of one regroup event: word k of Philox with counter
+    {k/4, island, event, PURPOSE_REGROUP}; rand_int(a, b) = a + w % (b - a + 1)
+    with no draw when a == b, as Random::RandIntUnif (src/random.cc:14-19)
+    with rand() replaced by the Philox word.  `words` (optional) injects the
+    words instead."""
+
+    def __init__(self, seed, island, event, words=None):
+        self.seed, self.island, self.event, self.k = seed, island, event, 0
+        self.words = words
+
+    def rand_int(self, a, b):
+        if a == b:
+            return a
+        if self.words is not None:
+            w = int(self.words[self.k])
+        else:
+            key = [self.seed & 0xFFFFFFFF, self.seed >> 32]
+            w = int(philox([self.k >> 2, self.island, self.event, PURPOSE_REGROUP], key)[self.k & 3])
+        self.k += 1
+        return a + w % (b - a + 1)
+
+
+def permutate(draws, n):
+    """Random::Permutate(n, n), src/random.cc:29-53: pick a random remaining
+    index, move the last remaining one into its place."""
+    g = list(range(n))
+    out = []
+    for i in range(n, 0, -1):
+        t = draws.rand_int(0, i - 1)
+        out.append(g[t])
+        g[t] = g[i - 1]
+    return out
+
+
+def island_regroup(seed, island, event, option, configure, np_, subs, perm, base, info,
+                   words=None):
+    """One regroup event (migrate.cc:147-186, DE_CUDA.cc:89-119): returns
+    (subisland_num, perm, [(CR, F, strategy, pop_size)] per sub-island).
+    Event 0 is DE_CUDA::Initilize's ConfigureEA.  base = (CR, F, strategy)."""
+    d = HostDraws(seed, island, event, words)
+    perm = list(perm)
+    configure_now = event == 0
+    if event > 0:
+        if option in (REGROUP_DYNAMIC_RANDOM, REGROUP_DYNAMIC_ORDERED):
+            max_index, p = 0, np_ // 4
+            while p > 1:
+                p //= 2
+                max_index += 1
+            size = 4 << d.rand_int(0, max_index)
+            subs = np_ // size
+            configure_now = True
+        if option in (REGROUP_DYNAMIC_RANDOM, REGROUP_STATIC_RANDOM):
+            perm = permutate(d, np_)
+    S = np_ // subs
+    out = []
+    for s in range(subs):
+        if configure_now and configure == CONFIGURE_DYNAMIC:
+            cr = np.float32([0.1, 0.9][d.rand_int(0, 1)])
+            st = [4, 6][d.rand_int(0, 1)]
+            out.append((float(cr), float(np.float32(base[1])), st, S))
+        elif configure_now:
+            out.append((float(np.float32(base[0])), float(np.float32(base[1])), int(base[2]), S))
+        else:
+            cr, f, st, _ = info[s]
+            out.append((cr, f, st, S))
+    return subs, perm, out
+
+
 def migrate_ring(X, fX, ld, off, dims, dst, k):
     X = np.array(X, np.float32, copy=True)
     fX = np.array(fX, np.float32, copy=True)

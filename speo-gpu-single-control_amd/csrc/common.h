@@ -49,6 +49,7 @@ enum Purpose : uint32_t {
     kPurposeShift = 16, // problem shift vector
     kPurposeGauss = 17, // problem N(0,1) matrix
     kPurposeProblem = 18,
+    kPurposeRegroup = 19,      // host draws of sub-island regroups and DE configuration
     kPurposeSurrogate = 32,    // MLP init
     kPurposeSurrBatch = 33,    // training minibatch rows
     kPurposeCandidate = 64     // candidate c >= 1: donors 64 + 2c, crossover 65 + 2c

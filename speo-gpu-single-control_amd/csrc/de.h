@@ -106,6 +106,9 @@ int de_generate(const GenArgs &a, hipStream_t s);
 int de_select(float *X, float *fX, const float *U, float *fU, int ld, int n, int dim,
               uint8_t *accept_out, int clear_trials, hipStream_t s);
 int segment_argmin(const float *f, const int *off, int nseg, int *best_out, hipStream_t s);
+// Best slot of each of `subs` sub-islands of size S (members perm[s S ..]),
+// ties to the lower slot, NaN last.
+int subisland_best(const float *f, const int *perm, int S, int subs, int *best_out, hipStream_t s);
 int migrate_ring(float *X, float *fX, const IslandDesc *isl_desc_dev, const int *row_off_dev,
                  const int *dst_dev, int *best_buf, int *worst_buf, int I, int k,
                  hipStream_t s);

@@ -211,6 +211,12 @@ __global__ void k_subisland_best(const float *f, const int *perm, int S, int *be
     block_argmin(f, perm, blockIdx.x * S, S, best_out + blockIdx.x);
 }
 
+int subisland_best(const float *f, const int *perm, int S, int subs, int *best_out, hipStream_t s) {
+    if (subs <= 0) return SPEO_OK;
+    hipLaunchKernelGGL(k_subisland_best, dim3(subs), dim3(256), 0, s, f, perm, S, best_out);
+    return check_launch("k_subisland_best");
+}
+
 int segment_argmin(const float *f, const int *off, int nseg, int *best_out, hipStream_t s) {
     if (nseg <= 0) return SPEO_OK;
     hipLaunchKernelGGL(k_segment_argmin, dim3(nseg), dim3(256), 0, s, f, off, best_out);

@@ -380,6 +380,21 @@ struct Engine {
     std::vector<speo_problem> problems;
     bool all_small = true;  // every dim <= kTLD
     bool any_generic = false;  // some problem needs the generic kernel (plain mode only)
+    // The reference island's sub-islands and DE configuration (plain mode):
+    // per island the sub-island count, permutation and DEInfo per sub-island.
+    bool configured = false;
+    int regroup_events = 0;
+    std::vector<int> subs_h;
+    std::vector<std::vector<int>> perm_h;
+    std::vector<std::vector<DEInfo>> info_h;
+    std::vector<DEInfo> base_h;
+    std::vector<int> doff;           // island -> first DEInfo / best slot
+    int *perm_dev = nullptr, *best_sub_dev = nullptr;
+    DEInfo *info_dev = nullptr;
+    int configure_setup();
+    int regroup(bool initial);
+    int generate_configured(int gen_offset, hipStream_t s);
+    int subislands(int island, int *subs_out, int *perm_out, DEInfo *info_out);
     int eval_generic_islands(const float *src, float *out, hipStream_t s);
 
     IslandDesc *isl_dev = nullptr;
@@ -474,7 +489,7 @@ Engine::~Engine() {
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
                     blocks_dev, Xnext, donor_buf, row_island_dev,
                     X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
-                    AX, Ay, Xb, tb, Xh, yh, predh};
+                    AX, Ay, Xb, tb, Xh, yh, predh, perm_dev, best_sub_dev, info_dev};
     for (void *b : bufs) (void)hipFree(b);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -628,6 +643,29 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
         hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream, fU[b], total_rows, INFINITY);
         SPEO_TRY(check_launch("k_fill"));
     }
+    configured = c.subisland_size != 0 || c.de_strategy != 0 || c.configure_EA != 0 ||
+                 c.regroup_option != 0;
+    if (configured) {
+        if (c.mode != SPEO_MODE_PLAIN || c.use_graph || c.surrogate_candidates > 0)
+            return set_error(SPEO_ERR_UNSUPPORTED, "speo_engine_create: sub-islands, strategies and "
+                                                   "regroups run in SPEO_MODE_PLAIN without graphs "
+                                                   "or the surrogate");
+        if (c.de_strategy < 0 || c.de_strategy > 9 || c.configure_EA < 0 || c.configure_EA > 1 ||
+            c.regroup_option < 0 || c.regroup_option > 3 || c.subisland_size < 0 ||
+            (c.subisland_size > 0 && c.subisland_size < 4))
+            return set_error(SPEO_ERR_ARG, "speo_engine_create: bad sub-island options");
+        for (int i = 0; i < I; ++i) {
+            const int np = isl[(size_t)i].np;
+            if (c.subisland_size > 0 && np % c.subisland_size)
+                return set_error(SPEO_ERR_ARG, "speo_engine_create: subisland_size must divide "
+                                               "every island size");
+            if ((c.regroup_option == SPEO_REGROUP_DYNAMIC_RANDOM ||
+                 c.regroup_option == SPEO_REGROUP_DYNAMIC_ORDERED) &&
+                ((np & (np - 1)) != 0 || np < 4))
+                return set_error(SPEO_ERR_ARG, "speo_engine_create: dynamic regroups need islands "
+                                               "of 4 * 2^k rows");
+        }
+    }
     if (c.surrogate_candidates > 0 && (c.mode != SPEO_MODE_PLAIN || c.use_graph))
         return set_error(SPEO_ERR_UNSUPPORTED, "speo_engine_create: the surrogate runs in "
                                                "SPEO_MODE_PLAIN without graphs");
@@ -635,6 +673,7 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     rt = c.rows_per_block;
     if (rt == 0) SPEO_TRY(autotune());
     if (c.surrogate_candidates > 0) SPEO_TRY(surrogate_setup());
+    if (configured) SPEO_TRY(configure_setup());
     SPEO_TRY(build_tiles(rt));
     // Initial evaluation (island_EA.cc:81 counts it as island_size FEs).
     GenerationArgs a{};
@@ -956,6 +995,97 @@ int Engine::launch(int mode, int gen_offset, int rt_, hipStream_t s, int clear) 
     return launch_any(rt_, cfg.precision, mode, a, ntiles[rt_], s);
 }
 
+// ------------------------------------------------- sub-islands (plain mode)
+
+int Engine::configure_setup() {
+    doff.assign((size_t)I + 1, 0);
+    subs_h.assign((size_t)I, 1);
+    perm_h.resize((size_t)I);
+    info_h.resize((size_t)I);
+    base_h.resize((size_t)I);
+    for (int i = 0; i < I; ++i) {
+        const IslandDesc &d = isl[(size_t)i];
+        doff[(size_t)i + 1] = doff[(size_t)i] + d.np / 4;
+        subs_h[(size_t)i] = cfg.subisland_size > 0 ? d.np / cfg.subisland_size : 1;
+        perm_h[(size_t)i].resize((size_t)d.np);
+        for (int r = 0; r < d.np; ++r) perm_h[(size_t)i][(size_t)r] = r;
+        info_h[(size_t)i].assign((size_t)(d.np / 4), DEInfo{});
+        // DE_CUDA::Initilize: CR 0.9, F 0.5, strategy 4 (DE_CUDA.cc:61-63);
+        // here the engine's per-island F, CR and strategy
+        base_h[(size_t)i] = DEInfo{d.CR, d.F, cfg.de_strategy ? cfg.de_strategy - 1 : 4, -1};
+    }
+    SPEO_TRY_HIP(hipMalloc(&perm_dev, sizeof(int) * (size_t)total_rows));
+    SPEO_TRY_HIP(hipMalloc(&info_dev, sizeof(DEInfo) * (size_t)std::max(1, doff[(size_t)I])));
+    SPEO_TRY_HIP(hipMalloc(&best_sub_dev, sizeof(int) * (size_t)std::max(1, doff[(size_t)I])));
+    return regroup(true);
+}
+
+// One regroup event of every island (event 0: the initial ConfigureEA),
+// then the new partitions and configurations uploaded.
+int Engine::regroup(bool initial) {
+    const int event = initial ? 0 : ++regroup_events;
+    const bool noop = !initial && cfg.regroup_option == SPEO_REGROUP_STATIC_ORDERED;
+    if (noop) return SPEO_OK;
+    SPEO_TRY_HIP(hipStreamSynchronize(stream));  // the previous upload's host arrays
+    for (int i = 0; i < I; ++i) {
+        const IslandDesc &d = isl[(size_t)i];
+        int subs = subs_h[(size_t)i];
+        SPEO_TRY(speo_island_regroup(cfg.seed, i, event, cfg.regroup_option, cfg.configure_EA,
+                                     d.np, subs, perm_h[(size_t)i].data(), &base_h[(size_t)i],
+                                     info_h[(size_t)i].data(), &subs));
+        subs_h[(size_t)i] = subs;
+        SPEO_TRY_HIP(hipMemcpy(perm_dev + d.row_off, perm_h[(size_t)i].data(), sizeof(int) * d.np,
+                               hipMemcpyHostToDevice));
+        SPEO_TRY_HIP(hipMemcpy(info_dev + doff[(size_t)i], info_h[(size_t)i].data(),
+                               sizeof(DEInfo) * (size_t)subs, hipMemcpyHostToDevice));
+    }
+    return SPEO_OK;
+}
+
+// The generation of every island through the reference's operator kernel
+// (sub-islands, permutation, per sub-island DEInfo, strategies 0-8), one
+// launch per island; strategies that use the best member get each
+// sub-island's best slot first (global_DE_FindBestIndividualInIsland).
+int Engine::generate_configured(int gen_offset, hipStream_t s) {
+    const uint32_t gen = (uint32_t)(gens + gen_offset);
+    for (int i = 0; i < I; ++i) {
+        const IslandDesc &d = isl[(size_t)i];
+        const int subs = subs_h[(size_t)i], S = d.np / subs;
+        bool need_best = false;
+        for (int q = 0; q < subs; ++q) {
+            const int st = info_h[(size_t)i][(size_t)q].strategy_ID;
+            need_best = need_best || st == 2 || st == 3 || st == 6 || st == 7;
+        }
+        int *best = best_sub_dev + doff[(size_t)i];
+        if (need_best) SPEO_TRY(subisland_best(fX[0] + d.row_off, perm_dev + d.row_off, S, subs, best, s));
+        GenArgs a{U[0] + d.elem_off, X + d.elem_off, d.ld, d.np, d.dim, S, perm_dev + d.row_off,
+                  info_dev + doff[(size_t)i], need_best ? best : nullptr, cfg.min_bound,
+                  cfg.max_bound, cfg.seed, gen, (uint32_t)d.row_off, nullptr, 0, nullptr, nullptr,
+                  nullptr};
+        SPEO_TRY(de_generate(a, s));
+    }
+    return SPEO_OK;
+}
+
+int Engine::subislands(int island, int *subs_out, int *perm_out, DEInfo *info_out) {
+    if (island < 0 || island >= I) return set_error(SPEO_ERR_ARG, "speo_engine_subislands: bad island");
+    const int np = isl[(size_t)island].np;
+    if (!configured) {
+        if (subs_out) *subs_out = 1;
+        if (perm_out)
+            for (int r = 0; r < np; ++r) perm_out[r] = r;
+        if (info_out) info_out[0] = DEInfo{isl[(size_t)island].CR, isl[(size_t)island].F, 4, np};
+        return SPEO_OK;
+    }
+    SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    const int subs = subs_h[(size_t)island];
+    if (subs_out) *subs_out = subs;
+    if (perm_out) std::copy(perm_h[(size_t)island].begin(), perm_h[(size_t)island].end(), perm_out);
+    if (info_out)
+        for (int q = 0; q < subs; ++q) info_out[q] = info_h[(size_t)island][(size_t)q];
+    return SPEO_OK;
+}
+
 // Islands whose problem the generic kernel evaluates: one launch per island.
 int Engine::eval_generic_islands(const float *src, float *out, hipStream_t s) {
     if (!any_generic) return SPEO_OK;
@@ -998,7 +1128,10 @@ int Engine::enqueue_chunk(int L, bool do_migrate, hipStream_t s) {
                 p = 1 - p;
                 pending = true;
             } else {
-                SPEO_TRY(launch(kModeGen, g, rt, s));
+                if (configured)
+                    SPEO_TRY(generate_configured(g, s));
+                else
+                    SPEO_TRY(launch(kModeGen, g, rt, s));
                 SPEO_TRY(launch(kModeEval, g, rt, s));
                 SPEO_TRY(eval_generic_islands(U[0], fU[0], s));
                 SPEO_TRY(launch(kModeSelect, g, rt, s));
@@ -1052,6 +1185,21 @@ int Engine::run(int generations, bool dry) {
             fes += total_rows;
             if (mig) ++migrations;
             if (gens % cfg.surrogate_train_interval == 0) SPEO_TRY(surrogate_train(stream));
+        }
+        return SPEO_OK;
+    }
+    if (configured && !dry) {
+        // generation by generation: after generation g with g % interval == 0
+        // the ring migration and then the regroup of every island
+        // (island_EA.cc:94-99; migrate.cc:59)
+        for (int g = 0; g < generations; ++g) {
+            const bool point = cfg.interval > 0 && gens % cfg.interval == 0;
+            const bool mig = point && cfg.migration_size > 0;
+            SPEO_TRY(enqueue_chunk(1, mig, stream));
+            ++gens;
+            fes += total_rows;
+            if (mig) ++migrations;
+            if (point) SPEO_TRY(regroup(false));
         }
         return SPEO_OK;
     }
@@ -1137,18 +1285,23 @@ int Engine::profile(int generations, double *avg_ms) {
             for (int m = 0; m < 3; ++m) {
                 SPEO_TRY(record(m));
                 SPEO_TRY_HIP(hipEventRecord(ev[ev.size() - 2], stream));
-                SPEO_TRY(launch(modes[m], 0, rt, stream));
+                if (modes[m] == kModeGen && configured)
+                    SPEO_TRY(generate_configured(0, stream));
+                else
+                    SPEO_TRY(launch(modes[m], 0, rt, stream));
                 if (modes[m] == kModeEval) SPEO_TRY(eval_generic_islands(U[0], fU[0], stream));
                 SPEO_TRY_HIP(hipEventRecord(ev.back(), stream));
             }
         }
-        const bool mig = cfg.interval > 0 && cfg.migration_size > 0 && gi % cfg.interval == 0;
+        const bool point = cfg.interval > 0 && gi % cfg.interval == 0;
+        const bool mig = point && cfg.migration_size > 0;
         if (mig) SPEO_TRY(migrate(stream));
         hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, stream, gen_base, 1u);
         SPEO_TRY(check_launch("k_add_u32"));
         ++gens;
         fes += total_rows;
         if (mig) ++migrations;
+        if (point && configured) SPEO_TRY(regroup(false));
     }
     SPEO_TRY_HIP(hipStreamSynchronize(stream));
     double sum[roles] = {0, 0, 0, 0, 0, 0};
@@ -1423,6 +1576,12 @@ int speo_engine_run_fes(speo_engine e, int64_t max_FEs, int *generations_out) {
     if (g > 0x7fffffff) return set_error(SPEO_ERR_ARG, "speo_engine_run_fes: budget beyond 2^31 generations");
     if (generations_out) *generations_out = (int)g;
     return e->e.run((int)g);
+}
+
+int speo_engine_subislands(speo_engine e, int island, int *subisland_num_out, int *perm_out,
+                           DEInfo *info_out) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_subislands: null engine");
+    return e->e.subislands(island, subisland_num_out, perm_out, info_out);
 }
 
 int speo_engine_prepare(speo_engine e, int generations) {

@@ -25,7 +25,9 @@ from ._lib import (DEInfo, EngineConfig, EngineStats, GPU_IslandInfo, GPU_Popula
                    OBJ_ELLIPTIC, OBJ_ESCAFFER6, OBJ_GRIE_ROSEN, OBJ_GRIEWANK, OBJ_HAPPYCAT,
                    OBJ_HGBAT, OBJ_HYBRID_1, OBJ_HYBRID_2, OBJ_HYBRID_3, OBJ_HYBRID_4,
                    OBJ_HYBRID_5, OBJ_HYBRID_6, OBJ_KATSUURA, OBJ_RASTRIGIN, OBJ_ROSENBROCK,
-                   OBJ_SCHWEFEL, OBJ_WEIERSTRASS, PREC_BF16,
+                   OBJ_SCHWEFEL, OBJ_WEIERSTRASS, PREC_BF16, REGROUP_STATIC_ORDERED,
+                   REGROUP_STATIC_RANDOM, REGROUP_DYNAMIC_RANDOM, REGROUP_DYNAMIC_ORDERED,
+                   CONFIGURE_CONSTANT, CONFIGURE_DYNAMIC,
                    PREC_BF16X3, PREC_BF16X6, PREC_FP16, PREC_FP16X3, PREC_FP32, PREC_NAMES,
                    ProblemInfo, SPEO_RandState, SpeoError, check, last_error, lib)
 
@@ -38,6 +40,8 @@ __all__ = [
     "OBJ_HYBRID_3", "OBJ_HYBRID_4", "OBJ_HYBRID_5", "OBJ_HYBRID_6", "PREC_FP32", "PREC_BF16", "PREC_FP16",
     "PREC_BF16X3", "PREC_FP16X3", "PREC_BF16X6", "PREC_NAMES", "MODE_PLAIN", "MODE_FUSED",
     "MODE_EPOCH",    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
+    "REGROUP_STATIC_ORDERED", "REGROUP_STATIC_RANDOM", "REGROUP_DYNAMIC_RANDOM",
+    "REGROUP_DYNAMIC_ORDERED", "CONFIGURE_CONSTANT", "CONFIGURE_DYNAMIC", "island_regroup",
     "generations_for_fes", "run_seed", "load_cec_data", "de_parameters", "result_header", "result_record",
 ]
 
@@ -127,6 +131,23 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+
+def island_regroup(seed, island, event, option, configure, np_, subs, perm, base, info=None):
+    """speo_island_regroup: (subisland_num, perm, [(CR, F, strategy, pop_size)]).
+    base = (CR, F, strategy); info = the current per-sub-island configuration."""
+    perm_a = np.ascontiguousarray(perm, np.int32).copy()
+    arr = (DEInfo * max(1, np_ // 4))()
+    if info is not None:
+        for s, (cr, f, st, ps) in enumerate(info):
+            arr[s] = DEInfo(cr, f, st, ps)
+    b = DEInfo(base[0], base[1], base[2], -1)
+    out = c_int()
+    check(lib().speo_island_regroup(seed, island, event, option, configure, np_, subs,
+                                    _np_ptr(perm_a), byref(b), ctypes.cast(arr, c_void_p),
+                                    byref(out)))
+    return out.value, perm_a, [(arr[s].CR, arr[s].F, arr[s].strategy_ID, arr[s].pop_size)
+                               for s in range(out.value)]
 
 
 def load_cec_data(directory, func_id: int, dim: int, ncomp: int = 1):
@@ -243,7 +264,9 @@ class Engine:
                  mode=MODE_FUSED, use_graph=False, rows_per_block=0, island_F=None,
                  island_CR=None, surrogate_candidates=0, surrogate_train_interval=10,
                  surrogate_archive=65536, surrogate_steps=40, surrogate_batch=1024,
-                 surrogate_lr=1e-3, surrogate_rho_min=0.5, epoch_parts=0):
+                 surrogate_lr=1e-3, surrogate_rho_min=0.5, epoch_parts=0, subisland_size=0,
+                 strategy=4, configure_EA=CONFIGURE_CONSTANT,
+                 regroup_option=REGROUP_STATIC_ORDERED):
         self.problems = list(problems)
         self.island_size = np.ascontiguousarray(island_size, np.int32)
         self.island_problem = np.ascontiguousarray(island_problem, np.int32)
@@ -271,6 +294,10 @@ class Engine:
         cfg.surrogate_lr = surrogate_lr
         cfg.surrogate_rho_min = surrogate_rho_min
         cfg.epoch_parts = epoch_parts
+        cfg.subisland_size = subisland_size
+        cfg.de_strategy = 0 if strategy == 4 else strategy + 1
+        cfg.configure_EA = configure_EA
+        cfg.regroup_option = regroup_option
         self.cfg = cfg
         h = c_void_p()
         check(lib().speo_engine_create(byref(cfg), None, byref(h)))
@@ -282,6 +309,17 @@ class Engine:
 
     def run(self, generations: int):
         check(lib().speo_engine_run(self.handle, generations))
+
+    def subislands(self, island: int):
+        """(subisland_num, perm [island_size], [(CR, F, strategy, pop_size)])"""
+        n = int(self.island_size[island])
+        subs = c_int()
+        perm = np.zeros(n, np.int32)
+        info = (DEInfo * max(1, n // 4))()
+        check(lib().speo_engine_subislands(self.handle, island, byref(subs), _np_ptr(perm),
+                                           ctypes.cast(info, c_void_p)))
+        return subs.value, perm, [(info[s].CR, info[s].F, info[s].strategy_ID, info[s].pop_size)
+                                  for s in range(subs.value)]
 
     def prepare(self, generations: int):
         """Capture the graphs the next run(generations) launches (no work)."""

@@ -26,6 +26,8 @@ PREC_FP32, PREC_BF16, PREC_FP16, PREC_BF16X3, PREC_FP16X3, PREC_BF16X6 = range(6
 PREC_NAMES = {PREC_FP32: "fp32", PREC_BF16: "bf16", PREC_FP16: "fp16", PREC_BF16X3: "bf16x3",
               PREC_FP16X3: "fp16x3", PREC_BF16X6: "bf16x6"}
 MODE_PLAIN, MODE_FUSED, MODE_EPOCH = 0, 1, 2
+REGROUP_STATIC_ORDERED, REGROUP_STATIC_RANDOM, REGROUP_DYNAMIC_RANDOM, REGROUP_DYNAMIC_ORDERED = range(4)
+CONFIGURE_CONSTANT, CONFIGURE_DYNAMIC = 0, 1
 
 
 class SpeoError(RuntimeError):
@@ -69,7 +71,9 @@ class EngineConfig(Structure):
                 ("surrogate_candidates", c_int), ("surrogate_train_interval", c_int),
                 ("surrogate_archive", c_int), ("surrogate_steps", c_int),
                 ("surrogate_batch", c_int), ("surrogate_lr", c_float),
-                ("surrogate_rho_min", c_float), ("epoch_parts", c_int)]
+                ("surrogate_rho_min", c_float), ("epoch_parts", c_int),
+                ("subisland_size", c_int), ("de_strategy", c_int), ("configure_EA", c_int),
+                ("regroup_option", c_int)]
 
 
 class RunRecord(Structure):
@@ -130,6 +134,9 @@ _SIGS = {
     "speo_engine_destroy": (c_int, [c_void_p]),
     "speo_engine_run": (c_int, [c_void_p, c_int]),
     "speo_engine_prepare": (c_int, [c_void_p, c_int]),
+    "speo_engine_subislands": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "speo_island_regroup": (c_int, [c_uint64, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
     "speo_engine_sync": (c_int, [c_void_p]),
     "speo_engine_best": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "speo_engine_read": (c_int, [c_void_p, c_void_p, c_void_p]),
