@@ -551,6 +551,49 @@ typedef struct speo_run_record {
         *EA_parameters;
 } speo_run_record;
 
+/* Runs until the evaluation count reaches max_FEs, as speo_engine_run_fes,
+ * and measures it: *seconds_out is the wall time from the call to the
+ * completion of the last generation, *comm_seconds_out the part spent at
+ * migration points (migration and regroup run between synchronisations, as
+ * the reference brackets MigrateOut / MigrateIn with MPI_Wtime,
+ * island_EA.cc:94-104).  Results equal speo_engine_run_fes's.  Synchronous. */
+SPEO_EXPORT int speo_engine_run_fes_timed(speo_engine e, int64_t max_FEs, int *generations_out,
+                                          double *seconds_out, double *comm_seconds_out);
+
+/* The reference's function x run sweep on this GPU (ConstructAndExecuteIslandModule,
+ * src/main.cc:124-150; IslandEA::Initilize / Execute / PrintResult,
+ * src/island_EA.cc:35-171): for every function and run, the seed
+ * speo_run_seed(node_ID, f, r), the CEC2014 problem from data_dir, an engine of
+ * island_num islands of island_size rows, the evaluation budget
+ * max_base_FEs * dim over all islands, the FP64-verified best, the measured
+ * time and communication share, and one row appended to record_file (created
+ * with the header line when absent, island_EA.cc:13-31).  Strings of the
+ * record: migration_topology "ring", buffer_manage "none" (ring migration of
+ * the best rows needs no immigrant buffer), regroup_option and configure_EA
+ * as the reference spells them. */
+typedef struct {
+    const char *data_dir;      /* the reference's input_data directory            */
+    const char *record_file;   /* CSV path; NULL = no file                        */
+    int dim;
+    const int *function_IDs;   /* host [num_functions], 1..30                      */
+    int num_functions;
+    const int *run_IDs;        /* host [num_runs]                                  */
+    int num_runs;
+    int node_ID, node_num, GPU_num;
+    int island_num, island_size;
+    int max_base_FEs;
+    int interval, migration_size;
+    double migration_rate;     /* recorded only                                    */
+    int buffer_capacity;       /* recorded only                                    */
+    int subisland_size, configure_EA, regroup_option;
+    int mode, precision, use_graph;
+    float min_bound, max_bound;
+} speo_driver_config;
+
+/* rows_out [num_functions * num_runs] (may be NULL) receives the records; their
+ * string fields point to static storage.  Synchronous. */
+SPEO_EXPORT int speo_island_driver(const speo_driver_config *cfg, speo_run_record *rows_out);
+
 /* The header line of the record file (no newline). */
 SPEO_EXPORT const char *speo_result_header(void);
 /* The row as the reference's stream output writes it (numbers with six

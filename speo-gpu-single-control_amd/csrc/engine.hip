@@ -384,6 +384,13 @@ struct Engine {
     // per island the sub-island count, permutation and DEInfo per sub-island.
     bool configured = false;
     int regroup_events = 0;
+    // measured communication (run_fes_timed): migrations (and regroups) run
+    // eagerly between synchronisations and their wall time is summed, as the
+    // reference's MPI_Wtime brackets around MigrateOut / MigrateIn
+    // (island_EA.cc:94-104)
+    bool time_comm = false;
+    double comm_seconds = 0.0;
+    int timed_point(bool mig, bool point);
     std::vector<int> subs_h;
     std::vector<std::vector<int>> perm_h;
     std::vector<std::vector<DEInfo>> info_h;
@@ -1160,6 +1167,19 @@ int Engine::next_chunk(int left, bool *mig) const {
     return left;
 }
 
+// A migration point of a timed run: the work before it completes, then the
+// migration (and, for configured engines, the regroup) runs eagerly and its
+// wall time until completion is added to comm_seconds.
+int Engine::timed_point(bool mig, bool point) {
+    SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (mig) SPEO_TRY(migrate(stream));
+    if (point && configured) SPEO_TRY(regroup(false));
+    SPEO_TRY_HIP(hipStreamSynchronize(stream));
+    comm_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return SPEO_OK;
+}
+
 int Engine::prepare(int generations) {
     if (C > 0 || !cfg.use_graph || graph_failed) return SPEO_OK;
     const int64_t g0 = gens, f0 = fes, m0 = migrations, gl0 = graph_launches;
@@ -1195,18 +1215,23 @@ int Engine::run(int generations, bool dry) {
         for (int g = 0; g < generations; ++g) {
             const bool point = cfg.interval > 0 && gens % cfg.interval == 0;
             const bool mig = point && cfg.migration_size > 0;
-            SPEO_TRY(enqueue_chunk(1, mig, stream));
+            SPEO_TRY(enqueue_chunk(1, mig && !time_comm, stream));
             ++gens;
             fes += total_rows;
             if (mig) ++migrations;
-            if (point) SPEO_TRY(regroup(false));
+            if (time_comm && point) {
+                SPEO_TRY(timed_point(mig, true));
+            } else if (point) {
+                SPEO_TRY(regroup(false));
+            }
         }
         return SPEO_OK;
     }
     int left = generations;
     while (left > 0) {
-        bool mig = false;
-        const int L = next_chunk(left, &mig);
+        bool mig_point = false;
+        const int L = next_chunk(left, &mig_point);
+        const bool mig = mig_point && !(time_comm && !dry);  // timed: migrate after the chunk
         const bool use_graph = cfg.use_graph && !graph_failed;
         if (use_graph) {
             const auto key = std::make_tuple(L, cfg.mode == SPEO_MODE_FUSED ? p : ep_par,
@@ -1249,9 +1274,10 @@ int Engine::run(int generations, bool dry) {
         } else if (!dry) {
             SPEO_TRY(enqueue_chunk(L, mig, stream));
         }
+        if (mig_point && !mig) SPEO_TRY(timed_point(true, false));
         gens += L;
         fes += (int64_t)L * total_rows;
-        if (mig) ++migrations;
+        if (mig_point) ++migrations;
         left -= L;
     }
     return SPEO_OK;
@@ -1576,6 +1602,26 @@ int speo_engine_run_fes(speo_engine e, int64_t max_FEs, int *generations_out) {
     if (g > 0x7fffffff) return set_error(SPEO_ERR_ARG, "speo_engine_run_fes: budget beyond 2^31 generations");
     if (generations_out) *generations_out = (int)g;
     return e->e.run((int)g);
+}
+
+int speo_engine_run_fes_timed(speo_engine e, int64_t max_FEs, int *generations_out,
+                              double *seconds_out, double *comm_seconds_out) {
+    if (!e) return set_error(SPEO_ERR_ARG, "speo_engine_run_fes_timed: null engine");
+    Engine &E = e->e;
+    const int64_t g = speo_generations_for_fes(max_FEs, E.fes, E.total_rows);
+    if (g > 0x7fffffff) return set_error(SPEO_ERR_ARG, "speo_engine_run_fes_timed: budget beyond 2^31 generations");
+    if (generations_out) *generations_out = (int)g;
+    SPEO_TRY_HIP(hipStreamSynchronize(E.stream));
+    E.time_comm = true;
+    E.comm_seconds = 0.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = E.run((int)g);
+    if (rc == SPEO_OK) rc = check_hip(hipStreamSynchronize(E.stream), "speo_engine_run_fes_timed");
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    E.time_comm = false;
+    if (seconds_out) *seconds_out = dt;
+    if (comm_seconds_out) *comm_seconds_out = E.comm_seconds;
+    return rc;
 }
 
 int speo_engine_subislands(speo_engine e, int island, int *subisland_num_out, int *perm_out,

@@ -20,7 +20,7 @@ from ctypes import POINTER, byref, c_double, c_float, c_int, c_void_p
 
 import numpy as np
 
-from ._lib import (DEInfo, EngineConfig, EngineStats, GPU_IslandInfo, GPU_Population,
+from ._lib import (DEInfo, DriverConfig, EngineConfig, RunRecord, EngineStats, GPU_IslandInfo, GPU_Population,
                    MODE_EPOCH, MODE_FUSED, MODE_PLAIN, OBJ_ACKLEY, OBJ_BENT_CIGAR, OBJ_DISCUS,
                    OBJ_ELLIPTIC, OBJ_ESCAFFER6, OBJ_GRIE_ROSEN, OBJ_GRIEWANK, OBJ_HAPPYCAT,
                    OBJ_HGBAT, OBJ_HYBRID_1, OBJ_HYBRID_2, OBJ_HYBRID_3, OBJ_HYBRID_4,
@@ -42,7 +42,7 @@ __all__ = [
     "MODE_EPOCH",    "DEInfo", "GPU_Population", "GPU_IslandInfo", "ProblemInfo", "SPEO_RandState", "lib",
     "REGROUP_STATIC_ORDERED", "REGROUP_STATIC_RANDOM", "REGROUP_DYNAMIC_RANDOM",
     "REGROUP_DYNAMIC_ORDERED", "CONFIGURE_CONSTANT", "CONFIGURE_DYNAMIC", "island_regroup",
-    "generations_for_fes", "run_seed", "load_cec_data", "de_parameters", "result_header", "result_record",
+    "island_driver", "generations_for_fes", "run_seed", "load_cec_data", "de_parameters", "result_header", "result_record",
 ]
 
 
@@ -148,6 +148,35 @@ def island_regroup(seed, island, event, option, configure, np_, subs, perm, base
                                     byref(out)))
     return out.value, perm_a, [(arr[s].CR, arr[s].F, arr[s].strategy_ID, arr[s].pop_size)
                                for s in range(out.value)]
+
+
+def island_driver(data_dir, functions, runs, dim, island_num, island_size, max_base_FEs,
+                  record_file=None, node_ID=0, node_num=1, GPU_num=1, interval=10,
+                  migration_size=2, migration_rate=0.5, buffer_capacity=32, subisland_size=0,
+                  configure_EA=CONFIGURE_CONSTANT, regroup_option=REGROUP_STATIC_ORDERED,
+                  mode=MODE_PLAIN, precision=PREC_FP32, use_graph=False, lo=-100.0, hi=100.0):
+    """speo_island_driver: the function x run sweep; returns the records as dicts."""
+    f = np.ascontiguousarray(functions, np.int32)
+    r = np.ascontiguousarray(runs, np.int32)
+    c = DriverConfig()
+    c.data_dir = str(data_dir).encode()
+    c.record_file = None if record_file is None else str(record_file).encode()
+    c.dim, c.num_functions, c.num_runs = dim, f.size, r.size
+    c.function_IDs, c.run_IDs = _np_ptr(f), _np_ptr(r)
+    c.node_ID, c.node_num, c.GPU_num = node_ID, node_num, GPU_num
+    c.island_num, c.island_size, c.max_base_FEs = island_num, island_size, max_base_FEs
+    c.interval, c.migration_size = interval, migration_size
+    c.migration_rate, c.buffer_capacity = migration_rate, buffer_capacity
+    c.subisland_size, c.configure_EA, c.regroup_option = subisland_size, configure_EA, regroup_option
+    c.mode, c.precision, c.use_graph = mode, precision, int(use_graph)
+    c.min_bound, c.max_bound = lo, hi
+    rows = (RunRecord * max(1, f.size * r.size))()
+    check(lib().speo_island_driver(byref(c), ctypes.cast(rows, c_void_p)))
+    out = []
+    for k in range(f.size * r.size):
+        d = {name: getattr(rows[k], name) for name, _ in RunRecord._fields_}
+        out.append({k2: (v.decode() if isinstance(v, bytes) else v) for k2, v in d.items()})
+    return out
 
 
 def load_cec_data(directory, func_id: int, dim: int, ncomp: int = 1):
@@ -324,6 +353,13 @@ class Engine:
     def prepare(self, generations: int):
         """Capture the graphs the next run(generations) launches (no work)."""
         check(lib().speo_engine_prepare(self.handle, generations))
+
+    def run_fes_timed(self, max_fes: int):
+        """(generations, seconds, communication seconds) of the budget loop."""
+        g, t, c = c_int(0), c_double(0.0), c_double(0.0)
+        check(lib().speo_engine_run_fes_timed(self.handle, int(max_fes), byref(g), byref(t),
+                                              byref(c)))
+        return g.value, t.value, c.value
 
     def run_fes(self, max_fes: int) -> int:
         """Generations until the evaluation count reaches max_fes (the budget

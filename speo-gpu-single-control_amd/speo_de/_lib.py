@@ -88,6 +88,18 @@ class RunRecord(Structure):
                 ("regroup_option", c_char_p), ("configure_EA", c_char_p), ("EA_parameters", c_char_p)]
 
 
+class DriverConfig(Structure):
+    """speo_driver_config: the reference's function x run sweep on one GPU."""
+    _fields_ = [("data_dir", c_char_p), ("record_file", c_char_p), ("dim", c_int),
+                ("function_IDs", c_void_p), ("num_functions", c_int), ("run_IDs", c_void_p),
+                ("num_runs", c_int), ("node_ID", c_int), ("node_num", c_int), ("GPU_num", c_int),
+                ("island_num", c_int), ("island_size", c_int), ("max_base_FEs", c_int),
+                ("interval", c_int), ("migration_size", c_int), ("migration_rate", c_double),
+                ("buffer_capacity", c_int), ("subisland_size", c_int), ("configure_EA", c_int),
+                ("regroup_option", c_int), ("mode", c_int), ("precision", c_int),
+                ("use_graph", c_int), ("min_bound", c_float), ("max_bound", c_float)]
+
+
 class EngineStats(Structure):
     _fields_ = [("generations", c_int64), ("evaluations", c_int64), ("migrations", c_int64),
                 ("graph_launches", c_int64), ("rows_per_block", c_int),
@@ -134,6 +146,9 @@ _SIGS = {
     "speo_engine_destroy": (c_int, [c_void_p]),
     "speo_engine_run": (c_int, [c_void_p, c_int]),
     "speo_engine_prepare": (c_int, [c_void_p, c_int]),
+    "speo_engine_run_fes_timed": (c_int, [c_void_p, c_int64, POINTER(c_int), POINTER(c_double),
+                                          POINTER(c_double)]),
+    "speo_island_driver": (c_int, [c_void_p, c_void_p]),
     "speo_engine_subislands": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "speo_island_regroup": (c_int, [c_uint64, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
                                     c_void_p, c_void_p, c_void_p]),
