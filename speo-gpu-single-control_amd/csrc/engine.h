@@ -55,6 +55,7 @@ struct EpochBlock {
     float rate, lambda, bias;
     const float *shift, *B32s, *colw;
     const uint16_t *M16s[3];
+    HeavyCol heavy;            // heavy.j < 0: none
 };
 
 struct EpochArgs {

@@ -257,6 +257,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 
                 reduce_rows<RT>(s1, s2, red, o1, o2);
                 if (tid < t.nrows) gval[c * RT + tid] = finish_objective_ext(comp.objective, o1[tid], o2[tid], dim);
             }
+            if (comp.heavy.j >= 0) {
+                // the heavy column's FP64 term, one wave per row
+                __syncthreads();
+                const int wave = tid >> 6, lane = tid & 63;
+                for (int r = wave; r < t.nrows; r += 4) {
+                    const double h = heavy_term(comp.heavy, src.rows + (size_t)r * src.ld, dim, lane);
+                    if (lane == 0) gval[c * RT + r] = with_heavy(gval[c * RT + r], (float)h);
+                }
+                __syncthreads();
+            }
         }
         if (tid < t.nrows) {
             float f;
@@ -779,6 +789,7 @@ int Engine::epoch_setup() {
         e.B32s = c.B32s;
         e.colw = c.colw;
         for (int q = 0; q < 3; ++q) e.M16s[q] = c.M16s[q];
+        e.heavy = c.heavy;
     }
     SPEO_TRY(upload_vec(&blocks_dev, blocks));
     std::vector<int> row_island((size_t)total_rows);

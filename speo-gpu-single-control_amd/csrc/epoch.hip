@@ -540,6 +540,15 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                 }
             }
             __syncthreads();
+            if (blk.heavy.j >= 0) {
+                // the heavy column's FP64 term, one wave per row, parked in
+                // the row's fU slot until the combine replaces it
+                for (int rl = wave; rl < nr; rl += kEpochThreads / 64) {
+                    const double h = heavy_term(blk.heavy, rows + aoff[rl], D, lane);
+                    if (lane == 0) fUs[p0 + rl] = (float)h;
+                }
+                __syncthreads();
+            }
             // the four column groups' partial sums, in column-group order
             for (int rl = tid; rl < nr; rl += kEpochThreads) {
                 float s1 = 0.f, s2 = 0.f;
@@ -548,6 +557,7 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
                     s1 += part[w * PR + rl];
                     s2 += part[(4 + w) * PR + rl];
                 }
+                if (blk.heavy.j >= 0) s1 = with_heavy(s1, fUs[p0 + rl]);
                 fUs[p0 + rl] = fmaf(lambda, finish_objective(blk.objective, s1, s2, D), bias);
             }
             __syncthreads();
@@ -910,6 +920,15 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
         __syncthreads();
     }
 
+    if (blk.heavy.j >= 0) {
+        // the heavy column's FP64 term, one wave per row, parked in the row's
+        // fU slot until the combine replaces it
+        for (int rl = wave; rl < nown; rl += kSplitThreads / 64) {
+            const double h = heavy_term(blk.heavy, trial + rl * ST, D, lane);
+            if (lane == 0) fUs[rl] = (float)h;
+        }
+        __syncthreads();
+    }
     // ---- combine (column groups 0..3 in order) and selection, strict "<"
     if (tid < nown) {
         float s1 = 0.f, s2 = 0.f;
@@ -918,6 +937,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
             s1 += part[w * OWN + tid];
             s2 += part[(4 + w) * OWN + tid];
         }
+        if (blk.heavy.j >= 0) s1 = with_heavy(s1, fUs[tid]);
         const float fu = fmaf(lambda, finish_objective(blk.objective, s1, s2, D), bias);
         const float fx = fXs[tid];
         const bool acc = fu < fx;

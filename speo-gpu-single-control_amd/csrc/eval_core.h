@@ -73,6 +73,21 @@ template <> struct Prec<SPEO_PREC_BF16X6> {
     static constexpr bool half = false;
 };
 
+// A weighted-sphere column that outweighs all others together (discus's z_0;
+// elliptic and bent cigar at small D): its weight is 0 in the FP32 weight
+// table and w z_j^2 is added in FP64 (heavy_term), since any FP32 rotation
+// loses the digits of a z_j that cancels.  j < 0: none.
+struct HeavyCol {
+    int j;
+    double w;
+    const double *m64;   // row j of M (FP64), or null: m32
+    const float *m32;    // M[j][k] at m32[k * ld]
+    int ld;
+    const double *o64;   // shift (FP64), or null: o32
+    const float *o32;
+    double rate;
+};
+
 // One objective component as the kernels see it.
 struct CompDev {
     int objective;           // SPEO_OBJ_*
@@ -93,6 +108,7 @@ struct CompDev {
     const float *B32s;
     const uint16_t *M16s[3];    // [128][kp32]: 16-bit parts of M (FP16 scaled by 2^14)
     int kp32;                   // round_up(dim, 32)
+    HeavyCol heavy;             // heavy.j < 0: none
 };
 
 constexpr int kSmallD = 128;  // widest dimension of the register-resident-B path
@@ -255,6 +271,28 @@ __device__ __forceinline__ float finish_objective_ext(int objective, float s1, f
         return sqrtf(fabsf((s1 - s2) * (s1 + s2))) + (0.5f * s1 + s2) / D + 0.5f;
     if (objective == SPEO_OBJ_SCHWEFEL) return fmaf(418.9828872724338f, D, s1 + s2 / D);
     return finish_objective(objective, s1, s2, dim);
+}
+
+// w z_j^2 of a heavy column for the row x (FP32 values,
+// global or LDS), z_j = sum_k M_jk (x_k - o_k) rate in FP64: lane-strided
+// partial sums, then the xor butterfly, so every kernel gets the same bits.
+// All 64 lanes of the wave call it; the value is returned on every lane.
+__device__ __forceinline__ double heavy_term(const HeavyCol &c, const float *x, int dim, int lane) {
+    double z = 0.0;
+    for (int k = lane; k < dim; k += 64) {
+        const double m = c.m64 ? c.m64[k] : (double)c.m32[(size_t)k * c.ld];
+        const double o = c.o64 ? c.o64[k] : (double)c.o32[k];
+        z = fma(m, ((double)x[k] - o) * c.rate, z);
+    }
+    for (int off = 32; off > 0; off >>= 1) z += __shfl_xor(z, off, 64);
+    return c.w * z * z;
+}
+
+// The FP32 row sum with the heavy column's term added (the term rounded to
+// FP32 first, so that kernels that park it in an FP32 slot of LDS get the
+// same bits).
+__device__ __forceinline__ float with_heavy(float s1, float h) {
+    return (float)((double)s1 + (double)h);
 }
 
 // Rows of the A tile: row r of the tile is `rows + r * ld` (global or LDS),

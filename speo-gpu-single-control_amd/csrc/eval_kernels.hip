@@ -76,6 +76,16 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch p) {
         o2 = o1 + RT;
         reduce_rows<RT>(s1, s2, red, o1, o2);
     }
+    if (p.comp.heavy.j >= 0 && chunk == 0) {
+        // the heavy column's FP64 term, one wave per row, into the row sums
+        // (the row staging area is free after the reductions' barriers)
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int r = wave; r < nrows; r += 4) {
+            const double h = heavy_term(p.comp.heavy, p.X + (size_t)(row0 + r) * p.ld, p.dim, lane);
+            if (lane == 0) o1[r] = with_heavy(o1[r], (float)h);
+        }
+        __syncthreads();
+    }
     const int t = threadIdx.x;
     if (t < RT && row0 + t < p.n) {
         const int row = row0 + t;
@@ -403,6 +413,15 @@ extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *d_rotat
     c.B32 = rotate ? d_rotated_data : t.identity;
     c.ldb32 = problem_dim;  // reference layout [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255)
     c.colw = t.colw[weights];
+    c.heavy.j = -1;
+    if (c.objective == SPEO_OBJ_ELLIPTIC) {
+        static const int kinds[3] = {SPEO_OBJ_ELLIPTIC, SPEO_OBJ_BENT_CIGAR, SPEO_OBJ_DISCUS};
+        c.heavy.j = heavy_column(kinds[weights], problem_dim, &c.heavy.w);
+        c.heavy.m32 = c.B32 + (c.heavy.j >= 0 ? c.heavy.j : 0);  // column j of B = row j of M
+        c.heavy.ld = problem_dim;
+        c.heavy.o32 = d_shift_data;
+        c.heavy.rate = 1.0;
+    }
     const int ld = next_pow2_dim(problem_dim);
     // Above 128 columns the per-chunk row sums go to the caller's rotated-row
     // buffer (pop_size * next_pow2(dim) floats, at least 2 * chunks * pop_size).

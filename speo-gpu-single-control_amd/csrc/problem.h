@@ -16,6 +16,10 @@ constexpr int kMaxComp = 8;
 // elliptic 10^(6 j / (D - 1)), bent cigar (1, 10^6, 10^6, ...), discus
 // (10^6, 1, 1, ...).  The FP32 kernels evaluate all three through one path.
 bool weighted_sphere(int objective);
+// The column that outweighs all others together (index), or -1; *w_out its
+// FP64 weight.  column_weights gives it weight 0: the kernels add its term
+// in FP64 (CompDev::heavy).
+int heavy_column(int objective, int dim, double *w_out);
 void column_weights(int objective, int dim, float *w);
 
 struct CompHost {

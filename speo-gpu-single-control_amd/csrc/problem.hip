@@ -50,15 +50,30 @@ bool weighted_sphere(int objective) {
 }
 
 // CEC2014.cc:488-496 (elliptic), 507-515 (bent cigar), 521-529 (discus).
-void column_weights(int objective, int dim, float *w) {
+static double weight64(int objective, int dim, int j) {
+    if (objective == SPEO_OBJ_BENT_CIGAR) return j == 0 ? 1.0 : 1.0e6;
+    if (objective == SPEO_OBJ_DISCUS) return j == 0 ? 1.0e6 : 1.0;
+    return std::pow(10.0, 6.0 * (double)j / (dim > 1 ? dim - 1 : 1));
+}
+
+int heavy_column(int objective, int dim, double *w_out) {
+    if (!weighted_sphere(objective) || dim < 2) return -1;
+    int jmax = 0;
+    double total = 0.0;
     for (int j = 0; j < dim; ++j) {
-        if (objective == SPEO_OBJ_BENT_CIGAR)
-            w[j] = j == 0 ? 1.0f : 1.0e6f;
-        else if (objective == SPEO_OBJ_DISCUS)
-            w[j] = j == 0 ? 1.0e6f : 1.0f;
-        else
-            w[j] = (float)std::pow(10.0, 6.0 * (double)j / (dim > 1 ? dim - 1 : 1));
+        total += weight64(objective, dim, j);
+        if (weight64(objective, dim, j) > weight64(objective, dim, jmax)) jmax = j;
     }
+    const double wj = weight64(objective, dim, jmax);
+    if (!(wj > total - wj)) return -1;
+    if (w_out) *w_out = wj;
+    return jmax;
+}
+
+void column_weights(int objective, int dim, float *w) {
+    for (int j = 0; j < dim; ++j) w[j] = (float)weight64(objective, dim, j);
+    const int h = heavy_column(objective, dim, nullptr);
+    if (h >= 0) w[h] = 0.0f;  // added in FP64 by the kernels (CompDev::heavy)
 }
 
 CompDev ProblemHost::view(int c, int precision) const {
@@ -78,6 +93,11 @@ CompDev ProblemHost::view(int c, int precision) const {
     d.B32s = h.B32s;
     for (int p = 0; p < 3; ++p) d.M16s[p] = half ? h.Mhs[p] : h.Mbfs[p];
     d.kp32 = round_up(dim, 32);
+    d.heavy.j = heavy_column(h.objective, dim, &d.heavy.w);
+    d.heavy.m64 = d.heavy.j >= 0 ? h.M64 + (size_t)d.heavy.j * dim : nullptr;
+    d.heavy.o64 = h.shift64;
+    d.heavy.o32 = h.shift32;
+    d.heavy.rate = h.rate64;
     return d;
 }
 
