@@ -478,6 +478,24 @@ SPEO_EXPORT int speo_island_regroup(uint64_t seed, int island, int event, int re
                                     const DEInfo *base, DEInfo *info_out,
                                     int *subisland_num_out);
 
+/* The surrogate's state of problem `problem` (host copies; each output may be
+ * NULL): *cands_out candidates per target in the last generation (1 before
+ * the first training round, else surrogate_candidates), cand_out
+ * [rows][cands][round_up(dim, 32)] those candidates (rows = the problem's rows
+ * in island order), pred_out [rows][cands] their scores (the network's
+ * log(1 + f) predictions; unused before training), archive_X
+ * [surrogate_archive][round_up(dim, 32)] and archive_y [surrogate_archive]
+ * the ring archive of true evaluations (y = log(1 + f)), *head_out its next
+ * slot and *appended_out the rows ever appended.  Synchronous. */
+SPEO_EXPORT int speo_engine_surrogate_dump(speo_engine e, int problem, int *cands_out,
+                                           float *cand_out, float *pred_out, float *archive_X,
+                                           float *archive_y, int *head_out,
+                                           int64_t *appended_out);
+/* Overwrites the archive's labels of `problem` (archive_y [surrogate_archive]),
+ * e.g. to test the held-out correlation check.  Synchronous. */
+SPEO_EXPORT int speo_engine_surrogate_set_labels(speo_engine e, int problem,
+                                                 const float *archive_y);
+
 /* Device pointers of the engine's buffers (for profiling and tests). */
 SPEO_EXPORT int speo_engine_buffers(speo_engine e, float **X, float **fX, int64_t *elem_off,
                                     int *row_off, int *ld);

@@ -446,6 +446,7 @@ struct Engine {
     float *cand = nullptr, *pred = nullptr, *rho_dev = nullptr, *AX = nullptr, *Ay = nullptr;
     float *Xb = nullptr, *tb = nullptr, *Xh = nullptr, *yh = nullptr, *predh = nullptr;
     bool any_trained = false;
+    int last_Ce = 1;  // candidates per target in the last surrogate generation
     int64_t surr_pred = 0, surr_rounds = 0;
     int surrogate_setup();
     int surrogate_generation(int gen_offset, hipStream_t s);
@@ -913,6 +914,7 @@ int Engine::surrogate_setup() {
 // the picked trial, archive append, selection.
 int Engine::surrogate_generation(int gen_offset, hipStream_t s) {
     const int Ce = any_trained ? C : 1;
+    last_Ce = Ce;
     SPEO_TRY(gen_candidates(isl_dev, srows_dev, Q, Ce, X, cfg.min_bound, cfg.max_bound, cfg.seed,
                             gen_base, (uint32_t)gen_offset, cand, Kp, s));
     if (any_trained) {
@@ -1704,6 +1706,44 @@ int speo_engine_surrogate_stats(speo_engine e, float *rho_out, int *active_out, 
             active_out[p] = (E.C > 0 && E.surr[(size_t)p]->trained && rho[(size_t)p] >= E.cfg.surrogate_rho_min) ? 1 : 0;
         if (loss_out) loss_out[p] = loss[(size_t)p];
     }
+    return SPEO_OK;
+}
+
+int speo_engine_surrogate_dump(speo_engine e, int problem, int *cands_out, float *cand_out,
+                               float *pred_out, float *archive_X, float *archive_y, int *head_out,
+                               int64_t *appended_out) {
+    if (!e || e->e.C <= 0 || problem < 0 || problem >= e->e.cfg.num_problems)
+        return set_error(SPEO_ERR_ARG, "speo_engine_surrogate_dump: no surrogate or bad problem");
+    Engine &E = e->e;
+    const auto &sp = *E.surr[(size_t)problem];
+    const size_t Kp = (size_t)E.Kp, Ce = (size_t)E.last_Ce, cap = (size_t)E.cfg.surrogate_archive;
+    SPEO_TRY_HIP(hipStreamSynchronize(E.stream));
+    if (cands_out) *cands_out = E.last_Ce;
+    if (cand_out)
+        SPEO_TRY_HIP(hipMemcpy(cand_out, E.cand + (size_t)sp.qoff * Ce * Kp,
+                               sizeof(float) * (size_t)sp.rows * Ce * Kp, hipMemcpyDeviceToHost));
+    if (pred_out)
+        SPEO_TRY_HIP(hipMemcpy(pred_out, E.pred + (size_t)sp.qoff * Ce, sizeof(float) * (size_t)sp.rows * Ce,
+                               hipMemcpyDeviceToHost));
+    if (archive_X)
+        SPEO_TRY_HIP(hipMemcpy(archive_X, E.AX + (size_t)problem * cap * Kp, sizeof(float) * cap * Kp,
+                               hipMemcpyDeviceToHost));
+    if (archive_y)
+        SPEO_TRY_HIP(hipMemcpy(archive_y, E.Ay + (size_t)problem * cap, sizeof(float) * cap,
+                               hipMemcpyDeviceToHost));
+    if (head_out) *head_out = sp.head;
+    if (appended_out) *appended_out = sp.appended;
+    return SPEO_OK;
+}
+
+int speo_engine_surrogate_set_labels(speo_engine e, int problem, const float *archive_y) {
+    if (!e || e->e.C <= 0 || problem < 0 || problem >= e->e.cfg.num_problems || !archive_y)
+        return set_error(SPEO_ERR_ARG, "speo_engine_surrogate_set_labels: no surrogate or bad arguments");
+    Engine &E = e->e;
+    const size_t cap = (size_t)E.cfg.surrogate_archive;
+    SPEO_TRY_HIP(hipStreamSynchronize(E.stream));
+    SPEO_TRY_HIP(hipMemcpy(E.Ay + (size_t)problem * cap, archive_y, sizeof(float) * cap,
+                           hipMemcpyHostToDevice));
     return SPEO_OK;
 }
 

@@ -339,6 +339,30 @@ class Engine:
     def run(self, generations: int):
         check(lib().speo_engine_run(self.handle, generations))
 
+    def surrogate_dump(self, problem: int, candidates: int, archive: int):
+        """(cands, cand [rows][cands][Kp], pred [rows][cands], archive X, archive y,
+        head, appended) of one problem's surrogate."""
+        rows = int(sum(self.island_size[self.island_problem == problem]))
+        Kp = (self.max_dim + 31) // 32 * 32
+        ce = c_int()
+        cand = np.zeros((rows, candidates, Kp), np.float32)
+        pred = np.zeros((rows, candidates), np.float32)
+        AX = np.zeros((archive, Kp), np.float32)
+        Ay = np.zeros(archive, np.float32)
+        head = c_int()
+        app = ctypes.c_int64()
+        check(lib().speo_engine_surrogate_dump(self.handle, problem, byref(ce), _np_ptr(cand),
+                                               _np_ptr(pred), _np_ptr(AX), _np_ptr(Ay),
+                                               byref(head), byref(app)))
+        n = ce.value
+        flat_c = cand.reshape(-1)[:rows * n * Kp].reshape(rows, n, Kp)
+        flat_p = pred.reshape(-1)[:rows * n].reshape(rows, n)
+        return n, flat_c, flat_p, AX, Ay, head.value, app.value
+
+    def surrogate_set_labels(self, problem: int, y):
+        y = np.ascontiguousarray(y, np.float32)
+        check(lib().speo_engine_surrogate_set_labels(self.handle, problem, _np_ptr(y)))
+
     def subislands(self, island: int):
         """(subisland_num, perm [island_size], [(CR, F, strategy, pop_size)])"""
         n = int(self.island_size[island])

@@ -146,6 +146,9 @@ _SIGS = {
     "speo_engine_destroy": (c_int, [c_void_p]),
     "speo_engine_run": (c_int, [c_void_p, c_int]),
     "speo_engine_prepare": (c_int, [c_void_p, c_int]),
+    "speo_engine_surrogate_dump": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_void_p]),
+    "speo_engine_surrogate_set_labels": (c_int, [c_void_p, c_int, c_void_p]),
     "speo_engine_run_fes_timed": (c_int, [c_void_p, c_int64, POINTER(c_int), POINTER(c_double),
                                           POINTER(c_double)]),
     "speo_island_driver": (c_int, [c_void_p, c_void_p]),

@@ -1,0 +1,113 @@
+"""The surrogate engine at the configuration of the surrogate-assisted study:
+32 islands x NP = 256, D = 100, shifted-rotated Rastrigin and elliptic, a
+100-256-256-1 network per problem trained every 10 generations (40 Adam steps
+of batch 1024, lr 1e-3) on a ring archive of 65,536 true evaluations of
+log(1 + f), 8 candidates per target, fallback below a held-out Spearman rho
+of 0.5.  Checked against the oracle: the truly evaluated fitness, the pick
+(the argmin of the network's scores, or candidate 0 when inactive), the
+archive rows, and the fallback when the archive's labels are shuffled."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ISL, NP, D, C, CAP = 32, 256, 100, 8, 65536
+
+
+def _setup(speo, oracle, **kw):
+    objs = [speo.OBJ_RASTRIGIN, speo.OBJ_ELLIPTIC]
+    data, probs = [], []
+    for k, obj in enumerate(objs):
+        o, M = oracle.gen_problem(9100 + k, D)
+        data.append((obj, o, M))
+        probs.append(speo.Problem([obj], o[None, :], M[None, :, :]))
+    pidx = [i % 2 for i in range(ISL)]
+    args = dict(F=0.5, CR=0.9, seed=909, interval=50, migration_size=2, mode=speo.MODE_PLAIN,
+                rows_per_block=32)
+    args.update(kw)
+    eng = speo.Engine(probs, [NP] * ISL, pidx, **args)
+    return eng, probs, data, np.array(pidx)
+
+
+def _surr_kw(rho_min=0.5):
+    return dict(surrogate_candidates=C, surrogate_train_interval=10, surrogate_archive=CAP,
+                surrogate_steps=40, surrogate_batch=1024, surrogate_lr=1e-3,
+                surrogate_rho_min=rho_min)
+
+
+def _rows_of(eng, pidx, p):
+    rows = []
+    for i in range(ISL):
+        if pidx[i] == p:
+            rows.extend(range(int(eng.row_off[i]), int(eng.row_off[i + 1])))
+    return np.array(rows)
+
+
+def test_inactive_surrogate_equals_plain_full_size(cuda, speo, oracle):
+    plain, probs, _, _ = _setup(speo, oracle)
+    surr, probs2, _, _ = _setup(speo, oracle, **_surr_kw(rho_min=2.0))
+    plain.run(25)
+    surr.run(25)
+    Xp, fp = plain.read()
+    Xs, fs = surr.read()
+    np.testing.assert_array_equal(Xp, Xs)
+    np.testing.assert_array_equal(fp, fs)
+    assert surr.stats()["surrogate_rounds"] == 2
+    for e in (plain, surr):
+        e.close()
+    for p in probs + probs2:
+        p.close()
+
+
+def test_active_surrogate_pick_fitness_and_archive(cuda, speo, oracle):
+    # rho_min = -1: every trained network screens, whatever its correlation
+    eng, probs, data, pidx = _setup(speo, oracle, **_surr_kw(rho_min=-1.0))
+    eng.run(20)
+    heads = [eng.surrogate_dump(p, C, CAP)[5] for p in range(2)]
+    eng.run(1)
+    U, fU = eng.read_trials()
+    for p, (obj, o, M) in enumerate(data):
+        n, cand, pred, AX, Ay, head, app = eng.surrogate_dump(p, C, CAP)
+        rows = _rows_of(eng, pidx, p)
+        assert n == C and app == 21 * rows.size
+        assert np.all(np.isfinite(pred))
+        # the truly evaluated trial is the candidate the network scores lowest
+        pick = np.argmin(pred, axis=1)
+        np.testing.assert_array_equal(U[rows, :D], cand[np.arange(rows.size), pick, :D])
+        # its fitness against the FP64 oracle
+        ref = oracle.evaluate(obj, U[rows, :D], o, M)
+        np.testing.assert_allclose(fU[rows], ref, rtol=1e-5)
+        # the generation's rows went to the archive slots after the previous
+        # head, in row order, with y = log(1 + f)
+        slots = (heads[p] + np.arange(rows.size)) % CAP
+        assert head == (heads[p] + rows.size) % CAP
+        np.testing.assert_array_equal(AX[slots, :D], U[rows, :D])
+        assert np.all(AX[slots, D:] == 0.0)
+        np.testing.assert_allclose(Ay[slots], np.log1p(fU[rows].astype(np.float64)), rtol=2e-7)
+    eng.close()
+    for p in probs:
+        p.close()
+
+
+def test_shuffled_labels_trigger_the_fallback(cuda, speo, oracle):
+    eng, probs, data, pidx = _setup(speo, oracle, **_surr_kw(rho_min=0.5))
+    eng.run(29)
+    rho, act, _ = eng.surrogate_stats()
+    assert act[1] == 1 and rho[1] > 0.5, rho       # elliptic: the network ranks well
+    # shuffle problem 1's labels; the training round after generation 30
+    # measures the held-out rho on the shuffled archive
+    _, _, _, _, Ay, _, _ = eng.surrogate_dump(1, C, CAP)
+    rng = np.random.default_rng(0)
+    eng.surrogate_set_labels(1, rng.permutation(Ay))
+    eng.run(1)
+    rho2, act2, _ = eng.surrogate_stats()
+    assert act2[1] == 0 and rho2[1] < 0.5, rho2
+    # the next generation takes candidate 0 (the plain trial) for problem 1
+    eng.run(1)
+    U, fU = eng.read_trials()
+    n, cand, pred, _, _, _, _ = eng.surrogate_dump(1, C, CAP)
+    rows = _rows_of(eng, pidx, 1)
+    np.testing.assert_array_equal(U[rows, :D], cand[:, 0, :D])
+    eng.close()
+    for p in probs:
+        p.close()
