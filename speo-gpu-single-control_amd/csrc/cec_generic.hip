@@ -257,13 +257,14 @@ __global__ __launch_bounds__(256) void k_eval_generic(GenLaunch p) {
                     }
                 }
             }
-            // accumulator element i of lane l: row 4 (l / 16) + i, column l % 16
+            // accumulator element i of lane l of v_mfma_f64_16x16x4f64: row
+            // l / 16 + 4 i, column l % 16
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const int t = wave + 4 * q, col = 16 * t + lr;
                 if (t < T && col < dim)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) Zs[(4 * lk + i) * ldz + col] = acc[q][i];
+                    for (int i = 0; i < 4; ++i) Zs[(lk + 4 * i) * ldz + col] = acc[q][i];
             }
         } else {
             __syncthreads();

@@ -111,7 +111,8 @@ extern "C" int speo_island_driver(const speo_driver_config *c, speo_run_record *
             r.buffer_manage = "none";
             r.regroup_option = regroup_name(c->regroup_option);
             r.configure_EA = c->configure_EA == SPEO_CONFIGURE_DYNAMIC ? "dynamic" : "constant";
-            SPEO_TRY(speo_de_parameters(ec.CR, ec.F, 4, params, (int)sizeof(params)));
+            const int plen = speo_de_parameters(ec.CR, ec.F, 4, params, (int)sizeof(params));
+            if (plen < 0) return plen;
             r.EA_parameters = params;
             if (c->record_file) {
                 char line[1024];

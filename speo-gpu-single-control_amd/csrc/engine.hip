@@ -19,8 +19,13 @@
 // fitness arrays are double-buffered.  The result equals the plain engine's.
 #pragma clang fp contract(off)
 
+#include <sys/stat.h>
+
 #include <chrono>
 #include <algorithm>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -377,6 +382,49 @@ static int launch_any(int rt, int prec, int mode, const GenerationArgs &a, int n
     }
 }
 
+// ------------------------------------------------------------ tuning cache
+//
+// The autotuner's choices, persisted across processes: one "key value" line
+// per choice in $SPEO_TUNE_CACHE (empty: no file), else
+// $HOME/.cache/speo_de/autotune-v1.txt.  Keys name the mode, the precision
+// and every island's (dimension, NP rounded up to a power of two).
+
+namespace {
+std::mutex g_tune_mu;
+
+std::string tune_path() {
+    if (const char *e = std::getenv("SPEO_TUNE_CACHE")) return e;
+    const char *h = std::getenv("HOME");
+    return h ? std::string(h) + "/.cache/speo_de/autotune-v1.txt" : std::string();
+}
+
+bool tune_lookup(const std::string &key, int *value) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string path = tune_path();
+    if (path.empty()) return false;
+    std::ifstream f(path);
+    std::string k;
+    int v = 0;
+    bool found = false;
+    while (f >> k >> v)
+        if (k == key) {
+            *value = v;  // the last line for a key wins
+            found = true;
+        }
+    return found;
+}
+
+void tune_store(const std::string &key, int value) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string path = tune_path();
+    if (path.empty()) return;
+    for (size_t i = 1; i < path.size(); ++i)  // create the directories on the way
+        if (path[i] == '/') (void)::mkdir(path.substr(0, i).c_str(), 0755);
+    std::ofstream f(path, std::ios::app);
+    if (f) f << key << ' ' << value << '\n';
+}
+}  // namespace
+
 // ------------------------------------------------------------------ engine
 
 struct Engine {
@@ -493,6 +541,12 @@ struct Engine {
     int prof_end(hipStream_t s);
     int next_chunk(int left, bool *mig) const;
     int autotune();
+    // epoch mode: workgroups per island measured over the candidates on
+    // throwaway engines of the same configuration (cfg.epoch_parts == 0)
+    bool tuning_child = false;
+    int tuned_from_cache = 0;  // bit 0: row tile, bit 1: epoch parts
+    std::string tune_key(const char *what) const;
+    int tune_epoch_parts();
     int best(int *row_out, float *f32_out, double *f64_out);
     int read(float *Xo, float *fo);
     int read_rows(const float *src, const float *fsrc, float *Xo, float *fo);
@@ -687,6 +741,8 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
     if (c.surrogate_candidates > 0 && (c.mode != SPEO_MODE_PLAIN || c.use_graph))
         return set_error(SPEO_ERR_UNSUPPORTED, "speo_engine_create: the surrogate runs in "
                                                "SPEO_MODE_PLAIN without graphs");
+    if (cfg.mode == SPEO_MODE_EPOCH && cfg.epoch_parts == 0 && !tuning_child)
+        SPEO_TRY(tune_epoch_parts());
     if (cfg.mode == SPEO_MODE_EPOCH) SPEO_TRY(epoch_setup());
     rt = c.rows_per_block;
     if (rt == 0) SPEO_TRY(autotune());
@@ -1426,6 +1482,15 @@ int Engine::autotune() {
             return SPEO_OK;
         }
     }
+    const std::string pkey = tune_key("rt");
+    int stored = 0;
+    if (tune_lookup(pkey, &stored) && (stored == 16 || stored == 32 || stored == 64)) {
+        rt = stored;
+        tuned_from_cache |= 1;
+        std::lock_guard<std::mutex> lk(mu);
+        cache[key] = rt;
+        return SPEO_OK;
+    }
     float *saveX = X, *savefX[2] = {fX[0], fX[1]}, *saveU[2] = {U[0], U[1]},
           *savefU[2] = {fU[0], fU[1]};
     const size_t E = (size_t)total_elems, R = (size_t)total_rows;
@@ -1488,8 +1553,79 @@ int Engine::autotune() {
     (void)hipFree(tU[1]);
     for (auto &b : tf) (void)hipFree(b);
     rt = best_rt;
+    tune_store(pkey, rt);
     std::lock_guard<std::mutex> lk(mu);
     cache[key] = rt;
+    return SPEO_OK;
+}
+
+std::string Engine::tune_key(const char *what) const {
+    std::vector<std::pair<int, int>> dn;
+    for (const auto &d : isl) {
+        int b = 1;
+        while (b < d.np) b <<= 1;
+        dn.emplace_back(d.dim, b);
+    }
+    std::sort(dn.begin(), dn.end());
+    std::ostringstream k;
+    k << what << ":m" << cfg.mode << ":p" << cfg.precision << ":n" << I;
+    for (size_t i = 0; i < dn.size();) {  // runs of equal (dim, bucket) as dim/bucket*count
+        size_t j = i;
+        while (j < dn.size() && dn[j] == dn[i]) ++j;
+        k << ':' << dn[i].first << '/' << dn[i].second << '*' << (j - i);
+        i = j;
+    }
+    return k.str();
+}
+
+// Workgroups per island for epoch mode: every feasible power of two up to 8
+// (islands x parts at most four per compute unit) timed on a throwaway
+// engine of this configuration over 24 generations after a warm-up that
+// captures the graphs; the fastest is kept and persisted.
+int Engine::tune_epoch_parts() {
+    const std::string key = tune_key("epoch_parts");
+    int stored = 0;
+    if (tune_lookup(key, &stored) && (stored == 1 || stored == 2 || stored == 4 || stored == 8)) {
+        cfg.epoch_parts = stored;
+        tuned_from_cache |= 2;
+        return SPEO_OK;
+    }
+    int dev = 0, cus = 0;
+    SPEO_TRY_HIP(hipGetDevice(&dev));
+    SPEO_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<int> cands;
+    for (int parts = 1; parts <= kEpochMaxParts; parts *= 2) {
+        std::vector<EpochPlan> plans;
+        int lds = 0;
+        if ((int64_t)I * parts > 4 * (int64_t)cus) break;
+        if (epoch_plan_all(parts, &plans, &lds) == SPEO_OK) cands.push_back(parts);
+    }
+    if (cands.empty()) return SPEO_OK;  // epoch_setup reports the reason
+    int best = cands[0];
+    if (cands.size() > 1) {
+        double best_s = 1e30;
+        for (int parts : cands) {
+            speo_engine_config c2 = cfg;
+            c2.epoch_parts = parts;
+            c2.use_graph = 1;
+            c2.rows_per_block = cfg.rows_per_block ? cfg.rows_per_block : 32;
+            auto child = std::make_unique<Engine>();
+            child->tuning_child = true;
+            SPEO_TRY(child->create(c2, stream));
+            SPEO_TRY(child->run(24));
+            SPEO_TRY_HIP(hipStreamSynchronize(stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            SPEO_TRY(child->run(24));
+            SPEO_TRY_HIP(hipStreamSynchronize(stream));
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (dt < best_s) {
+                best_s = dt;
+                best = parts;
+            }
+        }
+    }
+    cfg.epoch_parts = best;
+    tune_store(key, best);
     return SPEO_OK;
 }
 

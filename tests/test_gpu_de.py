@@ -264,7 +264,7 @@ def test_dropin_unsupported_function_reports(cuda, speo):
     f = torch.zeros(16, dtype=torch.float32, device=cuda)
     L.API_evaluateFitness(ctypes.c_void_p(f.data_ptr()), None, None,
                           ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(f.data_ptr()),
-                          ctypes.c_void_p(f.data_ptr()), None, None, None, None, 16, 10, 0.0, 23)
+                          ctypes.c_void_p(f.data_ptr()), None, None, None, None, 16, 10, 0.0, 31)
     assert L.API_CheckCUDAError() == -3
-    assert "23" in speo.last_error()
+    assert "31" in speo.last_error()
     assert L.API_CheckCUDAError() == 0       # reported once

@@ -159,11 +159,24 @@ def test_epoch_split_edge_dims_equal_plain(cuda, speo, oracle, prec):
     _assert_same(plain, split)
 
 
-def test_epoch_automatic_split(cuda, speo, oracle):
+def test_epoch_automatic_split(cuda, speo, oracle, tmp_path, monkeypatch):
+    # epoch_parts = 0: the tuner times every feasible split on throwaway
+    # engines, keeps the fastest and persists it; a second engine of the same
+    # shape reads the choice back
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("SPEO_TUNE_CACHE", str(cache))
     probs = _problems(speo, oracle, [(100, 1)], base=2800)
     e = speo.Engine(probs, [128] * 8, [0] * 8, mode=speo.MODE_EPOCH, interval=10, migration_size=2,
                     seed=SEED)
-    assert e.stats()["epoch_parts"] == 8     # 8 islands on 256 compute units
+    chosen = e.stats()["epoch_parts"]
+    assert chosen in (1, 2, 4, 8)
+    lines = cache.read_text().split()
+    assert lines[0].startswith("epoch_parts:m2:p0:n8:100/128*8") and int(lines[1]) == chosen
+    cache.write_text(lines[0] + " 2\n")          # a stored choice is used as it is
+    e2 = speo.Engine(probs, [128] * 8, [0] * 8, mode=speo.MODE_EPOCH, interval=10,
+                     migration_size=2, seed=SEED)
+    assert e2.stats()["epoch_parts"] == 2
+    e2.close()
     one = speo.Engine(probs, [128] * 8, [0] * 8, mode=speo.MODE_EPOCH, interval=10,
                       migration_size=2, seed=SEED, epoch_parts=1)
     e.run(25)

@@ -276,8 +276,10 @@ def test_composition_components_sharing_a_shift(cuda, speo, oracle):
     f64 = torch.zeros(3, dtype=torch.float64, device=cuda)
     speo.evaluate_f64(prob, _t(X, cuda), f64)
     torch.cuda.synchronize()
-    np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-5)
-    np.testing.assert_allclose(f64.cpu().numpy(), ref, rtol=1e-12)
+    # row 1 sits on component 1's shift: f = 0 up to the other components'
+    # weights of ~1e-100 in FP64
+    np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-5, atol=1e-90)
+    np.testing.assert_allclose(f64.cpu().numpy(), ref, rtol=1e-12, atol=1e-90)
 
 
 def test_strided_rows_and_ragged_counts(cuda, speo, oracle):

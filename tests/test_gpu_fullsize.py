@@ -65,7 +65,7 @@ def test_headline_workload_epoch_equals_plain(cuda, speo, oracle):
     probs, pidx = _round_robin(speo, oracle, 100, [1, 2, 3], 64, 4100)
     kw = dict(F=0.5, CR=0.9, seed=SEED, interval=50, migration_size=2, use_graph=True)
     plain = speo.Engine(probs, [128] * 64, pidx, mode=speo.MODE_PLAIN, **kw)
-    fast = speo.Engine(probs, [128] * 64, pidx, mode=speo.MODE_EPOCH, **kw)
+    fast = speo.Engine(probs, [128] * 64, pidx, mode=speo.MODE_EPOCH, epoch_parts=4, **kw)
     assert fast.stats()["epoch_parts"] == 4
     _run_pair(speo, plain, fast, [1, 40, 60, 19])
     st = fast.stats()
@@ -80,7 +80,7 @@ def test_time_to_error_workload_epoch_equals_plain(cuda, speo, oracle, prec):
     kw = dict(F=0.5, CR=0.9, seed=SEED, interval=50, migration_size=2, use_graph=True,
               precision=prec)
     plain = speo.Engine(probs, [256] * 32, pidx, mode=speo.MODE_PLAIN, **kw)
-    fast = speo.Engine(probs, [256] * 32, pidx, mode=speo.MODE_EPOCH, **kw)
+    fast = speo.Engine(probs, [256] * 32, pidx, mode=speo.MODE_EPOCH, epoch_parts=8, **kw)
     assert fast.stats()["epoch_parts"] == 8
     _run_pair(speo, plain, fast, [1, 49, 30])
 
