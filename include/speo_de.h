@@ -132,17 +132,19 @@ SPEO_EXPORT int API_DE_SelectSurvival(GPU_Population d_population, GPU_Populatio
                                       GPU_IslandInfo GPU_island_info, ProblemInfo problem_info);
 
 /* Replaces API_evaluateFitness (CUDA_EA_library/CEC2014_CUDA.h:35-37,
- * kernel_CEC2014_CUDA.cu:1672-1797) for function_ID 1 (rotated elliptic),
- * 2 (rotated bent cigar), 3 (rotated discus), 5 (rotated Ackley), 8 (shifted
- * Rastrigin, rate 0.0512), 9 (rotated Rastrigin, rate 0.0512), 10 (shifted
- * Schwefel, rate 10), 11 (rotated Schwefel, rate 10), 13 (rotated HappyCat,
- * rate 0.05) and 14 (rotated HGBat, rate 0.05), in the reference's layouts: d_original
- * [pop_size][next_pow2(dim)], d_shift [next_pow2(dim)], d_rotated_data =
- * M transposed and row-padded, [next_pow2(dim)][dim] (CEC2014_CUDA.cc:255).
- * The shift-rotate GEMM and the objective run as one fused MFMA kernel, so
- * d_shuffled_elements, d_rotated_elements, d_shuffle_data, d_tmp_fitness,
- * d_wi and d_sigma are accepted and unused.  Other IDs record
- * SPEO_ERR_UNSUPPORTED, reported by API_CheckCUDAError(). */
+ * kernel_CEC2014_CUDA.cu:1672-1797) for every function_ID 1..30, in the
+ * reference's layouts: d_original [pop_size][next_pow2(dim)], d_shift
+ * [ncomp][next_pow2(dim)], d_rotated_data = M transposed and row-padded,
+ * [ncomp][next_pow2(dim)][dim] (CEC2014_CUDA.cc:255), d_shuffle_data the
+ * 1-based shuffle of the hybrids (padded to next_pow2(dim) per component),
+ * d_sigma the compositions' widths (read).  IDs 1, 2, 3, 5, 8, 9, 10, 11, 13
+ * and 14 run as one fused FP32 MFMA kernel (above 128 columns its per-chunk
+ * row sums use d_rotated_elements as workspace); IDs 4, 6, 7, 12 and 15-30
+ * take the FP64 generic kernel (dim <= 1024; values rounded to real).  As in
+ * the reference, IDs 1-16 ignore `bias` and IDs 17-30 add it.  The other
+ * scratch arguments (d_shuffled_elements, d_tmp_fitness, d_wi) are accepted
+ * and unused.  IDs outside 1..30 record SPEO_ERR_UNSUPPORTED, reported by
+ * API_CheckCUDAError(). */
 SPEO_EXPORT void API_evaluateFitness(real *d_fitness_value, real *d_shuffled_elements,
                                      real *d_rotated_elements, real *d_original_elements,
                                      real *d_shift_data, real *d_rotated_data,
@@ -265,13 +267,13 @@ SPEO_EXPORT uint64_t speo_problem_seed(uint64_t base_seed, uint32_t index);
 SPEO_EXPORT int speo_load_cec_data(const char *dir, int func_id, int dim, int ncomp,
                                    double *shift_out, double *rotation_out);
 
-/* The problem of CEC2014 function `func_id` at `dim` from those files, with
- * the reference GPU path's objective, rate and rotation flag
- * (kernel_CEC2014_CUDA.cu:1099-1190): 1, 2, 3, 5, 8 (no rotation), 9, 10 (no
- * rotation), 11, 13, 14, and the compositions 24 and 25 (cf02 / cf03,
- * CEC2014.cc:1235-1267: three components with the reference's sigma, bias,
- * lambda and rates); others SPEO_ERR_UNSUPPORTED.  Values are the raw
- * objective (no 100 * func_id), as the reference's API_evaluateFitness. */
+/* The problem of CEC2014 function `func_id` (1..30) at `dim` from those files,
+ * with the reference's objectives, rates, rotation flags and composition
+ * parameters (CEC2014.cc:14-145, 954-1372; kernel_CEC2014_CUDA.cu:1099-1671);
+ * the hybrids, and the compositions built on them (F29, F30), also read
+ * <dir>/shuffle_data_<func_id>_D<dim>.txt (ncomp * dim 1-based indices,
+ * CEC2014.cc:440-458).  Values exclude the function's bias 100 * func_id
+ * (API_evaluateFitness takes it as its `bias` argument). */
 SPEO_EXPORT int speo_problem_create_cec(const char *dir, int func_id, int dim, speo_problem *out);
 
 /* f_out[i] = f(X[i]) for n rows of stride ld (device pointers), FP32
@@ -528,6 +530,16 @@ SPEO_EXPORT int speo_mlp_loss_grad(speo_mlp m, const float *X, int ld, int n, co
                                    float t_mean, float t_std, float *loss_out, float *grad_out);
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the last gradient.  Synchronous. */
 SPEO_EXPORT int speo_mlp_adam_step(speo_mlp m, float lr);
+/* The engine's training step: a minibatch of `batch` rows drawn from the
+ * archive's first n_valid slots (device AX [slot][round_up(dim, 32)], labels
+ * Ay [slot]; slot of row b = floor(u n_valid) with u the u01 of word b % 4 of
+ * Philox block {b / 4, round, step, 33} under `seed`, clamped to n_valid - 1,
+ * minus one when slot % 8 == 7), labels standardised by t_mean / t_std, loss,
+ * gradients and one Adam step in two kernels.  *loss_out (host, may be NULL)
+ * receives the loss.  Synchronous. */
+SPEO_EXPORT int speo_mlp_train_step(speo_mlp m, const float *AX, const float *Ay, int n_valid,
+                                    uint64_t seed, uint32_t round, uint32_t step, int batch,
+                                    float t_mean, float t_std, float lr, float *loss_out);
 /* Spearman rank correlation (average ranks) of device arrays a, b of n <=
  * 4096 values into *rho_out (device). */
 SPEO_EXPORT int speo_spearman(const float *a, const float *b, int n, float *rho_out,

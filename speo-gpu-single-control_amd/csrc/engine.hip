@@ -1006,15 +1006,10 @@ int Engine::surrogate_train(hipStream_t s) {
         if (n_valid < std::max(cfg.surrogate_batch, 512)) continue;
         const float *ax = AX + p * cap * Kp, *ay = Ay + p * cap;
         SPEO_TRY(mean_std(ay, n_valid, sp.net.stats, s));
-        for (int step = 0; step < cfg.surrogate_steps; ++step) {
-            SPEO_TRY(gather_batch(ax, ay, n_valid, Kp, cfg.seed ^ (0xD1B54A32D192ED03ull * (p + 1)),
-                                  (uint32_t)surr_rounds, (uint32_t)step, cfg.surrogate_batch, Xb,
-                                  tb, s));
-            SPEO_TRY(sp.net.loss_grad(Xb, Kp, cfg.surrogate_batch, tb, s,
-                                      step == cfg.surrogate_steps - 1));
-            SPEO_TRY(sp.net.adam(cfg.surrogate_lr, s));
-        }
-        SPEO_TRY(sp.net.refresh_bf16(s));
+        for (int step = 0; step < cfg.surrogate_steps; ++step)
+            SPEO_TRY(sp.net.train_step(ax, ay, n_valid, cfg.seed ^ (0xD1B54A32D192ED03ull * (p + 1)),
+                                       (uint32_t)surr_rounds, (uint32_t)step, cfg.surrogate_batch,
+                                       cfg.surrogate_lr, step == cfg.surrogate_steps - 1, Xb, tb, s));
         const int H = std::min(1024, n_valid / 8);
         SPEO_TRY(gather_holdout(ax, ay, H, Kp, Xh, yh, s));
         SPEO_TRY(sp.net.infer(Xh, Kp, H, predh, s));

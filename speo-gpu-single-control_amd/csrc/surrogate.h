@@ -14,6 +14,9 @@ constexpr float kXScale = 0.01f;  // inputs in [-100, 100] -> [-1, 1]
 // Flat parameter layout (floats): W1 [256][Kp], b1 [256], W2 [256][256],
 // b2 [256], w3 [256], b3 [4 (1 used)].
 constexpr int kMaxSplitK = 8;
+constexpr int kFusedMaxKp = 128;  // widest input of the fused inference and training kernels
+constexpr int kTrainRows = 32;    // minibatch rows per workgroup of the fused training step
+constexpr int kTrainPart = 1024;  // floats of partial sums per such workgroup
 
 struct MlpLayout {
     int D = 0, Kp = 0;
@@ -41,8 +44,16 @@ struct Mlp {
     float *y = nullptr, *dy = nullptr;                                 // [max_batch]
     float *stats = nullptr;  // [0] loss, [1] target mean, [2] target std, [3] rho
     uint16_t *W1b = nullptr, *W2b = nullptr;  // BF16 copies for the fused inference kernel
+    uint16_t *W2Tb = nullptr;                 // BF16 W2 transposed (backward of the fused step)
+    bool bf16_stale = true;                   // the BF16 copies lag the FP32 weights
     float *part = nullptr;                    // split-K partial products of the weight gradients
     float *lterm = nullptr;                   // [max_batch] per-row loss terms
+    // Fused training step (Kp <= 128): activations transposed, [rows][Bp]
+    // BF16 with Bp = round_up(max_batch, 32), and per-row-tile partial sums of
+    // the bias and output-layer gradients, [tiles][1024] floats.
+    uint16_t *XT = nullptr, *H1T = nullptr, *dH2T = nullptr, *dH1T = nullptr;
+    float *tpart = nullptr;
+    int Bp = 0;
     int64_t adam_t = 0;
     ~Mlp();
     int alloc(int dim, int batch);
@@ -50,10 +61,19 @@ struct Mlp {
     // y = MLP(X) for n rows of X (stride ld >= D, raw inputs), batched through
     // the workspace; pred receives n values.
     int forward(const float *X, int ld, int n, float *pred, hipStream_t s);
-    // Loss (0.5 mean (y - t)^2) and gradients into G for n <= max_batch rows.
+    // Loss (0.5 mean (y - t)^2) and gradients into G for n <= max_batch rows
+    // (the fused step when Kp <= 128, else GEMMs and reductions layer by layer).
     int loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s,
                   bool want_loss = true);
+    int loss_grad_unfused(const float *X, int ld, int n, const float *t, hipStream_t s,
+                          bool want_loss);
     int adam(float lr, hipStream_t s);
+    // One Adam step on a minibatch of the archive (rows AX [slot][Kp], labels
+    // Ay): the slots drawn as gather_batch draws them, loss and gradients,
+    // Adam, and fresh BF16 copies.  Two kernels when Kp <= 128.
+    int train_step(const float *AX, const float *Ay, int n_valid, uint64_t seed, uint32_t round,
+                   uint32_t step, int batch, float lr, bool want_loss, float *Xb, float *tb,
+                   hipStream_t s);
     // Refreshes the BF16 weight copies from the FP32 master weights.
     int refresh_bf16(hipStream_t s);
     // Fused inference (one kernel): pred[i] = MLP(X[i]); uses the BF16 copies.

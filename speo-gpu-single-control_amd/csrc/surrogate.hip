@@ -341,17 +341,25 @@ __global__ void k_sum(const float *v, int n, float *out) {
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-// Adam with bias correction (Kingma & Ba, 2015).
-__global__ void k_adam(float *P, const float *G, float *m, float *v, size_t n, float lr, float c1,
-                       float c2) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float g = G[i];
+// Adam with bias correction (Kingma & Ba, 2015) on parameter i with gradient
+// g; c1 = 1 - 0.9^t, c2 = 1 - 0.999^t.  Returns the new value.  Shared by
+// k_adam and the fused training step, so both give the same bits.
+__device__ __forceinline__ float adam_update(float *P, float *m, float *v, size_t i, float g,
+                                             float lr, float c1, float c2) {
     const float mi = 0.9f * m[i] + 0.1f * g;
     const float vi = 0.999f * v[i] + 0.001f * g * g;
     m[i] = mi;
     v[i] = vi;
-    P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    const float p = P[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    P[i] = p;
+    return p;
+}
+
+__global__ void k_adam(float *P, const float *G, float *m, float *v, size_t n, float lr, float c1,
+                       float c2) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    (void)adam_update(P, m, v, i, G[i], lr, c1, c2);
 }
 
 // He-uniform init from Philox: W1 ~ U(+-sqrt(6/D)), W2 ~ U(+-sqrt(6/256)),
@@ -388,7 +396,8 @@ static int zero(float *p, size_t n, hipStream_t s) {
 // ------------------------------------------------------------------ Mlp
 
 Mlp::~Mlp() {
-    void *b[] = {P, G, m, v, A1, H1, A2, H2, dH1, dH2, y, dy, stats, W1b, W2b, part, lterm};
+    void *b[] = {P,   G,   m,     v,   A1,  H1,   A2,   H2,   dH1,   dH2,  y,
+                 dy,  stats, W1b, W2b, part, lterm, W2Tb, XT, H1T, dH2T, dH1T, tpart};
     for (void *p : b) (void)hipFree(p);
 }
 
@@ -407,10 +416,18 @@ int Mlp::alloc(int dim, int batch) {
     const int splits = std::min(kMaxSplitK, (batch + 127) / 128);
     SPEO_TRY_HIP(hipMalloc(&part, sizeof(float) * (size_t)std::max(splits, 1) * kHidden *
                                       (size_t)std::max(kHidden, L.Kp)));
+    if (L.Kp <= kFusedMaxKp) {
+        Bp = round_up(batch, kTrainRows);
+        SPEO_TRY_HIP(hipMalloc(&XT, sizeof(uint16_t) * (size_t)L.Kp * Bp));
+        for (uint16_t **p : {&H1T, &dH2T, &dH1T})
+            SPEO_TRY_HIP(hipMalloc(p, sizeof(uint16_t) * (size_t)kHidden * Bp));
+        SPEO_TRY_HIP(hipMalloc(&tpart, sizeof(float) * (size_t)(Bp / kTrainRows) * kTrainPart));
+    }
     return SPEO_OK;
 }
 
 int Mlp::init(uint64_t seed, hipStream_t s) {
+    bf16_stale = true;
     hipLaunchKernelGGL(k_mlp_init, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, P, L, seed);
     SPEO_TRY(check_launch("k_mlp_init"));
     SPEO_TRY_HIP(hipMemsetAsync(m, 0, sizeof(float) * L.n, s));
@@ -454,7 +471,8 @@ int Mlp::forward(const float *X, int ld, int n, float *pred, hipStream_t s) {
     return SPEO_OK;
 }
 
-int Mlp::loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s, bool want_loss) {
+int Mlp::loss_grad_unfused(const float *X, int ld, int n, const float *t, hipStream_t s,
+                           bool want_loss) {
     if (n > max_batch) return set_error(SPEO_ERR_ARG, "mlp: batch larger than the workspace");
     SPEO_TRY(gemm(layer1(*this, X, ld, n), s));
     SPEO_TRY(gemm(layer2(*this, n), s));
@@ -503,6 +521,7 @@ int Mlp::adam(float lr, hipStream_t s) {
     const float c2 = 1.0f - powf(0.999f, (float)adam_t);
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, P, G, m, v,
                        L.n, lr, c1, c2);
+    bf16_stale = true;
     return check_launch("k_adam");
 }
 
@@ -512,6 +531,12 @@ __global__ void k_to_bf16(const float *src, uint16_t *dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         dst[i] = to_bits(src[i], false);
+}
+
+// dst[j][i] = BF16(src[i][j]) for a 256 x 256 matrix (block j, thread i).
+__global__ void k_to_bf16_t(const float *src, uint16_t *dst) {
+    const int j = blockIdx.x, i = threadIdx.x;
+    dst[(size_t)j * kHidden + i] = to_bits(src[(size_t)i * kHidden + j], false);
 }
 
 // 64 rows per block, 4 waves; wave w owns hidden columns [64w, 64w + 64) of
@@ -639,15 +664,18 @@ int Mlp::refresh_bf16(hipStream_t s) {
     if (!W1b) {
         SPEO_TRY_HIP(hipMalloc(&W1b, sizeof(uint16_t) * (size_t)kHidden * L.Kp));
         SPEO_TRY_HIP(hipMalloc(&W2b, sizeof(uint16_t) * (size_t)kHidden * kHidden));
+        SPEO_TRY_HIP(hipMalloc(&W2Tb, sizeof(uint16_t) * (size_t)kHidden * kHidden));
     }
     hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, s, P + L.oW1, W1b, (size_t)kHidden * L.Kp);
     hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, s, P + L.oW2, W2b, (size_t)kHidden * kHidden);
+    hipLaunchKernelGGL(k_to_bf16_t, dim3(kHidden), dim3(kHidden), 0, s, P + L.oW2, W2Tb);
+    bf16_stale = false;
     return check_launch("k_to_bf16");
 }
 
 int Mlp::infer(const float *X, int ld, int n, float *pred, hipStream_t s) {
     if (n <= 0) return SPEO_OK;
-    if (!W1b) SPEO_TRY(refresh_bf16(s));
+    if (bf16_stale || !W1b) SPEO_TRY(refresh_bf16(s));
     const dim3 grid((n + 63) / 64), block(256);
     const float *b1 = P + L.ob1, *b2 = P + L.ob2, *w3 = P + L.ow3, *b3 = P + L.ob3;
     switch (L.Kp) {
@@ -658,6 +686,474 @@ int Mlp::infer(const float *X, int ld, int n, float *pred, hipStream_t s) {
     default: return forward(X, ld, n, pred, s);  // wide inputs: the unfused path
     }
     return check_launch("k_mlp_infer");
+}
+
+// ------------------------------------------------------ fused training step
+//
+// One Adam step in two launches instead of a dozen: k_train_rows runs the
+// forward and backward passes of 32 minibatch rows per workgroup up to the
+// activation gradients, and k_train_update forms the weight gradients as
+// GEMMs over the whole minibatch and applies Adam to them.  Same rounding
+// points as the unfused path (BF16 operands, FP32 accumulation, FP32 SiLU,
+// output layer, loss and Adam); the sums run in other fixed orders.
+
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+struct TrainRowsArgs {
+    const float *X;        // direct rows (stride ld) and targets, or null:
+    int ld;                //   then minibatch slots of the archive AX / Ay
+    const float *t;
+    const float *AX, *Ay;  // archive rows [slot][Kp], labels [slot]
+    int n_valid;
+    uint64_t seed;
+    uint32_t round, step;
+    int n, D;
+    const float *stats;    // [1] mean, [2] std of the labels
+    const uint16_t *W1b, *W2b, *W2Tb;
+    const float *b1, *b2, *w3, *b3;
+    uint16_t *XT, *H1T, *dH2T, *dH1T;  // [rows][Bp]: column b = minibatch row b
+    int Bp;
+    float *part;           // [tile][kTrainPart]: db1 | db2 | dw3 | (db3, loss)
+};
+
+// Rows [32 w, 32 w + 32) of the minibatch.  Wave v owns hidden columns
+// [64 v, 64 v + 64) of every layer for the 32 rows (2 x 4 MFMA 16x16 tiles).
+// Rows >= n are zero inputs with dy = 0, so they add nothing to any gradient.
+template <int KP>
+__global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
+    constexpr int TM = kTrainRows, LX = KP + 8, LH = kHidden + 8;
+    __shared__ __align__(16) uint16_t Xs[TM * LX];
+    __shared__ __align__(16) uint16_t Hs[TM * LH];  // H1, then dH1
+    __shared__ __align__(16) uint16_t Gs[TM * LH];  // dH2
+    __shared__ float red[4][TM];
+    __shared__ float ts[TM], dys[TM], lts[TM];
+    __shared__ int slots[TM];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int row0 = blockIdx.x * TM;
+    float *part = a.part + (size_t)blockIdx.x * kTrainPart;
+    if (tid < TM) {
+        const int b = row0 + tid;
+        int slot = b;
+        float t = 0.f;
+        if (b < a.n) {
+            if (a.AX) {  // the slot gather_batch draws for row b
+                const U4 w = philox_block(a.seed, (uint32_t)(b >> 2), a.round, a.step, kPurposeSurrBatch);
+                slot = (int)(u01(word(w, (uint32_t)(b & 3))) * (float)a.n_valid);
+                slot = slot >= a.n_valid ? a.n_valid - 1 : slot;
+                if (slot % 8 == 7) slot -= 1;
+                t = a.Ay[slot];
+            } else {
+                t = a.t[b];
+            }
+            t = (t - a.stats[1]) / a.stats[2];
+        }
+        slots[tid] = slot;
+        ts[tid] = t;
+    }
+    __syncthreads();
+    {
+        const float *src = a.AX ? a.AX : a.X;
+        const size_t ld = a.AX ? (size_t)KP : (size_t)a.ld;
+        constexpr int G = TM * KP / 256;
+        float x[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int e = tid + g * 256, r = e / KP, k = e % KP;
+            x[g] = (row0 + r < a.n && k < a.D) ? src[(size_t)slots[r] * ld + k] : 0.f;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int e = tid + g * 256, r = e / KP, k = e % KP;
+            Xs[r * LX + k] = to_bits(x[g] * kXScale, false);
+        }
+    }
+    __syncthreads();
+    // transposed copy of a [TM][stride] BF16 tile: T[c][row0 + r]
+    auto store_t = [&](const uint16_t *S, int stride, int ncols, uint16_t *T) __attribute__((always_inline)) {
+        for (int e = tid; e < ncols * (TM / 8); e += 256) {
+            const int c = e / (TM / 8), r8 = (e % (TM / 8)) * 8;
+            u16x8 v;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = S[(r8 + u) * stride + c];
+            *reinterpret_cast<u16x8 *>(T + (size_t)c * a.Bp + row0 + r8) = v;
+        }
+    };
+    store_t(Xs, LX, KP, a.XT);
+    const int c0 = wave * 64;
+    floatx4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // layer 1
+    for (int ks = 0; ks < KP / 32; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+        bf16x8 av[2], bv[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) av[i] = *reinterpret_cast<const bf16x8 *>(&Xs[(i * 16 + lr) * LX + kb]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bv[j] = *reinterpret_cast<const bf16x8 *>(&a.W1b[(size_t)(c0 + j * 16 + lr) * KP + kb]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    float d1[2][4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = c0 + j * 16 + lr;
+        const float bias = a.b1[col];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float pre = acc[i][j][q] + bias;
+                d1[i][j][q] = dsiluf_(pre);
+                Hs[(i * 16 + lk * 4 + q) * LH + col] = to_bits(siluf_(pre), false);
+                acc[i][j][q] = 0.f;
+            }
+    }
+    __syncthreads();
+    // layer 2
+    for (int ks = 0; ks < kHidden / 32; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+        bf16x8 av[2], bv[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) av[i] = *reinterpret_cast<const bf16x8 *>(&Hs[(i * 16 + lr) * LH + kb]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bv[j] = *reinterpret_cast<const bf16x8 *>(&a.W2b[(size_t)(c0 + j * 16 + lr) * kHidden + kb]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    store_t(Hs, LH, kHidden, a.H1T);
+    float d2[2][4][4], ysum[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ysum[i][q] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = c0 + j * 16 + lr;
+        const float bias = a.b2[col], w = a.w3[col];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float pre = acc[i][j][q] + bias, h = siluf_(pre);
+                d2[i][j][q] = dsiluf_(pre);
+                acc[i][j][q] = h;  // H2
+                ysum[i][q] = fmaf(h, w, ysum[i][q]);
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float v = sum16(ysum[i][q]);
+            if (lr == 0) red[wave][i * 16 + lk * 4 + q] = v;
+        }
+    __syncthreads();
+    if (tid < TM) {
+        float dy = 0.f, lt = 0.f;
+        if (row0 + tid < a.n) {
+            const float y = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid] + a.b3[0];
+            const float e = y - ts[tid];
+            dy = e / (float)a.n;
+            lt = 0.5f * e * e / (float)a.n;
+        }
+        dys[tid] = dy;
+        lts[tid] = lt;
+    }
+    __syncthreads();
+    // dH2 = dy w3 silu'(A2); partial sums of db2 = sum dH2 and dw3 = sum H2 dy
+    // over the tile's rows (this lane's 8 rows, then the four lane groups)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = c0 + j * 16 + lr;
+        const float w = a.w3[col];
+        float pb2 = 0.f, pw3 = 0.f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = i * 16 + lk * 4 + q;
+                const float dy = dys[r];
+                const float g = dy * w * d2[i][j][q];
+                Gs[r * LH + col] = to_bits(g, false);
+                pb2 += g;
+                pw3 = fmaf(acc[i][j][q], dy, pw3);
+            }
+        pb2 += __shfl_xor(pb2, 16, 64);
+        pb2 += __shfl_xor(pb2, 32, 64);
+        pw3 += __shfl_xor(pw3, 16, 64);
+        pw3 += __shfl_xor(pw3, 32, 64);
+        if (lk == 0) {
+            part[kHidden + col] = pb2;
+            part[2 * kHidden + col] = pw3;
+        }
+    }
+    if (tid == 0) {
+        float sdy = 0.f, sl = 0.f;
+        for (int r = 0; r < TM; ++r) {
+            sdy += dys[r];
+            sl += lts[r];
+        }
+        part[3 * kHidden] = sdy;
+        part[3 * kHidden + 1] = sl;
+    }
+    __syncthreads();
+    store_t(Gs, LH, kHidden, a.dH2T);
+    // dH1 = (dH2 W2) silu'(A1)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < kHidden / 32; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+        bf16x8 av[2], bv[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) av[i] = *reinterpret_cast<const bf16x8 *>(&Gs[(i * 16 + lr) * LH + kb]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bv[j] = *reinterpret_cast<const bf16x8 *>(&a.W2Tb[(size_t)(c0 + j * 16 + lr) * kHidden + kb]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = c0 + j * 16 + lr;
+        float pb1 = 0.f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float g = acc[i][j][q] * d1[i][j][q];
+                Hs[(i * 16 + lk * 4 + q) * LH + col] = to_bits(g, false);
+                pb1 += g;
+            }
+        pb1 += __shfl_xor(pb1, 16, 64);
+        pb1 += __shfl_xor(pb1, 32, 64);
+        if (lk == 0) part[col] = pb1;
+    }
+    __syncthreads();
+    store_t(Hs, LH, kHidden, a.dH1T);
+}
+
+struct TrainUpdateArgs {
+    const uint16_t *XT, *H1T, *dH2T, *dH1T;
+    int Bp, K;      // K = 32 * tiles: the minibatch rows the GEMMs sum over
+    const float *part;
+    int tiles;
+    float *P, *G, *m, *v;  // G: gradients out (or null); Adam when adam != 0
+    MlpLayout L;
+    uint16_t *W1b, *W2b, *W2Tb;
+    int adam;
+    float lr, c1, c2;
+    float *stats;   // [0] = loss when want_loss
+    int want_loss;
+};
+
+// Blocks [0, 64): 32 x 32 tiles of dW2 = dH2^T H1; then (256 / 32) (Kp / 32)
+// tiles of dW1 = dH1^T (X / 100); the last block: b1, b2, w3, b3 from the
+// row tiles' partial sums (in tile order).  A GEMM block's four waves take
+// every fourth 32-row step of the minibatch, and their 32 x 32 results are
+// added in wave order.  Each block then writes the gradient and/or applies
+// Adam to its parameters and refreshes their BF16 copies.
+__global__ __launch_bounds__(256) void k_train_update(TrainUpdateArgs a) {
+    __shared__ float red[4][32][33];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    constexpr int nW2 = (kHidden / 32) * (kHidden / 32);
+    const int kt = a.L.Kp / 32, nW1 = (kHidden / 32) * kt;
+    const int bid = blockIdx.x;
+    if (bid == nW2 + nW1) {
+        const int c = tid;
+        float g1 = 0.f, g2 = 0.f, g3 = 0.f;
+        for (int w = 0; w < a.tiles; ++w) {
+            const float *pw = a.part + (size_t)w * kTrainPart;
+            g1 += pw[c];
+            g2 += pw[kHidden + c];
+            g3 += pw[2 * kHidden + c];
+        }
+        const size_t o1 = a.L.ob1 + c, o2 = a.L.ob2 + c, o3 = a.L.ow3 + c;
+        if (a.G) {
+            a.G[o1] = g1;
+            a.G[o2] = g2;
+            a.G[o3] = g3;
+        }
+        if (a.adam) {
+            (void)adam_update(a.P, a.m, a.v, o1, g1, a.lr, a.c1, a.c2);
+            (void)adam_update(a.P, a.m, a.v, o2, g2, a.lr, a.c1, a.c2);
+            (void)adam_update(a.P, a.m, a.v, o3, g3, a.lr, a.c1, a.c2);
+        }
+        if (tid == 0) {
+            float gb = 0.f, loss = 0.f;
+            for (int w = 0; w < a.tiles; ++w) {
+                gb += a.part[(size_t)w * kTrainPart + 3 * kHidden];
+                loss += a.part[(size_t)w * kTrainPart + 3 * kHidden + 1];
+            }
+            if (a.G) a.G[a.L.ob3] = gb;
+            if (a.adam) (void)adam_update(a.P, a.m, a.v, a.L.ob3, gb, a.lr, a.c1, a.c2);
+            if (a.want_loss) a.stats[0] = loss;
+        }
+        return;
+    }
+    const bool w2 = bid < nW2;
+    const int t = w2 ? bid : bid - nW2, ncolt = w2 ? kHidden / 32 : kt;
+    const int i0 = (t / ncolt) * 32, j0 = (t % ncolt) * 32;
+    const uint16_t *A = w2 ? a.dH2T : a.dH1T;
+    const uint16_t *B = w2 ? a.H1T : a.XT;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int steps = a.K / 32;
+#pragma unroll 4
+    for (int st = wave; st < steps; st += 4) {
+        const int kb = st * 32 + lk * 8;
+        bf16x8 av[2], bv[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            av[i] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + i * 16 + lr) * a.Bp + kb);
+            bv[i] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + i * 16 + lr) * a.Bp + kb);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) red[wave][i * 16 + lk * 4 + q][j * 16 + lr] = acc[i][j][q];
+    __syncthreads();
+    const int mr = tid >> 3;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int nc = (tid & 7) * 4 + e;
+        const float g = ((red[0][mr][nc] + red[1][mr][nc]) + red[2][mr][nc]) + red[3][mr][nc];
+        const int I = i0 + mr, J = j0 + nc;
+        const size_t off = w2 ? a.L.oW2 + (size_t)I * kHidden + J : a.L.oW1 + (size_t)I * a.L.Kp + J;
+        if (a.G) a.G[off] = g;
+        if (a.adam) {
+            const float p = adam_update(a.P, a.m, a.v, off, g, a.lr, a.c1, a.c2);
+            const uint16_t h = to_bits(p, false);
+            if (w2) {
+                a.W2b[(size_t)I * kHidden + J] = h;
+                a.W2Tb[(size_t)J * kHidden + I] = h;
+            } else {
+                a.W1b[(size_t)I * a.L.Kp + J] = h;
+            }
+        }
+    }
+}
+
+// One fused step: rows (X, ld, t) or archive slots (AX, Ay, ...), gradients
+// into G (grad) and/or Adam (lr > 0).
+static int fused_step(Mlp &N, const TrainRowsArgs &r0, bool grad, float lr, bool want_loss,
+                      hipStream_t s) {
+    TrainRowsArgs r = r0;
+    r.D = N.L.D;
+    r.stats = N.stats;
+    r.W1b = N.W1b;
+    r.W2b = N.W2b;
+    r.W2Tb = N.W2Tb;
+    r.b1 = N.P + N.L.ob1;
+    r.b2 = N.P + N.L.ob2;
+    r.w3 = N.P + N.L.ow3;
+    r.b3 = N.P + N.L.ob3;
+    r.XT = N.XT;
+    r.H1T = N.H1T;
+    r.dH2T = N.dH2T;
+    r.dH1T = N.dH1T;
+    r.Bp = N.Bp;
+    r.part = N.tpart;
+    const int tiles = (r.n + kTrainRows - 1) / kTrainRows;
+    const dim3 grid(tiles), block(256);
+    switch (N.L.Kp) {
+    case 32: hipLaunchKernelGGL(k_train_rows<32>, grid, block, 0, s, r); break;
+    case 64: hipLaunchKernelGGL(k_train_rows<64>, grid, block, 0, s, r); break;
+    case 96: hipLaunchKernelGGL(k_train_rows<96>, grid, block, 0, s, r); break;
+    case 128: hipLaunchKernelGGL(k_train_rows<128>, grid, block, 0, s, r); break;
+    default: return set_error(SPEO_ERR_ARG, "mlp: fused step needs Kp <= 128");
+    }
+    SPEO_TRY(check_launch("k_train_rows"));
+    TrainUpdateArgs u{};
+    u.XT = N.XT;
+    u.H1T = N.H1T;
+    u.dH2T = N.dH2T;
+    u.dH1T = N.dH1T;
+    u.Bp = N.Bp;
+    u.K = tiles * kTrainRows;
+    u.part = N.tpart;
+    u.tiles = tiles;
+    u.P = N.P;
+    u.G = grad ? N.G : nullptr;
+    u.m = N.m;
+    u.v = N.v;
+    u.L = N.L;
+    u.W1b = N.W1b;
+    u.W2b = N.W2b;
+    u.W2Tb = N.W2Tb;
+    u.adam = lr > 0.f ? 1 : 0;
+    if (u.adam) {
+        ++N.adam_t;
+        u.lr = lr;
+        u.c1 = 1.0f - powf(0.9f, (float)N.adam_t);
+        u.c2 = 1.0f - powf(0.999f, (float)N.adam_t);
+    }
+    u.stats = N.stats;
+    u.want_loss = want_loss ? 1 : 0;
+    const int blocks = (kHidden / 32) * (kHidden / 32) + (kHidden / 32) * (N.L.Kp / 32) + 1;
+    hipLaunchKernelGGL(k_train_update, dim3(blocks), dim3(256), 0, s, u);
+    return check_launch("k_train_update");
+}
+
+int Mlp::loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s, bool want_loss) {
+    if (n > max_batch) return set_error(SPEO_ERR_ARG, "mlp: batch larger than the workspace");
+    if (L.Kp > kFusedMaxKp) return loss_grad_unfused(X, ld, n, t, s, want_loss);
+    if (bf16_stale) SPEO_TRY(refresh_bf16(s));
+    TrainRowsArgs r{};
+    r.X = X;
+    r.ld = ld;
+    r.t = t;
+    r.n = n;
+    return fused_step(*this, r, true, 0.f, want_loss, s);
+}
+
+int Mlp::train_step(const float *AX, const float *Ay, int n_valid, uint64_t seed, uint32_t round,
+                    uint32_t step, int batch, float lr, bool want_loss, float *Xb, float *tb,
+                    hipStream_t s) {
+    if (batch > max_batch) return set_error(SPEO_ERR_ARG, "mlp: batch larger than the workspace");
+    if (L.Kp > kFusedMaxKp) {  // wide inputs: gather, the unfused gradients, Adam
+        SPEO_TRY(gather_batch(AX, Ay, n_valid, L.Kp, seed, round, step, batch, Xb, tb, s));
+        SPEO_TRY(loss_grad_unfused(Xb, L.Kp, batch, tb, s, want_loss));
+        return adam(lr, s);
+    }
+    if (bf16_stale) SPEO_TRY(refresh_bf16(s));
+    TrainRowsArgs r{};
+    r.AX = AX;
+    r.Ay = Ay;
+    r.n_valid = n_valid;
+    r.seed = seed;
+    r.round = round;
+    r.step = step;
+    r.n = batch;
+    return fused_step(*this, r, false, lr, want_loss, s);
 }
 
 // ------------------------------------------------------------- statistics
@@ -955,6 +1451,7 @@ int speo_mlp_params(speo_mlp h, float *get_out, const float *set_in) {
         SPEO_TRY_HIP(hipMemset(h->m.m, 0, sizeof(float) * h->m.L.n));
         SPEO_TRY_HIP(hipMemset(h->m.v, 0, sizeof(float) * h->m.L.n));
         h->m.adam_t = 0;
+        h->m.bf16_stale = true;
     }
     if (get_out) SPEO_TRY_HIP(hipMemcpy(get_out, h->m.P, sizeof(float) * h->m.L.n, hipMemcpyDeviceToHost));
     return SPEO_OK;
@@ -993,6 +1490,30 @@ int speo_mlp_adam_step(speo_mlp h, float lr) {
     if (!h) return set_error(SPEO_ERR_ARG, "speo_mlp_adam_step: null handle");
     SPEO_TRY(h->m.adam(lr, nullptr));
     return check_hip(hipDeviceSynchronize(), "speo_mlp_adam_step");
+}
+
+int speo_mlp_train_step(speo_mlp h, const float *AX, const float *Ay, int n_valid, uint64_t seed,
+                        uint32_t round, uint32_t step, int batch, float t_mean, float t_std,
+                        float lr, float *loss_out) {
+    if (!h || !AX || !Ay || n_valid < 8 || batch < 1 || batch > h->m.max_batch || !(t_std > 0) ||
+        !(lr > 0))
+        return set_error(SPEO_ERR_ARG, "speo_mlp_train_step: bad arguments");
+    const float st[3] = {0.f, t_mean, t_std};
+    SPEO_TRY_HIP(hipMemcpy(h->m.stats, st, sizeof(st), hipMemcpyHostToDevice));
+    float *Xb = nullptr, *tb = nullptr;
+    if (h->m.L.Kp > kFusedMaxKp) {  // the unfused fallback gathers into buffers
+        SPEO_TRY_HIP(hipMalloc(&Xb, sizeof(float) * (size_t)batch * h->m.L.Kp));
+        SPEO_TRY_HIP(hipMalloc(&tb, sizeof(float) * (size_t)batch));
+    }
+    int rc = h->m.train_step(AX, Ay, n_valid, seed, round, step, batch, lr, loss_out != nullptr, Xb,
+                             tb, nullptr);
+    if (rc == SPEO_OK) rc = check_hip(hipDeviceSynchronize(), "speo_mlp_train_step");
+    (void)hipFree(Xb);
+    (void)hipFree(tb);
+    if (rc == SPEO_OK && loss_out)
+        rc = check_hip(hipMemcpy(loss_out, h->m.stats, sizeof(float), hipMemcpyDeviceToHost),
+                       "speo_mlp_train_step");
+    return rc;
 }
 
 int speo_spearman(const float *a, const float *b, int n, float *rho_out, speo_stream_t stream) {

@@ -176,6 +176,8 @@ _SIGS = {
     "speo_mlp_loss_grad": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_float, c_float,
                                    c_void_p, c_void_p]),
     "speo_mlp_adam_step": (c_int, [c_void_p, c_float]),
+    "speo_mlp_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_uint64, c_uint32,
+                                    c_uint32, c_int, c_float, c_float, c_float, c_void_p]),
     "speo_spearman": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "speo_generations_for_fes": (c_int64, [c_int64, c_int64, c_int64]),
     "speo_engine_run_fes": (c_int, [c_void_p, c_int64, POINTER(c_int)]),

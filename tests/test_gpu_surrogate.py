@@ -158,6 +158,56 @@ def test_adam_step_matches_oracle(cuda, speo):
     net.close()
 
 
+def _batch_slots(oracle, seed, rnd, step, batch, n_valid):
+    """The archive slots of a training minibatch: word b % 4 of Philox block
+    {b / 4, round, step, 33} as u01, times n_valid in FP32, truncated, clamped,
+    held-out slots (slot % 8 == 7) moved down by one."""
+    b = np.arange(batch)
+    w = oracle.philox_np(seed, b >> 2, rnd, step, 33)
+    word = np.choose(b & 3, w).astype(np.uint32)
+    u = ((word >> 8).astype(np.float64) + 1.0) * 2.0 ** -24
+    slot = (u.astype(np.float32) * np.float32(n_valid)).astype(np.int64)
+    slot = np.minimum(slot, n_valid - 1)
+    return np.where(slot % 8 == 7, slot - 1, slot)
+
+
+@pytest.mark.parametrize("D", [100, 30])
+def test_fused_train_step_equals_gather_grad_adam(cuda, speo, oracle, D):
+    # the engine's two-kernel training step (archive gather, gradients, Adam,
+    # BF16 refresh) against the slots drawn on the host, the gradient entry
+    # point and the Adam entry point: the same parameters, bit for bit, over
+    # three steps; the gradients themselves are pinned to the oracle above
+    rng = np.random.default_rng(7)
+    Kp, n_valid, batch, seed, rnd = (D + 31) // 32 * 32, 3000, 500, 0xABCDEF12345, 5
+    AX = np.zeros((4096, Kp), np.float32)
+    AX[:, :D] = rng.uniform(-100, 100, (4096, D))
+    Ay = rng.normal(4.0, 1.5, 4096).astype(np.float32)
+    a, b = Net(speo, D, 512), Net(speo, D, 512)
+    f = _random_params(a, rng)
+    a.set_flat(f)
+    b.set_flat(f)
+    AXd, Ayd = _t(AX, cuda), _t(Ay, cuda)
+    for step in range(3):
+        la = np.zeros(1, np.float32)
+        speo.check(a.L.speo_mlp_train_step(a.h, ctypes.c_void_p(AXd.data_ptr()),
+                                           ctypes.c_void_p(Ayd.data_ptr()), n_valid, seed, rnd,
+                                           step, batch, 4.0, 1.5, 1e-3,
+                                           la.ctypes.data_as(ctypes.c_void_p)))
+        slots = _batch_slots(oracle, seed, rnd, step, batch, n_valid)
+        assert slots.min() >= 0 and slots.max() < n_valid and not np.any(slots % 8 == 7)
+        Xb, tb = _t(AX[slots, :D], cuda), _t(Ay[slots], cuda)
+        lb = np.zeros(1, np.float32)
+        speo.check(b.L.speo_mlp_loss_grad(b.h, ctypes.c_void_p(Xb.data_ptr()), D, batch,
+                                          ctypes.c_void_p(tb.data_ptr()), 4.0, 1.5,
+                                          lb.ctypes.data_as(ctypes.c_void_p), None))
+        speo.check(b.L.speo_mlp_adam_step(b.h, 1e-3))
+        assert la[0] == lb[0]
+        np.testing.assert_array_equal(a.flat(), b.flat())
+    assert not np.array_equal(a.flat(), f)
+    a.close()
+    b.close()
+
+
 def test_spearman_kernel(cuda, speo):
     import surrogate as SO
     import torch

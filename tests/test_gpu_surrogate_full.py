@@ -93,20 +93,23 @@ def test_shuffled_labels_trigger_the_fallback(cuda, speo, oracle):
     eng, probs, data, pidx = _setup(speo, oracle, **_surr_kw(rho_min=0.5))
     eng.run(29)
     rho, act, _ = eng.surrogate_stats()
-    assert act[1] == 1 and rho[1] > 0.5, rho       # elliptic: the network ranks well
-    # shuffle problem 1's labels; the training round after generation 30
+    # Rastrigin: after 29 generations the network ranks the held-out rows with
+    # rho of about 0.68 (elliptic's, whose log-values span more decades, is
+    # still about 0.2, so that problem falls back already)
+    assert act[0] == 1 and rho[0] > 0.5, rho
+    # shuffle problem 0's labels; the training round after generation 30
     # measures the held-out rho on the shuffled archive
-    _, _, _, _, Ay, _, _ = eng.surrogate_dump(1, C, CAP)
+    _, _, _, _, Ay, _, _ = eng.surrogate_dump(0, C, CAP)
     rng = np.random.default_rng(0)
-    eng.surrogate_set_labels(1, rng.permutation(Ay))
+    eng.surrogate_set_labels(0, rng.permutation(Ay))
     eng.run(1)
     rho2, act2, _ = eng.surrogate_stats()
-    assert act2[1] == 0 and rho2[1] < 0.5, rho2
-    # the next generation takes candidate 0 (the plain trial) for problem 1
+    assert act2[0] == 0 and rho2[0] < 0.5, rho2
+    # the next generation takes candidate 0 (the plain trial) for problem 0
     eng.run(1)
     U, fU = eng.read_trials()
-    n, cand, pred, _, _, _, _ = eng.surrogate_dump(1, C, CAP)
-    rows = _rows_of(eng, pidx, 1)
+    n, cand, pred, _, _, _, _ = eng.surrogate_dump(0, C, CAP)
+    rows = _rows_of(eng, pidx, 0)
     np.testing.assert_array_equal(U[rows, :D], cand[:, 0, :D])
     eng.close()
     for p in probs:
