@@ -264,6 +264,96 @@ def precision_study(S, n_points=10000, D=1000, reps=5):
     return out
 
 
+def write_cec_files(S, directory, fid, D, ncomp=5, seed=BASE_SEED + 500):
+    """Problem files of CEC2014 function `fid` in the reference's input_data
+    format (M_<id>_D<D>.txt, shift_data_<id>.txt, shuffle_data_<id>_D<D>.txt),
+    filled from the library's seeded generator: `ncomp` components, the most
+    any CEC2014 function reads, so every function finds what it needs."""
+    rng = np.random.default_rng(seed + fid)
+    shifts, rots = [], []
+    for c in range(ncomp):
+        o, M = S.generate_problem_data(S.problem_seed(seed + fid, c), D)
+        shifts.append(o)
+        rots.append(M)
+    d = Path(directory)
+    with open(d / f"M_{fid}_D{D}.txt", "w") as f:
+        for M in rots:
+            np.savetxt(f, M, fmt="%.17e")
+    with open(d / f"shift_data_{fid}.txt", "w") as f:
+        for o in shifts:
+            f.write(" ".join(f"{v:.17e}" for v in o) + "\n")
+    with open(d / f"shuffle_data_{fid}_D{D}.txt", "w") as f:
+        for _ in range(ncomp):
+            f.write(" ".join(str(int(v)) for v in rng.permutation(D) + 1) + "\n")
+
+
+def cec_suite(S, D=100, n=8192, reps=20):
+    """Evaluation rate of each CEC2014 function F1-F30 at dimension D on n
+    random rows (speo_evaluate, FP32 rotation for F1-F3, F5, F8-F11, F13,
+    F14; the FP64 generic kernel for the rest), with the rotation FLOP of
+    each (2 D^2 per row and rotated component)."""
+    import torch
+    rng = np.random.default_rng(13)
+    X = torch.from_numpy(rng.uniform(-100, 100, (n, D)).astype(np.float32)).cuda()
+    f = torch.zeros(n, dtype=torch.float32, device="cuda")
+    out = {"dim": D, "rows": n, "functions": {}}
+    with tempfile.TemporaryDirectory() as td:
+        for fid in range(1, 31):
+            write_cec_files(S, td, fid, D)
+            prob = S.Problem.from_cec(td, fid, D)
+            S.evaluate(prob, X, f)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                S.evaluate(prob, X, f)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            flops = 2.0 * D * D * n * prob.ncomp
+            out["functions"][f"F{fid}"] = {"components": prob.ncomp,
+                                          "evals_per_s": n / dt, "us_per_call": dt * 1e6,
+                                          "rotation_TFLOPs": flops / dt / 1e12}
+            prob.close()
+    return out
+
+
+def surrogate_cost(S, gens=40, warm=20, seed=BASE_SEED + 400):
+    """Surrogate-assisted configuration: 32 islands x NP=256, D=100,
+    Rastrigin / elliptic, 8 candidates per target, training every 10
+    generations (40 Adam steps of batch 1024) on a 65,536-row archive.
+    Wall time per generation (training rounds included) after `warm`
+    generations, next to the plain engine on the same islands."""
+    D, I, NP = 100, 32, 256
+    probs = []
+    for k, obj in enumerate((S.OBJ_RASTRIGIN, S.OBJ_ELLIPTIC)):
+        o, M = S.generate_problem_data(S.problem_seed(seed, k), D)
+        probs.append(S.Problem([obj], o[None, :], M[None, :, :]))
+    pidx = [i % 2 for i in range(I)]
+    out = {}
+    for name, extra in (("plain", {}),
+                        ("surrogate", dict(surrogate_candidates=8, surrogate_train_interval=10,
+                                           surrogate_archive=65536, surrogate_steps=40,
+                                           surrogate_batch=1024, surrogate_lr=1e-3,
+                                           surrogate_rho_min=0.5))):
+        eng = S.Engine(probs, [NP] * I, pidx, F=0.5, CR=0.9, seed=seed, interval=50,
+                       migration_size=2, mode=S.MODE_PLAIN, rows_per_block=0, **extra)
+        eng.run(warm)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.run(gens)
+        eng.sync()
+        dt = time.perf_counter() - t0
+        out[name] = {"ms_per_generation": dt / gens * 1e3}
+        if extra:
+            rho, act, loss = eng.surrogate_stats()
+            out[name].update({"rho": [float(r) for r in rho], "active": [int(a) for a in act],
+                              "rounds": eng.stats()["surrogate_rounds"]})
+        eng.close()
+    out["ratio"] = out["surrogate"]["ms_per_generation"] / out["plain"]["ms_per_generation"]
+    for p in probs:
+        p.close()
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -441,6 +531,13 @@ def main():
                     help="also run the uneven 256-island batch (D in {10,30,50,100}, NP 50..512)")
     ap.add_argument("--precision-study", action="store_true",
                     help="also run the D=1000 precision study (10k points)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="also time a surrogate-assisted generation against a plain one "
+                         "(32 islands x 256, D=100, 8 candidates)")
+    ap.add_argument("--cec-suite", action="store_true",
+                    help="also time the evaluation of every CEC2014 function F1-F30 at D=100")
+    ap.add_argument("--only-extras", action="store_true",
+                    help="skip the headline run; run only the requested extra legs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (scales rocprof byte counters)")
     args = ap.parse_args()
@@ -465,6 +562,23 @@ def main():
         print(f"# copy calibration: 3 x {args.calibrate_copy_mib} MiB read + written",
               file=sys.stderr, flush=True)
         del src, dst
+    if args.only_extras:
+        extra = {}
+        if args.surrogate:
+            print("# surrogate cost", file=sys.stderr, flush=True)
+            extra["surrogate_cost"] = surrogate_cost(S)
+        if args.uneven:
+            print("# uneven batch", file=sys.stderr, flush=True)
+            extra["uneven_batch"] = {f"{m}_{p}": uneven_batch(S, precision=p, mode=m)
+                                     for m, p in (("epoch", "fp32"), ("fused", "fp32"))}
+        if args.precision_study:
+            print("# precision study", file=sys.stderr, flush=True)
+            extra["precision_study"] = precision_study(S)
+        if args.cec_suite:
+            print("# CEC2014 suite", file=sys.stderr, flush=True)
+            extra["cec_suite"] = cec_suite(S)
+        print(json.dumps(extra), flush=True)
+        return
     eng, problems, data = build_engine(S, args)
     rows = eng.total_rows
     D = args.dim
@@ -525,6 +639,7 @@ def main():
             "islands": args.islands, "island_size": args.island_size, "dim": D,
             "mode": args.mode, "precision": args.precision, "graph": not args.no_graph,
             "rows_per_block": eng.stats()["rows_per_block"],
+            "epoch_parts": eng.stats()["epoch_parts"],
         },
         "roofline": {
             "kernel": kernel_name,
@@ -556,6 +671,12 @@ def main():
     if args.precision_study:
         print("# precision study", file=sys.stderr, flush=True)
         out["precision_study"] = precision_study(S)
+    if args.surrogate:
+        print("# surrogate cost", file=sys.stderr, flush=True)
+        out["surrogate_cost"] = surrogate_cost(S)
+    if args.cec_suite:
+        print("# CEC2014 suite", file=sys.stderr, flush=True)
+        out["cec_suite"] = cec_suite(S)
     if not args.no_tte:
         print("# time-to-error runs", file=sys.stderr, flush=True)
         out["time_to_error"] = time_to_error(S, D, 32, 256, args.tte_gens, 10)
