@@ -285,6 +285,20 @@ SPEO_EXPORT int speo_evaluate(speo_problem p, const float *X, int ld, int n, flo
 SPEO_EXPORT int speo_evaluate_f64(speo_problem p, const float *X, int ld, int n, double *f_out,
                                   speo_stream_t stream);
 
+/* speo_evaluate without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace (per-chunk row sums above 128 columns, per-component values of a
+ * composition: 2 ceil(dim/128) n + ncomp n floats, the first term only when
+ * dim > 128, the second only when ncomp > 1) is a fixed 4 MiB (2^20 floats)
+ * device array per GPU, held until the process ends; larger inputs are
+ * rejected with SPEO_ERR_ARG before anything is launched.  Every call on one
+ * GPU shares that array, which keeps the last call's contents: calls that
+ * overlap on one GPU (other streams, other threads) race.  Same results as
+ * speo_evaluate, bit for bit. */
+SPEO_EXPORT int speo_evaluate_noalloc(speo_problem p, const float *X, int ld, int n,
+                                      float *f_out, int precision, speo_stream_t stream);
+
 /* ---- DE operators on caller-owned buffers (device pointers) ---------- */
 
 /* x = lo + u (hi - lo) for rows [0, n) of stride ld (padding set to 0);

@@ -33,7 +33,7 @@ from ._lib import (DEInfo, DriverConfig, EngineConfig, RunRecord, EngineStats, G
 
 __all__ = [
     "Problem", "Engine", "generate_problem_data", "problem_seed", "init_population",
-    "de_generate", "de_select", "evaluate", "evaluate_f64", "segment_argmin", "SpeoError",
+    "de_generate", "de_select", "evaluate", "evaluate_f64", "evaluate_noalloc", "segment_argmin", "SpeoError",
     "OBJ_ELLIPTIC", "OBJ_RASTRIGIN", "OBJ_ACKLEY", "OBJ_BENT_CIGAR", "OBJ_DISCUS", "OBJ_HAPPYCAT",
     "OBJ_HGBAT", "OBJ_SCHWEFEL", "OBJ_ROSENBROCK", "OBJ_WEIERSTRASS", "OBJ_GRIEWANK",
     "OBJ_KATSUURA", "OBJ_GRIE_ROSEN", "OBJ_ESCAFFER6", "OBJ_HYBRID_1", "OBJ_HYBRID_2",
@@ -195,6 +195,16 @@ def evaluate(problem: Problem, X, f_out, precision=PREC_FP32, ld=None, n=None, s
     n = X.shape[0] if n is None else n
     check(lib().speo_evaluate(problem.handle, _p(X), ld, n, _p(f_out), precision,
                               _stream(stream)))
+
+
+def evaluate_noalloc(problem: Problem, X, f_out, precision=PREC_FP32, ld=None, n=None,
+                     stream=None):
+    """evaluate() with no device allocation: the workspace is a fixed 2^20-float
+    array per GPU, shared by every call on that GPU (overlapping calls race)."""
+    ld = X.shape[1] if ld is None else ld
+    n = X.shape[0] if n is None else n
+    check(lib().speo_evaluate_noalloc(problem.handle, _p(X), ld, n, _p(f_out), precision,
+                                      _stream(stream)))
 
 
 def evaluate_f64(problem: Problem, X, f_out, ld=None, n=None, stream=None):

@@ -134,6 +134,8 @@ _SIGS = {
     "speo_problem_create_cec": (c_int, [c_char_p, c_int, c_int, POINTER(c_void_p)]),
     "speo_evaluate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
     "speo_evaluate_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "speo_evaluate_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
+                                      c_void_p]),
     "speo_init_population": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_float, c_uint64,
                                      c_uint32, c_void_p]),
     "speo_de_generate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p,
