@@ -641,8 +641,15 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
         c.rows_per_block != 64)
         return set_error(SPEO_ERR_ARG, "speo_engine_create: rows_per_block must be 0, 16, 32, 64");
     problems.assign(c.problems, c.problems + c.num_problems);
-    for (auto *q : problems)
+    for (auto *q : problems) {
         if (!q) return set_error(SPEO_ERR_ARG, "speo_engine_create: null problem");
+        if (c.precision != SPEO_PREC_FP32 && !q->h.generic)
+            for (int k = 0; k < q->h.ncomp; ++k)
+                if (q->h.comp[k].unrotated)
+                    return set_error(SPEO_ERR_UNSUPPORTED,
+                                     "speo_engine_create: a 16-bit rotation precision would round "
+                                     "an unrotated component's z; use SPEO_PREC_FP32 for it");
+    }
     I = c.num_islands;
     for (int i = 0; i < I; ++i) {
         const int pr = c.island_problem[i];
@@ -999,19 +1006,48 @@ int Engine::surrogate_generation(int gen_offset, hipStream_t s) {
 
 int Engine::surrogate_train(hipStream_t s) {
     const size_t cap = (size_t)cfg.surrogate_archive;
+    std::vector<size_t> todo;
     for (size_t p = 0; p < surr.size(); ++p) {
         SurrProblem &sp = *surr[p];
         if (sp.rows == 0) continue;
         const int n_valid = (int)std::min<int64_t>(sp.appended, (int64_t)cap);
         if (n_valid < std::max(cfg.surrogate_batch, 512)) continue;
-        const float *ax = AX + p * cap * Kp, *ay = Ay + p * cap;
-        SPEO_TRY(mean_std(ay, n_valid, sp.net.stats, s));
+        SPEO_TRY(mean_std(Ay + p * cap, n_valid, sp.net.stats, s));
+        todo.push_back(p);
+    }
+    auto seed_of = [&](size_t p) { return cfg.seed ^ (0xD1B54A32D192ED03ull * (p + 1)); };
+    auto n_valid_of = [&](size_t p) { return (int)std::min<int64_t>(surr[p]->appended, (int64_t)cap); };
+    if (!todo.empty() && Kp <= kFusedMaxKp) {
+        // every problem's network takes its step in the same two launches
+        std::vector<Mlp *> nets;
+        std::vector<const float *> ax, ay;
+        std::vector<int> nv;
+        std::vector<uint64_t> seeds;
+        for (size_t p : todo) {
+            nets.push_back(&surr[p]->net);
+            ax.push_back(AX + p * cap * Kp);
+            ay.push_back(Ay + p * cap);
+            nv.push_back(n_valid_of(p));
+            seeds.push_back(seed_of(p));
+        }
         for (int step = 0; step < cfg.surrogate_steps; ++step)
-            SPEO_TRY(sp.net.train_step(ax, ay, n_valid, cfg.seed ^ (0xD1B54A32D192ED03ull * (p + 1)),
-                                       (uint32_t)surr_rounds, (uint32_t)step, cfg.surrogate_batch,
-                                       cfg.surrogate_lr, step == cfg.surrogate_steps - 1, Xb, tb, s));
-        const int H = std::min(1024, n_valid / 8);
-        SPEO_TRY(gather_holdout(ax, ay, H, Kp, Xh, yh, s));
+            SPEO_TRY(train_steps(nets.data(), ax.data(), ay.data(), nv.data(), seeds.data(),
+                                 (int)todo.size(), (uint32_t)surr_rounds, (uint32_t)step,
+                                 cfg.surrogate_batch, cfg.surrogate_lr,
+                                 step == cfg.surrogate_steps - 1, s));
+    } else {
+        for (size_t p : todo)
+            for (int step = 0; step < cfg.surrogate_steps; ++step)
+                SPEO_TRY(surr[p]->net.train_step(AX + p * cap * Kp, Ay + p * cap, n_valid_of(p),
+                                                 seed_of(p), (uint32_t)surr_rounds, (uint32_t)step,
+                                                 cfg.surrogate_batch, cfg.surrogate_lr,
+                                                 step == cfg.surrogate_steps - 1, Xb, tb, s));
+    }
+    for (size_t p : todo) {
+        SurrProblem &sp = *surr[p];
+        if (sp.net.bf16_stale) SPEO_TRY(sp.net.refresh_bf16(s));
+        const int H = std::min(1024, n_valid_of(p) / 8);
+        SPEO_TRY(gather_holdout(AX + p * cap * Kp, Ay + p * cap, H, Kp, Xh, yh, s));
         SPEO_TRY(sp.net.infer(Xh, Kp, H, predh, s));
         SPEO_TRY(spearman(predh, yh, H, rho_dev + p, s));
         sp.trained = true;

@@ -240,14 +240,17 @@ int eval_problem(const ProblemHost &P, const float *X, int ld, int n, float *f_o
     const int nchunks = (P.dim + kNC - 1) / kNC;
     float *part = nchunks > 1 ? scratch : nullptr;
     float *gbuf = scratch + (nchunks > 1 ? (size_t)nchunks * 2 * n : 0);
+    // an unrotated component's z = (x - o) rate is exact in FP32; a 16-bit
+    // identity GEMM would round it to the operand precision
+    auto prec_of = [&](int c) { return P.comp[c].unrotated ? (int)SPEO_PREC_FP32 : precision; };
     if (P.ncomp == 1) {
         const CompHost &c = P.comp[0];
-        return eval_component(P.view(0, precision), X, ld, n, P.dim, f_out, part,
-                              (float)c.lambda, (float)c.bias, 0, precision, rows_per_block, s);
+        return eval_component(P.view(0, prec_of(0)), X, ld, n, P.dim, f_out, part,
+                              (float)c.lambda, (float)c.bias, 0, prec_of(0), rows_per_block, s);
     }
     for (int c = 0; c < P.ncomp; ++c)
-        SPEO_TRY(eval_component(P.view(c, precision), X, ld, n, P.dim, gbuf + (size_t)c * n, part,
-                                1.f, 0.f, 1, precision, rows_per_block, s));
+        SPEO_TRY(eval_component(P.view(c, prec_of(c)), X, ld, n, P.dim, gbuf + (size_t)c * n, part,
+                                1.f, 0.f, 1, prec_of(c), rows_per_block, s));
     const int waves_per_block = 4;
     hipLaunchKernelGGL(k_comp_combine, dim3((n + waves_per_block - 1) / waves_per_block),
                        dim3(64 * waves_per_block), 0, s, P.params_dev, P.ncomp, P.dim, X, ld, n,

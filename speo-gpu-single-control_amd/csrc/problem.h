@@ -38,6 +38,8 @@ struct CompHost {
     uint16_t *Mbfs[3] = {nullptr, nullptr, nullptr};         // [128][round_up(dim,32)]
     uint16_t *Mhs[3] = {nullptr, nullptr, nullptr};
     float *colw = nullptr;      // [dim] bent cigar / discus weights (elliptic: ProblemHost::colw)
+    bool unrotated = false;     // M = I: evaluated in FP32 whatever the requested precision,
+                                // where the identity GEMM is exact
 };
 
 // Per-component parameters of the FP64 path and of the composition combine,

@@ -203,6 +203,7 @@ static int create_problem(int dim, int ncomp, const int *objectives, const doubl
             std::memcpy(M.data(), rotation + (size_t)c * D * D, sizeof(double) * D * D);
         else
             for (size_t i = 0; i < D; ++i) M[i * D + i] = 1.0;
+        h.unrotated = is_identity(M.data(), dim);
         std::vector<double> o(shift + (size_t)c * D, shift + (size_t)(c + 1) * D);
         std::vector<float> o32(D);
         for (size_t j = 0; j < D; ++j) o32[j] = (float)o[j];

@@ -109,6 +109,11 @@ int gather_batch(const float *AX, const float *Ay, int n_valid, int Kp, uint64_t
 int gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh, float *yh,
                    hipStream_t s);
 int mean_std(const float *y, int n, float *stats, hipStream_t s);
+// One training step (Mlp::train_step's, Kp <= 128) of nn networks of equal
+// width on their own archives, in two launches per eight networks.
+int train_steps(Mlp *const *nets, const float *const *AX, const float *const *Ay,
+                const int *n_valid, const uint64_t *seed, int nn, uint32_t round, uint32_t step,
+                int batch, float lr, bool want_loss, hipStream_t s);
 int spearman(const float *a, const float *b, int n, float *rho_out, hipStream_t s);
 
 }  // namespace speo

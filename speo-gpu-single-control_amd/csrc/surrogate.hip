@@ -716,22 +716,48 @@ struct TrainRowsArgs {
     float *part;           // [tile][kTrainPart]: db1 | db2 | dw3 | (db3, loss)
 };
 
-// Rows [32 w, 32 w + 32) of the minibatch.  Wave v owns hidden columns
-// [64 v, 64 v + 64) of every layer for the 32 rows (2 x 4 MFMA 16x16 tiles).
-// Rows >= n are zero inputs with dy = 0, so they add nothing to any gradient.
+// Rows [32 w, 32 w + 32) of the minibatch, 16 waves: wave v owns hidden
+// columns [16 v, 16 v + 16) of every layer for the 32 rows (two MFMA 16x16
+// tiles).  A wave's weight fragments (W1, W2 and W2^T for its columns, 20
+// 16-byte loads per lane) are all issued at the start, so the kernel waits
+// on one L2 round trip for them instead of one per K step.  Rows >= n are
+// zero inputs with dy = 0, so they add nothing to any gradient.
+// Networks trained in one launch (one per problem), each a run of blocks.
+constexpr int kMaxTrainNets = 8;
+struct TrainRowsBatch {
+    TrainRowsArgs net[kMaxTrainNets];
+    int tiles;  // row tiles per network
+};
+
 template <int KP>
-__global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
-    constexpr int TM = kTrainRows, LX = KP + 8, LH = kHidden + 8;
+__global__ __launch_bounds__(1024) void k_train_rows(TrainRowsBatch batch) {
+    const TrainRowsArgs &a = batch.net[blockIdx.x / batch.tiles];
+    constexpr int TM = kTrainRows, LX = KP + 8, LH = kHidden + 8, NW = kHidden / 16;
+    constexpr int K1 = KP / 32, K2 = kHidden / 32;
     __shared__ __align__(16) uint16_t Xs[TM * LX];
-    __shared__ __align__(16) uint16_t Hs[TM * LH];  // H1, then dH1
+    __shared__ __align__(16) uint16_t Hs[TM * LH];  // H1
     __shared__ __align__(16) uint16_t Gs[TM * LH];  // dH2
-    __shared__ float red[4][TM];
+    __shared__ __align__(16) uint16_t Ds[TM * LH];  // dH1
+    __shared__ float red[NW][TM];
     __shared__ float ts[TM], dys[TM], lts[TM];
     __shared__ int slots[TM];
+    const int tile = blockIdx.x % batch.tiles;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
-    const int row0 = blockIdx.x * TM;
-    float *part = a.part + (size_t)blockIdx.x * kTrainPart;
+    const int row0 = tile * TM;
+    const int col = wave * 16 + lr;  // this lane's hidden column in every layer
+    float *part = a.part + (size_t)tile * kTrainPart;
+    // weight fragments first: B(k, n = col) for K steps of 32 (8 per lane)
+    bf16x8 w1[K1], w2[K2], w2t[K2];
+#pragma unroll
+    for (int ks = 0; ks < K1; ++ks)
+        w1[ks] = *reinterpret_cast<const bf16x8 *>(&a.W1b[(size_t)col * KP + ks * 32 + lk * 8]);
+#pragma unroll
+    for (int ks = 0; ks < K2; ++ks) {
+        w2[ks] = *reinterpret_cast<const bf16x8 *>(&a.W2b[(size_t)col * kHidden + ks * 32 + lk * 8]);
+        w2t[ks] = *reinterpret_cast<const bf16x8 *>(&a.W2Tb[(size_t)col * kHidden + ks * 32 + lk * 8]);
+    }
+    const float bias1 = a.b1[col], bias2 = a.b2[col], wout = a.w3[col], bias3 = a.b3[0];
     if (tid < TM) {
         const int b = row0 + tid;
         int slot = b;
@@ -755,23 +781,23 @@ __global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
     {
         const float *src = a.AX ? a.AX : a.X;
         const size_t ld = a.AX ? (size_t)KP : (size_t)a.ld;
-        constexpr int G = TM * KP / 256;
+        constexpr int G = TM * KP / 1024;
         float x[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const int e = tid + g * 256, r = e / KP, k = e % KP;
+            const int e = tid + g * 1024, r = e / KP, k = e % KP;
             x[g] = (row0 + r < a.n && k < a.D) ? src[(size_t)slots[r] * ld + k] : 0.f;
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const int e = tid + g * 256, r = e / KP, k = e % KP;
+            const int e = tid + g * 1024, r = e / KP, k = e % KP;
             Xs[r * LX + k] = to_bits(x[g] * kXScale, false);
         }
     }
     __syncthreads();
     // transposed copy of a [TM][stride] BF16 tile: T[c][row0 + r]
     auto store_t = [&](const uint16_t *S, int stride, int ncols, uint16_t *T) __attribute__((always_inline)) {
-        for (int e = tid; e < ncols * (TM / 8); e += 256) {
+        for (int e = tid; e < ncols * (TM / 8); e += 1024) {
             const int c = e / (TM / 8), r8 = (e % (TM / 8)) * 8;
             u16x8 v;
 #pragma unroll
@@ -779,91 +805,56 @@ __global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
             *reinterpret_cast<u16x8 *>(T + (size_t)c * a.Bp + row0 + r8) = v;
         }
     };
-    store_t(Xs, LX, KP, a.XT);
-    const int c0 = wave * 64;
-    floatx4 acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    floatx4 acc[2];
     // layer 1
-    for (int ks = 0; ks < KP / 32; ++ks) {
+    acc[0] = acc[1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K1; ++ks) {
         const int kb = ks * 32 + lk * 8;
-        bf16x8 av[2], bv[4];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) av[i] = *reinterpret_cast<const bf16x8 *>(&Xs[(i * 16 + lr) * LX + kb]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            bv[j] = *reinterpret_cast<const bf16x8 *>(&a.W1b[(size_t)(c0 + j * 16 + lr) * KP + kb]);
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                *reinterpret_cast<const bf16x8 *>(&Xs[(i * 16 + lr) * LX + kb]), w1[ks], acc[i], 0, 0, 0);
     }
-    float d1[2][4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = c0 + j * 16 + lr;
-        const float bias = a.b1[col];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float pre = acc[i][j][q] + bias;
-                d1[i][j][q] = dsiluf_(pre);
-                Hs[(i * 16 + lk * 4 + q) * LH + col] = to_bits(siluf_(pre), false);
-                acc[i][j][q] = 0.f;
-            }
-    }
-    __syncthreads();
-    // layer 2
-    for (int ks = 0; ks < kHidden / 32; ++ks) {
-        const int kb = ks * 32 + lk * 8;
-        bf16x8 av[2], bv[4];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) av[i] = *reinterpret_cast<const bf16x8 *>(&Hs[(i * 16 + lr) * LH + kb]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            bv[j] = *reinterpret_cast<const bf16x8 *>(&a.W2b[(size_t)(c0 + j * 16 + lr) * kHidden + kb]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
-    }
-    store_t(Hs, LH, kHidden, a.H1T);
-    float d2[2][4][4], ysum[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) ysum[i][q] = 0.f;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = c0 + j * 16 + lr;
-        const float bias = a.b2[col], w = a.w3[col];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float pre = acc[i][j][q] + bias, h = siluf_(pre);
-                d2[i][j][q] = dsiluf_(pre);
-                acc[i][j][q] = h;  // H2
-                ysum[i][q] = fmaf(h, w, ysum[i][q]);
-            }
-    }
+    float d1[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float v = sum16(ysum[i][q]);
+            const float pre = acc[i][q] + bias1;
+            d1[i][q] = dsiluf_(pre);
+            Hs[(i * 16 + lk * 4 + q) * LH + col] = to_bits(siluf_(pre), false);
+        }
+    __syncthreads();
+    // layer 2
+    acc[0] = acc[1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K2; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                *reinterpret_cast<const bf16x8 *>(&Hs[(i * 16 + lr) * LH + kb]), w2[ks], acc[i], 0, 0, 0);
+    }
+    float d2[2][4], h2[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float pre = acc[i][q] + bias2, h = siluf_(pre);
+            d2[i][q] = dsiluf_(pre);
+            h2[i][q] = h;
+            const float v = sum16(h * wout);
             if (lr == 0) red[wave][i * 16 + lk * 4 + q] = v;
         }
     __syncthreads();
     if (tid < TM) {
         float dy = 0.f, lt = 0.f;
         if (row0 + tid < a.n) {
-            const float y = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid] + a.b3[0];
+            float y = 0.f;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) y += red[w][tid];
+            y += bias3;
             const float e = y - ts[tid];
             dy = e / (float)a.n;
             lt = 0.5f * e * e / (float)a.n;
@@ -874,10 +865,7 @@ __global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
     __syncthreads();
     // dH2 = dy w3 silu'(A2); partial sums of db2 = sum dH2 and dw3 = sum H2 dy
     // over the tile's rows (this lane's 8 rows, then the four lane groups)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = c0 + j * 16 + lr;
-        const float w = a.w3[col];
+    {
         float pb2 = 0.f, pw3 = 0.f;
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -885,10 +873,10 @@ __global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
             for (int q = 0; q < 4; ++q) {
                 const int r = i * 16 + lk * 4 + q;
                 const float dy = dys[r];
-                const float g = dy * w * d2[i][j][q];
+                const float g = dy * wout * d2[i][q];
                 Gs[r * LH + col] = to_bits(g, false);
                 pb2 += g;
-                pw3 = fmaf(acc[i][j][q], dy, pw3);
+                pw3 = fmaf(h2[i][q], dy, pw3);
             }
         pb2 += __shfl_xor(pb2, 16, 64);
         pb2 += __shfl_xor(pb2, 32, 64);
@@ -899,46 +887,36 @@ __global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
             part[2 * kHidden + col] = pw3;
         }
     }
-    if (tid == 0) {
-        float sdy = 0.f, sl = 0.f;
-        for (int r = 0; r < TM; ++r) {
-            sdy += dys[r];
-            sl += lts[r];
+    if (wave == 0) {  // db3 and loss partials: the tile's rows by an xor butterfly
+        float sdy = lane < TM ? dys[lane] : 0.f, sl = lane < TM ? lts[lane] : 0.f;
+        for (int off = 1; off < 64; off <<= 1) {
+            sdy += __shfl_xor(sdy, off, 64);
+            sl += __shfl_xor(sl, off, 64);
         }
-        part[3 * kHidden] = sdy;
-        part[3 * kHidden + 1] = sl;
+        if (lane == 0) {
+            part[3 * kHidden] = sdy;
+            part[3 * kHidden + 1] = sl;
+        }
     }
     __syncthreads();
-    store_t(Gs, LH, kHidden, a.dH2T);
     // dH1 = (dH2 W2) silu'(A1)
+    acc[0] = acc[1] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < kHidden / 32; ++ks) {
+    for (int ks = 0; ks < K2; ++ks) {
         const int kb = ks * 32 + lk * 8;
-        bf16x8 av[2], bv[4];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) av[i] = *reinterpret_cast<const bf16x8 *>(&Gs[(i * 16 + lr) * LH + kb]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            bv[j] = *reinterpret_cast<const bf16x8 *>(&a.W2Tb[(size_t)(c0 + j * 16 + lr) * kHidden + kb]);
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                *reinterpret_cast<const bf16x8 *>(&Gs[(i * 16 + lr) * LH + kb]), w2t[ks], acc[i], 0, 0, 0);
     }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = c0 + j * 16 + lr;
+    {
         float pb1 = 0.f;
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float g = acc[i][j][q] * d1[i][j][q];
-                Hs[(i * 16 + lk * 4 + q) * LH + col] = to_bits(g, false);
+                const float g = acc[i][q] * d1[i][q];
+                Ds[(i * 16 + lk * 4 + q) * LH + col] = to_bits(g, false);
                 pb1 += g;
             }
         pb1 += __shfl_xor(pb1, 16, 64);
@@ -946,7 +924,12 @@ __global__ __launch_bounds__(256) void k_train_rows(TrainRowsArgs a) {
         if (lk == 0) part[col] = pb1;
     }
     __syncthreads();
-    store_t(Hs, LH, kHidden, a.dH1T);
+    // the four transposed activation tiles, stored last so that no barrier
+    // above waits for them
+    store_t(Xs, LX, KP, a.XT);
+    store_t(Hs, LH, kHidden, a.H1T);
+    store_t(Gs, LH, kHidden, a.dH2T);
+    store_t(Ds, LH, kHidden, a.dH1T);
 }
 
 struct TrainUpdateArgs {
@@ -969,21 +952,39 @@ struct TrainUpdateArgs {
 // every fourth 32-row step of the minibatch, and their 32 x 32 results are
 // added in wave order.  Each block then writes the gradient and/or applies
 // Adam to its parameters and refreshes their BF16 copies.
-__global__ __launch_bounds__(256) void k_train_update(TrainUpdateArgs a) {
+struct TrainUpdateBatch {
+    TrainUpdateArgs net[kMaxTrainNets];
+    int blocks;  // blocks per network
+};
+
+__global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
+    const TrainUpdateArgs &a = batch.net[blockIdx.x / batch.blocks];
     __shared__ float red[4][32][33];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
     constexpr int nW2 = (kHidden / 32) * (kHidden / 32);
     const int kt = a.L.Kp / 32, nW1 = (kHidden / 32) * kt;
-    const int bid = blockIdx.x;
+    const int bid = blockIdx.x % batch.blocks;
     if (bid == nW2 + nW1) {
+        // tile partials in tile order, eight tiles' loads in flight at a time
         const int c = tid;
         float g1 = 0.f, g2 = 0.f, g3 = 0.f;
-        for (int w = 0; w < a.tiles; ++w) {
-            const float *pw = a.part + (size_t)w * kTrainPart;
-            g1 += pw[c];
-            g2 += pw[kHidden + c];
-            g3 += pw[2 * kHidden + c];
+        for (int w0 = 0; w0 < a.tiles; w0 += 8) {
+            float v1[8], v2[8], v3[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float *pw = a.part + (size_t)min(w0 + u, a.tiles - 1) * kTrainPart;
+                v1[u] = pw[c];
+                v2[u] = pw[kHidden + c];
+                v3[u] = pw[2 * kHidden + c];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (w0 + u < a.tiles) {
+                    g1 += v1[u];
+                    g2 += v2[u];
+                    g3 += v3[u];
+                }
         }
         const size_t o1 = a.L.ob1 + c, o2 = a.L.ob2 + c, o3 = a.L.ow3 + c;
         if (a.G) {
@@ -998,9 +999,20 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateArgs a) {
         }
         if (tid == 0) {
             float gb = 0.f, loss = 0.f;
-            for (int w = 0; w < a.tiles; ++w) {
-                gb += a.part[(size_t)w * kTrainPart + 3 * kHidden];
-                loss += a.part[(size_t)w * kTrainPart + 3 * kHidden + 1];
+            for (int w0 = 0; w0 < a.tiles; w0 += 8) {
+                float vb[8], vl[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float *pw = a.part + (size_t)min(w0 + u, a.tiles - 1) * kTrainPart;
+                    vb[u] = pw[3 * kHidden];
+                    vl[u] = pw[3 * kHidden + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (w0 + u < a.tiles) {
+                        gb += vb[u];
+                        loss += vl[u];
+                    }
             }
             if (a.G) a.G[a.L.ob3] = gb;
             if (a.adam) (void)adam_update(a.P, a.m, a.v, a.L.ob3, gb, a.lr, a.c1, a.c2);
@@ -1018,21 +1030,30 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateArgs a) {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // wave w takes steps w, w + 4, ... of 32 rows; eight steps' fragments are
+    // loaded before the first is used (steps past K read as zero)
     const int steps = a.K / 32;
-#pragma unroll 4
-    for (int st = wave; st < steps; st += 4) {
-        const int kb = st * 32 + lk * 8;
-        bf16x8 av[2], bv[2];
+    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
+    for (int base = wave; base < steps; base += 32) {
+        bf16x8 av[8][2], bv[8][2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            av[i] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + i * 16 + lr) * a.Bp + kb);
-            bv[i] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + i * 16 + lr) * a.Bp + kb);
+        for (int u = 0; u < 8; ++u) {
+            const int st = base + 4 * u, kb = st * 32 + lk * 8;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                av[u][i] = st < steps ? *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + i * 16 + lr) * a.Bp + kb)
+                                      : zero8;
+                bv[u][i] = st < steps ? *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + i * 16 + lr) * a.Bp + kb)
+                                      : zero8;
+            }
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int u = 0; u < 8; ++u)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[u][i], bv[u][j], acc[i][j], 0, 0, 0);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -1062,65 +1083,107 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateArgs a) {
     }
 }
 
-// One fused step: rows (X, ld, t) or archive slots (AX, Ay, ...), gradients
-// into G (grad) and/or Adam (lr > 0).
-static int fused_step(Mlp &N, const TrainRowsArgs &r0, bool grad, float lr, bool want_loss,
-                      hipStream_t s) {
-    TrainRowsArgs r = r0;
-    r.D = N.L.D;
-    r.stats = N.stats;
-    r.W1b = N.W1b;
-    r.W2b = N.W2b;
-    r.W2Tb = N.W2Tb;
-    r.b1 = N.P + N.L.ob1;
-    r.b2 = N.P + N.L.ob2;
-    r.w3 = N.P + N.L.ow3;
-    r.b3 = N.P + N.L.ob3;
-    r.XT = N.XT;
-    r.H1T = N.H1T;
-    r.dH2T = N.dH2T;
-    r.dH1T = N.dH1T;
-    r.Bp = N.Bp;
-    r.part = N.tpart;
-    const int tiles = (r.n + kTrainRows - 1) / kTrainRows;
-    const dim3 grid(tiles), block(256);
-    switch (N.L.Kp) {
-    case 32: hipLaunchKernelGGL(k_train_rows<32>, grid, block, 0, s, r); break;
-    case 64: hipLaunchKernelGGL(k_train_rows<64>, grid, block, 0, s, r); break;
-    case 96: hipLaunchKernelGGL(k_train_rows<96>, grid, block, 0, s, r); break;
-    case 128: hipLaunchKernelGGL(k_train_rows<128>, grid, block, 0, s, r); break;
+// One fused step of nn networks with equal Kp and n: rows (X, ld, t) or
+// archive slots (AX, Ay, ...) per network, gradients into G (grad) and/or
+// Adam (lr > 0); two launches for all of them.
+static int fused_steps(Mlp *const *nets, const TrainRowsArgs *r0, int nn, bool grad, float lr,
+                       bool want_loss, hipStream_t s) {
+    if (nn < 1 || nn > kMaxTrainNets) return set_error(SPEO_ERR_ARG, "mlp: 1..8 networks per step");
+    const int Kp = nets[0]->L.Kp, n = r0[0].n;
+    const int tiles = (n + kTrainRows - 1) / kTrainRows;
+    const int blocks = (kHidden / 32) * (kHidden / 32) + (kHidden / 32) * (Kp / 32) + 1;
+    TrainRowsBatch rb{};
+    TrainUpdateBatch ub{};
+    rb.tiles = tiles;
+    ub.blocks = blocks;
+    for (int k = 0; k < nn; ++k) {
+        Mlp &N = *nets[k];
+        if (N.L.Kp != Kp || r0[k].n != n) return set_error(SPEO_ERR_ARG, "mlp: networks of one step differ");
+        TrainRowsArgs &r = rb.net[k];
+        r = r0[k];
+        r.D = N.L.D;
+        r.stats = N.stats;
+        r.W1b = N.W1b;
+        r.W2b = N.W2b;
+        r.W2Tb = N.W2Tb;
+        r.b1 = N.P + N.L.ob1;
+        r.b2 = N.P + N.L.ob2;
+        r.w3 = N.P + N.L.ow3;
+        r.b3 = N.P + N.L.ob3;
+        r.XT = N.XT;
+        r.H1T = N.H1T;
+        r.dH2T = N.dH2T;
+        r.dH1T = N.dH1T;
+        r.Bp = N.Bp;
+        r.part = N.tpart;
+        TrainUpdateArgs &u = ub.net[k];
+        u.XT = N.XT;
+        u.H1T = N.H1T;
+        u.dH2T = N.dH2T;
+        u.dH1T = N.dH1T;
+        u.Bp = N.Bp;
+        u.K = tiles * kTrainRows;
+        u.part = N.tpart;
+        u.tiles = tiles;
+        u.P = N.P;
+        u.G = grad ? N.G : nullptr;
+        u.m = N.m;
+        u.v = N.v;
+        u.L = N.L;
+        u.W1b = N.W1b;
+        u.W2b = N.W2b;
+        u.W2Tb = N.W2Tb;
+        u.adam = lr > 0.f ? 1 : 0;
+        if (u.adam) {
+            ++N.adam_t;
+            u.lr = lr;
+            u.c1 = 1.0f - powf(0.9f, (float)N.adam_t);
+            u.c2 = 1.0f - powf(0.999f, (float)N.adam_t);
+        }
+        u.stats = N.stats;
+        u.want_loss = want_loss ? 1 : 0;
+    }
+    const dim3 grid(tiles * nn), block(1024);
+    switch (Kp) {
+    case 32: hipLaunchKernelGGL(k_train_rows<32>, grid, block, 0, s, rb); break;
+    case 64: hipLaunchKernelGGL(k_train_rows<64>, grid, block, 0, s, rb); break;
+    case 96: hipLaunchKernelGGL(k_train_rows<96>, grid, block, 0, s, rb); break;
+    case 128: hipLaunchKernelGGL(k_train_rows<128>, grid, block, 0, s, rb); break;
     default: return set_error(SPEO_ERR_ARG, "mlp: fused step needs Kp <= 128");
     }
     SPEO_TRY(check_launch("k_train_rows"));
-    TrainUpdateArgs u{};
-    u.XT = N.XT;
-    u.H1T = N.H1T;
-    u.dH2T = N.dH2T;
-    u.dH1T = N.dH1T;
-    u.Bp = N.Bp;
-    u.K = tiles * kTrainRows;
-    u.part = N.tpart;
-    u.tiles = tiles;
-    u.P = N.P;
-    u.G = grad ? N.G : nullptr;
-    u.m = N.m;
-    u.v = N.v;
-    u.L = N.L;
-    u.W1b = N.W1b;
-    u.W2b = N.W2b;
-    u.W2Tb = N.W2Tb;
-    u.adam = lr > 0.f ? 1 : 0;
-    if (u.adam) {
-        ++N.adam_t;
-        u.lr = lr;
-        u.c1 = 1.0f - powf(0.9f, (float)N.adam_t);
-        u.c2 = 1.0f - powf(0.999f, (float)N.adam_t);
-    }
-    u.stats = N.stats;
-    u.want_loss = want_loss ? 1 : 0;
-    const int blocks = (kHidden / 32) * (kHidden / 32) + (kHidden / 32) * (N.L.Kp / 32) + 1;
-    hipLaunchKernelGGL(k_train_update, dim3(blocks), dim3(256), 0, s, u);
+    hipLaunchKernelGGL(k_train_update, dim3(blocks * nn), dim3(256), 0, s, ub);
     return check_launch("k_train_update");
+}
+
+static int fused_step(Mlp &N, const TrainRowsArgs &r0, bool grad, float lr, bool want_loss,
+                      hipStream_t s) {
+    Mlp *net = &N;
+    return fused_steps(&net, &r0, 1, grad, lr, want_loss, s);
+}
+
+int train_steps(Mlp *const *nets, const float *const *AX, const float *const *Ay,
+                const int *n_valid, const uint64_t *seed, int nn, uint32_t round, uint32_t step,
+                int batch, float lr, bool want_loss, hipStream_t s) {
+    for (int k0 = 0; k0 < nn; k0 += kMaxTrainNets) {
+        const int m = std::min(kMaxTrainNets, nn - k0);
+        TrainRowsArgs r[kMaxTrainNets] = {};
+        for (int k = 0; k < m; ++k) {
+            Mlp &N = *nets[k0 + k];
+            if (batch > N.max_batch || N.L.Kp > kFusedMaxKp)
+                return set_error(SPEO_ERR_ARG, "mlp: batch or width beyond the fused step");
+            if (N.bf16_stale) SPEO_TRY(N.refresh_bf16(s));
+            r[k].AX = AX[k0 + k];
+            r[k].Ay = Ay[k0 + k];
+            r[k].n_valid = n_valid[k0 + k];
+            r[k].seed = seed[k0 + k];
+            r[k].round = round;
+            r[k].step = step;
+            r[k].n = batch;
+        }
+        SPEO_TRY(fused_steps(nets + k0, r, m, false, lr, want_loss, s));
+    }
+    return SPEO_OK;
 }
 
 int Mlp::loss_grad(const float *X, int ld, int n, const float *t, hipStream_t s, bool want_loss) {

@@ -2,13 +2,13 @@
 FP64 oracle (CEC2014.cc forms, pinned to the reference's compiled CEC2014.cc).
 
 Tolerances: FP32 (and the FP32-recovering splits BF16x6 / FP16x3): 1e-5
-relative on random points of [-100, 100]^D.  For the weighted-sphere
-objectives (elliptic, bent cigar, discus) a row whose 1e6-weighted
-coordinates cancel in the rotation is ill-conditioned for any FP32 GEMM and
-is held to 4 u kappa instead (tolerance.py: kappa is the row's condition
-number, u the precision's unit roundoff).  BF16x3 (two-part BF16 split):
-1e-4.  Single BF16 / FP16: reported, bounded loosely (2e-2 / 5e-3).
-FP64 device path: 1e-12 relative.
+relative on every row of random points of [-100, 100]^D (the kernels sum the
+weighted-sphere column that outweighs all others in FP64).  In the 16-bit
+precisions a weighted-sphere row whose 1e6-weighted coordinates cancel in the
+rotation is ill-conditioned and is held to 4 u kappa (tolerance.py: kappa is
+the row's condition number, u the precision's unit roundoff).  BF16x3
+(two-part BF16 split): 1e-4.  Single BF16 / FP16: reported, bounded loosely
+(2e-2 / 5e-3).  FP64 device path: 1e-12 relative.
 """
 import numpy as np
 import pytest
@@ -198,6 +198,31 @@ def test_precision_modes(cuda, speo, oracle, D):
     print("\nmax relative error vs FP64, D=%d:" % D)
     for k, v in sorted(report.items()):
         print("  obj=%d %-7s %.3e" % (k[0], k[1], v))
+
+
+@pytest.mark.parametrize("D", [10, 100, 300])
+def test_unrotated_components_stay_fp32_in_16bit_modes(cuda, speo, oracle, D):
+    # z = (x - o) rate needs no rotation: every precision gives the FP32
+    # result bit for bit (a 16-bit identity GEMM would round z to BF16 /
+    # FP16); the engine refuses a 16-bit precision for such a problem
+    import torch
+    rng = np.random.default_rng(D + 5)
+    n = 96
+    X = _points(rng, n, D, D)
+    o = oracle.gen_shift(31, D)
+    for obj, rate in ((speo.OBJ_SCHWEFEL, 10.0), (speo.OBJ_RASTRIGIN, 0.0512)):
+        prob = speo.Problem([obj], o[None, :], None, rate=[rate])
+        ref = torch.zeros(n, dtype=torch.float32, device=cuda)
+        speo.evaluate(prob, _t(X, cuda), ref, precision=0)
+        for prec in range(1, 6):
+            f = torch.zeros(n, dtype=torch.float32, device=cuda)
+            speo.evaluate(prob, _t(X, cuda), f, precision=prec)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(f.cpu().numpy(), ref.cpu().numpy())
+        with pytest.raises(speo.SpeoError):
+            speo.Engine([prob], [16, 16], [0, 0], seed=1, precision=speo.PREC_BF16X3,
+                        mode=speo.MODE_PLAIN)
+        prob.close()
 
 
 @pytest.mark.parametrize("D", [10, 100, 1000])
