@@ -502,7 +502,10 @@ SPEO_EXPORT int speo_island_regroup(uint64_t seed, int island, int event, int re
  * log(1 + f) predictions; unused before training), archive_X
  * [surrogate_archive][round_up(dim, 32)] and archive_y [surrogate_archive]
  * the ring archive of true evaluations (y = log(1 + f)), *head_out its next
- * slot and *appended_out the rows ever appended.  Synchronous. */
+ * slot and *appended_out the rows ever appended.  Candidates are written to
+ * memory only from the first call on (until then each generation scores its
+ * candidates without storing them, with the same results), so cand_out holds
+ * those of the last generation run after an earlier call.  Synchronous. */
 SPEO_EXPORT int speo_engine_surrogate_dump(speo_engine e, int problem, int *cands_out,
                                            float *cand_out, float *pred_out, float *archive_X,
                                            float *archive_y, int *head_out,

@@ -493,8 +493,10 @@ struct Engine {
     SurrRowDesc *srows_dev = nullptr;
     float *cand = nullptr, *pred = nullptr, *rho_dev = nullptr, *AX = nullptr, *Ay = nullptr;
     float *Xb = nullptr, *tb = nullptr, *Xh = nullptr, *yh = nullptr, *predh = nullptr;
+    float2 *ranks_ws = nullptr;  // Spearman workspace
     bool any_trained = false;
     int last_Ce = 1;  // candidates per target in the last surrogate generation
+    bool retain_cand = false;  // candidates also written to memory (after a dump request)
     int64_t surr_pred = 0, surr_rounds = 0;
     int surrogate_setup();
     int surrogate_generation(int gen_offset, hipStream_t s);
@@ -561,7 +563,7 @@ Engine::~Engine() {
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
                     blocks_dev, Xnext, donor_buf, row_island_dev,
                     X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
-                    AX, Ay, Xb, tb, Xh, yh, predh, perm_dev, best_sub_dev, info_dev};
+                    AX, Ay, Xb, tb, Xh, yh, predh, ranks_ws, perm_dev, best_sub_dev, info_dev};
     for (void *b : bufs) (void)hipFree(b);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -969,6 +971,7 @@ int Engine::surrogate_setup() {
     SPEO_TRY_HIP(hipMalloc(&Xh, sizeof(float) * 1024 * (size_t)Kp));
     SPEO_TRY_HIP(hipMalloc(&yh, sizeof(float) * 1024));
     SPEO_TRY_HIP(hipMalloc(&predh, sizeof(float) * 1024));
+    SPEO_TRY_HIP(hipMalloc(&ranks_ws, sizeof(float2) * 4096));
     return SPEO_OK;
 }
 
@@ -978,17 +981,39 @@ int Engine::surrogate_setup() {
 int Engine::surrogate_generation(int gen_offset, hipStream_t s) {
     const int Ce = any_trained ? C : 1;
     last_Ce = Ce;
-    SPEO_TRY(gen_candidates(isl_dev, srows_dev, Q, Ce, X, cfg.min_bound, cfg.max_bound, cfg.seed,
-                            gen_base, (uint32_t)gen_offset, cand, Kp, s));
-    if (any_trained) {
-        for (auto &sp : surr)
-            if (sp->rows > 0)
-                SPEO_TRY(sp->net.infer(cand + (size_t)sp->qoff * Ce * Kp, Kp, sp->rows * Ce,
-                                       pred + (size_t)sp->qoff * Ce, s));
-        surr_pred += (int64_t)Q * Ce;
+    const float rho_min = any_trained ? cfg.surrogate_rho_min : INFINITY;
+    const float lo = cfg.min_bound, hi = cfg.max_bound;
+    if (retain_cand || Kp > kFusedMaxKp) {
+        // candidates written to memory, scored, the pick copied
+        SPEO_TRY(gen_candidates(isl_dev, srows_dev, Q, Ce, X, lo, hi, cfg.seed, gen_base,
+                                (uint32_t)gen_offset, cand, Kp, s));
+        if (any_trained) {
+            for (auto &sp : surr)
+                if (sp->rows > 0)
+                    SPEO_TRY(sp->net.infer(cand + (size_t)sp->qoff * Ce * Kp, Kp, sp->rows * Ce,
+                                           pred + (size_t)sp->qoff * Ce, s));
+            surr_pred += (int64_t)Q * Ce;
+        }
+        SPEO_TRY(pick_candidates(isl_dev, srows_dev, Q, Ce, cand, Kp, pred, rho_dev, rho_min, U[0], s));
+    } else {
+        // candidates generated and scored in one kernel per problem, the pick
+        // generated again into U: the same trials without the candidate buffer
+        if (any_trained) {
+            std::vector<Mlp *> nets;
+            std::vector<int> qo, nq;
+            for (auto &sp : surr)
+                if (sp->rows > 0) {
+                    nets.push_back(&sp->net);
+                    qo.push_back(sp->qoff);
+                    nq.push_back(sp->rows);
+                }
+            SPEO_TRY(cand_infer(nets.data(), qo.data(), nq.data(), (int)nets.size(), isl_dev, srows_dev,
+                                Ce, X, lo, hi, cfg.seed, gen_base, (uint32_t)gen_offset, pred, s));
+            surr_pred += (int64_t)Q * Ce;
+        }
+        SPEO_TRY(pick_regen(isl_dev, srows_dev, Q, Ce, pred, rho_dev, rho_min, X, lo, hi, cfg.seed,
+                            gen_base, (uint32_t)gen_offset, U[0], s));
     }
-    SPEO_TRY(pick_candidates(isl_dev, srows_dev, Q, Ce, cand, Kp, pred, rho_dev,
-                             any_trained ? cfg.surrogate_rho_min : INFINITY, U[0], s));
     SPEO_TRY(launch(kModeEval, gen_offset, rt, s));
     SurrHeads h{};
     for (size_t p = 0; p < surr.size(); ++p) {
@@ -1049,7 +1074,7 @@ int Engine::surrogate_train(hipStream_t s) {
         const int H = std::min(1024, n_valid_of(p) / 8);
         SPEO_TRY(gather_holdout(AX + p * cap * Kp, Ay + p * cap, H, Kp, Xh, yh, s));
         SPEO_TRY(sp.net.infer(Xh, Kp, H, predh, s));
-        SPEO_TRY(spearman(predh, yh, H, rho_dev + p, s));
+        SPEO_TRY(spearman(predh, yh, H, rho_dev + p, ranks_ws, s));
         sp.trained = true;
         any_trained = true;
     }
@@ -1900,6 +1925,7 @@ int speo_engine_surrogate_dump(speo_engine e, int problem, int *cands_out, float
                                hipMemcpyDeviceToHost));
     if (head_out) *head_out = sp.head;
     if (appended_out) *appended_out = sp.appended;
+    E.retain_cand = true;  // later generations keep their candidates for the next dump
     return SPEO_OK;
 }
 

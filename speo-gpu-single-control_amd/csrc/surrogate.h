@@ -34,6 +34,9 @@ struct MlpLayout {
     }
 };
 
+struct IslandDesc;
+struct SurrRowDesc;
+
 // Device buffers of one MLP with its training workspace.
 struct Mlp {
     MlpLayout L;
@@ -43,14 +46,15 @@ struct Mlp {
     float *dH1 = nullptr, *dH2 = nullptr;                              // [max_batch][256]
     float *y = nullptr, *dy = nullptr;                                 // [max_batch]
     float *stats = nullptr;  // [0] loss, [1] target mean, [2] target std, [3] rho
-    uint16_t *W1b = nullptr, *W2b = nullptr;  // BF16 copies for the fused inference kernel
+    uint16_t *W1b = nullptr, *W2b = nullptr;  // BF16 copies (MFMA fragment order) for inference
     uint16_t *W2Tb = nullptr;                 // BF16 W2 transposed (backward of the fused step)
     bool bf16_stale = true;                   // the BF16 copies lag the FP32 weights
     float *part = nullptr;                    // split-K partial products of the weight gradients
     float *lterm = nullptr;                   // [max_batch] per-row loss terms
     // Fused training step (Kp <= 128): activations transposed, [rows][Bp]
-    // BF16 with Bp = round_up(max_batch, 32), and per-row-tile partial sums of
-    // the bias and output-layer gradients, [tiles][1024] floats.
+    // BF16 in MFMA fragment order with Bp = round_up(max_batch, 32), and
+    // per-row-tile partial sums of the bias and output-layer gradients,
+    // [tiles][1024] floats.
     uint16_t *XT = nullptr, *H1T = nullptr, *dH2T = nullptr, *dH1T = nullptr;
     float *tpart = nullptr;
     int Bp = 0;
@@ -101,6 +105,24 @@ int gen_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C,
 int pick_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C,
                     const float *cand, int Kp, const float *pred, const float *rho,
                     float rho_min, float *U, hipStream_t s);
+// Per-problem networks of one fused screening launch.
+struct CandInferBatch {
+    const uint16_t *W1b[kMaxSurrProblems], *W2b[kMaxSurrProblems];
+    const float *P[kMaxSurrProblems];
+    int qoff[kMaxSurrProblems], nq[kMaxSurrProblems], first[kMaxSurrProblems];
+    int nnets;
+    size_t ob1, ob2, ow3, ob3;
+};
+// gen_candidates for every problem's targets [qoff[k], qoff[k] + nq[k]) and
+// each problem's network on them, in one kernel that never stores the
+// candidates: pred[q C + c], Kp <= 128.
+int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, int nn, const IslandDesc *isl,
+               const SurrRowDesc *rows, int C, const float *X, float lo, float hi, uint64_t seed,
+               const uint32_t *gen_base, uint32_t gen_offset, float *pred, hipStream_t s);
+// pick_candidates with the chosen candidate generated again into U.
+int pick_regen(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C, const float *pred,
+               const float *rho, float rho_min, const float *X, float lo, float hi, uint64_t seed,
+               const uint32_t *gen_base, uint32_t gen_offset, float *U, hipStream_t s);
 int archive_append(const IslandDesc *isl, const SurrRowDesc *rows, int Q, const float *U,
                    const float *fU, float *AX, float *Ay, int cap, int Kp, SurrHeads heads,
                    hipStream_t s);
@@ -114,6 +136,7 @@ int mean_std(const float *y, int n, float *stats, hipStream_t s);
 int train_steps(Mlp *const *nets, const float *const *AX, const float *const *Ay,
                 const int *n_valid, const uint64_t *seed, int nn, uint32_t round, uint32_t step,
                 int batch, float lr, bool want_loss, hipStream_t s);
-int spearman(const float *a, const float *b, int n, float *rho_out, hipStream_t s);
+// Spearman's rho of n <= 4096 values; ranks: workspace of 4096 float2.
+int spearman(const float *a, const float *b, int n, float *rho_out, float2 *ranks, hipStream_t s);
 
 }  // namespace speo

@@ -527,58 +527,148 @@ int Mlp::adam(float lr, hipStream_t s) {
 
 // ------------------------------------------------------ fused inference
 
-__global__ void k_to_bf16(const float *src, uint16_t *dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = to_bits(src[i], false);
+// BF16 weight copies are stored in MFMA B-fragment order: for B(k, n) of a
+// layer (n = output neuron in 16-column tiles t, k in steps of 32), lane
+// l = n % 16 + 16 ((k % 32) / 8) of tile t, step ks holds k % 8 = 0..7 at
+// ((t KS + ks) 64 + l) 8, so one wave's fragment is 1 KB contiguous.
+__host__ __device__ __forceinline__ size_t frag_index(int n, int k, int KS) {
+    return ((size_t)((n >> 4) * KS + (k >> 5)) * 64 + (n & 15) + 16 * ((k & 31) >> 3)) * 8 + (k & 7);
 }
 
-// dst[j][i] = BF16(src[i][j]) for a 256 x 256 matrix (block j, thread i).
-__global__ void k_to_bf16_t(const float *src, uint16_t *dst) {
-    const int j = blockIdx.x, i = threadIdx.x;
-    dst[(size_t)j * kHidden + i] = to_bits(src[(size_t)i * kHidden + j], false);
+// dst = BF16 of a [256][K] row-major matrix as B(k, n) = src[n][k]
+// (transposed: B(k, n) = src[k][n], K = 256), in fragment order.
+__global__ void k_to_frag(const float *src, uint16_t *dst, int K, int transposed) {
+    const int n = blockIdx.x, KS = K / 32;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const float v = transposed ? src[(size_t)k * kHidden + n] : src[(size_t)n * K + k];
+        dst[frag_index(n, k, KS)] = to_bits(v, false);
+    }
 }
 
-// 64 rows per block, 4 waves; wave w owns hidden columns [64w, 64w + 64) of
-// both layers for all 64 rows (4 x 4 MFMA 16x16 tiles).  X (scaled, BF16) and
-// H1 (BF16) live in LDS; weights are read as BF16 from L2 (16 bytes per lane
-// per fragment); the output layer is reduced across lanes and waves in LDS.
-// Same rounding points as the unfused path: BF16 operands, FP32 accumulation,
-// FP32 SiLU and output layer.
+// The network on a tile of 64 rows whose scaled BF16 inputs are in Xs
+// ([64][KP + 8]), 16 waves: wave w owns hidden columns [16w, 16w + 16) of both
+// layers for all 64 rows (4 MFMA 16x16 tiles); its weight fragments (W1 and
+// W2, frag_index order, 16 bytes per lane) come from mlp_prefetch.  H1 (BF16)
+// in Hs; the output layer reduced across lanes and waves in
+// LDS.  pred[r] for rows r < nrows of the tile.  BF16 operands, FP32
+// accumulation, FP32 SiLU and output layer.
+template <int KP> struct MlpFrags {
+    bf16x8 w1[KP / 32], w2[kHidden / 32];
+    float bias1, bias2, wout, bias3;
+};
+
+// A wave's weight fragments and column parameters for mlp_tile64 (issued
+// early, so that their loads overlap whatever the block does meanwhile).
 template <int KP>
-__global__ __launch_bounds__(256) void k_mlp_infer(const float *X, int ld, int n, int D,
-                                                   const uint16_t *W1b, const float *b1,
-                                                   const uint16_t *W2b, const float *b2,
-                                                   const float *w3, const float *b3,
-                                                   float *pred) {
+__device__ __forceinline__ void mlp_prefetch(MlpFrags<KP> &f, const uint16_t *W1b, const float *b1,
+                                             const uint16_t *W2b, const float *b2, const float *w3,
+                                             const float *b3) {
+    constexpr int K1 = KP / 32, K2 = kHidden / 32;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = wave * 16 + (lane & 15);
+#pragma unroll
+    for (int ks = 0; ks < K1; ++ks)
+        f.w1[ks] = *reinterpret_cast<const bf16x8 *>(&W1b[((size_t)(wave * K1 + ks) * 64 + lane) * 8]);
+#pragma unroll
+    for (int ks = 0; ks < K2; ++ks)
+        f.w2[ks] = *reinterpret_cast<const bf16x8 *>(&W2b[((size_t)(wave * K2 + ks) * 64 + lane) * 8]);
+    f.bias1 = b1[col];
+    f.bias2 = b2[col];
+    f.wout = w3[col];
+    f.bias3 = b3[0];
+}
+
+template <int KP>
+__device__ __forceinline__ void mlp_tile64(const uint16_t *Xs, uint16_t *Hs, float (*red)[64],
+                                           const MlpFrags<KP> &f, int nrows, float *pred) {
+    constexpr int TM = 64, LX = KP + 8, LH = kHidden + 8, K1 = KP / 32, K2 = kHidden / 32;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int col = wave * 16 + lr;
+    const bf16x8 *w1 = f.w1, *w2 = f.w2;
+    const float bias1 = f.bias1, bias2 = f.bias2, wout = f.wout, bias3 = f.bias3;
+    floatx4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K1; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                *reinterpret_cast<const bf16x8 *>(&Xs[(i * 16 + lr) * LX + kb]), w1[ks], acc[i], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Hs[(i * 16 + lk * 4 + q) * LH + col] = to_bits(siluf_(acc[i][q] + bias1), false);
+            acc[i][q] = 0.f;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < K2; ++ks) {
+        const int kb = ks * 32 + lk * 8;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                *reinterpret_cast<const bf16x8 *>(&Hs[(i * 16 + lr) * LH + kb]), w2[ks], acc[i], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float v = sum16(siluf_(acc[i][q] + bias2) * wout);
+            if (lr == 0) red[wave][i * 16 + lk * 4 + q] = v;
+        }
+    __syncthreads();
+    if (tid < TM && tid < nrows) {
+        float y = 0.f;
+#pragma unroll
+        for (int w = 0; w < kHidden / 16; ++w) y += red[w][tid];
+        pred[tid] = y + bias3;
+    }
+}
+
+
+// 64 rows of X per block (1024 threads): the rows scaled and rounded to
+// BF16 in LDS, then mlp_tile64.
+template <int KP>
+__global__ __launch_bounds__(1024) void k_mlp_infer(const float *X, int ld, int n, int D,
+                                                    const uint16_t *W1b, const float *b1,
+                                                    const uint16_t *W2b, const float *b2,
+                                                    const float *w3, const float *b3,
+                                                    float *pred) {
     constexpr int TM = 64, LX = KP + 8, LH = kHidden + 8;
     __shared__ __align__(16) uint16_t Xs[TM * LX];
     __shared__ __align__(16) uint16_t Hs[TM * LH];
-    __shared__ float red[4][TM];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int lr = lane & 15, lk = lane >> 4;
+    __shared__ float red[kHidden / 16][TM];
+    const int tid = threadIdx.x;
     const int row0 = blockIdx.x * TM;
+    MlpFrags<KP> f;
+    mlp_prefetch<KP>(f, W1b, b1, W2b, b2, w3, b3);
+    typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
     if ((ld & 3) == 0 && ld >= KP && (reinterpret_cast<uintptr_t>(X) & 15) == 0) {
         // 16-byte loads, all of the tile issued before the first is converted
-        constexpr int G = TM * KP / 4 / 256;
+        constexpr int G = (TM * KP / 4 + 1023) / 1024;
         floatx4 x[G];
 #pragma unroll
         for (int i = 0; i < G; ++i) {
-            const int e = tid + i * 256, r = e / (KP / 4), k = (e % (KP / 4)) * 4;
-            x[i] = gld(reinterpret_cast<const floatx4 *>(X + (row0 + r < n ? (size_t)(row0 + r) * ld + k : 0)));
+            const int e = tid + i * 1024, r = e / (KP / 4), k = (e % (KP / 4)) * 4;
+            x[i] = e < TM * KP / 4 && row0 + r < n
+                       ? gld(reinterpret_cast<const floatx4 *>(X + (size_t)(row0 + r) * ld + k))
+                       : floatx4{0.f, 0.f, 0.f, 0.f};
         }
 #pragma unroll
         for (int i = 0; i < G; ++i) {
-            const int e = tid + i * 256, r = e / (KP / 4), k = (e % (KP / 4)) * 4;
-            typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+            const int e = tid + i * 1024, r = e / (KP / 4), k = (e % (KP / 4)) * 4;
+            if (e >= TM * KP / 4) continue;
             u16x4 h;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                h[q] = to_bits(row0 + r < n && k + q < D ? x[i][q] * kXScale : 0.f, false);
+            for (int q = 0; q < 4; ++q) h[q] = to_bits(k + q < D ? x[i][q] * kXScale : 0.f, false);
             *reinterpret_cast<u16x4 *>(&Xs[r * LX + k]) = h;
         }
     } else {
-        for (int e = tid; e < TM * KP; e += 256) {
+        for (int e = tid; e < TM * KP; e += 1024) {
             const int r = e / KP, k = e % KP;
             float v = 0.f;
             if (row0 + r < n && k < D) v = X[(size_t)(row0 + r) * ld + k] * kXScale;
@@ -586,78 +676,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *X, int ld, int n
         }
     }
     __syncthreads();
-    const int c0 = wave * 64;
-    floatx4 acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < KP / 32; ++ks) {
-        const int kb = ks * 32 + lk * 8;
-        bf16x8 a[4], b[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&Xs[(i * 16 + lr) * LX + kb]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            b[j] = *reinterpret_cast<const bf16x8 *>(&W1b[(size_t)(c0 + j * 16 + lr) * KP + kb]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = c0 + j * 16 + lr;
-        const float bias = b1[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = i * 16 + lk * 4 + q;
-                Hs[r * LH + col] = to_bits(siluf_(acc[i][j][q] + bias), false);
-                acc[i][j][q] = 0.f;
-            }
-    }
-    __syncthreads();
-    for (int ks = 0; ks < kHidden / 32; ++ks) {
-        const int kb = ks * 32 + lk * 8;
-        bf16x8 a[4], b[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&Hs[(i * 16 + lr) * LH + kb]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            b[j] = *reinterpret_cast<const bf16x8 *>(&W2b[(size_t)(c0 + j * 16 + lr) * kHidden + kb]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-    float part[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[i][q] = 0.f;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = c0 + j * 16 + lr;
-        const float bias = b2[col], w = w3[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) part[i][q] = fmaf(siluf_(acc[i][j][q] + bias), w, part[i][q]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float v = sum16(part[i][q]);
-            if (lr == 0) red[wave][i * 16 + lk * 4 + q] = v;
-        }
-    __syncthreads();
-    if (tid < TM && row0 + tid < n)
-        pred[row0 + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid] + b3[0];
+    mlp_tile64<KP>(Xs, Hs, red, f, n - row0, pred + row0);
 }
 
 int Mlp::refresh_bf16(hipStream_t s) {
@@ -666,17 +685,17 @@ int Mlp::refresh_bf16(hipStream_t s) {
         SPEO_TRY_HIP(hipMalloc(&W2b, sizeof(uint16_t) * (size_t)kHidden * kHidden));
         SPEO_TRY_HIP(hipMalloc(&W2Tb, sizeof(uint16_t) * (size_t)kHidden * kHidden));
     }
-    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, s, P + L.oW1, W1b, (size_t)kHidden * L.Kp);
-    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, s, P + L.oW2, W2b, (size_t)kHidden * kHidden);
-    hipLaunchKernelGGL(k_to_bf16_t, dim3(kHidden), dim3(kHidden), 0, s, P + L.oW2, W2Tb);
+    hipLaunchKernelGGL(k_to_frag, dim3(kHidden), dim3(128), 0, s, P + L.oW1, W1b, L.Kp, 0);
+    hipLaunchKernelGGL(k_to_frag, dim3(kHidden), dim3(128), 0, s, P + L.oW2, W2b, kHidden, 0);
+    hipLaunchKernelGGL(k_to_frag, dim3(kHidden), dim3(128), 0, s, P + L.oW2, W2Tb, kHidden, 1);
     bf16_stale = false;
-    return check_launch("k_to_bf16");
+    return check_launch("k_to_frag");
 }
 
 int Mlp::infer(const float *X, int ld, int n, float *pred, hipStream_t s) {
     if (n <= 0) return SPEO_OK;
     if (bf16_stale || !W1b) SPEO_TRY(refresh_bf16(s));
-    const dim3 grid((n + 63) / 64), block(256);
+    const dim3 grid((n + 63) / 64), block(1024);
     const float *b1 = P + L.ob1, *b2 = P + L.ob2, *w3 = P + L.ow3, *b3 = P + L.ob3;
     switch (L.Kp) {
     case 32: hipLaunchKernelGGL(k_mlp_infer<32>, grid, block, 0, s, X, ld, n, L.D, W1b, b1, W2b, b2, w3, b3, pred); break;
@@ -711,7 +730,7 @@ struct TrainRowsArgs {
     const float *stats;    // [1] mean, [2] std of the labels
     const uint16_t *W1b, *W2b, *W2Tb;
     const float *b1, *b2, *w3, *b3;
-    uint16_t *XT, *H1T, *dH2T, *dH1T;  // [rows][Bp]: column b = minibatch row b
+    uint16_t *XT, *H1T, *dH2T, *dH1T;  // [rows][Bp] in fragment order: k = minibatch row
     int Bp;
     float *part;           // [tile][kTrainPart]: db1 | db2 | dw3 | (db3, loss)
 };
@@ -751,11 +770,11 @@ __global__ __launch_bounds__(1024) void k_train_rows(TrainRowsBatch batch) {
     bf16x8 w1[K1], w2[K2], w2t[K2];
 #pragma unroll
     for (int ks = 0; ks < K1; ++ks)
-        w1[ks] = *reinterpret_cast<const bf16x8 *>(&a.W1b[(size_t)col * KP + ks * 32 + lk * 8]);
+        w1[ks] = *reinterpret_cast<const bf16x8 *>(&a.W1b[((size_t)(wave * K1 + ks) * 64 + lane) * 8]);
 #pragma unroll
     for (int ks = 0; ks < K2; ++ks) {
-        w2[ks] = *reinterpret_cast<const bf16x8 *>(&a.W2b[(size_t)col * kHidden + ks * 32 + lk * 8]);
-        w2t[ks] = *reinterpret_cast<const bf16x8 *>(&a.W2Tb[(size_t)col * kHidden + ks * 32 + lk * 8]);
+        w2[ks] = *reinterpret_cast<const bf16x8 *>(&a.W2b[((size_t)(wave * K2 + ks) * 64 + lane) * 8]);
+        w2t[ks] = *reinterpret_cast<const bf16x8 *>(&a.W2Tb[((size_t)(wave * K2 + ks) * 64 + lane) * 8]);
     }
     const float bias1 = a.b1[col], bias2 = a.b2[col], wout = a.w3[col], bias3 = a.b3[0];
     if (tid < TM) {
@@ -795,14 +814,16 @@ __global__ __launch_bounds__(1024) void k_train_rows(TrainRowsBatch batch) {
         }
     }
     __syncthreads();
-    // transposed copy of a [TM][stride] BF16 tile: T[c][row0 + r]
+    // transposed copy of a [TM][stride] BF16 tile: element (c, row0 + r) of
+    // T, a [ncols][Bp] matrix as B-fragment operand (k = minibatch row), in
+    // fragment order (frag_index), 8 consecutive rows per 16-byte store
     auto store_t = [&](const uint16_t *S, int stride, int ncols, uint16_t *T) __attribute__((always_inline)) {
         for (int e = tid; e < ncols * (TM / 8); e += 1024) {
             const int c = e / (TM / 8), r8 = (e % (TM / 8)) * 8;
             u16x8 v;
 #pragma unroll
             for (int u = 0; u < 8; ++u) v[u] = S[(r8 + u) * stride + c];
-            *reinterpret_cast<u16x8 *>(T + (size_t)c * a.Bp + row0 + r8) = v;
+            *reinterpret_cast<u16x8 *>(T + frag_index(c, row0 + r8, a.Bp / 32)) = v;
         }
     };
     floatx4 acc[2];
@@ -1032,7 +1053,7 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
     // wave w takes steps w, w + 4, ... of 32 rows; eight steps' fragments are
     // loaded before the first is used (steps past K read as zero)
-    const int steps = a.K / 32;
+    const int steps = a.K / 32, kbs = a.Bp / 32;
     const bf16x8 zero8 = __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
     for (int base = wave; base < steps; base += 32) {
         bf16x8 av[8][2], bv[8][2];
@@ -1041,9 +1062,9 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
             const int st = base + 4 * u, kb = st * 32 + lk * 8;
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                av[u][i] = st < steps ? *reinterpret_cast<const bf16x8 *>(A + (size_t)(i0 + i * 16 + lr) * a.Bp + kb)
+                av[u][i] = st < steps ? *reinterpret_cast<const bf16x8 *>(A + frag_index(i0 + i * 16 + lr, kb, kbs))
                                       : zero8;
-                bv[u][i] = st < steps ? *reinterpret_cast<const bf16x8 *>(B + (size_t)(j0 + i * 16 + lr) * a.Bp + kb)
+                bv[u][i] = st < steps ? *reinterpret_cast<const bf16x8 *>(B + frag_index(j0 + i * 16 + lr, kb, kbs))
                                       : zero8;
             }
         }
@@ -1074,10 +1095,10 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
             const float p = adam_update(a.P, a.m, a.v, off, g, a.lr, a.c1, a.c2);
             const uint16_t h = to_bits(p, false);
             if (w2) {
-                a.W2b[(size_t)I * kHidden + J] = h;
-                a.W2Tb[(size_t)J * kHidden + I] = h;
+                a.W2b[frag_index(I, J, kHidden / 32)] = h;
+                a.W2Tb[frag_index(J, I, kHidden / 32)] = h;
             } else {
-                a.W1b[(size_t)I * a.L.Kp + J] = h;
+                a.W1b[frag_index(I, J, a.L.Kp / 32)] = h;
             }
         }
     }
@@ -1222,29 +1243,38 @@ int Mlp::train_step(const float *AX, const float *Ay, int n_valid, uint64_t seed
 // ------------------------------------------------------------- statistics
 
 // Spearman rank correlation of a[0..n) and b[0..n) (average ranks for ties),
-// one block of 1024 threads with both inputs in LDS; O(n^2) ranks, n <= 4096.
-__global__ __launch_bounds__(1024) void k_spearman(const float *a, const float *b, int n,
-                                                   float *rho_out) {
-    __shared__ double sh[3][1024];
+// n <= 4096.  k_spearman_ranks: 256 values per block, each thread counting
+// the values below and equal to its own over the whole array in LDS, writes
+// the centred ranks; k_spearman_corr: one block sums their products in a
+// fixed order.
+__global__ __launch_bounds__(256) void k_spearman_ranks(const float *a, const float *b, int n,
+                                                        float2 *ranks) {
     __shared__ float as[4096], bs[4096];
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int i = threadIdx.x; i < n; i += 256) {
         as[i] = a[i];
         bs[i] = b[i];
     }
     __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int la = 0, ea = 0, lb = 0, eb = 0;
+    const float ai = as[i], bi = bs[i];
+    for (int j = 0; j < n; ++j) {
+        const float aj = as[j], bj = bs[j];
+        la += aj < ai;
+        ea += aj == ai;
+        lb += bj < bi;
+        eb += bj == bi;
+    }
+    const float mean = 0.5f * (float)(n - 1);  // ranks are multiples of 1/2: exact in FP32
+    ranks[i] = float2{(float)la + 0.5f * (float)(ea - 1) - mean, (float)lb + 0.5f * (float)(eb - 1) - mean};
+}
+
+__global__ __launch_bounds__(1024) void k_spearman_corr(const float2 *ranks, int n, float *rho_out) {
+    __shared__ double sh[3][1024];
     double sab = 0.0, saa = 0.0, sbb = 0.0;
-    const double mean = 0.5 * (n - 1);
     for (int i = threadIdx.x; i < n; i += 1024) {
-        int la = 0, ea = 0, lb = 0, eb = 0;
-        const float ai = as[i], bi = bs[i];
-        for (int j = 0; j < n; ++j) {
-            const float aj = as[j], bj = bs[j];
-            la += aj < ai;
-            ea += aj == ai;
-            lb += bj < bi;
-            eb += bj == bi;
-        }
-        const double ra = la + 0.5 * (ea - 1) - mean, rb = lb + 0.5 * (eb - 1) - mean;
+        const double ra = ranks[i].x, rb = ranks[i].y;
         sab += ra * rb;
         saa += ra * ra;
         sbb += rb * rb;
@@ -1264,10 +1294,12 @@ __global__ __launch_bounds__(1024) void k_spearman(const float *a, const float *
     }
 }
 
-int spearman(const float *a, const float *b, int n, float *rho_out, hipStream_t s) {
+int spearman(const float *a, const float *b, int n, float *rho_out, float2 *ranks, hipStream_t s) {
     if (n < 2 || n > 4096) return set_error(SPEO_ERR_ARG, "spearman: need 2 <= n <= 4096");
-    hipLaunchKernelGGL(k_spearman, dim3(1), dim3(1024), 0, s, a, b, n, rho_out);
-    return check_launch("k_spearman");
+    hipLaunchKernelGGL(k_spearman_ranks, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n, ranks);
+    SPEO_TRY(check_launch("k_spearman_ranks"));
+    hipLaunchKernelGGL(k_spearman_corr, dim3(1), dim3(1024), 0, s, ranks, n, rho_out);
+    return check_launch("k_spearman_corr");
 }
 
 // ------------------------------------------------------ candidate screening
@@ -1277,15 +1309,67 @@ __device__ __forceinline__ float pick4(const float4 &v, int q) {
     return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
 }
 
-// 16 lanes per (target q, candidate c).  Candidate 0 uses the plain DE
-// streams (purposes 0 / 1), so it is exactly the plain engine's trial;
-// candidate c >= 1 uses purposes 64 + 2c / 65 + 2c.
+// Candidate c of a target row: its donor rows, j_rand and crossover stream.
+// Candidate 0 uses the plain DE streams (purposes 0 / 1), so it is exactly
+// the plain engine's trial; candidate c >= 1 uses purposes 64 + 2c / 65 + 2c.
+struct CandSrc {
+    const float *p0, *p1, *p2, *own;
+    int jr, dim;
+    float F, CR;
+    uint32_t pc, gr, gen;
+};
+
+__device__ __forceinline__ CandSrc cand_src(const IslandDesc &I, const SurrRowDesc &rd, int c,
+                                            const float *X, uint64_t seed, uint32_t gen) {
+    const uint32_t pd = c == 0 ? (uint32_t)kPurposeDonor : (uint32_t)(kPurposeCandidate + 2 * c);
+    DonorStream ds(seed, (uint32_t)rd.gr, gen, pd);
+    int r[kMaxR], jr;
+    draw_donors(ds, nullptr, 0, I.np, I.dim, r, jr);
+    const float *base = X + I.elem_off;
+    CandSrc cs;
+    cs.p0 = base + (size_t)r[0] * I.ld;
+    cs.p1 = base + (size_t)r[1] * I.ld;
+    cs.p2 = base + (size_t)r[2] * I.ld;
+    cs.own = base + (size_t)rd.li * I.ld;
+    cs.jr = jr;
+    cs.dim = I.dim;
+    cs.F = I.F;
+    cs.CR = I.CR;
+    cs.pc = c == 0 ? (uint32_t)kPurposeCross : (uint32_t)(kPurposeCandidate + 2 * c + 1);
+    cs.gr = (uint32_t)rd.gr;
+    cs.gen = gen;
+    return cs;
+}
+
+// Coordinates 4g .. 4g + 3 of the candidate (0 past the dimension).
+__device__ __forceinline__ float4 cand_group(const CandSrc &cs, int g, uint64_t seed, float lo,
+                                             float hi) {
+#pragma clang fp contract(off)
+    if (4 * g >= cs.dim) return float4{0.f, 0.f, 0.f, 0.f};
+    const U4 w = philox_block(seed, (uint32_t)g, cs.gr, cs.gen, cs.pc);
+    const float4 e0 = ld4s(cs.p0 + 4 * g), e1 = ld4s(cs.p1 + 4 * g), e2 = ld4s(cs.p2 + 4 * g);
+    const float4 xo = ld4s(cs.own + 4 * g);
+    float v[4];
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+        const int j = 4 * g + qq;
+        if (j < cs.dim) {
+            const bool take = (j == cs.jr) || (u01(word(w, qq)) < cs.CR);
+            const float m = take ? fmaf(cs.F, pick4(e1, qq) - pick4(e2, qq), pick4(e0, qq)) : pick4(xo, qq);
+            v[qq] = check_bound(m, lo, hi);
+        } else {
+            v[qq] = 0.f;
+        }
+    }
+    return float4{v[0], v[1], v[2], v[3]};
+}
+
+// 16 lanes per (target q, candidate c): the candidate's Kp columns.
 __global__ __launch_bounds__(256) void k_gen_candidates(const IslandDesc *isl,
                                                         const SurrRowDesc *rows, int Q, int C,
                                                         const float *X, float lo, float hi,
                                                         uint64_t seed, const uint32_t *gen_base,
                                                         uint32_t gen_offset, float *cand, int Kp) {
-#pragma clang fp contract(off)
     const int item = blockIdx.x * 16 + (threadIdx.x >> 4);
     const int l = threadIdx.x & 15;
     if (item >= Q * C) return;
@@ -1293,37 +1377,91 @@ __global__ __launch_bounds__(256) void k_gen_candidates(const IslandDesc *isl,
     const SurrRowDesc rd = rows[q];
     const IslandDesc I = isl[rd.island];
     const uint32_t gen = (gen_base ? *gen_base : 0u) + gen_offset;
-    const uint32_t pd = c == 0 ? (uint32_t)kPurposeDonor : (uint32_t)(kPurposeCandidate + 2 * c);
-    const uint32_t pc = c == 0 ? (uint32_t)kPurposeCross : (uint32_t)(kPurposeCandidate + 2 * c + 1);
-    DonorStream ds(seed, (uint32_t)rd.gr, gen, pd);
-    int r[kMaxR], jr;
-    draw_donors(ds, nullptr, 0, I.np, I.dim, r, jr);
-    const float *base = X + I.elem_off;
-    const float *p0 = base + (size_t)r[0] * I.ld, *p1 = base + (size_t)r[1] * I.ld;
-    const float *p2 = base + (size_t)r[2] * I.ld, *own = base + (size_t)rd.li * I.ld;
+    const CandSrc cs = cand_src(I, rd, c, X, seed, gen);
     float *out = cand + (size_t)item * Kp;
-    const int ngroups = (I.dim + 3) >> 2;
-    for (int g = l; g < ngroups; g += 16) {
-        const U4 w = philox_block(seed, (uint32_t)g, (uint32_t)rd.gr, gen, pc);
-        const float4 e0 = ld4s(p0 + 4 * g), e1 = ld4s(p1 + 4 * g), e2 = ld4s(p2 + 4 * g);
-        const float4 xo = ld4s(own + 4 * g);
-        float v[4];
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-            const int j = 4 * g + qq;
-            if (j < I.dim) {
-                const bool take = (j == jr) || (u01(word(w, qq)) < I.CR);
-                const float m = take ? fmaf(I.F, pick4(e1, qq) - pick4(e2, qq), pick4(e0, qq))
-                                     : pick4(xo, qq);
-                v[qq] = check_bound(m, lo, hi);
-            } else {
-                v[qq] = 0.f;
-            }
+    for (int g = l; g < (Kp >> 2); g += 16)
+        *reinterpret_cast<float4 *>(out + 4 * g) = cand_group(cs, g, seed, lo, hi);
+}
+
+// The candidates of each problem's targets generated straight into LDS (64
+// per block, 16 lanes per candidate) and scored by that problem's network
+// (mlp_tile64): the same predictions as k_gen_candidates followed by
+// k_mlp_infer, without writing the candidates to memory.  All problems in one
+// launch: problem k owns blocks [first[k], first[k + 1]).
+template <int KP>
+__global__ __launch_bounds__(1024) void k_cand_infer(CandInferBatch B, const IslandDesc *isl,
+                                                     const SurrRowDesc *rows, int C, const float *X,
+                                                     float lo, float hi, uint64_t seed,
+                                                     const uint32_t *gen_base, uint32_t gen_offset,
+                                                     float *pred) {
+    constexpr int TM = 64, LX = KP + 8, LH = kHidden + 8;
+    __shared__ __align__(16) uint16_t Xs[TM * LX];
+    __shared__ __align__(16) uint16_t Hs[TM * LH];
+    __shared__ float red[kHidden / 16][TM];
+    int k = 0;
+    while (k + 1 < B.nnets && (int)blockIdx.x >= B.first[k + 1]) ++k;
+    const int qoff = B.qoff[k], n = B.nq[k] * C;
+    const float *P = B.P[k];
+    MlpFrags<KP> f;
+    mlp_prefetch<KP>(f, B.W1b[k], P + B.ob1, B.W2b[k], P + B.ob2, P + B.ow3, P + B.ob3);
+    const int r = threadIdx.x >> 4, l = threadIdx.x & 15;  // candidate r of the block, lane l
+    const int item0 = ((int)blockIdx.x - B.first[k]) * TM, item = item0 + r;
+    const uint32_t gen = (gen_base ? *gen_base : 0u) + gen_offset;
+    typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+    if (item < n) {
+        const SurrRowDesc rd = rows[qoff + item / C];
+        const CandSrc cs = cand_src(isl[rd.island], rd, item % C, X, seed, gen);
+        for (int g = l; g < KP / 4; g += 16) {
+            const float4 v = cand_group(cs, g, seed, lo, hi);
+            *reinterpret_cast<u16x4 *>(&Xs[r * LX + 4 * g]) =
+                u16x4{to_bits(v.x * kXScale, false), to_bits(v.y * kXScale, false),
+                      to_bits(v.z * kXScale, false), to_bits(v.w * kXScale, false)};
         }
-        *reinterpret_cast<float4 *>(out + 4 * g) = float4{v[0], v[1], v[2], v[3]};
+    } else {
+        for (int g = l; g < KP / 4; g += 16) *reinterpret_cast<u16x4 *>(&Xs[r * LX + 4 * g]) = u16x4{0, 0, 0, 0};
     }
-    for (int g = ngroups + l; g < (Kp >> 2); g += 16)
-        *reinterpret_cast<float4 *>(out + 4 * g) = float4{0.f, 0.f, 0.f, 0.f};
+    __syncthreads();
+    mlp_tile64<KP>(Xs, Hs, red, f, n - item0, pred + (size_t)qoff * C + item0);
+}
+
+int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, int nn, const IslandDesc *isl,
+               const SurrRowDesc *rows, int C, const float *X, float lo, float hi, uint64_t seed,
+               const uint32_t *gen_base, uint32_t gen_offset, float *pred, hipStream_t s) {
+    if (nn < 1 || nn > kMaxSurrProblems) return set_error(SPEO_ERR_ARG, "cand_infer: 1..16 problems");
+    CandInferBatch B{};
+    B.nnets = nn;
+    int blocks = 0;
+    const MlpLayout &L = nets[0]->L;
+    if (L.Kp > kFusedMaxKp) return set_error(SPEO_ERR_ARG, "mlp: fused screening needs Kp <= 128");
+    for (int k = 0; k < nn; ++k) {
+        Mlp &N = *nets[k];
+        if (N.L.Kp != L.Kp) return set_error(SPEO_ERR_ARG, "cand_infer: networks differ in width");
+        if (N.bf16_stale || !N.W1b) SPEO_TRY(N.refresh_bf16(s));
+        B.W1b[k] = N.W1b;
+        B.W2b[k] = N.W2b;
+        B.P[k] = N.P;
+        B.qoff[k] = qoff[k];
+        B.nq[k] = nq[k];
+        B.first[k] = blocks;
+        blocks += (nq[k] * C + 63) / 64;
+    }
+    B.ob1 = L.ob1;
+    B.ob2 = L.ob2;
+    B.ow3 = L.ow3;
+    B.ob3 = L.ob3;
+    if (blocks == 0) return SPEO_OK;
+    const dim3 grid(blocks), block(1024);
+#define SPEO_CAND_INFER(KP)                                                                          \
+    hipLaunchKernelGGL(k_cand_infer<KP>, grid, block, 0, s, B, isl, rows, C, X, lo, hi, seed, gen_base, \
+                       gen_offset, pred)
+    switch (L.Kp) {
+    case 32: SPEO_CAND_INFER(32); break;
+    case 64: SPEO_CAND_INFER(64); break;
+    case 96: SPEO_CAND_INFER(96); break;
+    default: SPEO_CAND_INFER(128); break;
+    }
+#undef SPEO_CAND_INFER
+    return check_launch("k_cand_infer");
 }
 
 int gen_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C, const float *X,
@@ -1362,6 +1500,44 @@ __global__ void k_pick_candidates(const IslandDesc *isl, const SurrRowDesc *rows
     float *dst = U + I.elem_off + (size_t)rd.li * I.ld;
     for (int g = l; g < (I.ld >> 2); g += 16)
         *reinterpret_cast<float4 *>(dst + 4 * g) = *reinterpret_cast<const float4 *>(src + 4 * g);
+}
+
+// The same pick, the chosen candidate generated again into U instead of
+// copied (its coordinates are a function of the counters and X only).
+__global__ void k_pick_regen(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C,
+                             const float *pred, const float *rho, float rho_min, const float *X,
+                             float lo, float hi, uint64_t seed, const uint32_t *gen_base,
+                             uint32_t gen_offset, float *U) {
+    const int q = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int l = threadIdx.x & 15;
+    if (q >= Q) return;
+    const SurrRowDesc rd = rows[q];
+    const IslandDesc I = isl[rd.island];
+    int best = 0;
+    if (rho[rd.problem] >= rho_min) {
+        float bv = pred[(size_t)q * C];
+        for (int c = 1; c < C; ++c) {
+            const float v = pred[(size_t)q * C + c];
+            if (v < bv || (isnan(bv) && !isnan(v))) {
+                bv = v;
+                best = c;
+            }
+        }
+    }
+    const uint32_t gen = (gen_base ? *gen_base : 0u) + gen_offset;
+    const CandSrc cs = cand_src(I, rd, best, X, seed, gen);
+    float *dst = U + I.elem_off + (size_t)rd.li * I.ld;
+    for (int g = l; g < (I.ld >> 2); g += 16)
+        *reinterpret_cast<float4 *>(dst + 4 * g) = cand_group(cs, g, seed, lo, hi);
+}
+
+int pick_regen(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C, const float *pred,
+               const float *rho, float rho_min, const float *X, float lo, float hi, uint64_t seed,
+               const uint32_t *gen_base, uint32_t gen_offset, float *U, hipStream_t s) {
+    if (Q <= 0) return SPEO_OK;
+    hipLaunchKernelGGL(k_pick_regen, dim3((Q + 15) / 16), dim3(256), 0, s, isl, rows, Q, C, pred,
+                       rho, rho_min, X, lo, hi, seed, gen_base, gen_offset, U);
+    return check_launch("k_pick_regen");
 }
 
 int pick_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C,
@@ -1581,7 +1757,12 @@ int speo_mlp_train_step(speo_mlp h, const float *AX, const float *Ay, int n_vali
 
 int speo_spearman(const float *a, const float *b, int n, float *rho_out, speo_stream_t stream) {
     if (!a || !b || !rho_out) return set_error(SPEO_ERR_ARG, "speo_spearman: null argument");
-    return spearman(a, b, n, rho_out, as_stream(stream));
+    float2 *ranks = nullptr;
+    SPEO_TRY_HIP(hipMalloc(&ranks, sizeof(float2) * 4096));
+    int rc = spearman(a, b, n, rho_out, ranks, as_stream(stream));
+    if (rc == SPEO_OK) rc = check_hip(hipStreamSynchronize(as_stream(stream)), "speo_spearman");
+    (void)hipFree(ranks);
+    return rc;
 }
 
 }  // extern "C"

@@ -114,3 +114,24 @@ def test_shuffled_labels_trigger_the_fallback(cuda, speo, oracle):
     eng.close()
     for p in probs:
         p.close()
+
+
+def test_screening_without_stored_candidates_equals_stored(cuda, speo, oracle):
+    # generations score their candidates in one kernel per problem and
+    # generate the pick again, unless a dump asked for the candidates: both
+    # forms give the same trials, so two engines, one of them dumping from
+    # the start, stay equal bit for bit through training rounds
+    a, pa, _, _ = _setup(speo, oracle, **_surr_kw(rho_min=-1.0))
+    b, pb, _, _ = _setup(speo, oracle, **_surr_kw(rho_min=-1.0))
+    b.surrogate_dump(0, C, CAP)
+    a.run(25)
+    b.run(25)
+    Xa, fa = a.read()
+    Xb, fb = b.read()
+    np.testing.assert_array_equal(Xa, Xb)
+    np.testing.assert_array_equal(fa, fb)
+    assert a.stats()["surrogate_rounds"] == 2 and a.stats()["surrogate_predictions"] > 0
+    for e in (a, b):
+        e.close()
+    for p in pa + pb:
+        p.close()
