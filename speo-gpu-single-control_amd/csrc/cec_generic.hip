@@ -102,14 +102,16 @@ __device__ double seg_value(int obj, const Seg &s, int lane) {
         b = wave_sum(b);
         return kE - 20.0 * exp(-0.2 * sqrt(a / (double)n)) + 20.0 - exp(b / (double)n);
     case SPEO_OBJ_WEIERSTRASS: {  // :615-644, a = 0.5, b = 3, k_max = 20
+        // a^k and b^k as running products: 2^-k and 3^k (< 2^53) are exact in
+        // FP64, so these are the values pow(a, k), pow(b, k) return
         for (int i = lane; i < n; i += 64) {
             const double y = s.y(i);
-            double t = 0.0;
-            for (int k = 0; k <= 20; ++k) t += pow(0.5, k) * cos(2.0 * kPi * pow(3.0, k) * (y + 0.5));
+            double t = 0.0, ak = 1.0, bk = 1.0;
+            for (int k = 0; k <= 20; ++k, ak *= 0.5, bk *= 3.0) t += ak * cos(2.0 * kPi * bk * (y + 0.5));
             a += t;
         }
-        double c = 0.0;
-        for (int k = 0; k <= 20; ++k) c += pow(0.5, k) * cos(2.0 * kPi * pow(3.0, k) * 0.5);
+        double c = 0.0, ak = 1.0, bk = 1.0;
+        for (int k = 0; k <= 20; ++k, ak *= 0.5, bk *= 3.0) c += ak * cos(2.0 * kPi * bk * 0.5);
         return wave_sum(a) - n * c;
     }
     case SPEO_OBJ_GRIEWANK: {  // :646-666
