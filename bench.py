@@ -55,7 +55,7 @@ TTE_VARIANTS = {
     "fused_fp32": ("fused", "fp32", True, 0),
     "batched_fp32": ("epoch", "fp32", True, 0),
     "batched_bf16x3": ("epoch", "bf16x3", True, 0),
-    "surrogate_bf16x3": ("plain", "bf16x3", False, 8),
+    "surrogate_bf16x3": ("plain", "bf16x3", True, 8),
 }
 
 
@@ -316,12 +316,15 @@ def cec_suite(S, D=100, n=8192, reps=20):
     return out
 
 
-def surrogate_cost(S, gens=40, warm=20, seed=BASE_SEED + 400):
+def surrogate_cost(S, gens=40, warm=60, seed=BASE_SEED + 400):
     """Surrogate-assisted configuration: 32 islands x NP=256, D=100,
     Rastrigin / elliptic, 8 candidates per target, training every 10
     generations (40 Adam steps of batch 1024) on a 65,536-row archive.
     Wall time per generation (training rounds included) after `warm`
-    generations, next to the plain engine on the same islands."""
+    generations, next to the plain engine on the same islands; both capture
+    their launches in hipGraphs (the surrogate's rounds once its archives are
+    full, both of its round graphs, with and without a migration, during the
+    warm-up; the plain engine's before the timed region)."""
     D, I, NP = 100, 32, 256
     probs = []
     for k, obj in enumerate((S.OBJ_RASTRIGIN, S.OBJ_ELLIPTIC)):
@@ -335,14 +338,16 @@ def surrogate_cost(S, gens=40, warm=20, seed=BASE_SEED + 400):
                                            surrogate_batch=1024, surrogate_lr=1e-3,
                                            surrogate_rho_min=0.5))):
         eng = S.Engine(probs, [NP] * I, pidx, F=0.5, CR=0.9, seed=seed, interval=50,
-                       migration_size=2, mode=S.MODE_PLAIN, rows_per_block=0, **extra)
+                       migration_size=2, mode=S.MODE_PLAIN, use_graph=True, rows_per_block=0,
+                       **extra)
         eng.run(warm)
+        eng.prepare(gens)
         eng.sync()
         t0 = time.perf_counter()
         eng.run(gens)
         eng.sync()
         dt = time.perf_counter() - t0
-        out[name] = {"ms_per_generation": dt / gens * 1e3}
+        out[name] = {"ms_per_generation": dt / gens * 1e3, "graph_launches": eng.stats()["graph_launches"]}
         if extra:
             rho, act, loss = eng.surrogate_stats()
             out[name].update({"rho": [float(r) for r in rho], "active": [int(a) for a in act],

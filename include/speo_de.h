@@ -363,7 +363,10 @@ typedef struct {
      * evaluations ranks them and only the best-ranked one is truly evaluated.
      * Until the MLP is trained, and whenever its Spearman rho on held-out
      * archive points is below surrogate_rho_min, candidate 0 is used (plain
-     * DE).  All islands must then share one dimension. */
+     * DE).  All islands must then share one dimension, and mode must be
+     * SPEO_MODE_PLAIN.  With use_graph, once every network screens and every
+     * archive is full, each round (the generations up to a training point and
+     * the training) is captured once and replayed. */
     int surrogate_candidates;     /* e.g. 8                                       */
     int surrogate_train_interval; /* generations between training rounds (10)    */
     int surrogate_archive;        /* ring archive rows per problem (65536)        */

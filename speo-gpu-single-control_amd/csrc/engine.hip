@@ -494,12 +494,17 @@ struct Engine {
     float *cand = nullptr, *pred = nullptr, *rho_dev = nullptr, *AX = nullptr, *Ay = nullptr;
     float *Xb = nullptr, *tb = nullptr, *Xh = nullptr, *yh = nullptr, *predh = nullptr;
     float2 *ranks_ws = nullptr;  // Spearman workspace
+    int *heads_dev = nullptr;         // next archive slot per problem (device)
+    uint32_t *round_dev = nullptr;    // training rounds run (device; keys the minibatch draws)
     bool any_trained = false;
     int last_Ce = 1;  // candidates per target in the last surrogate generation
     bool retain_cand = false;  // candidates also written to memory (after a dump request)
     int64_t surr_pred = 0, surr_rounds = 0;
     int surrogate_setup();
     int surrogate_generation(int gen_offset, hipStream_t s);
+    int enqueue_surrogate_round(int TI, int mig_pos, hipStream_t s);
+    void surrogate_round_host(int TI);
+    bool surrogate_round_replayable() const;
     int surrogate_train(hipStream_t s);
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
     bool graph_failed = false;
@@ -563,7 +568,8 @@ Engine::~Engine() {
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
                     blocks_dev, Xnext, donor_buf, row_island_dev,
                     X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
-                    AX, Ay, Xb, tb, Xh, yh, predh, ranks_ws, perm_dev, best_sub_dev, info_dev};
+                    AX, Ay, Xb, tb, Xh, yh, predh, ranks_ws, heads_dev, round_dev, perm_dev,
+                    best_sub_dev, info_dev};
     for (void *b : bufs) (void)hipFree(b);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -747,9 +753,9 @@ int Engine::create(const speo_engine_config &c, hipStream_t s) {
                                                "of 4 * 2^k rows");
         }
     }
-    if (c.surrogate_candidates > 0 && (c.mode != SPEO_MODE_PLAIN || c.use_graph))
+    if (c.surrogate_candidates > 0 && c.mode != SPEO_MODE_PLAIN)
         return set_error(SPEO_ERR_UNSUPPORTED, "speo_engine_create: the surrogate runs in "
-                                               "SPEO_MODE_PLAIN without graphs");
+                                               "SPEO_MODE_PLAIN");
     if (cfg.mode == SPEO_MODE_EPOCH && cfg.epoch_parts == 0 && !tuning_child)
         SPEO_TRY(tune_epoch_parts());
     if (cfg.mode == SPEO_MODE_EPOCH) SPEO_TRY(epoch_setup());
@@ -972,6 +978,10 @@ int Engine::surrogate_setup() {
     SPEO_TRY_HIP(hipMalloc(&yh, sizeof(float) * 1024));
     SPEO_TRY_HIP(hipMalloc(&predh, sizeof(float) * 1024));
     SPEO_TRY_HIP(hipMalloc(&ranks_ws, sizeof(float2) * 4096));
+    SPEO_TRY_HIP(hipMalloc(&heads_dev, sizeof(int) * kMaxSurrProblems));
+    SPEO_TRY_HIP(hipMemset(heads_dev, 0, sizeof(int) * kMaxSurrProblems));
+    SPEO_TRY_HIP(hipMalloc(&round_dev, sizeof(uint32_t)));
+    SPEO_TRY_HIP(hipMemset(round_dev, 0, sizeof(uint32_t)));
     return SPEO_OK;
 }
 
@@ -1017,12 +1027,12 @@ int Engine::surrogate_generation(int gen_offset, hipStream_t s) {
     SPEO_TRY(launch(kModeEval, gen_offset, rt, s));
     SurrHeads h{};
     for (size_t p = 0; p < surr.size(); ++p) {
-        h.head[p] = surr[p]->head;
         h.qoff[p] = surr[p]->qoff;
+        h.rows[p] = surr[p]->rows;
     }
     SPEO_TRY(archive_append(isl_dev, srows_dev, Q, U[0], fU[0], AX, Ay, cfg.surrogate_archive, Kp,
-                            h, s));
-    for (auto &sp : surr) {
+                            h, (int)surr.size(), heads_dev, s));
+    for (auto &sp : surr) {  // host copies of the device heads
         sp->head = (int)(((int64_t)sp->head + sp->rows) % cfg.surrogate_archive);
         sp->appended += sp->rows;
     }
@@ -1057,9 +1067,9 @@ int Engine::surrogate_train(hipStream_t s) {
         }
         for (int step = 0; step < cfg.surrogate_steps; ++step)
             SPEO_TRY(train_steps(nets.data(), ax.data(), ay.data(), nv.data(), seeds.data(),
-                                 (int)todo.size(), (uint32_t)surr_rounds, (uint32_t)step,
-                                 cfg.surrogate_batch, cfg.surrogate_lr,
-                                 step == cfg.surrogate_steps - 1, s));
+                                 (int)todo.size(), round_dev, (uint32_t)step, cfg.surrogate_batch,
+                                 cfg.surrogate_lr, step == cfg.surrogate_steps - 1, s));
+        for (Mlp *n : nets) SPEO_TRY(add_i32(n->t_dev, cfg.surrogate_steps, s));
     } else {
         for (size_t p : todo)
             for (int step = 0; step < cfg.surrogate_steps; ++step)
@@ -1079,7 +1089,8 @@ int Engine::surrogate_train(hipStream_t s) {
         any_trained = true;
     }
     ++surr_rounds;
-    return SPEO_OK;
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, s, round_dev, 1u);
+    return check_launch("k_add_u32");
 }
 
 int Engine::launch(int mode, int gen_offset, int rt_, hipStream_t s, int clear) {
@@ -1321,15 +1332,100 @@ int Engine::prepare(int generations) {
     return rc;
 }
 
+// A surrogate round: the train_interval generations up to a training point
+// (migration after generation mig_pos when >= 0), the generation counter, then
+// the training.  The same launches surrogate generations and training enqueue
+// one by one, with the generations' counters offset within the round.
+int Engine::enqueue_surrogate_round(int TI, int mig_pos, hipStream_t s) {
+    for (int k = 0; k < TI; ++k) {
+        SPEO_TRY(surrogate_generation(k, s));
+        if (k == mig_pos) SPEO_TRY(migrate(s));
+    }
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, s, gen_base, (uint32_t)TI);
+    SPEO_TRY(check_launch("k_add_u32"));
+    return surrogate_train(s);
+}
+
+// The host-side counters a replayed surrogate round advances (a captured
+// round advanced them once while it was enqueued).
+void Engine::surrogate_round_host(int TI) {
+    const int64_t cap = cfg.surrogate_archive;
+    for (int k = 0; k < TI; ++k) {
+        last_Ce = any_trained ? C : 1;
+        if (any_trained) surr_pred += (int64_t)Q * last_Ce;
+        for (auto &sp : surr) {
+            sp->head = (int)(((int64_t)sp->head + sp->rows) % cap);
+            sp->appended += sp->rows;
+        }
+    }
+    ++surr_rounds;
+}
+
+// Rounds are replayable once every network screens and every archive is full:
+// then no launch argument changes from one round to the next.
+bool Engine::surrogate_round_replayable() const {
+    if (!any_trained || Kp > kFusedMaxKp || cfg.surrogate_archive < std::max(cfg.surrogate_batch, 512))
+        return false;
+    for (const auto &sp : surr)
+        if (sp->rows > 0 && (!sp->trained || sp->appended < cfg.surrogate_archive)) return false;
+    return true;
+}
+
 int Engine::run(int generations, bool dry) {
     if (C > 0) {
-        for (int g = 0; g < generations; ++g) {
-            const bool mig = cfg.interval > 0 && cfg.migration_size > 0 && gens % cfg.interval == 0;
+        const int TI = cfg.surrogate_train_interval;
+        const bool mig_on = cfg.interval > 0 && cfg.migration_size > 0;
+        int left = generations;
+        while (left > 0) {
+            if (cfg.use_graph && !graph_failed && !dry && gens % TI == 0 && left >= TI &&
+                surrogate_round_replayable()) {
+                int mig_pos = -1, nmig = 0;
+                for (int k = 0; k < TI && mig_on; ++k)
+                    if ((gens + k) % cfg.interval == 0) {
+                        mig_pos = k;
+                        ++nmig;
+                    }
+                if (nmig <= 1) {
+                    const auto key = std::make_tuple(-TI, mig_pos, retain_cand ? 1 : 0);
+                    auto it = graphs.find(key);
+                    bool captured = false;
+                    if (it == graphs.end()) {
+                        hipGraph_t g = nullptr;
+                        hipGraphExec_t ge = nullptr;
+                        SPEO_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                        const int rc = enqueue_surrogate_round(TI, mig_pos, stream);
+                        hipError_t e = hipStreamEndCapture(stream, &g);
+                        if (rc != SPEO_OK || e != hipSuccess ||
+                            hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+                            // the host counters advanced during the attempt: this
+                            // engine cannot continue consistently
+                            if (g) (void)hipGraphDestroy(g);
+                            (void)hipGetLastError();
+                            graph_failed = true;
+                            return rc != SPEO_OK ? rc
+                                                 : set_error(SPEO_ERR_HIP, "surrogate round capture failed");
+                        }
+                        (void)hipGraphDestroy(g);
+                        it = graphs.emplace(key, ge).first;
+                        captured = true;
+                    }
+                    SPEO_TRY_HIP(hipGraphLaunch(it->second, stream));
+                    ++graph_launches;
+                    if (!captured) surrogate_round_host(TI);
+                    gens += TI;
+                    fes += (int64_t)TI * total_rows;
+                    migrations += nmig;
+                    left -= TI;
+                    continue;
+                }
+            }
+            const bool mig = mig_on && gens % cfg.interval == 0;
             SPEO_TRY(enqueue_chunk(1, mig, stream));
             ++gens;
             fes += total_rows;
             if (mig) ++migrations;
-            if (gens % cfg.surrogate_train_interval == 0) SPEO_TRY(surrogate_train(stream));
+            if (gens % TI == 0) SPEO_TRY(surrogate_train(stream));
+            --left;
         }
         return SPEO_OK;
     }

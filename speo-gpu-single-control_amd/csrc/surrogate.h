@@ -58,7 +58,7 @@ struct Mlp {
     uint16_t *XT = nullptr, *H1T = nullptr, *dH2T = nullptr, *dH1T = nullptr;
     float *tpart = nullptr;
     int Bp = 0;
-    int64_t adam_t = 0;
+    int *t_dev = nullptr;  // Adam steps taken (device), read by the update kernels
     ~Mlp();
     int alloc(int dim, int batch);
     int init(uint64_t seed, hipStream_t s);
@@ -94,8 +94,8 @@ struct SurrRowDesc {
 
 constexpr int kMaxSurrProblems = 16;
 struct SurrHeads {
-    int head[kMaxSurrProblems];  // next archive slot per problem
     int qoff[kMaxSurrProblems];  // first q of each problem
+    int rows[kMaxSurrProblems];  // targets of each problem
 };
 
 struct IslandDesc;
@@ -123,9 +123,14 @@ int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, int nn, const I
 int pick_regen(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C, const float *pred,
                const float *rho, float rho_min, const float *X, float lo, float hi, uint64_t seed,
                const uint32_t *gen_base, uint32_t gen_offset, float *U, hipStream_t s);
+// Appends every target's trial and log(1 + f) at its problem's next archive
+// slots (head_dev[p] on the device), then advances head_dev[p] by the
+// problem's rows modulo cap.
 int archive_append(const IslandDesc *isl, const SurrRowDesc *rows, int Q, const float *U,
                    const float *fU, float *AX, float *Ay, int cap, int Kp, SurrHeads heads,
-                   hipStream_t s);
+                   int nproblems, int *head_dev, hipStream_t s);
+// *p += v (one thread; also inside captured graphs).
+int add_i32(int *p, int v, hipStream_t s);
 int gather_batch(const float *AX, const float *Ay, int n_valid, int Kp, uint64_t seed,
                  uint32_t round, uint32_t step, int batch, float *Xb, float *tb, hipStream_t s);
 int gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh, float *yh,
@@ -133,9 +138,12 @@ int gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh, f
 int mean_std(const float *y, int n, float *stats, hipStream_t s);
 // One training step (Mlp::train_step's, Kp <= 128) of nn networks of equal
 // width on their own archives, in two launches per eight networks.
+// The minibatch draw uses the device round counter *round_dev; Adam's step
+// count is each network's *t_dev + step + 1 (the caller advances t_dev after
+// the round).
 int train_steps(Mlp *const *nets, const float *const *AX, const float *const *Ay,
-                const int *n_valid, const uint64_t *seed, int nn, uint32_t round, uint32_t step,
-                int batch, float lr, bool want_loss, hipStream_t s);
+                const int *n_valid, const uint64_t *seed, int nn, const uint32_t *round_dev,
+                uint32_t step, int batch, float lr, bool want_loss, hipStream_t s);
 // Spearman's rho of n <= 4096 values; ranks: workspace of 4096 float2.
 int spearman(const float *a, const float *b, int n, float *rho_out, float2 *ranks, hipStream_t s);
 

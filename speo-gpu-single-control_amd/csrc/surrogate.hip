@@ -355,11 +355,27 @@ __device__ __forceinline__ float adam_update(float *P, float *m, float *v, size_
     return p;
 }
 
-__global__ void k_adam(float *P, const float *G, float *m, float *v, size_t n, float lr, float c1,
-                       float c2) {
+// Bias corrections of Adam step t (1-based).
+__device__ __forceinline__ void adam_corrections(int t, float &c1, float &c2) {
+    c1 = 1.0f - powf(0.9f, (float)t);
+    c2 = 1.0f - powf(0.999f, (float)t);
+}
+
+// Step *t_dev + 1 on every parameter.
+__global__ void k_adam(float *P, const float *G, float *m, float *v, size_t n, float lr,
+                       const int *t_dev) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    float c1, c2;
+    adam_corrections(*t_dev + 1, c1, c2);
     (void)adam_update(P, m, v, i, G[i], lr, c1, c2);
+}
+
+__global__ void k_add_i32(int *p, int v) { *p += v; }
+
+int add_i32(int *p, int v, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_i32, dim3(1), dim3(1), 0, s, p, v);
+    return check_launch("k_add_i32");
 }
 
 // He-uniform init from Philox: W1 ~ U(+-sqrt(6/D)), W2 ~ U(+-sqrt(6/256)),
@@ -396,8 +412,8 @@ static int zero(float *p, size_t n, hipStream_t s) {
 // ------------------------------------------------------------------ Mlp
 
 Mlp::~Mlp() {
-    void *b[] = {P,   G,   m,     v,   A1,  H1,   A2,   H2,   dH1,   dH2,  y,
-                 dy,  stats, W1b, W2b, part, lterm, W2Tb, XT, H1T, dH2T, dH1T, tpart};
+    void *b[] = {P,   G,   m,     v,   A1,  H1,   A2,   H2,   dH1,   dH2,  y,     dy,
+                 stats, W1b, W2b, part, lterm, W2Tb, XT, H1T, dH2T, dH1T, tpart, t_dev};
     for (void *p : b) (void)hipFree(p);
 }
 
@@ -411,6 +427,8 @@ int Mlp::alloc(int dim, int batch) {
     SPEO_TRY_HIP(hipMalloc(&dy, sizeof(float) * batch));
     SPEO_TRY_HIP(hipMalloc(&lterm, sizeof(float) * batch));
     SPEO_TRY_HIP(hipMalloc(&stats, sizeof(float) * 8));
+    SPEO_TRY_HIP(hipMalloc(&t_dev, sizeof(int)));
+    SPEO_TRY_HIP(hipMemset(t_dev, 0, sizeof(int)));
     SPEO_TRY_HIP(hipMemset(G, 0, sizeof(float) * L.n));
     SPEO_TRY_HIP(hipMemset(stats, 0, sizeof(float) * 8));
     const int splits = std::min(kMaxSplitK, (batch + 127) / 128);
@@ -435,7 +453,7 @@ int Mlp::init(uint64_t seed, hipStream_t s) {
     const float st[8] = {0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     SPEO_TRY_HIP(hipMemcpyAsync(stats, st, sizeof(st), hipMemcpyHostToDevice, s));
     SPEO_TRY_HIP(hipStreamSynchronize(s));
-    adam_t = 0;
+    SPEO_TRY_HIP(hipMemset(t_dev, 0, sizeof(int)));
     return SPEO_OK;
 }
 
@@ -516,13 +534,11 @@ int Mlp::loss_grad_unfused(const float *X, int ld, int n, const float *t, hipStr
 }
 
 int Mlp::adam(float lr, hipStream_t s) {
-    ++adam_t;
-    const float c1 = 1.0f - powf(0.9f, (float)adam_t);
-    const float c2 = 1.0f - powf(0.999f, (float)adam_t);
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, P, G, m, v,
-                       L.n, lr, c1, c2);
+                       L.n, lr, t_dev);
     bf16_stale = true;
-    return check_launch("k_adam");
+    SPEO_TRY(check_launch("k_adam"));
+    return add_i32(t_dev, 1, s);
 }
 
 // ------------------------------------------------------ fused inference
@@ -725,7 +741,8 @@ struct TrainRowsArgs {
     const float *AX, *Ay;  // archive rows [slot][Kp], labels [slot]
     int n_valid;
     uint64_t seed;
-    uint32_t round, step;
+    uint32_t round, step;  // the minibatch draw's counters (round: *round_dev when set)
+    const uint32_t *round_dev;
     int n, D;
     const float *stats;    // [1] mean, [2] std of the labels
     const uint16_t *W1b, *W2b, *W2Tb;
@@ -783,7 +800,8 @@ __global__ __launch_bounds__(1024) void k_train_rows(TrainRowsBatch batch) {
         float t = 0.f;
         if (b < a.n) {
             if (a.AX) {  // the slot gather_batch draws for row b
-                const U4 w = philox_block(a.seed, (uint32_t)(b >> 2), a.round, a.step, kPurposeSurrBatch);
+                const uint32_t round = a.round_dev ? *a.round_dev : a.round;
+                const U4 w = philox_block(a.seed, (uint32_t)(b >> 2), round, a.step, kPurposeSurrBatch);
                 slot = (int)(u01(word(w, (uint32_t)(b & 3))) * (float)a.n_valid);
                 slot = slot >= a.n_valid ? a.n_valid - 1 : slot;
                 if (slot % 8 == 7) slot -= 1;
@@ -962,7 +980,9 @@ struct TrainUpdateArgs {
     MlpLayout L;
     uint16_t *W1b, *W2b, *W2Tb;
     int adam;
-    float lr, c1, c2;
+    float lr;
+    const int *t_dev;  // Adam step of this update: *t_dev + t_off + 1
+    int t_off;
     float *stats;   // [0] = loss when want_loss
     int want_loss;
 };
@@ -980,6 +1000,8 @@ struct TrainUpdateBatch {
 
 __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
     const TrainUpdateArgs &a = batch.net[blockIdx.x / batch.blocks];
+    float c1 = 1.f, c2 = 1.f;
+    if (a.adam) adam_corrections(*a.t_dev + a.t_off + 1, c1, c2);
     __shared__ float red[4][32][33];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
@@ -1014,9 +1036,9 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
             a.G[o3] = g3;
         }
         if (a.adam) {
-            (void)adam_update(a.P, a.m, a.v, o1, g1, a.lr, a.c1, a.c2);
-            (void)adam_update(a.P, a.m, a.v, o2, g2, a.lr, a.c1, a.c2);
-            (void)adam_update(a.P, a.m, a.v, o3, g3, a.lr, a.c1, a.c2);
+            (void)adam_update(a.P, a.m, a.v, o1, g1, a.lr, c1, c2);
+            (void)adam_update(a.P, a.m, a.v, o2, g2, a.lr, c1, c2);
+            (void)adam_update(a.P, a.m, a.v, o3, g3, a.lr, c1, c2);
         }
         if (tid == 0) {
             float gb = 0.f, loss = 0.f;
@@ -1036,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
                     }
             }
             if (a.G) a.G[a.L.ob3] = gb;
-            if (a.adam) (void)adam_update(a.P, a.m, a.v, a.L.ob3, gb, a.lr, a.c1, a.c2);
+            if (a.adam) (void)adam_update(a.P, a.m, a.v, a.L.ob3, gb, a.lr, c1, c2);
             if (a.want_loss) a.stats[0] = loss;
         }
         return;
@@ -1092,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
         const size_t off = w2 ? a.L.oW2 + (size_t)I * kHidden + J : a.L.oW1 + (size_t)I * a.L.Kp + J;
         if (a.G) a.G[off] = g;
         if (a.adam) {
-            const float p = adam_update(a.P, a.m, a.v, off, g, a.lr, a.c1, a.c2);
+            const float p = adam_update(a.P, a.m, a.v, off, g, a.lr, c1, c2);
             const uint16_t h = to_bits(p, false);
             if (w2) {
                 a.W2b[frag_index(I, J, kHidden / 32)] = h;
@@ -1108,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_train_update(TrainUpdateBatch batch) {
 // archive slots (AX, Ay, ...) per network, gradients into G (grad) and/or
 // Adam (lr > 0); two launches for all of them.
 static int fused_steps(Mlp *const *nets, const TrainRowsArgs *r0, int nn, bool grad, float lr,
-                       bool want_loss, hipStream_t s) {
+                       int t_off, bool want_loss, hipStream_t s) {
     if (nn < 1 || nn > kMaxTrainNets) return set_error(SPEO_ERR_ARG, "mlp: 1..8 networks per step");
     const int Kp = nets[0]->L.Kp, n = r0[0].n;
     const int tiles = (n + kTrainRows - 1) / kTrainRows;
@@ -1155,12 +1177,9 @@ static int fused_steps(Mlp *const *nets, const TrainRowsArgs *r0, int nn, bool g
         u.W2b = N.W2b;
         u.W2Tb = N.W2Tb;
         u.adam = lr > 0.f ? 1 : 0;
-        if (u.adam) {
-            ++N.adam_t;
-            u.lr = lr;
-            u.c1 = 1.0f - powf(0.9f, (float)N.adam_t);
-            u.c2 = 1.0f - powf(0.999f, (float)N.adam_t);
-        }
+        u.lr = lr;
+        u.t_dev = N.t_dev;
+        u.t_off = t_off;
         u.stats = N.stats;
         u.want_loss = want_loss ? 1 : 0;
     }
@@ -1177,15 +1196,17 @@ static int fused_steps(Mlp *const *nets, const TrainRowsArgs *r0, int nn, bool g
     return check_launch("k_train_update");
 }
 
+// One step of one network; with Adam (lr > 0) its step count advances by one.
 static int fused_step(Mlp &N, const TrainRowsArgs &r0, bool grad, float lr, bool want_loss,
                       hipStream_t s) {
     Mlp *net = &N;
-    return fused_steps(&net, &r0, 1, grad, lr, want_loss, s);
+    SPEO_TRY(fused_steps(&net, &r0, 1, grad, lr, 0, want_loss, s));
+    return lr > 0.f ? add_i32(N.t_dev, 1, s) : SPEO_OK;
 }
 
 int train_steps(Mlp *const *nets, const float *const *AX, const float *const *Ay,
-                const int *n_valid, const uint64_t *seed, int nn, uint32_t round, uint32_t step,
-                int batch, float lr, bool want_loss, hipStream_t s) {
+                const int *n_valid, const uint64_t *seed, int nn, const uint32_t *round_dev,
+                uint32_t step, int batch, float lr, bool want_loss, hipStream_t s) {
     for (int k0 = 0; k0 < nn; k0 += kMaxTrainNets) {
         const int m = std::min(kMaxTrainNets, nn - k0);
         TrainRowsArgs r[kMaxTrainNets] = {};
@@ -1198,11 +1219,11 @@ int train_steps(Mlp *const *nets, const float *const *AX, const float *const *Ay
             r[k].Ay = Ay[k0 + k];
             r[k].n_valid = n_valid[k0 + k];
             r[k].seed = seed[k0 + k];
-            r[k].round = round;
+            r[k].round_dev = round_dev;
             r[k].step = step;
             r[k].n = batch;
         }
-        SPEO_TRY(fused_steps(nets + k0, r, m, false, lr, want_loss, s));
+        SPEO_TRY(fused_steps(nets + k0, r, m, false, lr, (int)step, want_loss, s));
     }
     return SPEO_OK;
 }
@@ -1553,27 +1574,35 @@ int pick_candidates(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C
 // x is stored with Kp columns (zero padded) and y = log(1 + f).
 __global__ void k_archive_append(const IslandDesc *isl, const SurrRowDesc *rows, int Q,
                                  const float *U, const float *fU, float *AX, float *Ay, int cap,
-                                 int Kp, SurrHeads heads) {
+                                 int Kp, SurrHeads heads, const int *head_dev) {
     const int q = blockIdx.x * 16 + (threadIdx.x >> 4);
     const int l = threadIdx.x & 15;
     if (q >= Q) return;
     const SurrRowDesc rd = rows[q];
     const IslandDesc I = isl[rd.island];
     const int p = rd.problem;
-    const int slot = (int)(((int64_t)heads.head[p] + q - heads.qoff[p]) % cap);
+    const int slot = (int)(((int64_t)head_dev[p] + q - heads.qoff[p]) % cap);
     float *dst = AX + ((size_t)p * cap + slot) * Kp;
     const float *src = U + I.elem_off + (size_t)rd.li * I.ld;
     for (int j = l; j < Kp; j += 16) dst[j] = j < I.dim ? src[j] : 0.f;
     if (l == 0) Ay[(size_t)p * cap + slot] = log1pf(fmaxf(fU[rd.gr], 0.f));
 }
 
+__global__ void k_archive_advance(int *head_dev, SurrHeads heads, int nproblems, int cap) {
+    const int p = threadIdx.x;
+    if (p < nproblems) head_dev[p] = (int)(((int64_t)head_dev[p] + heads.rows[p]) % cap);
+}
+
 int archive_append(const IslandDesc *isl, const SurrRowDesc *rows, int Q, const float *U,
                    const float *fU, float *AX, float *Ay, int cap, int Kp, SurrHeads heads,
-                   hipStream_t s) {
+                   int nproblems, int *head_dev, hipStream_t s) {
     if (Q <= 0) return SPEO_OK;
     hipLaunchKernelGGL(k_archive_append, dim3((Q + 15) / 16), dim3(256), 0, s, isl, rows, Q, U, fU,
-                       AX, Ay, cap, Kp, heads);
-    return check_launch("k_archive_append");
+                       AX, Ay, cap, Kp, heads, head_dev);
+    SPEO_TRY(check_launch("k_archive_append"));
+    hipLaunchKernelGGL(k_archive_advance, dim3(1), dim3(kMaxSurrProblems), 0, s, head_dev, heads,
+                       nproblems, cap);
+    return check_launch("k_archive_advance");
 }
 
 // Minibatch of the valid archive slots [0, n_valid), skipping the held-out
@@ -1689,7 +1718,7 @@ int speo_mlp_params(speo_mlp h, float *get_out, const float *set_in) {
         SPEO_TRY_HIP(hipMemcpy(h->m.P, set_in, sizeof(float) * h->m.L.n, hipMemcpyHostToDevice));
         SPEO_TRY_HIP(hipMemset(h->m.m, 0, sizeof(float) * h->m.L.n));
         SPEO_TRY_HIP(hipMemset(h->m.v, 0, sizeof(float) * h->m.L.n));
-        h->m.adam_t = 0;
+        SPEO_TRY_HIP(hipMemset(h->m.t_dev, 0, sizeof(int)));
         h->m.bf16_stale = true;
     }
     if (get_out) SPEO_TRY_HIP(hipMemcpy(get_out, h->m.P, sizeof(float) * h->m.L.n, hipMemcpyDeviceToHost));

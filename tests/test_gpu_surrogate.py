@@ -275,3 +275,37 @@ def test_surrogate_gain_over_plain(cuda, speo, oracle):
     assert plain.stats()["evaluations"] == surr.stats()["evaluations"]
     assert rho[0] > 0.5
     assert fs.min() < fp.min()
+
+
+def test_surrogate_rounds_under_graphs_equal_eager(cuda, speo, oracle):
+    # once the networks screen and the archives are full, each round (the
+    # generations up to a training point, a migration among them or not, and
+    # the training) is captured once and replayed: the same bits as eager
+    # launches, generation counter, archive heads and Adam steps included
+    D, C = 30, 4
+    probs = []
+    for k, obj in enumerate((speo.OBJ_RASTRIGIN, speo.OBJ_ELLIPTIC)):
+        o, M = oracle.gen_problem(300 + k, D)
+        probs.append(speo.Problem([obj], o[None, :], M[None, :, :]))
+    kw = dict(seed=17, interval=10, migration_size=2, mode=speo.MODE_PLAIN, surrogate_candidates=C,
+              surrogate_train_interval=5, surrogate_archive=1024, surrogate_steps=5,
+              surrogate_batch=256, surrogate_rho_min=-1.0)
+    eager = speo.Engine(probs, [64] * 8, [i % 2 for i in range(8)], use_graph=False, **kw)
+    graph = speo.Engine(probs, [64] * 8, [i % 2 for i in range(8)], use_graph=True, **kw)
+    for chunk in (12, 3, 25):  # rounds start at generations 15, 20, ..., 35
+        eager.run(chunk)
+        graph.run(chunk)
+        Xe, fe = eager.read()
+        Xg, fg = graph.read()
+        np.testing.assert_array_equal(Xe, Xg)
+        np.testing.assert_array_equal(fe, fg)
+    se, sg = eager.stats(), graph.stats()
+    assert se["surrogate_rounds"] == sg["surrogate_rounds"] == 8
+    assert sg["graph_launches"] >= 4 and se["graph_launches"] == 0
+    rho_e, act_e, _ = eager.surrogate_stats()
+    rho_g, act_g, _ = graph.surrogate_stats()
+    np.testing.assert_array_equal(rho_e, rho_g)
+    for e in (eager, graph):
+        e.close()
+    for p in probs:
+        p.close()
