@@ -41,7 +41,11 @@ namespace speo {
 
 enum GenMode { kModeGen = 0, kModeEval = 1, kModeSelect = 2, kModeFused = 3 };
 
+// A block's work item: rows [row0, row0 + nrows) of an island, with a copy of
+// the island's descriptor (immutable after set-up), so a block reads its work
+// in one round trip instead of two dependent ones.
 struct TileDesc {
+    IslandDesc I;
     int island, row0, nrows, pad;
 };
 
@@ -84,17 +88,17 @@ __device__ __forceinline__ float comp4(const float4 &v, int q) {
     return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
 }
 
-// Row tile 32: four waves per SIMD (at most 128 registers).  The fused FP32
-// kernel would otherwise take 158 and run three; the few values it then
-// spills cost less than the lost occupancy (7 % faster on an uneven batch of
-// 256 islands, D 10..100).  The other row tiles lose by it (row tile 16: 12 %
-// slower on 32 islands x 256 at D = 100) and keep their allocation.
+// Row tile 32: three waves per SIMD (at most 168 registers).  Held to four
+// waves (128 registers) the fused FP32 kernel spills 44 registers and runs
+// 9 % slower on an uneven batch of 256 islands, D 10..100 (with 13 spilled
+// values, before the FP64 heavy-column step was added, four waves were 7 %
+// faster).  The other row tiles keep the compiler's allocation.
 template <int RT, int PREC, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 4 : 1))) void k_generation(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT == 32 ? 3 : 1))) void k_generation(
     GenerationArgs a) {
     extern __shared__ __align__(16) char lds[];
     const TileDesc t = a.tiles[blockIdx.x];
-    const IslandDesc I = a.isl[t.island];
+    const IslandDesc I = t.I;
     const int tid = threadIdx.x;
 
     if constexpr (MODE == kModeSelect) {
@@ -587,7 +591,13 @@ int Engine::build_tiles(int rt_) {
     std::vector<TileDesc> tiles;
     for (int i = 0; i < I; ++i)
         for (int r0 = 0; r0 < isl[(size_t)i].np; r0 += rt_)
-            tiles.push_back(TileDesc{i, r0, std::min(rt_, isl[(size_t)i].np - r0), 0});
+            tiles.push_back(TileDesc{isl[(size_t)i], i, r0, std::min(rt_, isl[(size_t)i].np - r0), 0});
+    // Heaviest tiles first (rows x dim^2, ties in island order): blocks are
+    // dispatched in this order, so with uneven islands the last wave of blocks
+    // holds the cheap tiles.  Tiles are independent: the order changes no result.
+    auto cost = [&](const TileDesc &t) { return (int64_t)t.nrows * t.I.dim * t.I.dim; };
+    std::stable_sort(tiles.begin(), tiles.end(),
+                     [&](const TileDesc &x, const TileDesc &y) { return cost(x) > cost(y); });
     TileDesc *d = nullptr;
     SPEO_TRY(upload_vec(&d, tiles));
     tiles_dev[rt_] = d;

@@ -349,7 +349,7 @@ def test_engine_rejects_bad_configs(cuda, speo, oracle, case):
     elif case == "surrogate_dims":
         kw, match = dict(surrogate_candidates=4, mode=speo.MODE_PLAIN), "share one dimension"
     elif case == "surrogate_mode":
-        kw, match = dict(surrogate_candidates=4, mode=speo.MODE_FUSED), "SPEO_MODE_PLAIN without graphs"
+        kw, match = dict(surrogate_candidates=4, mode=speo.MODE_FUSED), "surrogate runs in SPEO_MODE_PLAIN"
     if case == "run_negative":
         e = _engine(speo, probs, sizes, pidx)
         with pytest.raises(speo.SpeoError, match="bad arguments"):
