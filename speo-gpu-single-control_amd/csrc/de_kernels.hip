@@ -227,17 +227,40 @@ int segment_argmin(const float *f, const int *off, int nseg, int *best_out, hipS
 
 // Block per island: rank of each row in the order (f ascending, row
 // ascending, NaN last); the first k rows are its best, the last k its worst.
+// The island's fitness is compared from LDS (islands of up to 4096 rows; larger
+// ones from global memory).
+constexpr int kRankLds = 4096;
+
+// before(a, ia, b, ib) without branches (the rank loop runs it n^2 times).
+__device__ __forceinline__ int before_nb(float a, int ia, float b, int ib) {
+    const bool an = a != a, bn = b != b, idx = ia < ib;
+    const bool neither = !an && !bn;
+    return (int)((bn && !an) | (an && bn && idx) | (neither && ((a < b) | ((a == b) && idx))));
+}
 __global__ void k_rank_islands(const float *f, const int *row_off, int k, int *best_out,
                                int *worst_out) {
+    __shared__ float fs[kRankLds];
     const int isl = blockIdx.x;
     const int r0 = row_off[isl], np = row_off[isl + 1] - r0;
-    for (int r = threadIdx.x; r < np; r += blockDim.x) {
-        const float fr = f[r0 + r];
-        int rank = 0;
-        for (int q = 0; q < np; ++q) rank += before(f[r0 + q], r0 + q, fr, r0 + r) ? 1 : 0;
-        if (rank < k) best_out[isl * k + rank] = r0 + r;
-        if (rank >= np - k) worst_out[isl * k + (np - 1 - rank)] = r0 + r;
-    }
+    const bool staged = np <= kRankLds;
+    if (staged)
+        for (int r = threadIdx.x; r < np; r += blockDim.x) fs[r] = f[r0 + r];
+    __syncthreads();
+    auto rank_rows = [&](const auto *fi) {
+        for (int r = threadIdx.x; r < np; r += blockDim.x) {
+            const float fr = fi[r];
+            int rank = 0;
+#pragma unroll 8
+            for (int q = 0; q < np; ++q) rank += before_nb(fi[q], r0 + q, fr, r0 + r);
+            if (rank < k) best_out[isl * k + rank] = r0 + r;
+            if (rank >= np - k) worst_out[isl * k + (np - 1 - rank)] = r0 + r;
+        }
+    };
+    // two loops, so that the staged one reads LDS with LDS instructions
+    if (staged)
+        rank_rows(fs);
+    else
+        rank_rows(f + r0);
 }
 
 // Block per (island, migrant): island isl's t-th best row replaces the t-th
