@@ -533,19 +533,25 @@ def main():
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-error runs")
     ap.add_argument("--tte-gens", type=int, default=2000)
     ap.add_argument("--uneven", action="store_true",
-                    help="also run the uneven 256-island batch (D in {10,30,50,100}, NP 50..512)")
+                    help="run the uneven 256-island batch (D in {10,30,50,100}, NP 50..512)")
     ap.add_argument("--precision-study", action="store_true",
-                    help="also run the D=1000 precision study (10k points)")
+                    help="run the D=1000 precision study (10k points)")
     ap.add_argument("--surrogate", action="store_true",
-                    help="also time a surrogate-assisted generation against a plain one "
+                    help="time a surrogate-assisted generation against a plain one "
                          "(32 islands x 256, D=100, 8 candidates)")
     ap.add_argument("--cec-suite", action="store_true",
-                    help="also time the evaluation of every CEC2014 function F1-F30 at D=100")
+                    help="time the evaluation of every CEC2014 function F1-F30 at D=100")
     ap.add_argument("--only-extras", action="store_true",
                     help="skip the headline run; run only the requested extra legs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the legs that a plain run includes by default: the uneven batch, "
+                         "the precision study, the surrogate's cost and the F1-F30 rates")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (scales rocprof byte counters)")
     args = ap.parse_args()
+    if not args.only_extras and not args.no_extras:
+        # a plain run records every leg (about 30 s together on one MI355X)
+        args.uneven = args.precision_study = args.surrogate = args.cec_suite = True
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box in this build has a single MI355X, "
