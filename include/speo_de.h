@@ -281,6 +281,15 @@ SPEO_EXPORT int speo_problem_create_cec(const char *dir, int func_id, int dim, s
 SPEO_EXPORT int speo_evaluate(speo_problem p, const float *X, int ld, int n, float *f_out,
                               int precision, speo_stream_t stream);
 
+/* Kernel of speo_evaluate's 16-bit evaluations (BF16, FP16 and their
+ * splits) of problems wider than 128 columns on at least 1024 rows, where the
+ * arithmetic is that of 64-row tiles: enable 1 (default) a kernel with
+ * 256-column tiles, 0 the 64-row-tile kernel, -1 leaves the setting.  Both
+ * give the same bits; the switch is for tests and measurements.  Process-wide;
+ * returns the previous setting, and *launches (optional) the number of
+ * launches of the 256-column kernel so far. */
+SPEO_EXPORT int speo_wide_eval(int enable, long long *launches);
+
 /* FP64 evaluation on the device (re-verification path). */
 SPEO_EXPORT int speo_evaluate_f64(speo_problem p, const float *X, int ld, int n, double *f_out,
                                   speo_stream_t stream);

@@ -33,7 +33,15 @@ int eval_problem(const ProblemHost &P, const float *X, int ld, int n, float *f_o
 int eval_problem_f64(const ProblemHost &P, const float *X, int ld, int n, double *f_out,
                      hipStream_t s);
 
-int default_rows_per_block(int n, int dim);
+int default_rows_per_block(int n, int dim, int precision = SPEO_PREC_FP32);
+
+// k_eval_wide: the 16-bit evaluation of one component wider than 128 columns
+// with k_eval<64, precision>'s bits (partial sums into part, finalized by the
+// caller); applies when X, the shift and the 16-bit operand rows allow
+// 16-byte loads.
+bool eval_wide_applies(const CompDev &c, const float *X, int ld, int dim, int precision);
+int eval_wide(const CompDev &c, const float *X, int ld, int n, int dim, float *part, int precision,
+              hipStream_t s);
 
 // Problems with objectives outside the fused FP32 epilogues (ProblemHost::
 // generic): the FP64 generic kernel, FP32 and/or FP64 outputs; rows (device,

@@ -5,6 +5,7 @@
 // kernel_CEC2014_CUDA.cu:1672-1797: ShiftAndRotation -> global_fN, plus
 // CalW/CombineFitness for compositions) by one kernel per N chunk of Z.
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <unordered_map>
 
@@ -164,6 +165,10 @@ __global__ __launch_bounds__(256) void k_eval_f64(const CompParams *comps, int n
 
 // ------------------------------------------------------------------ launch
 
+// k_eval_wide for the 16-bit evaluations that k_eval<64> would run.
+static std::atomic<int> g_wide_eval{1};
+static std::atomic<long long> g_wide_launches{0};
+
 template <int RT, int PREC>
 static int launch_rt_prec(const EvalLaunch &p, int n, int nchunks, hipStream_t s) {
     static std::once_flag once;
@@ -200,7 +205,12 @@ int eval_component(const CompDev &comp, const float *X, int ld, int n, int dim, 
     const int small = (dim <= kSmallD && comp.B32s != nullptr) ? 1 : 0;
     EvalLaunch p{comp, X, ld, n, dim, f_out, part, nchunks, lambda, bias, raw, small};
     int rc;
-    switch (rows_per_block) {
+    if (!small && rows_per_block == 64 && g_wide_eval.load(std::memory_order_relaxed) &&
+        eval_wide_applies(comp, X, ld, dim, precision)) {
+        // the same bits as k_eval<64, precision>, on wider tiles
+        rc = eval_wide(comp, X, ld, n, dim, part, precision, s);
+        g_wide_launches.fetch_add(1, std::memory_order_relaxed);
+    } else switch (rows_per_block) {
     case 16: rc = launch_rt<16>(p, precision, n, nchunks, s); break;
     case 32: rc = launch_rt<32>(p, precision, n, nchunks, s); break;
     case 64: rc = launch_rt<64>(p, precision, n, nchunks, s); break;
@@ -271,9 +281,12 @@ int eval_problem_f64(const ProblemHost &P, const float *X, int ld, int n, double
     return check_launch("k_eval_f64");
 }
 
-int default_rows_per_block(int n, int dim) {
+int default_rows_per_block(int n, int dim, int precision) {
     // Enough blocks to cover the 256 CUs; larger tiles share B staging.
     const int nchunks = (dim + kNC - 1) / kNC;
+    // 16-bit evaluations wider than one chunk: 64-row arithmetic from 1024
+    // rows on, which k_eval_wide runs on its own wider tiles
+    if (precision != SPEO_PREC_FP32 && nchunks > 1 && n >= 1024) return 64;
     if ((long)((n + 63) / 64) * nchunks >= 512) return 64;
     if ((long)((n + 31) / 32) * nchunks >= 256) return 32;
     return 16;
@@ -361,6 +374,11 @@ static int dropin_generic(real *f, real *X, real *shift, real *B, int *shuffle, 
 }  // namespace speo
 
 using namespace speo;
+
+extern "C" int speo_wide_eval(int enable, long long *launches) {
+    if (launches) *launches = g_wide_launches.load();
+    return enable < 0 ? g_wide_eval.load() : g_wide_eval.exchange(enable ? 1 : 0);
+}
 
 extern "C" void API_evaluateFitness(real *d_fitness_value, real *, real *d_rotated_elements,
                                     real *d_original_elements, real *d_shift_data,

@@ -54,6 +54,6 @@ extern "C" int speo_evaluate_noalloc(speo_problem p, const float *X, int ld, int
                                            std::to_string(kNoAllocScratchFloats) + " it holds");
     float *ws = nullptr;
     SPEO_TRY(scratch_ptr(&ws));
-    return eval_problem(p->h, X, ld, n, f_out, ws, precision, default_rows_per_block(n, p->h.dim),
+    return eval_problem(p->h, X, ld, n, f_out, ws, precision, default_rows_per_block(n, p->h.dim, precision),
                         as_stream(stream));
 }

@@ -529,7 +529,7 @@ int speo_evaluate(speo_problem p, const float *X, int ld, int n, float *f_out, i
         p->scratch_cap = need;
     }
     return eval_problem(p->h, X, ld, n, f_out, p->scratch, precision,
-                        default_rows_per_block(n, p->h.dim), as_stream(stream));
+                        default_rows_per_block(n, p->h.dim, precision), as_stream(stream));
 }
 
 int speo_evaluate_f64(speo_problem p, const float *X, int ld, int n, double *f_out,

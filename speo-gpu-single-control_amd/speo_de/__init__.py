@@ -197,6 +197,16 @@ def evaluate(problem: Problem, X, f_out, precision=PREC_FP32, ld=None, n=None, s
                               _stream(stream)))
 
 
+def wide_eval(enable=-1):
+    """Selects the kernel of the 16-bit evaluations wider than 128 columns on
+    at least 1024 rows (1: 256-column tiles, 0: 64-row tiles, -1: unchanged;
+    the same bits either way).  Returns (previous setting, launches of the
+    256-column kernel so far)."""
+    n = ctypes.c_longlong(0)
+    prev = lib().speo_wide_eval(enable, ctypes.byref(n))
+    return prev, n.value
+
+
 def evaluate_noalloc(problem: Problem, X, f_out, precision=PREC_FP32, ld=None, n=None,
                      stream=None):
     """evaluate() with no device allocation: the workspace is a fixed 2^20-float

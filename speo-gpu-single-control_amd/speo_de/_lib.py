@@ -7,8 +7,8 @@ missing, every call raises.
 from __future__ import annotations
 
 import ctypes
-from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int64, c_uint8,
-                    c_uint32, c_uint64, c_void_p)
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int64, c_longlong,
+                    c_uint8, c_uint32, c_uint64, c_void_p)
 from pathlib import Path
 
 PKG_DIR = Path(__file__).resolve().parent.parent
@@ -134,6 +134,7 @@ _SIGS = {
     "speo_problem_create_cec": (c_int, [c_char_p, c_int, c_int, POINTER(c_void_p)]),
     "speo_evaluate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
     "speo_evaluate_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "speo_wide_eval": (c_int, [c_int, POINTER(c_longlong)]),
     "speo_evaluate_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
                                       c_void_p]),
     "speo_init_population": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_float, c_uint64,

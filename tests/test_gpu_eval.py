@@ -321,3 +321,52 @@ def test_strided_rows_and_ragged_counts(cuda, speo, oracle):
         out = f.cpu().numpy()
         np.testing.assert_allclose(out[:n], oracle.evaluate(2, X[:, :D], o, M), rtol=1e-5)
         assert out[n] == -1.0                 # nothing written past n
+
+
+@pytest.mark.parametrize("D,ld", [(1000, 1000), (301, 304), (200, 200)])
+def test_wide_kernel_equals_64_row_kernel(cuda, speo, oracle, D, ld):
+    # The 256-column-tile kernel of the wide 16-bit evaluations against the
+    # 64-row-tile kernel on the same rows, bit for bit, in every 16-bit
+    # precision: single objectives (discus with its FP64 heavy column, bent
+    # cigar) and a composition, on a row count that leaves the last row block
+    # partial, at dimensions with an odd number of 128-column chunks and with
+    # a K tail that is not a multiple of 32 or of 8.  The well-conditioned
+    # objectives are also held to the FP64 oracle at each precision's bound.
+    import torch
+    rng = np.random.default_rng(D + 17)
+    n = 1100
+    X = _points(rng, n, D, ld)
+    Xd = _t(X, cuda)
+    probs = []
+    for obj in (1, 2, 3, 4, 5):
+        o, M = oracle.gen_problem(3000 + obj, D)
+        probs.append((obj, o, M, speo.Problem([obj], o[None, :], M[None, :, :])))
+    shifts = np.stack([oracle.gen_shift(s, D) for s in (31, 32, 33)])
+    rots = np.stack([oracle.gen_rotation(s, D) for s in (31, 32, 33)])
+    probs.append((0, None, None, speo.Problem([1, 2, 3], shifts, rots, sigma=[10.0, 20.0, 30.0],
+                                              bias=[0.0, 100.0, 200.0])))
+    prev, _ = speo.wide_eval(-1)
+    try:
+        for obj, o, M, prob in probs:
+            ref64 = oracle.evaluate(obj, X[:, :D], o, M) if obj in (1, 2, 3) else None
+            for prec in range(1, 6):
+                print(f"wide D={D} obj={obj} prec={prec}", flush=True)
+                speo.wide_eval(0)
+                _, before = speo.wide_eval(-1)
+                ref = torch.zeros(n, dtype=torch.float32, device=cuda)
+                speo.evaluate(prob, Xd, ref, precision=prec)
+                speo.wide_eval(1)
+                f = torch.zeros(n, dtype=torch.float32, device=cuda)
+                speo.evaluate(prob, Xd, f, precision=prec)
+                torch.cuda.synchronize()
+                _, after = speo.wide_eval(-1)
+                assert after > before, "the 256-column kernel did not run"
+                got = f.cpu().numpy()
+                np.testing.assert_array_equal(got, ref.cpu().numpy())
+                if ref64 is not None:
+                    rel = float(np.max(np.abs(got - ref64) / np.abs(ref64)))
+                    assert rel <= TOL[prec], (obj, prec, rel)
+    finally:
+        speo.wide_eval(prev)
+        for *_, prob in probs:
+            prob.close()
