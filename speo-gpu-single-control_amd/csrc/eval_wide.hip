@@ -34,27 +34,27 @@ namespace {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-#ifndef SPEO_WIDE_NW1
-#define SPEO_WIDE_NW1 8
-#endif
-
 template <int PREC> struct Wide {
     static constexpr int P = Prec<PREC>::parts;
     static constexpr bool H = Prec<PREC>::half;
-    static constexpr int NW = 8;                     // waves per block
-    static constexpr int NT = 64 * NW;               // threads per block
-    static constexpr int RG = NW / 2;                // row groups (two column chunks)
-    static constexpr int RB = P == 1 ? 256 : 128;    // rows per block
-    static constexpr int WR = RB / RG;               // rows per wave
-    static constexpr int RTW = WR / 16;              // row tiles per wave
-    static constexpr int RTB = RB / 16;              // row tiles per block
-    static constexpr int A_BYTES = RTB * 2 * 1024;   // FP32 rows of a stage: two 1 KB halves per row tile
-    static constexpr int B_BYTES = P * 16 * 1024;    // operand fragments of 16 column tiles per part
+    static constexpr int NW = 8;                            // waves per block
+    static constexpr int NT = 64 * NW;                      // threads per block
+    static constexpr int RG = NW / 2;                       // row groups (two column chunks)
+    static constexpr int WR = P == 1 ? 64 : 32;             // rows per wave
+    static constexpr int RB = RG * WR;                      // rows per block
+    static constexpr int RTW = WR / 16;                     // row tiles per wave
+    static constexpr int RTB = RB / 16;                     // row tiles per block
+    static constexpr int A_BYTES = RTB * 2 * 1024;          // FP32 rows of a stage: two 1 KB halves per row tile
+    static constexpr int B_BYTES = P * 16 * 1024;           // operand fragments of 16 column tiles per part
     static constexpr int STAGE = A_BYTES + B_BYTES;
+    // three stage buffers where they fit (two for the three-part split).
+    // Four-wave blocks, two per CU with two buffers each, measured 14 %
+    // slower for BF16: their copies are requested one stage ahead instead of
+    // two and load more bytes per MFMA step.
     static constexpr int NBUF = 3 * STAGE <= 144 * 1024 ? 3 : 2;
     static constexpr int LDS = NBUF * STAGE;
-    static constexpr int NA = RTB * 2 / NW;          // A copies per wave and stage
-    static constexpr int NB = P * 16 / NW;           // B copies per wave and stage
+    static constexpr int NA = RTB * 2 / NW;                 // A copies per wave and stage
+    static constexpr int NB = P * 16 / NW;                  // B copies per wave and stage
     static constexpr int NLD = NA + NB;
     static_assert(RTB * 2 % NW == 0 && 16 % NW == 0, "copies must tile the waves");
     static_assert(2 * RB * 2 * 4 <= LDS, "row sums must fit in the staging area");
