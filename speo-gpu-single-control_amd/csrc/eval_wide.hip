@@ -33,6 +33,7 @@ namespace speo {
 namespace {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float floatx8 __attribute__((ext_vector_type(8)));
 
 template <int PREC> struct Wide {
     static constexpr int P = Prec<PREC>::parts;
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(Wide<PREC>::NT) __attribute__((amdgpu_waves_per_eu(
         const int k = s * 32 + 8 * lk;
         const floatx4 o0 = *reinterpret_cast<const floatx4 *>(shs + k);
         const floatx4 o1 = *reinterpret_cast<const floatx4 *>(shs + k + 4);
+        const floatx8 ov = __builtin_shufflevector(o0, o1, 0, 1, 2, 3, 4, 5, 6, 7);
         vec8 a[W::RTW][P], bb[8][P];
 #pragma unroll
         for (int i = 0; i < W::RTW; ++i) {
@@ -193,30 +195,20 @@ __global__ __launch_bounds__(Wide<PREC>::NT) __attribute__((amdgpu_waves_per_eu(
             const floatx4 x0 = *reinterpret_cast<const floatx4 *>(st + (rt * 2 + 0) * 1024 + lane * 16);
             const floatx4 x1 = *reinterpret_cast<const floatx4 *>(st + (rt * 2 + 1) * 1024 + lane * 16);
             const bool rv = row0 + rt * 16 + lr < p.n;
-            u32x4 w[P];
+            // eight coordinates as one vector: the conversions pair up into
+            // packed converts with no packing instructions after them
+            const floatx8 d = __builtin_shufflevector(x0, x1, 0, 1, 2, 3, 4, 5, 6, 7) - ov;
+            floatx8 rem = UNIT ? d * sa : d * rate * sa;
+            if constexpr (MASK) {
 #pragma unroll
-            for (int q2 = 0; q2 < 4; ++q2) {
-                uint32_t lo[P], hi[P];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int q = 2 * q2 + h;
-                    const float x = q < 4 ? x0[q] : x1[q - 4], o = q < 4 ? o0[q] : o1[q - 4];
-                    const float d = x - o;
-                    const float raw = UNIT ? d * sa : d * rate * sa;
-                    const float v = MASK ? ((rv & (k + q < dim)) ? raw : 0.f) : raw;
-                    float rem = v;
-#pragma unroll
-                    for (int pp = 0; pp < P; ++pp) {
-                        const uint16_t bits = to_bits(rem, H);
-                        (h ? hi : lo)[pp] = bits;
-                        rem -= from_bits(bits, H);
-                    }
-                }
-#pragma unroll
-                for (int pp = 0; pp < P; ++pp) w[pp][q2] = lo[pp] | (hi[pp] << 16);
+                for (int q = 0; q < 8; ++q)
+                    if (!(rv & (k + q < dim))) rem[q] = 0.f;
             }
 #pragma unroll
-            for (int pp = 0; pp < P; ++pp) a[i][pp] = __builtin_bit_cast(vec8, w[pp]);
+            for (int pp = 0; pp < P; ++pp) {
+                a[i][pp] = __builtin_convertvector(rem, vec8);
+                if (pp + 1 < P) rem -= __builtin_convertvector(a[i][pp], floatx8);
+            }
         }
         const char *bs = st + W::A_BYTES;
 #pragma unroll
