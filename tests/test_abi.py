@@ -48,3 +48,14 @@ def test_argument_errors_are_returned(speo):
     assert rc == _lib.SPEO_ERR_ARG
     with pytest.raises(speo.SpeoError):
         speo.generate_problem_data(1, 0)
+
+
+def test_wide_eval_switch_is_host_side(speo):
+    # the kernel switch of the wide 16-bit evaluations: 1 by default, -1
+    # reads it, and no call touches a GPU
+    prev, launches = speo.wide_eval(-1)
+    assert prev == 1 and launches >= 0
+    assert speo.wide_eval(0)[0] == 1
+    assert speo.wide_eval(-1)[0] == 0
+    assert speo.wide_eval(1)[0] == 0
+    assert speo.wide_eval(-1) == (1, launches)
