@@ -323,18 +323,20 @@ def test_strided_rows_and_ragged_counts(cuda, speo, oracle):
         assert out[n] == -1.0                 # nothing written past n
 
 
-@pytest.mark.parametrize("D,ld", [(1000, 1000), (301, 304), (200, 200)])
-def test_wide_kernel_equals_64_row_kernel(cuda, speo, oracle, D, ld):
+@pytest.mark.parametrize("D,ld,n", [(1000, 1000, 1100), (301, 304, 1100), (200, 200, 1100),
+                                    (1000, 1000, 16384)])
+def test_wide_kernel_equals_64_row_kernel(cuda, speo, oracle, D, ld, n):
     # The 256-column-tile kernel of the wide 16-bit evaluations against the
     # 64-row-tile kernel on the same rows, bit for bit, in every 16-bit
     # precision: single objectives (discus with its FP64 heavy column, bent
     # cigar) and a composition, on a row count that leaves the last row block
     # partial, at dimensions with an odd number of 128-column chunks and with
-    # a K tail that is not a multiple of 32 or of 8.  The well-conditioned
-    # objectives are also held to the FP64 oracle at each precision's bound.
+    # a K tail that is not a multiple of 32 or of 8, and at the precision
+    # study's full size (8 islands x 2048 rows, D = 1000).  The
+    # well-conditioned objectives are also held to the FP64 oracle at each
+    # precision's bound.
     import torch
     rng = np.random.default_rng(D + 17)
-    n = 1100
     X = _points(rng, n, D, ld)
     Xd = _t(X, cuda)
     probs = []
