@@ -425,7 +425,9 @@ typedef struct {
     int64_t migrations;    /* migration events                        */
     int64_t graph_launches;
     int rows_per_block;    /* tile in use                            */
-    int64_t surrogate_predictions; /* candidates scored by the surrogate */
+    int64_t surrogate_predictions; /* candidates scored by the surrogate (a problem
+                                      whose held-out rho is below the threshold has
+                                      its candidates skipped, not scored) */
     int64_t surrogate_rounds;      /* training rounds run               */
     int epoch_parts;               /* workgroups per island (epoch mode) */
 } speo_engine_stats;
@@ -464,6 +466,8 @@ SPEO_EXPORT int speo_engine_read(speo_engine e, float *X_out, float *f_out);
 SPEO_EXPORT int speo_engine_read_trials(speo_engine e, float *U_out, float *fU_out);
 /* Overwrites the population and fitness from host arrays (same layout). */
 SPEO_EXPORT int speo_engine_write(speo_engine e, const float *X_in, const float *f_in);
+/* Counters of the engine; with the surrogate on it waits for the engine's
+ * stream (the count of scored candidates is kept on the device). */
 SPEO_EXPORT int speo_engine_get_stats(speo_engine e, speo_engine_stats *out);
 SPEO_EXPORT int speo_engine_total_rows(speo_engine e);
 SPEO_EXPORT int speo_engine_max_dim(speo_engine e);

@@ -496,6 +496,7 @@ struct Engine {
     std::vector<std::unique_ptr<SurrProblem>> surr;
     SurrRowDesc *srows_dev = nullptr;
     float *cand = nullptr, *pred = nullptr, *rho_dev = nullptr, *AX = nullptr, *Ay = nullptr;
+    long long *scored_dev = nullptr;  // candidates scored per problem by the one-kernel screening
     float *Xb = nullptr, *tb = nullptr, *Xh = nullptr, *yh = nullptr, *predh = nullptr;
     float2 *ranks_ws = nullptr;  // Spearman workspace
     int *heads_dev = nullptr;         // next archive slot per problem (device)
@@ -571,7 +572,7 @@ Engine::~Engine() {
     void *bufs[] = {isl_dev, row_off_dev, dst_dev, best_buf, worst_buf, best_row_dev,
                     best_f64_dev, probs_dev, params_tab_dev, ncomp_dev, dims_dev, gen_base,
                     blocks_dev, Xnext, donor_buf, row_island_dev,
-                    X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev,
+                    X, fX[0], fX[1], U[0], U[1], fU[0], fU[1], srows_dev, cand, pred, rho_dev, scored_dev,
                     AX, Ay, Xb, tb, Xh, yh, predh, ranks_ws, heads_dev, round_dev, perm_dev,
                     best_sub_dev, info_dev};
     for (void *b : bufs) (void)hipFree(b);
@@ -980,6 +981,7 @@ int Engine::surrogate_setup() {
     SPEO_TRY_HIP(hipMalloc(&cand, sizeof(float) * (size_t)Q * C * Kp));
     SPEO_TRY_HIP(hipMalloc(&pred, sizeof(float) * (size_t)Q * C));
     SPEO_TRY(upload_vec(&rho_dev, std::vector<float>(P, -1.0f)));
+    SPEO_TRY(upload_vec(&scored_dev, std::vector<long long>(P, 0)));
     SPEO_TRY_HIP(hipMalloc(&AX, sizeof(float) * P * cap * Kp));
     SPEO_TRY_HIP(hipMalloc(&Ay, sizeof(float) * P * cap));
     SPEO_TRY_HIP(hipMalloc(&Xb, sizeof(float) * (size_t)cfg.surrogate_batch * Kp));
@@ -1020,16 +1022,17 @@ int Engine::surrogate_generation(int gen_offset, hipStream_t s) {
         // generated again into U: the same trials without the candidate buffer
         if (any_trained) {
             std::vector<Mlp *> nets;
-            std::vector<int> qo, nq;
-            for (auto &sp : surr)
-                if (sp->rows > 0) {
-                    nets.push_back(&sp->net);
-                    qo.push_back(sp->qoff);
-                    nq.push_back(sp->rows);
+            std::vector<int> qo, nq, pi;
+            for (size_t p = 0; p < surr.size(); ++p)
+                if (surr[p]->rows > 0) {
+                    nets.push_back(&surr[p]->net);
+                    qo.push_back(surr[p]->qoff);
+                    nq.push_back(surr[p]->rows);
+                    pi.push_back((int)p);
                 }
-            SPEO_TRY(cand_infer(nets.data(), qo.data(), nq.data(), (int)nets.size(), isl_dev, srows_dev,
-                                Ce, X, lo, hi, cfg.seed, gen_base, (uint32_t)gen_offset, pred, s));
-            surr_pred += (int64_t)Q * Ce;
+            SPEO_TRY(cand_infer(nets.data(), qo.data(), nq.data(), pi.data(), (int)nets.size(), isl_dev,
+                                srows_dev, Ce, X, lo, hi, cfg.seed, gen_base, (uint32_t)gen_offset,
+                                rho_dev, rho_min, scored_dev, pred, s));
         }
         SPEO_TRY(pick_regen(isl_dev, srows_dev, Q, Ce, pred, rho_dev, rho_min, X, lo, hi, cfg.seed,
                             gen_base, (uint32_t)gen_offset, U[0], s));
@@ -1362,7 +1365,8 @@ void Engine::surrogate_round_host(int TI) {
     const int64_t cap = cfg.surrogate_archive;
     for (int k = 0; k < TI; ++k) {
         last_Ce = any_trained ? C : 1;
-        if (any_trained) surr_pred += (int64_t)Q * last_Ce;
+        // (the one-kernel screening counts its candidates on the device)
+        if (any_trained && (retain_cand || Kp > kFusedMaxKp)) surr_pred += (int64_t)Q * last_Ce;
         for (auto &sp : surr) {
             sp->head = (int)(((int64_t)sp->head + sp->rows) % cap);
             sp->appended += sp->rows;
@@ -1980,6 +1984,14 @@ int speo_engine_get_stats(speo_engine e, speo_engine_stats *out) {
     out->graph_launches = e->e.graph_launches;
     out->rows_per_block = e->e.rt;
     out->surrogate_predictions = e->e.surr_pred;
+    if (e->e.scored_dev) {
+        // the one-kernel screening's count lives on the device
+        const size_t P = (size_t)e->e.cfg.num_problems;
+        std::vector<long long> scored(P, 0);
+        SPEO_TRY_HIP(hipStreamSynchronize(e->e.stream));
+        SPEO_TRY_HIP(hipMemcpy(scored.data(), e->e.scored_dev, sizeof(long long) * P, hipMemcpyDeviceToHost));
+        for (long long v : scored) out->surrogate_predictions += v;
+    }
     out->surrogate_rounds = e->e.surr_rounds;
     out->epoch_parts = e->e.cfg.mode == SPEO_MODE_EPOCH ? e->e.epoch_parts : 0;
     return SPEO_OK;

@@ -110,15 +110,22 @@ struct CandInferBatch {
     const uint16_t *W1b[kMaxSurrProblems], *W2b[kMaxSurrProblems];
     const float *P[kMaxSurrProblems];
     int qoff[kMaxSurrProblems], nq[kMaxSurrProblems], first[kMaxSurrProblems];
+    int prob[kMaxSurrProblems];  // problem index of network k (rho, scored)
     int nnets;
     size_t ob1, ob2, ow3, ob3;
+    const float *rho;  // held-out rho per problem (device)
+    float rho_min;
+    long long *scored;  // candidates scored per problem (device, accumulated)
 };
 // gen_candidates for every problem's targets [qoff[k], qoff[k] + nq[k]) and
 // each problem's network on them, in one kernel that never stores the
-// candidates: pred[q C + c], Kp <= 128.
-int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, int nn, const IslandDesc *isl,
-               const SurrRowDesc *rows, int C, const float *X, float lo, float hi, uint64_t seed,
-               const uint32_t *gen_base, uint32_t gen_offset, float *pred, hipStream_t s);
+// candidates: pred[q C + c], Kp <= 128.  A problem whose rho (device) is not
+// at least rho_min is skipped on the device (its pick is candidate 0 without
+// scores); scored[problem] counts the candidates scored.
+int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, const int *prob, int nn,
+               const IslandDesc *isl, const SurrRowDesc *rows, int C, const float *X, float lo,
+               float hi, uint64_t seed, const uint32_t *gen_base, uint32_t gen_offset,
+               const float *rho, float rho_min, long long *scored, float *pred, hipStream_t s);
 // pick_candidates with the chosen candidate generated again into U.
 int pick_regen(const IslandDesc *isl, const SurrRowDesc *rows, int Q, int C, const float *pred,
                const float *rho, float rho_min, const float *X, float lo, float hi, uint64_t seed,

@@ -1421,6 +1421,10 @@ __global__ __launch_bounds__(1024) void k_cand_infer(CandInferBatch B, const Isl
     __shared__ float red[kHidden / 16][TM];
     int k = 0;
     while (k + 1 < B.nnets && (int)blockIdx.x >= B.first[k + 1]) ++k;
+    // a problem below the rho threshold picks candidate 0 without scores
+    // (the whole block leaves before any barrier)
+    if (!(B.rho[B.prob[k]] >= B.rho_min)) return;
+    if (threadIdx.x == 0 && (int)blockIdx.x == B.first[k]) B.scored[B.prob[k]] += (long long)B.nq[k] * C;
     const int qoff = B.qoff[k], n = B.nq[k] * C;
     const float *P = B.P[k];
     MlpFrags<KP> f;
@@ -1445,9 +1449,10 @@ __global__ __launch_bounds__(1024) void k_cand_infer(CandInferBatch B, const Isl
     mlp_tile64<KP>(Xs, Hs, red, f, n - item0, pred + (size_t)qoff * C + item0);
 }
 
-int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, int nn, const IslandDesc *isl,
-               const SurrRowDesc *rows, int C, const float *X, float lo, float hi, uint64_t seed,
-               const uint32_t *gen_base, uint32_t gen_offset, float *pred, hipStream_t s) {
+int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, const int *prob, int nn,
+               const IslandDesc *isl, const SurrRowDesc *rows, int C, const float *X, float lo,
+               float hi, uint64_t seed, const uint32_t *gen_base, uint32_t gen_offset,
+               const float *rho, float rho_min, long long *scored, float *pred, hipStream_t s) {
     if (nn < 1 || nn > kMaxSurrProblems) return set_error(SPEO_ERR_ARG, "cand_infer: 1..16 problems");
     CandInferBatch B{};
     B.nnets = nn;
@@ -1463,6 +1468,7 @@ int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, int nn, const I
         B.P[k] = N.P;
         B.qoff[k] = qoff[k];
         B.nq[k] = nq[k];
+        B.prob[k] = prob[k];
         B.first[k] = blocks;
         blocks += (nq[k] * C + 63) / 64;
     }
@@ -1470,6 +1476,9 @@ int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, int nn, const I
     B.ob2 = L.ob2;
     B.ow3 = L.ow3;
     B.ob3 = L.ob3;
+    B.rho = rho;
+    B.rho_min = rho_min;
+    B.scored = scored;
     if (blocks == 0) return SPEO_OK;
     const dim3 grid(blocks), block(1024);
 #define SPEO_CAND_INFER(KP)                                                                          \
