@@ -116,6 +116,7 @@ struct CandInferBatch {
     const float *rho;  // held-out rho per problem (device)
     float rho_min;
     long long *scored;  // candidates scored per problem (device, accumulated)
+    int tiles;          // 64-candidate tiles of all problems
 };
 // gen_candidates for every problem's targets [qoff[k], qoff[k] + nq[k]) and
 // each problem's network on them, in one kernel that never stores the

@@ -7,6 +7,9 @@
 // FP32 on the vector ALUs.
 #include "surrogate.h"
 
+#include <algorithm>
+#include <atomic>
+
 #include "de.h"
 #include "eval_core.h"
 
@@ -1405,10 +1408,15 @@ __global__ __launch_bounds__(256) void k_gen_candidates(const IslandDesc *isl,
 }
 
 // The candidates of each problem's targets generated straight into LDS (64
-// per block, 16 lanes per candidate) and scored by that problem's network
+// per tile, 16 lanes per candidate) and scored by that problem's network
 // (mlp_tile64): the same predictions as k_gen_candidates followed by
 // k_mlp_infer, without writing the candidates to memory.  All problems in one
-// launch: problem k owns blocks [first[k], first[k + 1]).
+// launch: problem k owns tiles [first[k], first[k + 1]).  One block per CU
+// walks the tiles (b, b + grid, ...), so a wave loads its weight fragments
+// once per problem instead of once per 64 candidates.  mlp_tile64 reads Xs
+// before its first barrier and Hs before its second, and the next tile's
+// candidates are written after both, so the tiles need no barrier between
+// them.
 template <int KP>
 __global__ __launch_bounds__(1024) void k_cand_infer(CandInferBatch B, const IslandDesc *isl,
                                                      const SurrRowDesc *rows, int C, const float *X,
@@ -1419,34 +1427,57 @@ __global__ __launch_bounds__(1024) void k_cand_infer(CandInferBatch B, const Isl
     __shared__ __align__(16) uint16_t Xs[TM * LX];
     __shared__ __align__(16) uint16_t Hs[TM * LH];
     __shared__ float red[kHidden / 16][TM];
-    int k = 0;
-    while (k + 1 < B.nnets && (int)blockIdx.x >= B.first[k + 1]) ++k;
-    // a problem below the rho threshold picks candidate 0 without scores
-    // (the whole block leaves before any barrier)
-    if (!(B.rho[B.prob[k]] >= B.rho_min)) return;
-    if (threadIdx.x == 0 && (int)blockIdx.x == B.first[k]) B.scored[B.prob[k]] += (long long)B.nq[k] * C;
-    const int qoff = B.qoff[k], n = B.nq[k] * C;
-    const float *P = B.P[k];
-    MlpFrags<KP> f;
-    mlp_prefetch<KP>(f, B.W1b[k], P + B.ob1, B.W2b[k], P + B.ob2, P + B.ow3, P + B.ob3);
-    const int r = threadIdx.x >> 4, l = threadIdx.x & 15;  // candidate r of the block, lane l
-    const int item0 = ((int)blockIdx.x - B.first[k]) * TM, item = item0 + r;
+    const int r = threadIdx.x >> 4, l = threadIdx.x & 15;  // candidate r of the tile, lane l
     const uint32_t gen = (gen_base ? *gen_base : 0u) + gen_offset;
     typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
-    if (item < n) {
-        const SurrRowDesc rd = rows[qoff + item / C];
-        const CandSrc cs = cand_src(isl[rd.island], rd, item % C, X, seed, gen);
-        for (int g = l; g < KP / 4; g += 16) {
-            const float4 v = cand_group(cs, g, seed, lo, hi);
-            *reinterpret_cast<u16x4 *>(&Xs[r * LX + 4 * g]) =
-                u16x4{to_bits(v.x * kXScale, false), to_bits(v.y * kXScale, false),
-                      to_bits(v.z * kXScale, false), to_bits(v.w * kXScale, false)};
+    MlpFrags<KP> f;
+    int kf = -1;  // the network whose fragments f holds
+    for (int tile = blockIdx.x; tile < B.tiles; tile += gridDim.x) {
+        int k = 0;
+        while (k + 1 < B.nnets && tile >= B.first[k + 1]) ++k;
+        // a problem below the rho threshold picks candidate 0 without scores
+        // (every thread reads the same rho, so the whole block skips the tile)
+        if (!(B.rho[B.prob[k]] >= B.rho_min)) continue;
+        if (threadIdx.x == 0 && tile == B.first[k]) B.scored[B.prob[k]] += (long long)B.nq[k] * C;
+        const int qoff = B.qoff[k], n = B.nq[k] * C;
+        if (k != kf) {
+            const float *P = B.P[k];
+            mlp_prefetch<KP>(f, B.W1b[k], P + B.ob1, B.W2b[k], P + B.ob2, P + B.ow3, P + B.ob3);
+            kf = k;
         }
-    } else {
-        for (int g = l; g < KP / 4; g += 16) *reinterpret_cast<u16x4 *>(&Xs[r * LX + 4 * g]) = u16x4{0, 0, 0, 0};
+        const int item0 = (tile - B.first[k]) * TM, item = item0 + r;
+        if (item < n) {
+            const SurrRowDesc rd = rows[qoff + item / C];
+            const CandSrc cs = cand_src(isl[rd.island], rd, item % C, X, seed, gen);
+            for (int g = l; g < KP / 4; g += 16) {
+                const float4 v = cand_group(cs, g, seed, lo, hi);
+                *reinterpret_cast<u16x4 *>(&Xs[r * LX + 4 * g]) =
+                    u16x4{to_bits(v.x * kXScale, false), to_bits(v.y * kXScale, false),
+                          to_bits(v.z * kXScale, false), to_bits(v.w * kXScale, false)};
+            }
+        } else {
+            for (int g = l; g < KP / 4; g += 16)
+                *reinterpret_cast<u16x4 *>(&Xs[r * LX + 4 * g]) = u16x4{0, 0, 0, 0};
+        }
+        __syncthreads();
+        mlp_tile64<KP>(Xs, Hs, red, f, n - item0, pred + (size_t)qoff * C + item0);
     }
-    __syncthreads();
-    mlp_tile64<KP>(Xs, Hs, red, f, n - item0, pred + (size_t)qoff * C + item0);
+}
+
+// One resident block of k_cand_infer per compute unit of the current device.
+static int cand_infer_blocks() {
+    static std::atomic<int> cached[16];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 256;
+    int v = cached[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached[dev].store(cus, std::memory_order_relaxed);
+        v = cus;
+    }
+    return v;
 }
 
 int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, const int *prob, int nn,
@@ -1480,7 +1511,8 @@ int cand_infer(Mlp *const *nets, const int *qoff, const int *nq, const int *prob
     B.rho_min = rho_min;
     B.scored = scored;
     if (blocks == 0) return SPEO_OK;
-    const dim3 grid(blocks), block(1024);
+    B.tiles = blocks;
+    const dim3 grid(std::min(blocks, cand_infer_blocks())), block(1024);
 #define SPEO_CAND_INFER(KP)                                                                          \
     hipLaunchKernelGGL(k_cand_infer<KP>, grid, block, 0, s, B, isl, rows, C, X, lo, hi, seed, gen_base, \
                        gen_offset, pred)
