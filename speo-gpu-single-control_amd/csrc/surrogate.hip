@@ -49,7 +49,14 @@ static int extent_bytes(long rows, long K, long s_row, long s_k) {
     return e > 0 && e < 0x7fffffffL ? (int)e : 0;
 }
 
-__device__ __forceinline__ float sigmoidf_(float a) { return 1.0f / (1.0f + expf(-a)); }
+// sigmoid(a) = 1 / (1 + 2^(-a log2(e))) on the hardware base-2 exponential
+// (v_exp_f32) and reciprocal (v_rcp_f32), both within 1 ulp: about 1e-6
+// relative error for |a| < 20, far below the BF16 rounding of the
+// activations, at a fifth of the instructions of expf and an IEEE divide.
+// Large |a| saturate to 0 and 1 (2^(+big) = inf, 1 / inf = 0).
+__device__ __forceinline__ float sigmoidf_(float a) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(a * -1.44269504088896341f));
+}
 __device__ __forceinline__ float siluf_(float a) { return a * sigmoidf_(a); }
 __device__ __forceinline__ float dsiluf_(float a) {
     const float s = sigmoidf_(a);
