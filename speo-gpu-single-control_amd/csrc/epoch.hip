@@ -630,6 +630,10 @@ __global__ __launch_bounds__(kEpochThreads) void k_island_epoch(EpochArgs a) {
 //   combine (column groups 0..3 in order), selection, write-back of the
 //   selected row.
 constexpr int kSplitThreads = 1024;
+typedef __attribute__((address_space(3))) void *split_lds_ptr;
+// s_waitcnt vmcnt(0), expcnt and lgkmcnt left alone (gfx9 encoding: vmcnt in
+// bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int kWaitVmcnt0 = (7 << 4) | (15 << 8);
 
 template <int PREC>
 __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
@@ -675,8 +679,20 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
     const float *gXin = a.Xin + I.elem_off;
     constexpr int kBatch = 8;
     const int total = NP * ng;
+    // With an LDS row stride of D the island is one linear block in HBM and
+    // in LDS: copied straight into LDS, 1 KB per wave instruction (lane l's
+    // 16 bytes land at the base + 16 l), with no registers or LDS stores on
+    // the way.  Otherwise rows go through registers to their padded slots.
+    const bool linear = (D & 3) == 0 && S == D;
     floatx4 pre[kBatch];
-    {
+    if (linear) {
+        const int nchunk = (total + 63) >> 6;
+        for (int c = wave; c < nchunk; c += kSplitThreads / 64) {
+            const int it = c * 64 + lane;
+            if (it < total)
+                __builtin_amdgcn_global_load_lds(gXin + 4 * it, (split_lds_ptr)(rows + c * 256), 16, 0, 0);
+        }
+    } else {
         const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<float *>(gXin), (short)0, total * 16, kBufferWord3);
 #pragma unroll
@@ -723,25 +739,22 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
         doff[tid] = int4{d_r.x * S, d_r.y * S, d_r.z * S, r * S};
         jr[tid] = d_r.w;
     }
-    const bool linear = (D & 3) == 0 && S == D;
     auto put_row_group = [&](int it, float4 x) {
-        if (linear) {
-            st4(rows + 4 * it, x);
-            return;
-        }
         const int r = div_ng(it), q4 = it - r * ng, j = 4 * q4;
         if (j + 1 >= D) x.y = 0.f;
         if (j + 2 >= D) x.z = 0.f;
         if (j + 3 >= D) x.w = 0.f;
         st4(rows + r * S + 4 * q4, x);
     };
+    if (!linear) {
 #pragma unroll
-    for (int bb = 0; bb < kBatch; ++bb) {
-        const int it = tid + bb * kSplitThreads;
-        if (it < total) put_row_group(it, float4{pre[bb][0], pre[bb][1], pre[bb][2], pre[bb][3]});
+        for (int bb = 0; bb < kBatch; ++bb) {
+            const int it = tid + bb * kSplitThreads;
+            if (it < total) put_row_group(it, float4{pre[bb][0], pre[bb][1], pre[bb][2], pre[bb][3]});
+        }
+        for (int it = tid + kBatch * kSplitThreads; it < total; it += kSplitThreads)
+            put_row_group(it, ld4(gXin + 4 * it));
     }
-    for (int it = tid + kBatch * kSplitThreads; it < total; it += kSplitThreads)
-        put_row_group(it, ld4(gXin + 4 * it));
     if constexpr (P == 0) {
         // the shifted copies' columns [4 ng, ST) are read by the evaluation's
         // blocks of four K steps and stay zero
@@ -758,6 +771,9 @@ __global__ __launch_bounds__(kSplitThreads) void k_island_split(EpochArgs a) {
             a16[r * LDA + 4 * ng + (e - r * pad)] = 0;
         }
     }
+    // the copies into LDS count as outstanding vector-memory loads: every one
+    // of this wave's has landed before the barrier
+    if (linear) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
 
     // the rotation fragments, needed from the evaluation on: issued now, they
