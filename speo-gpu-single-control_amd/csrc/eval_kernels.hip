@@ -118,21 +118,57 @@ __global__ void k_eval_finalize(const float *part, int nchunks, int n, int dim, 
 // normalised by their maximum, so no weight underflows before the ratio is
 // formed; components with d_c == 0 share all the weight equally (cf_cal gives
 // each of them w = 1e99, beside which every other weight vanishes).
+// One wave per row: the row's squared distance to each component's shift
+// (per-lane partial sums, then a butterfly over the 64 lanes), the cf_cal
+// weights in the log domain and the weighted sum.  When the row stride, X and
+// every shift allow 16-byte loads and dim % 4 == 0, lane l takes coordinates
+// 4 (l + 64 i) .. + 3 and the row (dim <= 1024) is read once into registers
+// for all components; otherwise coordinates l + 64 i, one at a time.
 __global__ void k_comp_combine(const CompParams *comps, int ncomp, int dim, const float *X,
                                int ld, int n, const float *g, float *f_out) {
+    constexpr int kGroups = 4;  // float4 groups per lane held in registers
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (wave >= n) return;
     const float *x = X + (size_t)wave * ld;
+    const int ng = dim >> 2;
+    bool vec = (dim & 3) == 0 && (ld & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+               ng <= 64 * kGroups;
+    for (int c = 0; c < ncomp; ++c) vec = vec && (reinterpret_cast<uintptr_t>(comps[c].shift32) & 15) == 0;
+    float4 xr[kGroups];
+    if (vec) {
+#pragma unroll
+        for (int i = 0; i < kGroups; ++i) {
+            const int q = lane + 64 * i;
+            xr[i] = q < ng ? *reinterpret_cast<const float4 *>(x + 4 * q) : float4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
     float lw[kMaxComp];
     bool zero[kMaxComp];
     bool any_zero = false;
     float lwmax = -INFINITY;
     for (int c = 0; c < ncomp; ++c) {
+        const float *o = comps[c].shift32;
         float d = 0.f;
-        for (int j = lane; j < dim; j += 64) {
-            const float v = x[j] - comps[c].shift32[j];
-            d = fmaf(v, v, d);
+        if (vec) {
+#pragma unroll
+            for (int i = 0; i < kGroups; ++i) {
+                const int q = lane + 64 * i;
+                if (q < ng) {
+                    const float4 ov = *reinterpret_cast<const float4 *>(o + 4 * q);
+                    const float v0 = xr[i].x - ov.x, v1 = xr[i].y - ov.y, v2 = xr[i].z - ov.z,
+                                v3 = xr[i].w - ov.w;
+                    d = fmaf(v0, v0, d);
+                    d = fmaf(v1, v1, d);
+                    d = fmaf(v2, v2, d);
+                    d = fmaf(v3, v3, d);
+                }
+            }
+        } else {
+            for (int j = lane; j < dim; j += 64) {
+                const float v = x[j] - o[j];
+                d = fmaf(v, v, d);
+            }
         }
         for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
         zero[c] = d == 0.f;
