@@ -189,6 +189,93 @@ __global__ void k_comp_combine(const CompParams *comps, int ncomp, int dim, cons
     }
 }
 
+// k_comp_combine's 16-byte path for many rows: the shifts staged in LDS once
+// per block, kCombRows rows per wave with the next row's loads (X and the
+// component values g) in flight while the current row is combined.  Lane c
+// (c < ncomp) forms component c's weight and value, and lane 0 sums them in
+// component order: the same arithmetic as k_comp_combine, result for result.
+constexpr int kCombRows = 4;
+__global__ __launch_bounds__(256) void k_comp_combine_rows(const CompParams *comps, int ncomp, int dim,
+                                                           const float *X, int ld, int n,
+                                                           const float *g, float *f_out) {
+    constexpr int kGroups = 4;
+    extern __shared__ __align__(16) float osh[];  // [ncomp][dim]
+    __shared__ float sig[kMaxComp], lam[kMaxComp], bia[kMaxComp];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ng = dim >> 2;
+    for (int c = 0; c < ncomp; ++c)
+        for (int q = tid; q < ng; q += 256)
+            reinterpret_cast<float4 *>(osh + (size_t)c * dim)[q] =
+                reinterpret_cast<const float4 *>(comps[c].shift32)[q];
+    if (tid < ncomp) {
+        sig[tid] = (float)comps[tid].sigma;
+        lam[tid] = (float)comps[tid].lambda;
+        bia[tid] = (float)comps[tid].bias;
+    }
+    const int row0 = (blockIdx.x * 4 + wave) * kCombRows;
+    auto load = [&](int r, float4 (&xr)[kGroups], float &gv) {
+#pragma unroll
+        for (int i = 0; i < kGroups; ++i) {
+            const int q = lane + 64 * i;
+            xr[i] = r < n && q < ng ? *reinterpret_cast<const float4 *>(X + (size_t)r * ld + 4 * q)
+                                    : float4{0.f, 0.f, 0.f, 0.f};
+        }
+        gv = r < n && lane < ncomp ? g[(size_t)lane * n + r] : 0.f;
+    };
+    float4 cur[kGroups], nxt[kGroups];
+    float gcur, gnxt;
+    load(row0, cur, gcur);
+    __syncthreads();
+    for (int k = 0; k < kCombRows; ++k) {
+        const int r = row0 + k;
+        if (r >= n) break;  // wave-uniform
+        load(r + 1, nxt, gnxt);
+        float my_lw = -INFINITY, lwmax = -INFINITY;
+        bool my_zero = false, any_zero = false;
+        for (int c = 0; c < ncomp; ++c) {
+            const float *o = osh + (size_t)c * dim;
+            float d = 0.f;
+#pragma unroll
+            for (int i = 0; i < kGroups; ++i) {
+                const int q = lane + 64 * i;
+                if (q < ng) {
+                    const float4 ov = *reinterpret_cast<const float4 *>(o + 4 * q);
+                    const float v0 = cur[i].x - ov.x, v1 = cur[i].y - ov.y, v2 = cur[i].z - ov.z,
+                                v3 = cur[i].w - ov.w;
+                    d = fmaf(v0, v0, d);
+                    d = fmaf(v1, v1, d);
+                    d = fmaf(v2, v2, d);
+                    d = fmaf(v3, v3, d);
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+            const float s = sig[c];
+            const float lw = -0.5f * logf(d) - d / (2.0f * (float)dim * s * s);
+            any_zero = any_zero || d == 0.f;
+            lwmax = fmaxf(lwmax, lw);
+            if (lane == c) {
+                my_lw = lw;
+                my_zero = d == 0.f;
+            }
+        }
+        float w = 0.f, fit = 0.f;
+        if (lane < ncomp) {
+            w = any_zero ? (my_zero ? 1.f : 0.f) : expf(my_lw - lwmax);
+            fit = fmaf(lam[lane], gcur, bia[lane]);
+        }
+        float wsum = 0.f, acc = 0.f;
+        for (int c = 0; c < ncomp; ++c) {
+            const float wc = __shfl(w, c, 64), fc = __shfl(fit, c, 64);
+            wsum += wc;
+            acc = fmaf(wc, fc, acc);
+        }
+        if (lane == 0) f_out[r] = acc / wsum;
+#pragma unroll
+        for (int i = 0; i < kGroups; ++i) cur[i] = nxt[i];
+        gcur = gnxt;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_eval_f64(const CompParams *comps, int ncomp, int dim,
                                                   const float *X, int ld, int n, double *f_out) {
     extern __shared__ __align__(16) char lds_raw[];
@@ -297,6 +384,15 @@ int eval_problem(const ProblemHost &P, const float *X, int ld, int n, float *f_o
     for (int c = 0; c < P.ncomp; ++c)
         SPEO_TRY(eval_component(P.view(c, prec_of(c)), X, ld, n, P.dim, gbuf + (size_t)c * n, part,
                                 1.f, 0.f, 1, prec_of(c), rows_per_block, s));
+    bool rows = (P.dim & 3) == 0 && P.dim <= 1024 && (ld & 3) == 0 &&
+                (reinterpret_cast<uintptr_t>(X) & 15) == 0 && n >= 4096;
+    for (int c = 0; c < P.ncomp; ++c) rows = rows && (reinterpret_cast<uintptr_t>(P.comp[c].shift32) & 15) == 0;
+    if (rows) {
+        hipLaunchKernelGGL(k_comp_combine_rows, dim3((n + 4 * kCombRows - 1) / (4 * kCombRows)), dim3(256),
+                           sizeof(float) * (size_t)P.ncomp * P.dim, s, P.params_dev, P.ncomp, P.dim, X, ld,
+                           n, gbuf, f_out);
+        return check_launch("k_comp_combine_rows");
+    }
     const int waves_per_block = 4;
     hipLaunchKernelGGL(k_comp_combine, dim3((n + waves_per_block - 1) / waves_per_block),
                        dim3(64 * waves_per_block), 0, s, P.params_dev, P.ncomp, P.dim, X, ld, n,
