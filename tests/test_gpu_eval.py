@@ -278,10 +278,37 @@ def test_composition(cuda, speo, oracle, D, objs):
     np.testing.assert_allclose(f64.cpu().numpy(), ref, rtol=1e-12, atol=1e-9)
 
 
-def test_composition_components_sharing_a_shift(cuda, speo, oracle):
+@pytest.mark.parametrize("D", [100, 30])
+def test_composition_many_rows(cuda, speo, oracle, D):
+    # 4100 rows: at D = 100 the many-row combine (shifts in LDS, four rows per
+    # wave), at D = 30 (not a multiple of 4) the one-wave-per-row combine
+    import torch
+    rng = np.random.default_rng(D + 11)
+    os_ = np.stack([oracle.gen_shift(s, D) for s in (41, 42, 43)])
+    Ms = np.stack([oracle.gen_rotation(s, D) for s in (41, 42, 43)])
+    objs = [1, 2, 3]
+    sigma, bias = np.array([10.0, 20.0, 30.0]), np.array([0.0, 100.0, 200.0])
+    prob = speo.Problem(objs, os_, Ms, sigma=sigma, bias=bias)
+    n = 4100
+    X = _points(rng, n, D, D)
+    X[4099, :D] = os_[2]                      # the last row on component 3's shift
+    X[17, :D] = os_[0] + 1e-3
+    ref = oracle.composition(X[:, :D], objs, os_, Ms, sigma, bias)
+    rtol = composition_rtol(objs, X[:, :D], os_, Ms, base=1e-5, u=2.0 ** -24)
+    f = torch.zeros(n, dtype=torch.float32, device=cuda)
+    speo.evaluate(prob, _t(X, cuda), f)
+    torch.cuda.synchronize()
+    got = f.cpu().numpy()
+    assert np.all(np.abs(got - ref) <= rtol * np.abs(ref) + 1e-4), np.max(np.abs(got - ref) / np.abs(ref))
+    assert got[4099] == pytest.approx(200.0)
+
+
+@pytest.mark.parametrize("n", [3, 4100])
+def test_composition_components_sharing_a_shift(cuda, speo, oracle, n):
     # Two components with the same shift: at that point both have d = 0 and
     # cf_cal gives each w = 1e99 (CEC2014.cc:1495-1498), so they share the
-    # weight equally and the third gets none.
+    # weight equally and the third gets none.  4100 rows (the three cases
+    # repeated) take the many-row combine, 3 rows the one-wave-per-row form.
     import torch
     D = 20
     os_ = np.stack([oracle.gen_shift(s, D) for s in (31, 32, 33)])
@@ -294,11 +321,12 @@ def test_composition_components_sharing_a_shift(cuda, speo, oracle):
     X[0] = os_[1]
     X[1] = os_[0]
     X[2] = os_[1] + 0.5
+    X = np.ascontiguousarray(np.tile(X, (n // 3 + 1, 1))[:n])
     ref = oracle.composition(X, objs, os_, Ms, sigma, bias)
     assert ref[0] == pytest.approx(150.0)          # (fit_2 + fit_3) / 2 = (100 + 200) / 2
-    f = torch.zeros(3, dtype=torch.float32, device=cuda)
+    f = torch.zeros(n, dtype=torch.float32, device=cuda)
     speo.evaluate(prob, _t(X, cuda), f)
-    f64 = torch.zeros(3, dtype=torch.float64, device=cuda)
+    f64 = torch.zeros(n, dtype=torch.float64, device=cuda)
     speo.evaluate_f64(prob, _t(X, cuda), f64)
     torch.cuda.synchronize()
     # row 1 sits on component 1's shift: f = 0 up to the other components'
