@@ -1274,10 +1274,13 @@ int Mlp::train_step(const float *AX, const float *Ay, int n_valid, uint64_t seed
 // ------------------------------------------------------------- statistics
 
 // Spearman rank correlation of a[0..n) and b[0..n) (average ranks for ties),
-// n <= 4096.  k_spearman_ranks: 256 values per block, each thread counting
-// the values below and equal to its own over the whole array in LDS, writes
-// the centred ranks; k_spearman_corr: one block sums their products in a
-// fixed order.
+// n <= 4096.  k_spearman_ranks: the whole array in LDS, 32 values per block,
+// the values below and equal to value i counted by kRankLanes lanes, each
+// over its own stretch of the array, and the integer counts added across
+// those lanes (exact, so the split does not change them); it writes the
+// centred ranks.  k_spearman_corr: one block sums their products in a fixed
+// order.
+constexpr int kRankLanes = 8;
 __global__ __launch_bounds__(256) void k_spearman_ranks(const float *a, const float *b, int n,
                                                         float2 *ranks) {
     __shared__ float as[4096], bs[4096];
@@ -1286,17 +1289,26 @@ __global__ __launch_bounds__(256) void k_spearman_ranks(const float *a, const fl
         bs[i] = b[i];
     }
     __syncthreads();
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    const int i = (blockIdx.x * 256 + threadIdx.x) / kRankLanes, part = threadIdx.x % kRankLanes;
+    const bool valid = i < n;
+    const int span = (n + kRankLanes - 1) / kRankLanes, j0 = part * span, j1 = min(n, j0 + span);
     int la = 0, ea = 0, lb = 0, eb = 0;
-    const float ai = as[i], bi = bs[i];
-    for (int j = 0; j < n; ++j) {
+    const float ai = valid ? as[i] : 0.f, bi = valid ? bs[i] : 0.f;
+    for (int j = j0; j < j1; ++j) {
         const float aj = as[j], bj = bs[j];
         la += aj < ai;
         ea += aj == ai;
         lb += bj < bi;
         eb += bj == bi;
     }
+#pragma unroll
+    for (int off = 1; off < kRankLanes; off <<= 1) {
+        la += __shfl_xor(la, off, 64);
+        ea += __shfl_xor(ea, off, 64);
+        lb += __shfl_xor(lb, off, 64);
+        eb += __shfl_xor(eb, off, 64);
+    }
+    if (!valid || part != 0) return;
     const float mean = 0.5f * (float)(n - 1);  // ranks are multiples of 1/2: exact in FP32
     ranks[i] = float2{(float)la + 0.5f * (float)(ea - 1) - mean, (float)lb + 0.5f * (float)(eb - 1) - mean};
 }
@@ -1327,7 +1339,7 @@ __global__ __launch_bounds__(1024) void k_spearman_corr(const float2 *ranks, int
 
 int spearman(const float *a, const float *b, int n, float *rho_out, float2 *ranks, hipStream_t s) {
     if (n < 2 || n > 4096) return set_error(SPEO_ERR_ARG, "spearman: need 2 <= n <= 4096");
-    hipLaunchKernelGGL(k_spearman_ranks, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n, ranks);
+    hipLaunchKernelGGL(k_spearman_ranks, dim3((n * kRankLanes + 255) / 256), dim3(256), 0, s, a, b, n, ranks);
     SPEO_TRY(check_launch("k_spearman_ranks"));
     hipLaunchKernelGGL(k_spearman_corr, dim3(1), dim3(1024), 0, s, ranks, n, rho_out);
     return check_launch("k_spearman_corr");
