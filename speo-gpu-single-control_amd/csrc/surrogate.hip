@@ -1710,7 +1710,21 @@ int gather_holdout(const float *AX, const float *Ay, int H, int Kp, float *Xh, f
 __global__ __launch_bounds__(1024) void k_mean_std(const float *y, int n, float *stats) {
     __shared__ double s1[1024], s2[1024];
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    // thread t sums y[t], y[t + 1024], ... in that order; eight loads are
+    // issued before their sums instead of one round trip per value
+    constexpr int kB = 8;
+    int i = threadIdx.x;
+    for (; i + (kB - 1) * 1024 < n; i += kB * 1024) {
+        float v[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) v[u] = y[i + u * 1024];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            a += v[u];
+            b += (double)v[u] * v[u];
+        }
+    }
+    for (; i < n; i += 1024) {
         a += y[i];
         b += (double)y[i] * y[i];
     }
